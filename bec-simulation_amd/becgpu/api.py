@@ -1,0 +1,539 @@
+"""Public entry points mirroring SpinorBEC.jl, running the hot path on the GPU.
+
+make_workspace (initialization.jl:86-191), split_step! (split_step.jl:10-35),
+run_simulation! (simulation.jl:1-94), find_ground_state (ground_state.jl:80-163),
+find_ground_state_multistart (ground_state.jl:239-271), scan_continuation
+(ground_state.jl:328-378), run_simulation_yoshida! (yoshida.jl:11-98),
+total_energy (energy.jl:6-54), total_norm / magnetization (observables.jl:1-33).
+
+Julia's `!` suffix is dropped (split_step, run_simulation, ...); the functions
+mutate the workspace exactly as the reference's do.  psi arrays are numpy
+complex128 of shape (D, nz, ny, nx) -- byte-identical to the reference's
+column-major (nx, ny, nz, D) array.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .physics import (AdaptiveDtParams, Grid, HarmonicTrap, InteractionParams, NoPotential,
+                      SimParams, SpinSystem, ZeemanParams, c0c1_to_gS, c_extra_to_delta_gS,
+                      cell_volume, classify_phase, evaluate_potential, init_psi, spin_matrices,
+                      zeeman_energies)
+
+ITP_EXPONENT_LIMIT = 50.0          # ground_state.jl:1
+
+
+@dataclass
+class SimulationResult:
+    """simulation_utils.jl:1-7."""
+    times: list = field(default_factory=list)
+    energies: list = field(default_factory=list)
+    norms: list = field(default_factory=list)
+    magnetizations: list = field(default_factory=list)
+    psi_snapshots: list = field(default_factory=list)
+
+
+def _precision_code(precision):
+    p = str(precision).lower()
+    if p in ("fp64", "float64", "complexf64", "double"):
+        return L.BEC_FP64
+    if p in ("fp32", "float32", "complexf32", "fp32-bulk", "fp32_bulk"):
+        return L.BEC_FP32
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+class Workspace:
+    """Device-resident workspace of B instances of one shape (types.jl:387-407)."""
+
+    def __init__(self, *, grid: Grid, F: int, n_instances: int, sim_params: SimParams,
+                 precision="fp64", enable_ddi=False, secular_ddi=False, ddi_padding=False):
+        self.grid = grid
+        self.F = F
+        self.sys = SpinSystem(F)
+        self.D = 2 * F + 1
+        self.B = n_instances
+        self.sim_params = sim_params
+        self.precision = _precision_code(precision)
+        self.enable_ddi = bool(enable_ddi)
+        sm = spin_matrices(F)
+        cfg = L.BecConfig()
+        nd = grid.ndim
+        cfg.ndim = nd
+        for d in range(3):
+            cfg.n[d] = grid.config.n_points[d] if d < nd else 1
+            cfg.L[d] = grid.config.box_size[d] if d < nd else 1.0
+        cfg.F = F
+        cfg.n_instances = n_instances
+        cfg.precision = self.precision
+        cfg.imaginary_time = 1 if sim_params.imaginary_time else 0
+        cfg.normalize_every = int(sim_params.normalize_every)
+        cfg.enable_ddi = 1 if enable_ddi else 0
+        cfg.ddi_secular = 1 if secular_ddi else 0
+        cfg.ddi_padding = 1 if ddi_padding else 0
+        vecs = np.ascontiguousarray(sm["fy_vecs"], dtype=np.complex128)
+        xt = np.ascontiguousarray(np.concatenate(grid.x), dtype=np.float64)
+        kt = np.ascontiguousarray(np.concatenate(grid.k), dtype=np.float64)
+        self._keep = (vecs, xt, kt)
+        cfg.fy_vecs = vecs.ctypes.data_as(C.POINTER(C.c_double))
+        cfg.x_tab = xt.ctypes.data_as(C.POINTER(C.c_double))
+        cfg.k_tab = kt.ctypes.data_as(C.POINTER(C.c_double))
+        self._cfg = cfg
+        h = C.c_void_p()
+        L.check(L.lib().bec_ws_create(C.byref(cfg), C.byref(h)), "bec_ws_create")
+        self._h = h
+        self.t = np.zeros(n_instances)
+        self.step = np.zeros(n_instances, dtype=np.int64)
+        self.params = [None] * n_instances
+        self.interactions = [None] * n_instances
+        self.zeeman = [ZeemanParams()] * n_instances
+        self._active = list(range(n_instances))
+
+    # -- handle ------------------------------------------------------------
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("workspace destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().bec_ws_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def bytes_for(grid: Grid, F: int, n_instances=1, precision="fp64", enable_ddi=False):
+        cfg = L.BecConfig()
+        nd = grid.ndim
+        cfg.ndim = nd
+        for d in range(3):
+            cfg.n[d] = grid.config.n_points[d] if d < nd else 1
+        cfg.F, cfg.n_instances = F, n_instances
+        cfg.precision = _precision_code(precision)
+        cfg.enable_ddi = 1 if enable_ddi else 0
+        return int(L.lib().bec_workspace_bytes(C.byref(cfg)))
+
+    # -- parameters / state --------------------------------------------------
+    def set_instance(self, inst, *, interactions: InteractionParams, zeeman=ZeemanParams(),
+                     potential=NoPotential(), dt=None, c_dd=0.0):
+        p = L.BecInstParams()
+        p.dt = float(self.sim_params.dt if dt is None else dt)
+        p.c0, p.c1, p.c_lhy = interactions.c0, interactions.c1, interactions.c_lhy
+        p.c_dd = float(c_dd)
+        p.p, p.q = zeeman.p, zeeman.q
+        if isinstance(potential, HarmonicTrap):
+            for d in range(self.grid.ndim):
+                p.omega[d] = potential.omega[d]
+            L.check(L.lib().bec_set_params(self.handle, inst, C.byref(p)), "bec_set_params")
+        elif isinstance(potential, NoPotential) or potential is None:
+            L.check(L.lib().bec_set_params(self.handle, inst, C.byref(p)), "bec_set_params")
+        else:
+            L.check(L.lib().bec_set_params(self.handle, inst, C.byref(p)), "bec_set_params")
+            V = np.ascontiguousarray(evaluate_potential(potential, self.grid), dtype=np.float64)
+            L.check(L.lib().bec_set_potential(self.handle, inst, V.ctypes.data), "bec_set_potential")
+        self.params[inst] = p
+        self.interactions[inst] = interactions
+        self.zeeman[inst] = zeeman
+
+    def set_psi(self, psi, inst=0):
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        if a.shape != (self.D,) + self.grid.shape:
+            raise ValueError(f"psi shape {a.shape} != {(self.D,) + self.grid.shape}")
+        L.check(L.lib().bec_set_psi(self.handle, inst, a.ctypes.data), "bec_set_psi")
+
+    def get_psi(self, inst=0):
+        out = np.empty((self.D,) + self.grid.shape, dtype=np.complex128)
+        L.check(L.lib().bec_get_psi(self.handle, inst, out.ctypes.data), "bec_get_psi")
+        return out
+
+    @property
+    def psi(self):
+        return self.get_psi(0)
+
+    def set_active(self, insts):
+        a = np.ascontiguousarray(list(insts), dtype=np.int32)
+        L.check(L.lib().bec_set_active(self.handle, a.ctypes.data, len(a)), "bec_set_active")
+        self._active = [int(v) for v in a]
+
+    @property
+    def active(self):
+        return list(self._active)
+
+    # -- device ops ------------------------------------------------------------
+    def energy_terms(self):
+        out = np.zeros((self.B, L.E_COUNT))
+        L.check(L.lib().bec_energy(self.handle, out.ctypes.data), "bec_energy")
+        return out
+
+    def diff(self, a, b, mode):
+        out = np.zeros((self.B, 2))
+        L.check(L.lib().bec_diff(self.handle, a, b, mode, out.ctypes.data), "bec_diff")
+        return out
+
+    def copy_slot(self, dst, src):
+        L.check(L.lib().bec_copy_slot(self.handle, dst, src), "bec_copy_slot")
+
+    def itp_guard(self):
+        out = np.zeros((self.B, 2))
+        L.check(L.lib().bec_itp_guard(self.handle, out.ctypes.data), "bec_itp_guard")
+        return out
+
+    def profile(self, on: bool):
+        L.check(L.lib().bec_profile_enable(self.handle, 1 if on else 0), "bec_profile_enable")
+
+    def profile_read(self):
+        """{kernel class: (total_ms, launches)} since profile(True)."""
+        ms = np.zeros(len(L.K_NAMES))
+        cnt = np.zeros(len(L.K_NAMES), dtype=np.int64)
+        L.check(L.lib().bec_profile_read(self.handle, ms.ctypes.data, cnt.ctypes.data),
+                "bec_profile_read")
+        return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(L.K_NAMES)}
+
+    def synchronize(self):
+        L.check(L.lib().bec_synchronize(self.handle), "bec_synchronize")
+
+
+# ---------------------------------------------------------------------------
+# make_workspace
+# ---------------------------------------------------------------------------
+def _tensor_active(F, interactions: InteractionParams):
+    """initialization.jl:164-185: even-rank c_extra with 4 <= k <= 2F activates the
+    general-F tensor step."""
+    return any((i + 1) % 2 == 0 and 4 <= i + 1 <= 2 * F and abs(v) > 1e-30
+               for i, v in enumerate(interactions.c_extra, start=1))
+
+
+def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
+                   zeeman=ZeemanParams(), potential=NoPotential(), sim_params: SimParams,
+                   psi_init=None, enable_ddi=False, c_dd=float("nan"), secular_ddi=False,
+                   raman=None, loss=None, ddi_padding=False, precision="fp64", fft_flags=None):
+    F = atom.F
+    if raman is not None:
+        raise NotImplementedError("Raman coupling is outside the GPU hot path")
+    if loss is not None:
+        raise NotImplementedError("loss step is not built on the GPU path yet")
+    if _tensor_active(F, interactions):
+        raise NotImplementedError("general-F tensor interaction step is not built yet")
+    c_dd_val = 0.0
+    if enable_ddi:
+        if math.isnan(c_dd):
+            if atom.mu_mag > 0.0:
+                raise ValueError(
+                    f"enable_ddi=true for dipolar atom {atom.name} but c_dd not specified.")
+            c_dd_val = 0.0
+        else:
+            c_dd_val = float(c_dd)
+    ws = Workspace(grid=grid, F=F, n_instances=1, sim_params=sim_params, precision=precision,
+                   enable_ddi=enable_ddi, secular_ddi=secular_ddi, ddi_padding=ddi_padding)
+    ws.set_instance(0, interactions=interactions, zeeman=zeeman, potential=potential,
+                    c_dd=c_dd_val)
+    psi = init_psi(grid, SpinSystem(F)) if psi_init is None else np.array(psi_init, dtype=complex)
+    ws.set_psi(psi, 0)
+    ws.atom = atom
+    ws.potential = potential
+    return ws
+
+
+# ---------------------------------------------------------------------------
+# Observables
+# ---------------------------------------------------------------------------
+def energy_terms(ws: Workspace, inst=0):
+    row = ws.energy_terms()[inst]
+    return {name: float(row[i]) for i, name in enumerate(L.E_NAMES)}
+
+
+def total_energy(ws: Workspace, inst=0):
+    return float(ws.energy_terms()[inst, L.E_TOTAL])
+
+
+def total_norm(ws: Workspace, inst=0):
+    return float(ws.energy_terms()[inst, L.E_NORM])
+
+
+def magnetization(ws: Workspace, inst=0):
+    return float(ws.energy_terms()[inst, L.E_MZ])
+
+
+def _record_snapshot(ws, res: SimulationResult, keep_psi=True, inst=0):
+    e = ws.energy_terms()[inst]
+    res.times.append(float(ws.t[inst]))
+    res.energies.append(float(e[L.E_TOTAL]))
+    res.norms.append(float(e[L.E_NORM]))
+    res.magnetizations.append(float(e[L.E_MZ]))
+    if keep_psi:
+        res.psi_snapshots.append(ws.get_psi(inst))
+
+
+# ---------------------------------------------------------------------------
+# Integrators
+# ---------------------------------------------------------------------------
+def split_step(ws: Workspace, n: int = 1):
+    """split_step! (split_step.jl:10-35), n times."""
+    L.check(L.lib().bec_split_step(ws.handle, int(n)), "bec_split_step")
+    if not ws.sim_params.imaginary_time:
+        for i in ws.active:
+            for _ in range(n):
+                ws.t[i] += ws.params[i].dt
+    ws.step[ws.active] += n
+
+
+def run_simulation(ws: Workspace, callback=None, keep_snapshots=True):
+    """run_simulation! (simulation.jl:1-94): ITP -> split_step loop with saves every
+    save_every; real time -> leapfrog with merged V(dt) and splits at save points."""
+    sp = ws.sim_params
+    res = SimulationResult()
+    _record_snapshot(ws, res, keep_snapshots)
+    if sp.imaginary_time:
+        done = 0
+        while done < sp.n_steps:
+            k = min(sp.save_every - done % sp.save_every, sp.n_steps - done)
+            split_step(ws, k)
+            done += k
+            if done % sp.save_every == 0:
+                _record_snapshot(ws, res, keep_snapshots)
+                if callback is not None:
+                    callback(ws, done)
+        return res
+    dt = ws.params[0].dt
+    done = 0
+    while done < sp.n_steps:
+        k = min(sp.save_every - done % sp.save_every, sp.n_steps - done)
+        L.check(L.lib().bec_leapfrog_segment(ws.handle, k), "bec_leapfrog_segment")
+        for _ in range(k):
+            ws.t[0] += dt
+        ws.step[0] += k
+        done += k
+        if done % sp.save_every == 0:
+            _record_snapshot(ws, res, keep_snapshots)
+            if callback is not None:
+                callback(ws, done)
+    return res
+
+
+def strang_core(ws: Workspace, dt: float):
+    L.check(L.lib().bec_strang_core(ws.handle, float(dt)), "bec_strang_core")
+
+
+def yoshida_core(ws: Workspace, dt: float):
+    L.check(L.lib().bec_yoshida_core(ws.handle, float(dt)), "bec_yoshida_core")
+
+
+def run_simulation_yoshida(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end: float,
+                           save_interval: float, callback=None, keep_snapshots=True,
+                           max_accepted=None):
+    """run_simulation_yoshida! (yoshida.jl:11-98): adaptive S4 with the embedded S2
+    error estimate ||psi_Y - psi_S||^2/||psi_S||^2 and the reference's PI rules.
+    psi_old, psi_strang stay on the device (slots 2 and 3)."""
+    dt = min(max(adaptive.dt_init, adaptive.dt_min), adaptive.dt_max)
+    res = SimulationResult()
+    _record_snapshot(ws, res, keep_snapshots)
+    next_save = ws.t[0] + save_interval
+    n_acc = n_rej = 0
+    while ws.t[0] < t_end - 1e-14:
+        if max_accepted is not None and n_acc >= max_accepted:
+            break
+        t = ws.t[0]
+        dt_step = min(dt, t_end - t)
+        rem = next_save - t
+        if 1e-14 < rem < dt_step:
+            dt_step = rem
+        dt_step = max(dt_step, adaptive.dt_min)
+        clamped = dt_step < dt * 0.99
+        ws.copy_slot(2, 0)                 # psi_old
+        strang_core(ws, dt_step)
+        ws.copy_slot(3, 0)                 # psi_strang
+        ws.copy_slot(0, 2)
+        yoshida_core(ws, dt_step)
+        d = ws.diff(0, 3, 0)[0]
+        err = float(d[0] / max(d[1], 1e-300))
+        if (not clamped) and err > adaptive.tol and dt_step > adaptive.dt_min * 1.01:
+            ws.copy_slot(0, 2)
+            dt = max(dt_step * max(0.3, 0.9 * (adaptive.tol / err) ** 0.2), adaptive.dt_min)
+            n_rej += 1
+            continue
+        ws.t[0] += dt_step
+        ws.step[0] += 1
+        n_acc += 1
+        if not clamped:
+            fac = min(3.0, 0.9 * (adaptive.tol / err) ** 0.2) if err > 1e-300 else 3.0
+            dt = min(max(dt * fac, adaptive.dt_min), adaptive.dt_max)
+        if ws.t[0] >= next_save - 1e-14:
+            _record_snapshot(ws, res, keep_snapshots)
+            if callback is not None:
+                callback(ws, n_acc)
+            next_save += save_interval
+    if not res.times or abs(res.times[-1] - ws.t[0]) > 1e-12:
+        _record_snapshot(ws, res, keep_snapshots)
+    return dict(result=res, n_accepted=n_acc, n_rejected=n_rej, final_dt=dt)
+
+
+# ---------------------------------------------------------------------------
+# Ground state drivers (ground_state.jl)
+# ---------------------------------------------------------------------------
+def _validate_itp_zeeman(zeeman: ZeemanParams, F, dt):
+    """ground_state.jl:31-42."""
+    mx = max(abs(v) for v in zeeman_energies(zeeman, F))
+    if mx * dt / 4 > ITP_EXPONENT_LIMIT:
+        raise ValueError(f"Zeeman p={zeeman.p} with F={F} and dt={dt} causes overflow in "
+                         f"imaginary time (exponent={mx * dt / 4:.1f} > {ITP_EXPONENT_LIMIT}).")
+
+
+def _validate_itp_interactions(interactions, F, dt, psi=None, c_dd=0.0, grid=None):
+    """ground_state.jl:46-78."""
+    n_peak = float(np.max(np.sum(np.abs(psi) ** 2, axis=0))) if psi is not None else 1.0
+    mc = max(abs(interactions.c0), abs(interactions.c1))
+    if mc > 1e-30 and mc * n_peak * dt / 4 > ITP_EXPONENT_LIMIT:
+        raise ValueError(f"Spin interaction c0={interactions.c0}, c1={interactions.c1} may cause "
+                         f"overflow in imaginary time (estimated exponent="
+                         f"{mc * n_peak * dt / 4:.1f} > {ITP_EXPONENT_LIMIT}).")
+    if c_dd > 0 and c_dd * F * n_peak * dt / 2 > ITP_EXPONENT_LIMIT:
+        raise ValueError(f"DDI c_dd={c_dd} with F={F} may cause overflow in imaginary time.")
+
+
+def _check_itp_overflow(ws: Workspace, step: int, insts=None):
+    """ground_state.jl:4-29 on the device-reduced guard values."""
+    g = ws.itp_guard()
+    for i in (ws.active if insts is None else insts):
+        if not np.isfinite(g[i, 0]):
+            raise ValueError(f"NaN detected in ITP at step {step}. Likely DDI potential overflow. "
+                             "Reduce dt.")
+        if ws.enable_ddi:
+            exponent = g[i, 1] * ws.params[i].dt / 2
+            if exponent > ITP_EXPONENT_LIMIT:
+                raise ValueError(f"DDI potential overflow in ITP at step {step}: "
+                                 f"max|Phi|={g[i, 1]:.3g}, exponent={exponent:.1f} > "
+                                 f"{ITP_EXPONENT_LIMIT}. Reduce dt (current={ws.params[i].dt}).")
+
+
+def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int):
+    """The ITP convergence loop of find_ground_state for every active instance of ws
+    (ground_state.jl:132-162).  Instances retire independently when converged."""
+    insts = ws.active
+    E_prev = {i: float(ws.energy_terms()[i, L.E_TOTAL]) for i in insts}
+    ws.copy_slot(1, 0)
+    out = {i: dict(converged=False, dE=float("nan"), dpsi=float("nan"), steps=0) for i in insts}
+    running = list(insts)
+    step = 0
+    while step < n_steps and running:
+        if step < 10:
+            k = 1
+        else:
+            k = min(save_every - step % save_every, n_steps - step)
+        split_step(ws, k)
+        step += k
+        if step <= 10:
+            _check_itp_overflow(ws, step, running)
+        if step % save_every == 0:
+            et = ws.energy_terms()
+            dd = ws.diff(0, 1, 1)
+            still = []
+            for i in running:
+                E = float(et[i, L.E_TOTAL])
+                dE = abs(E - E_prev[i])
+                pmax = dd[i, 1]
+                dpsi = float(dd[i, 0] / pmax) if pmax > 0 else 0.0
+                out[i].update(dE=dE, dpsi=dpsi, steps=step)
+                if dE < tol and dpsi < tol:
+                    out[i]["converged"] = True
+                else:
+                    E_prev[i] = E
+                    still.append(i)
+            if still != running:
+                running = still
+                if running:
+                    ws.set_active(running)
+            if running:
+                ws.copy_slot(1, 0)
+        else:
+            for i in running:
+                out[i]["steps"] = step
+    ws.set_active(insts)
+    et = ws.energy_terms()
+    for i in insts:
+        out[i]["energy"] = float(et[i, L.E_TOTAL])
+    return out
+
+
+def find_ground_state(*, grid: Grid, atom, interactions: InteractionParams,
+                      zeeman=ZeemanParams(), potential=NoPotential(), dt=0.001, n_steps=10000,
+                      tol=1e-10, initial_state="polar", psi_init=None, enable_ddi=False,
+                      c_dd=float("nan"), secular_ddi=False, adaptive_dt=False, dt_max=None,
+                      fft_flags=None, target_magnetization=None, precision="fp64"):
+    """ground_state.jl:80-163."""
+    F = atom.F
+    psi0 = init_psi(grid, SpinSystem(F), initial_state) if psi_init is None else np.array(psi_init)
+    _validate_itp_zeeman(zeeman, F, dt)
+    eff_cdd = c_dd if (enable_ddi and not math.isnan(c_dd)) else 0.0
+    _validate_itp_interactions(interactions, F, dt, psi=psi0, c_dd=eff_cdd)
+    if adaptive_dt:
+        raise NotImplementedError("adaptive-dt ITP is not built on the GPU path yet")
+    if target_magnetization is not None:
+        raise NotImplementedError("constrained-Mz ITP is not built on the GPU path yet")
+    sp = SimParams(dt=dt, n_steps=n_steps, imaginary_time=True, normalize_every=1,
+                   save_every=max(1, n_steps // 10))
+    ws = make_workspace(grid=grid, atom=atom, interactions=interactions, zeeman=zeeman,
+                        potential=potential, sim_params=sp, psi_init=psi0, enable_ddi=enable_ddi,
+                        c_dd=c_dd, secular_ddi=secular_ddi, precision=precision)
+    r = _itp_loop(ws, n_steps, tol, sp.save_every)[0]
+    return dict(workspace=ws, converged=r["converged"], energy=r["energy"], dE=r["dE"],
+                dpsi=r["dpsi"], steps=r["steps"])
+
+
+def find_ground_state_multistart(*, initial_states=("polar", "ferromagnetic", "uniform",
+                                                    "antiferromagnetic"),
+                                 n_random=3, seed=42, grid, atom, interactions, **kw):
+    """ground_state.jl:239-271 (sequential semantics; the batched engine in
+    becgpu.batch runs the same solves in one device launch)."""
+    results = []
+    for state in initial_states:
+        if state == "random":
+            for i in range(1, n_random + 1):
+                psi0 = init_psi(grid, SpinSystem(atom.F), "random", seed=seed + i)
+                r = find_ground_state(grid=grid, atom=atom, interactions=interactions,
+                                      psi_init=psi0, **kw)
+                results.append(dict(initial_state="random", idx=i, **r))
+        else:
+            r = find_ground_state(grid=grid, atom=atom, interactions=interactions,
+                                  initial_state=state, **kw)
+            results.append(dict(initial_state=state, idx=0, **r))
+    best = min(results, key=lambda r: r["energy"])   # first minimum, like argmin
+    return dict(workspace=best["workspace"], converged=best["converged"], energy=best["energy"],
+                initial_state=best["initial_state"], all_results=results)
+
+
+def scan_continuation(*, param_values, make_interactions, grid, atom, initial_state="polar",
+                      energy_jump_threshold=0.1, n_steps_continuation=500, n_steps_fresh=5000,
+                      **kw):
+    """ground_state.jl:328-378."""
+    results = []
+    prev_psi = None
+    prev_E = float("nan")
+    for val in param_values:
+        inter = make_interactions(val)
+        if prev_psi is not None:
+            r = find_ground_state(grid=grid, atom=atom, interactions=inter, psi_init=prev_psi.copy(),
+                                  n_steps=n_steps_continuation, **kw)
+        else:
+            r = find_ground_state(grid=grid, atom=atom, interactions=inter,
+                                  initial_state=initial_state, n_steps=n_steps_fresh, **kw)
+        if (not math.isnan(prev_E)
+                and abs(r["energy"] - prev_E) / max(abs(prev_E), 1e-30) > energy_jump_threshold):
+            r = find_ground_state_multistart(grid=grid, atom=atom, interactions=inter,
+                                             n_steps=n_steps_fresh, **kw)
+        psi = r["workspace"].get_psi(0)
+        ph = classify_phase(psi, atom.F, grid)
+        results.append(dict(param=val, energy=r["energy"], converged=r["converged"],
+                            phase=ph["phase"], psi=psi.copy()))
+        prev_psi = psi
+        prev_E = r["energy"]
+    return results

@@ -1,0 +1,813 @@
+// C ABI of the split-step engine: workspace ownership, parameter upload and the
+// schedule compiler that turns the reference's operator sequence
+// (split_step.jl / simulation.jl / yoshida.jl) into fused HBM passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/becgpu.h"
+#include "bec_kernels.h"
+
+using namespace bec;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return fail(BEC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+struct bec_ws {
+  Geom g{};
+  int F = 0, D = 0, B = 0, prec = 0, itp = 0, normalize_every = 1, ddi = 0;
+  size_t csz = 16;
+  std::vector<InstParams> hp;
+  std::vector<bec_inst_params> user;
+  std::vector<int> active;
+  std::vector<double> xtab, ktab;  // host copies, concatenated per axis
+  std::vector<int> has_varr;
+  void* slot[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* scratch = nullptr;
+  void* fhat = nullptr;
+  double* d_ktab = nullptr;
+  double* d_axtab = nullptr;
+  double* d_varr = nullptr;
+  double* d_red = nullptr;
+  double* d_partials = nullptr;
+  double* d_vfy = nullptr;
+  InstParams* d_params = nullptr;
+  int* d_active = nullptr;
+  void* d_tw[10] = {};
+  double* h_red = nullptr;  // pinned
+  int max_blocks = 0;
+  int n_xblocks = 0;
+  bool params_dirty = true;
+  hipStream_t stream = nullptr;
+  DevTables tb{};
+  // per-launch HIP-event profiling (bec_profile_*)
+  bool prof_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> prof_cls;
+};
+
+// Kernel classes reported by bec_profile_read (order = BEC_K_* in becgpu.h).
+static int prof_begin(bec_ws* ws, int cls) {
+  if (!ws->prof_on) return -1;
+  if (ws->ev_used + 2 > ws->ev_pool.size()) {
+    for (int i = 0; i < 256; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      ws->ev_pool.push_back(e);
+    }
+  }
+  const int idx = (int)ws->ev_used;
+  ws->ev_used += 2;
+  ws->prof_cls.push_back(cls);
+  (void)hipEventRecord(ws->ev_pool[idx], ws->stream);
+  return idx;
+}
+static void prof_end(bec_ws* ws, int idx) {
+  if (idx >= 0) (void)hipEventRecord(ws->ev_pool[idx + 1], ws->stream);
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static bool is_pow2_supported(int n) {
+  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+}
+
+static void refresh_tables(bec_ws* ws) {
+  DevTables& t = ws->tb;
+  t.ktab = ws->d_ktab;
+  t.axtab = ws->d_axtab;
+  t.Varr = ws->d_varr && std::any_of(ws->has_varr.begin(), ws->has_varr.end(), [](int v) { return v; })
+               ? ws->d_varr
+               : nullptr;
+  t.params = ws->d_params;
+  t.active = ws->d_active;
+  for (int i = 0; i < 10; ++i) t.tw[i] = ws->d_tw[i];
+  t.vfy = ws->d_vfy;
+  t.red = ws->d_red;
+  t.partials = ws->d_partials;
+  t.max_blocks = ws->max_blocks;
+  t.maxB = ws->B;
+}
+
+static int sync_params(bec_ws* ws) {
+  if (!ws->params_dirty) return BEC_OK;
+  HIPCHK(hipMemcpyAsync(ws->d_params, ws->hp.data(), sizeof(InstParams) * ws->B,
+                        hipMemcpyHostToDevice, ws->stream));
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  ws->params_dirty = false;
+  return BEC_OK;
+}
+
+static int n_active(bec_ws* ws) { return (int)ws->active.size(); }
+
+template <typename T>
+static int xlaunch(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
+  const int pe = prof_begin(ws, 0);
+  HIPCHK(launch_xpass<T>(ws->stream, ws->g, ws->tb, p, src, dst, ws->fhat, n_active(ws)));
+  prof_end(ws, pe);
+  return BEC_OK;
+}
+static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
+  return ws->prec == BEC_FP32 ? xlaunch<float>(ws, p, src, dst) : xlaunch<double>(ws, p, src, dst);
+}
+static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_frac, double dt_ovr,
+                      int itp, int ddi_data, int* nb = nullptr) {
+  hipError_t e;
+  const int pe = prof_begin(ws, ddi_data ? 5 + (mode == SM_DDI ? 1 : 0) : 1 + (mode == SM_KIN ? 1 : mode == SM_INV ? 2 : mode == SM_KE ? 3 : 0));
+  if (ws->prec == BEC_FP32)
+    e = launch_strided<float>(ws->stream, ws->g, ws->tb, axis, mode, data, n_active(ws), kin_frac,
+                              dt_ovr, itp, ddi_data, nb);
+  else
+    e = launch_strided<double>(ws->stream, ws->g, ws->tb, axis, mode, data, n_active(ws), kin_frac,
+                               dt_ovr, itp, ddi_data, nb);
+  prof_end(ws, pe);
+  HIPCHK(e);
+  return BEC_OK;
+}
+static int do_reduce(bec_ws* ws, int nblocks, uint32_t slots, uint32_t maxmask = 0) {
+  const int pe = prof_begin(ws, 7);
+  HIPCHK(launch_reduce(ws->stream, ws->tb, ws->d_red, n_active(ws), nblocks, slots, maxmask));
+  prof_end(ws, pe);
+  return BEC_OK;
+}
+
+#define CHK(expr)                 \
+  do {                            \
+    int _r = (expr);              \
+    if (_r != BEC_OK) return _r;  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Schedule compiler
+// ---------------------------------------------------------------------------
+enum ItemKind { IT_V, IT_K, IT_NORM };
+struct Item {
+  ItemKind kind;
+  double frac;  // V: dt_half = frac*dt ; K: kinetic dt = frac*dt
+};
+
+struct Sched {
+  bec_ws* ws;
+  int itp;
+  double dt_ovr;
+  XProgram p{};
+  bool scale_pending = false;
+
+  void reset() {
+    std::memset(&p, 0, sizeof(p));
+    p.itp = itp;
+    p.dt_override = dt_ovr;
+  }
+  bool empty() const { return p.stages == 0 && p.nA == 0 && p.nB == 0; }
+  int flush() {
+    if (empty()) return BEC_OK;
+    if (p.nA) p.stages |= XS_OPS_A;
+    if (p.nB) p.stages |= XS_OPS_B;
+    int r = do_x(ws, p, ws->slot[0], ws->slot[0]);
+    reset();
+    return r;
+  }
+  int add_op(int kind, double frac) {
+    if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
+    if (scale_pending) {
+      scale_pending = false;
+      CHK(add_op(OP_SCALE, 0.0));
+    }
+    if (((p.stages & XS_KIN_1D) ? p.nB : p.nA) >= kMaxOps) CHK(flush());
+    const bool toB = (p.stages & XS_KIN_1D) != 0;
+    Op* ops = toB ? p.B : p.A;
+    int& n = toB ? p.nB : p.nA;
+    ops[n].kind = kind;
+    ops[n].frac = frac;
+    ++n;
+    return BEC_OK;
+  }
+  int ddi_conv() {
+    // F_hat half rows were written by the launch just flushed
+    const int nd = ws->g.ndim;
+    if (nd == 3) {
+      CHK(do_strided(ws, 1, SM_FWD, ws->fhat, 0, 0, itp, 1));
+      CHK(do_strided(ws, 2, SM_DDI, ws->fhat, 0, 0, itp, 1));
+      CHK(do_strided(ws, 1, SM_INV, ws->fhat, 0, 0, itp, 1));
+    } else {
+      CHK(do_strided(ws, 1, SM_DDI, ws->fhat, 0, 0, itp, 1));
+    }
+    return BEC_OK;
+  }
+  // _half_potential_step! (split_step.jl:48-133) for dt_half = f*dt
+  int V(double f) {
+    CHK(add_op(OP_DIAG, f / 2));
+    CHK(add_op(OP_SPINMIX, f / 2));
+    if (ws->ddi) {
+      if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
+      p.stages |= XS_WRITE_FHAT;
+      CHK(flush());
+      CHK(ddi_conv());
+      p.stages |= XS_LOAD_PHI;
+      CHK(add_op(OP_DDI_ROT, f));
+    }
+    CHK(add_op(OP_SPINMIX, f / 2));
+    CHK(add_op(OP_DIAG, f / 2));
+    return BEC_OK;
+  }
+  int K(double f) {
+    if (scale_pending) {  // renormalise before the (linear) kinetic step
+      scale_pending = false;
+      CHK(add_op(OP_SCALE, 0.0));
+    }
+    if (ws->g.ndim == 1) {
+      if (p.stages & (XS_KIN_1D | XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
+      p.stages |= XS_KIN_1D;
+      p.kin_frac = f;
+      return BEC_OK;
+    }
+    p.stages |= XS_FWD_FFT;
+    CHK(flush());
+    if (ws->g.ndim == 3) {
+      CHK(do_strided(ws, 1, SM_FWD, ws->slot[0], 0, 0, itp, 0));
+      CHK(do_strided(ws, 2, SM_KIN, ws->slot[0], f, dt_ovr, itp, 0));
+      CHK(do_strided(ws, 1, SM_INV, ws->slot[0], 0, 0, itp, 0));
+    } else {
+      CHK(do_strided(ws, 1, SM_KIN, ws->slot[0], f, dt_ovr, itp, 0));
+    }
+    p.stages |= XS_INV_FFT;
+    return BEC_OK;
+  }
+  int NORM() {
+    if (scale_pending) {
+      scale_pending = false;
+      CHK(add_op(OP_SCALE, 0.0));
+    }
+    p.stages |= XS_NORM;
+    CHK(flush());
+    CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_NORM));
+    scale_pending = true;
+    return BEC_OK;
+  }
+  int finish() {
+    CHK(flush());
+    if (scale_pending) {
+      scale_pending = false;
+      hipError_t e = ws->prec == BEC_FP32
+                         ? launch_scale_norm<float>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws))
+                         : launch_scale_norm<double>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws));
+      HIPCHK(e);
+    }
+    return BEC_OK;
+  }
+};
+
+static int run_items(bec_ws* ws, const std::vector<Item>& items, int itp, double dt_ovr) {
+  if (n_active(ws) == 0) return BEC_OK;
+  CHK(sync_params(ws));
+  Sched s;
+  s.ws = ws;
+  s.itp = itp;
+  s.dt_ovr = dt_ovr;
+  s.reset();
+  for (const Item& it : items) {
+    if (it.kind == IT_V) CHK(s.V(it.frac));
+    else if (it.kind == IT_K) CHK(s.K(it.frac));
+    else CHK(s.NORM());
+  }
+  return s.finish();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* bec_last_error(void) { return g_err.c_str(); }
+
+static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out) {
+  const int D = 2 * c->F + 1;
+  long long ns = 1;
+  for (int d = 0; d < 3; ++d) ns *= (d < c->ndim ? c->n[d] : 1);
+  const long long csz = c->precision == BEC_FP32 ? 8 : 16;
+  const int B = c->n_instances;
+  const int nx = c->n[0];
+  const int ny = c->ndim >= 2 ? c->n[1] : 1, nz = c->ndim >= 3 ? c->n[2] : 1;
+  const int nrows = ny * nz;
+  const int rows = xpass_rows(D, nx, nrows);
+  int nxb = nrows / rows;
+  int maxb = std::max(nxb, 1024);
+  // strided SM_KE partial count
+  for (int axis = 1; axis < c->ndim; ++axis) {
+    const int N = axis == 1 ? ny : nz;
+    const int tbw = strided_tb(N, 1, (int)csz);
+    const int other = axis == 1 ? nz : ny;
+    maxb = std::max(maxb, ((nx + tbw - 1) / tbw) * other * D);
+  }
+  if (max_blocks_out) *max_blocks_out = maxb;
+  if (nxb_out) *nxb_out = nxb;
+  long long bytes = 2LL * B * D * ns * csz;  // psi + scratch
+  if (c->enable_ddi) {
+    const int nxh = nx / 2 + 1;
+    const int pitch = (nxh + 15) / 16 * 16;
+    bytes += 3LL * B * pitch * ny * nz * csz;
+  }
+  bytes += (long long)RS_COUNT * B * maxb * 8 + (long long)B * RS_COUNT * 8;
+  bytes += (long long)B * 3 * std::max(nx, std::max(ny, nz)) * 8;
+  return bytes;
+}
+
+long long bec_workspace_bytes(const bec_config* cfg) {
+  if (!cfg) return -1;
+  return ws_bytes(cfg, nullptr, nullptr);
+}
+
+int bec_ws_create(const bec_config* c, bec_ws** out) {
+  if (!c || !out) return fail(BEC_EINVAL, "null argument");
+  *out = nullptr;
+  if (c->ndim < 1 || c->ndim > 3) return fail(BEC_EINVAL, "ndim must be 1, 2 or 3");
+  for (int d = 0; d < c->ndim; ++d) {
+    if (c->n[d] <= 0 || (c->n[d] % 2) != 0)
+      return fail(BEC_EINVAL, "n_points must be positive even integers");
+    if (!(c->L[d] > 0)) return fail(BEC_EINVAL, "box_size must be positive");
+    if (!is_pow2_supported(c->n[d]))
+      return fail(BEC_EUNSUP, "grid sides must be powers of two in [16, 512] on this build");
+  }
+  if (!(c->F == 1 || c->F == 2 || c->F == 6)) return fail(BEC_EUNSUP, "F must be 1, 2 or 6");
+  if (c->n_instances < 1) return fail(BEC_EINVAL, "n_instances must be >= 1");
+  if (c->precision != BEC_FP64 && c->precision != BEC_FP32)
+    return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
+  if (c->enable_ddi && c->ndim < 2) return fail(BEC_EUNSUP, "DDI needs a 2-D or 3-D grid here");
+  if (c->ddi_padding) return fail(BEC_EUNSUP, "zero-padded DDI is not built yet");
+  if (!c->fy_vecs || !c->x_tab || !c->k_tab) return fail(BEC_EINVAL, "missing tables");
+
+  bec_ws* ws = new bec_ws();
+  ws->F = c->F;
+  ws->D = 2 * c->F + 1;
+  ws->B = c->n_instances;
+  ws->prec = c->precision;
+  ws->itp = c->imaginary_time;
+  ws->normalize_every = c->normalize_every;
+  ws->ddi = c->enable_ddi;
+  ws->csz = ws->prec == BEC_FP32 ? 8 : 16;
+  Geom& g = ws->g;
+  g.ndim = c->ndim;
+  for (int d = 0; d < 3; ++d) g.n[d] = d < c->ndim ? c->n[d] : 1;
+  g.D = ws->D;
+  g.nrows = g.n[1] * g.n[2];
+  g.nsites = (long long)g.n[0] * g.n[1] * g.n[2];
+  g.inst_stride = g.nsites * ws->D;
+  g.inv_ntot = 1.0 / (double)g.nsites;
+  double dV = 1.0;
+  for (int d = 0; d < c->ndim; ++d) dV *= c->L[d] / c->n[d];
+  g.dV = dV;
+  g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
+  g.nxh = g.n[0] / 2 + 1;
+  g.nxh_pitch = (g.nxh + 15) / 16 * 16;
+  g.fhat_field_stride = (long long)g.nxh_pitch * g.n[1] * g.n[2];
+  g.fhat_inst_stride = 3 * g.fhat_field_stride;
+  g.ddi_secular = c->ddi_secular;
+  ws_bytes(c, &ws->max_blocks, &ws->n_xblocks);
+
+  ws->hp.assign(ws->B, InstParams{});
+  ws->user.assign(ws->B, bec_inst_params{});
+  ws->has_varr.assign(ws->B, 0);
+  ws->active.resize(ws->B);
+  for (int i = 0; i < ws->B; ++i) ws->active[i] = i;
+  int off = 0;
+  for (int d = 0; d < c->ndim; ++d) {
+    ws->xtab.insert(ws->xtab.end(), c->x_tab + off, c->x_tab + off + c->n[d]);
+    ws->ktab.insert(ws->ktab.end(), c->k_tab + off, c->k_tab + off + c->n[d]);
+    off += c->n[d];
+  }
+
+  auto cleanup = [&](int code, const std::string& m) {
+    bec_ws_destroy(ws);
+    return fail(code, m);
+  };
+  if (hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(BEC_EDEVICE, "hipStreamCreate failed");
+  const size_t psi_bytes = ws->csz * (size_t)g.inst_stride * ws->B;
+  if (hipMalloc(&ws->slot[0], psi_bytes) != hipSuccess) return cleanup(BEC_ENOMEM, "psi alloc");
+  if (hipMalloc(&ws->scratch, psi_bytes) != hipSuccess) return cleanup(BEC_ENOMEM, "scratch alloc");
+  if (hipMemset(ws->slot[0], 0, psi_bytes) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
+  if (ws->ddi) {
+    const size_t fb = ws->csz * (size_t)g.fhat_inst_stride * ws->B;
+    if (hipMalloc(&ws->fhat, fb) != hipSuccess) return cleanup(BEC_ENOMEM, "fhat alloc");
+    if (hipMemset(ws->fhat, 0, fb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
+  }
+  // k table [3][nmax]
+  std::vector<double> kt(3 * g.nmax, 0.0);
+  off = 0;
+  for (int d = 0; d < c->ndim; ++d) {
+    for (int i = 0; i < c->n[d]; ++i) kt[d * g.nmax + i] = c->k_tab[off + i];
+    off += c->n[d];
+  }
+  if (hipMalloc(&ws->d_ktab, kt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "ktab");
+  if (hipMemcpy(ws->d_ktab, kt.data(), kt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(BEC_EDEVICE, "ktab copy");
+  const size_t axb = (size_t)ws->B * 3 * g.nmax * 8;
+  if (hipMalloc(&ws->d_axtab, axb) != hipSuccess) return cleanup(BEC_ENOMEM, "axtab");
+  if (hipMemset(ws->d_axtab, 0, axb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
+  if (hipMalloc(&ws->d_params, sizeof(InstParams) * ws->B) != hipSuccess)
+    return cleanup(BEC_ENOMEM, "params");
+  if (hipMalloc(&ws->d_active, sizeof(int) * ws->B) != hipSuccess) return cleanup(BEC_ENOMEM, "active");
+  if (hipMemcpy(ws->d_active, ws->active.data(), sizeof(int) * ws->B, hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return cleanup(BEC_EDEVICE, "active copy");
+  const size_t redb = (size_t)ws->B * RS_COUNT * 8;
+  if (hipMalloc(&ws->d_red, redb) != hipSuccess) return cleanup(BEC_ENOMEM, "red");
+  if (hipMemset(ws->d_red, 0, redb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
+  const size_t pb = (size_t)RS_COUNT * ws->B * ws->max_blocks * 8;
+  if (hipMalloc(&ws->d_partials, pb) != hipSuccess) return cleanup(BEC_ENOMEM, "partials");
+  if (hipMemset(ws->d_partials, 0, pb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
+  if (hipHostMalloc(&ws->h_red, redb) != hipSuccess) return cleanup(BEC_ENOMEM, "pinned red");
+  // Fy eigenvectors
+  const size_t vb = (size_t)ws->D * ws->D * 16;
+  if (hipMalloc(&ws->d_vfy, vb) != hipSuccess) return cleanup(BEC_ENOMEM, "vfy");
+  if (hipMemcpy(ws->d_vfy, c->fy_vecs, vb, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(BEC_EDEVICE, "vfy copy");
+  // twiddles exp(-2 pi i m / N), computed in long double
+  for (int l = 3; l <= 9; ++l) {
+    const int N = 1 << l;
+    const size_t esz = ws->csz;
+    std::vector<unsigned char> buf(esz * N);
+    for (int m = 0; m < N; ++m) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+      const long double cr = cosl(ang), ci = sinl(ang);
+      if (ws->prec == BEC_FP32) {
+        float v[2] = {(float)cr, (float)ci};
+        std::memcpy(&buf[esz * m], v, 8);
+      } else {
+        double v[2] = {(double)cr, (double)ci};
+        std::memcpy(&buf[esz * m], v, 16);
+      }
+    }
+    if (hipMalloc(&ws->d_tw[l], buf.size()) != hipSuccess) return cleanup(BEC_ENOMEM, "twiddles");
+    if (hipMemcpy(ws->d_tw[l], buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "twiddle copy");
+  }
+  refresh_tables(ws);
+  *out = ws;
+  return BEC_OK;
+}
+
+void bec_ws_destroy(bec_ws* ws) {
+  if (!ws) return;
+  if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+  for (int i = 0; i < 4; ++i)
+    if (ws->slot[i]) (void)hipFree(ws->slot[i]);
+  if (ws->scratch) (void)hipFree(ws->scratch);
+  if (ws->fhat) (void)hipFree(ws->fhat);
+  for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials, ws->d_vfy})
+    if (p) (void)hipFree(p);
+  if (ws->d_params) (void)hipFree(ws->d_params);
+  if (ws->d_active) (void)hipFree(ws->d_active);
+  for (int i = 0; i < 10; ++i)
+    if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
+  if (ws->h_red) (void)hipHostFree(ws->h_red);
+  for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
+  if (ws->stream) (void)hipStreamDestroy(ws->stream);
+  delete ws;
+}
+
+int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p) {
+  if (!ws || !p) return fail(BEC_EINVAL, "null argument");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  if (!(p->dt > 0)) return fail(BEC_EINVAL, "dt must be positive");
+  ws->user[inst] = *p;
+  InstParams& q = ws->hp[inst];
+  q = InstParams{};
+  q.dt = p->dt;
+  q.c0 = p->c0;
+  q.c1 = p->c1;
+  q.c_lhy = p->c_lhy;
+  q.cdd = p->c_dd;
+  const int F = ws->F;
+  double zmin = 0.0;
+  for (int c = 0; c < ws->D; ++c) {
+    const double m = (double)(F - c);
+    q.zee[c] = -p->p * m + p->q * (m * m);
+    zmin = c == 0 ? q.zee[c] : std::min(zmin, q.zee[c]);
+  }
+  q.zee_shift = zmin;
+  q.active = 1;
+  ws->params_dirty = true;
+  // trap tables omega_d^2 x_i^2
+  const Geom& g = ws->g;
+  std::vector<double> ax(3 * g.nmax, 0.0);
+  int off = 0;
+  for (int d = 0; d < g.ndim; ++d) {
+    const double w2 = p->omega[d] * p->omega[d];
+    for (int i = 0; i < g.n[d]; ++i) {
+      const double x = ws->xtab[off + i];
+      ax[d * g.nmax + i] = w2 * (x * x);
+    }
+    off += g.n[d];
+  }
+  HIPCHK(hipMemcpyAsync(ws->d_axtab + (size_t)inst * 3 * g.nmax, ax.data(), ax.size() * 8,
+                        hipMemcpyHostToDevice, ws->stream));
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  return BEC_OK;
+}
+
+int bec_set_potential(bec_ws* ws, int inst, const double* V) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  const Geom& g = ws->g;
+  if (!V) {
+    ws->has_varr[inst] = 0;
+    if (std::any_of(ws->has_varr.begin(), ws->has_varr.end(), [](int v) { return v; }))
+      return fail(BEC_EUNSUP, "mixing explicit and harmonic potentials across instances");
+    refresh_tables(ws);
+    return BEC_OK;
+  }
+  if (!ws->d_varr) {
+    if (hipMalloc(&ws->d_varr, (size_t)ws->B * g.nsites * 8) != hipSuccess)
+      return fail(BEC_ENOMEM, "potential alloc");
+  }
+  HIPCHK(hipMemcpy(ws->d_varr + (size_t)inst * g.nsites, V, g.nsites * 8, hipMemcpyHostToDevice));
+  ws->has_varr[inst] = 1;
+  refresh_tables(ws);
+  return BEC_OK;
+}
+
+int bec_set_psi(bec_ws* ws, int inst, const double* psi) {
+  if (!ws || !psi) return fail(BEC_EINVAL, "null argument");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  const size_t n = (size_t)ws->g.inst_stride;
+  char* dst = (char*)ws->slot[0] + ws->csz * n * inst;
+  if (ws->prec == BEC_FP64) {
+    HIPCHK(hipMemcpy(dst, psi, n * 16, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(2 * n);
+    for (size_t i = 0; i < 2 * n; ++i) tmp[i] = (float)psi[i];
+    HIPCHK(hipMemcpy(dst, tmp.data(), n * 8, hipMemcpyHostToDevice));
+  }
+  return BEC_OK;
+}
+
+int bec_get_psi(bec_ws* ws, int inst, double* psi) {
+  if (!ws || !psi) return fail(BEC_EINVAL, "null argument");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  const size_t n = (size_t)ws->g.inst_stride;
+  const char* src = (const char*)ws->slot[0] + ws->csz * n * inst;
+  if (ws->prec == BEC_FP64) {
+    HIPCHK(hipMemcpy(psi, src, n * 16, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(2 * n);
+    HIPCHK(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 2 * n; ++i) psi[i] = tmp[i];
+  }
+  return BEC_OK;
+}
+
+int bec_set_active(bec_ws* ws, const int* insts, int n) {
+  if (!ws || (n > 0 && !insts)) return fail(BEC_EINVAL, "null argument");
+  if (n < 0 || n > ws->B) return fail(BEC_EINVAL, "bad active count");
+  std::vector<int> a(insts, insts + n);
+  for (int v : a)
+    if (v < 0 || v >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  if (n) HIPCHK(hipMemcpy(ws->d_active, a.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  ws->active = a;
+  return BEC_OK;
+}
+
+int bec_split_step(bec_ws* ws, int n_steps) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (n_steps < 0) return fail(BEC_EINVAL, "n_steps < 0");
+  std::vector<Item> items;
+  items.reserve(4 * (size_t)n_steps);
+  for (int s = 1; s <= n_steps; ++s) {
+    items.push_back({IT_V, 0.5});
+    items.push_back({IT_K, 1.0});
+    items.push_back({IT_V, 0.5});
+    if (ws->itp && ws->normalize_every > 0 && (s % ws->normalize_every) == 0)
+      items.push_back({IT_NORM, 0.0});
+  }
+  return run_items(ws, items, ws->itp, 0.0);
+}
+
+int bec_leapfrog_segment(bec_ws* ws, int n_steps) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (n_steps < 1) return fail(BEC_EINVAL, "n_steps < 1");
+  std::vector<Item> items;
+  items.push_back({IT_V, 0.5});
+  for (int s = 1; s <= n_steps; ++s) {
+    items.push_back({IT_K, 1.0});
+    items.push_back({IT_V, s == n_steps ? 0.5 : 1.0});
+  }
+  return run_items(ws, items, 0, 0.0);
+}
+
+int bec_half_potential(bec_ws* ws, double dt_half, int imaginary_time) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  return run_items(ws, {{IT_V, 1.0}}, imaginary_time, dt_half);
+}
+
+int bec_kinetic(bec_ws* ws, double dt, int imaginary_time) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  return run_items(ws, {{IT_K, 1.0}}, imaginary_time, dt);
+}
+
+int bec_strang_core(bec_ws* ws, double dt) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  return run_items(ws, {{IT_V, 0.5}, {IT_K, 1.0}, {IT_V, 0.5}}, 0, dt);
+}
+
+int bec_yoshida_core(bec_ws* ws, double dt) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  const double w1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0));
+  const double w0 = 1.0 - 2.0 * w1;
+  const double wm = (w1 + w0) / 2;
+  return run_items(ws,
+                   {{IT_V, w1 / 2}, {IT_K, w1}, {IT_V, wm}, {IT_K, w0}, {IT_V, wm}, {IT_K, w1},
+                    {IT_V, w1 / 2}},
+                   0, dt);
+}
+
+int bec_normalize(bec_ws* ws) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  return run_items(ws, {{IT_NORM, 0.0}}, ws->itp, 0.0);
+}
+
+int bec_synchronize(bec_ws* ws) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  return BEC_OK;
+}
+
+static int fetch_red(bec_ws* ws) {
+  HIPCHK(hipMemcpyAsync(ws->h_red, ws->d_red, (size_t)ws->B * RS_COUNT * 8, hipMemcpyDeviceToHost,
+                        ws->stream));
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  return BEC_OK;
+}
+
+int bec_energy(bec_ws* ws, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  if (n_active(ws) == 0) return BEC_OK;
+  CHK(sync_params(ws));
+  const Geom& g = ws->g;
+  XProgram p{};
+  p.itp = ws->itp;
+  if (ws->ddi) {
+    p.stages = XS_WRITE_FHAT | XS_NO_STORE;
+    CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+    Sched s;
+    s.ws = ws;
+    s.itp = ws->itp;
+    s.dt_ovr = 0;
+    CHK(s.ddi_conv());
+    p.stages = XS_LOAD_PHI | XS_ENERGY | XS_NO_STORE;
+  } else {
+    p.stages = XS_ENERGY | XS_NO_STORE;
+  }
+  CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+  const uint32_t emask = (1u << RS_NORM) | (1u << RS_TRAP) | (1u << RS_ZEE) | (1u << RS_C0) |
+                         (1u << RS_C1) | (1u << RS_LHY) | (1u << RS_DDI) | (1u << RS_MZ);
+  CHK(do_reduce(ws, ws->n_xblocks, emask));
+  // kinetic energy: forward FFT of every component, sum k^2 |psi_hat|^2 (energy.jl:56-65)
+  if (g.ndim == 1) {
+    p.stages = XS_FWD_FFT | XS_KE_1D | XS_NO_STORE;
+    CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+    CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_KIN));
+  } else {
+    p.stages = XS_FWD_FFT;
+    CHK(do_x(ws, p, ws->slot[0], ws->scratch));
+    int nb = 0;
+    if (g.ndim == 3) CHK(do_strided(ws, 1, SM_FWD, ws->scratch, 0, 0, 0, 0));
+    CHK(do_strided(ws, g.ndim - 1, SM_KE, ws->scratch, 0, 0, 0, 0, &nb));
+    CHK(do_reduce(ws, nb, 1u << RS_KIN));
+  }
+  CHK(fetch_red(ws));
+  const double dV = g.dV;
+  for (int inst : ws->active) {
+    const double* r = ws->h_red + (size_t)inst * RS_COUNT;
+    const InstParams& q = ws->hp[inst];
+    double* o = out + (size_t)inst * BEC_E_COUNT;
+    o[BEC_E_KIN] = 0.5 * (r[RS_KIN] * dV / (double)g.nsites);
+    o[BEC_E_TRAP] = r[RS_TRAP] * dV;
+    o[BEC_E_ZEEMAN] = r[RS_ZEE] * dV;
+    o[BEC_E_C0] = 0.5 * q.c0 * r[RS_C0] * dV;
+    o[BEC_E_C1] = 0.5 * q.c1 * r[RS_C1] * dV;
+    o[BEC_E_DDI] = ws->ddi ? 0.5 * r[RS_DDI] * dV : 0.0;
+    o[BEC_E_LHY] = q.c_lhy != 0.0 ? (2.0 / 5.0) * q.c_lhy * r[RS_LHY] * dV : 0.0;
+    o[BEC_E_TENSOR] = 0.0;
+    o[BEC_E_NORM] = r[RS_NORM] * dV;
+    o[BEC_E_MZ] = r[RS_MZ] * dV;
+    o[BEC_E_TOTAL] = o[BEC_E_KIN] + o[BEC_E_TRAP] + o[BEC_E_ZEEMAN] + o[BEC_E_C0] + o[BEC_E_C1] +
+                     o[BEC_E_DDI] + o[BEC_E_LHY] + o[BEC_E_TENSOR];
+  }
+  return BEC_OK;
+}
+
+static int ensure_slot(bec_ws* ws, int s) {
+  if (s < 0 || s > 3) return fail(BEC_EINVAL, "slot out of range");
+  if (!ws->slot[s]) {
+    const size_t b = ws->csz * (size_t)ws->g.inst_stride * ws->B;
+    if (hipMalloc(&ws->slot[s], b) != hipSuccess) return fail(BEC_ENOMEM, "slot alloc");
+    HIPCHK(hipMemsetAsync(ws->slot[s], 0, b, ws->stream));
+  }
+  return BEC_OK;
+}
+
+int bec_copy_slot(bec_ws* ws, int dst, int src) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  CHK(ensure_slot(ws, dst));
+  CHK(ensure_slot(ws, src));
+  const size_t b = ws->csz * (size_t)ws->g.inst_stride;
+  for (int inst : ws->active)
+    HIPCHK(hipMemcpyAsync((char*)ws->slot[dst] + b * inst, (char*)ws->slot[src] + b * inst, b,
+                          hipMemcpyDeviceToDevice, ws->stream));
+  return BEC_OK;
+}
+
+int bec_diff(bec_ws* ws, int a, int b, int mode, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  CHK(ensure_slot(ws, a));
+  CHK(ensure_slot(ws, b));
+  if (n_active(ws) == 0) return BEC_OK;
+  const int nb = 512;
+  hipError_t e = ws->prec == BEC_FP32
+                     ? launch_diff<float>(ws->stream, ws->g, ws->tb, ws->slot[a], ws->slot[b],
+                                          n_active(ws), mode, nb)
+                     : launch_diff<double>(ws->stream, ws->g, ws->tb, ws->slot[a], ws->slot[b],
+                                           n_active(ws), mode, nb);
+  HIPCHK(e);
+  const uint32_t m = (1u << RS_NORM) | (1u << RS_KIN);
+  CHK(do_reduce(ws, nb, m, mode ? m : 0u));
+  CHK(fetch_red(ws));
+  for (int inst : ws->active) {
+    out[2 * inst] = ws->h_red[(size_t)inst * RS_COUNT + RS_NORM];
+    out[2 * inst + 1] = ws->h_red[(size_t)inst * RS_COUNT + RS_KIN];
+  }
+  return BEC_OK;
+}
+
+int bec_itp_guard(bec_ws* ws, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  if (n_active(ws) == 0) return BEC_OK;
+  const int nb = 512;
+  hipError_t e = ws->prec == BEC_FP32
+                     ? launch_diff<float>(ws->stream, ws->g, ws->tb, ws->slot[0], ws->slot[0],
+                                          n_active(ws), 0, nb)
+                     : launch_diff<double>(ws->stream, ws->g, ws->tb, ws->slot[0], ws->slot[0],
+                                           n_active(ws), 0, nb);
+  HIPCHK(e);
+  CHK(do_reduce(ws, nb, 1u << RS_KIN));
+  if (ws->ddi) CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_PHIMAX, 1u << RS_PHIMAX));
+  CHK(fetch_red(ws));
+  for (int inst : ws->active) {
+    out[2 * inst] = ws->h_red[(size_t)inst * RS_COUNT + RS_KIN];
+    out[2 * inst + 1] = ws->ddi ? ws->h_red[(size_t)inst * RS_COUNT + RS_PHIMAX] : 0.0;
+  }
+  return BEC_OK;
+}
+
+int bec_profile_enable(bec_ws* ws, int on) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  ws->prof_on = on != 0;
+  ws->ev_used = 0;
+  ws->prof_cls.clear();
+  return BEC_OK;
+}
+
+int bec_profile_read(bec_ws* ws, double* ms, long long* count) {
+  if (!ws || !ms || !count) return fail(BEC_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  for (int k = 0; k < BEC_K_COUNT; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
+  }
+  for (size_t i = 0; i < ws->prof_cls.size(); ++i) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ws->ev_pool[2 * i], ws->ev_pool[2 * i + 1]));
+    const int c = ws->prof_cls[i];
+    if (c >= 0 && c < BEC_K_COUNT) {
+      ms[c] += t;
+      count[c] += 1;
+    }
+  }
+  ws->ev_used = 0;
+  ws->prof_cls.clear();
+  return BEC_OK;
+}
+
+}  // extern "C"

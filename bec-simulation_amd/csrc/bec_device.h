@@ -1,0 +1,387 @@
+// Device building blocks for the split-step hot path on gfx950:
+//   * cplx<T>          : plain complex arithmetic (no library types)
+//   * fft_team<T,N>    : Stockham radix-8/4/2 FFT of one length-N line held in LDS,
+//                        computed by a team of N/8 threads (8 values per thread in
+//                        VGPRs, one LDS round trip per radix pass)
+//   * per-site spinor propagators: diagonal (trap/Zeeman/contact/LHY),
+//     spin-1 Rodrigues rotation, general-F Euler rotation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bec_types.h"
+
+namespace bec {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> mk(T a, T b) { cplx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cplx<T> operator*(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> operator*(cplx<T> a, T s) { return mk(a.x * s, a.y * s); }
+template <typename T>
+__device__ __forceinline__ cplx<T> conjc(cplx<T> a) { return mk(a.x, -a.y); }
+template <typename T>
+__device__ __forceinline__ T abs2(cplx<T> a) { return a.x * a.x + a.y * a.y; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) {
+  return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+
+__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
+__device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
+// cis(-a) for a phase argument that is always formed in FP64
+template <typename T>
+__device__ __forceinline__ cplx<T> cis_neg(double a) {
+  double s, c;
+  sincos(a, &s, &c);
+  return mk<T>((T)c, (T)(-s));
+}
+
+// ---------------------------------------------------------------------------
+// In-register small DFTs
+// ---------------------------------------------------------------------------
+template <typename T, bool INV>
+__device__ __forceinline__ void dft2(cplx<T>& a, cplx<T>& b) {
+  cplx<T> t = a - b;
+  a = a + b;
+  b = t;
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cplx<T>& v0, cplx<T>& v1, cplx<T>& v2, cplx<T>& v3) {
+  cplx<T> a0 = v0 + v2, a1 = v0 - v2, a2 = v1 + v3, a3 = mul_mi<T, INV>(v1 - v3);
+  v0 = a0 + a2;
+  v2 = a0 - a2;
+  v1 = a1 + a3;
+  v3 = a1 - a3;
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cplx<T>* v) {
+  // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
+  dft4<T, INV>(v[0], v[2], v[4], v[6]);
+  dft4<T, INV>(v[1], v[3], v[5], v[7]);
+  const T r = (T)0.70710678118654752440;
+  // twiddle odd part by W8^k, W8 = exp(-+ i pi/4)
+  cplx<T> o1 = v[3], o2 = v[5], o3 = v[7];
+  // W8^1 = (1 -+ i)/sqrt2
+  o1 = INV ? mk((o1.x - o1.y) * r, (o1.x + o1.y) * r) : mk((o1.x + o1.y) * r, (o1.y - o1.x) * r);
+  o2 = mul_mi<T, INV>(o2);
+  // W8^3 = (-1 -+ i)/sqrt2
+  o3 = INV ? mk((-o3.x - o3.y) * r, (o3.x - o3.y) * r) : mk((o3.y - o3.x) * r, (-o3.x - o3.y) * r);
+  cplx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = e0 + o0;
+  v[4] = e0 - o0;
+  v[1] = e1 + o1;
+  v[5] = e1 - o1;
+  v[2] = e2 + o2;
+  v[6] = e2 - o2;
+  v[3] = e3 + o3;
+  v[7] = e3 - o3;
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dftR(cplx<T>* v) {
+  if constexpr (R == 2) dft2<T, INV>(v[0], v[1]);
+  else if constexpr (R == 4) dft4<T, INV>(v[0], v[1], v[2], v[3]);
+  else dft8<T, INV>(v);
+}
+
+// Radix plan for a length-N line: a list of radices (each 2, 4 or 8) whose
+// product is N, every pass processed by N/8 threads holding 8 values each.
+template <int N>
+struct RadixPlan;
+template <> struct RadixPlan<8>   { static constexpr int np = 1; static constexpr int r[3] = {8, 0, 0}; };
+template <> struct RadixPlan<16>  { static constexpr int np = 2; static constexpr int r[3] = {8, 2, 0}; };
+template <> struct RadixPlan<32>  { static constexpr int np = 2; static constexpr int r[3] = {8, 4, 0}; };
+template <> struct RadixPlan<64>  { static constexpr int np = 2; static constexpr int r[3] = {8, 8, 0}; };
+template <> struct RadixPlan<128> { static constexpr int np = 3; static constexpr int r[3] = {8, 4, 4}; };
+template <> struct RadixPlan<256> { static constexpr int np = 3; static constexpr int r[3] = {8, 8, 4}; };
+template <> struct RadixPlan<512> { static constexpr int np = 3; static constexpr int r[3] = {8, 8, 8}; };
+
+// One Stockham pass: butterflies j = t + b*(N/8), b < 8/R.
+// in[j + r*N/R] -> twiddle W_N^{(j mod Ns) r N/(Ns R)} -> DFT_R -> out[(j/Ns)Ns R + j mod Ns + r Ns]
+// The line lives at base[e*ES]; tw[m] = exp(-2 pi i m / N).
+template <typename T, int N, int R, bool INV>
+__device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool active, int Ns,
+                                              const cplx<T>* __restrict__ tw) {
+  constexpr int NB = 8 / R;      // butterflies per thread
+  constexpr int TEAM = N / 8;
+  cplx<T> v[8];
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j = t + b * TEAM;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b * R + r] = base[(j + r * (N / R)) * ES];
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j = t + b * TEAM;
+      int k = j % Ns;
+      if (Ns > 1) {
+        int stepm = k * (N / (Ns * R));
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> w = tw[r * stepm];
+          if (INV) w.y = -w.y;
+          v[b * R + r] = v[b * R + r] * w;
+        }
+      }
+      dftR<T, R, INV>(&v[b * R]);
+      int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[(o + r * Ns) * ES] = v[b * R + r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full length-N FFT (unnormalised) of the line at base[e*ES], in place in LDS.
+// Every thread of the block must call it (it synchronises); `active` marks the
+// threads that own a (line, team index) pair.
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool active,
+                                         const cplx<T>* __restrict__ tw) {
+  using P = RadixPlan<N>;
+  int Ns = 1;
+  stockham_pass<T, N, P::r[0], INV>(base, ES, t, active, Ns, tw);
+  Ns *= P::r[0];
+  if constexpr (P::np > 1) {
+    stockham_pass<T, N, P::r[1], INV>(base, ES, t, active, Ns, tw);
+    Ns *= P::r[1];
+  }
+  if constexpr (P::np > 2) {
+    stockham_pass<T, N, P::r[2], INV>(base, ES, t, active, Ns, tw);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-site spinor propagators
+// ---------------------------------------------------------------------------
+
+// propagators.jl:277-314: n from the pre-update spinor; ITP uses real
+// exponentials with the Zeeman shift min_c E_c, real time uses cis.
+template <typename T, int D>
+__device__ __forceinline__ void op_diag(cplx<T>* s, double Vtrap, const InstParams& P, double dtf,
+                                        bool itp, const cplx<T>* zf) {
+  T n = 0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) n += abs2(s[c]);
+  double nd = (double)n;
+  double Vint = P.c0 * nd + P.c_lhy * nd * sqrt(nd);
+  double a = (Vtrap + Vint) * dtf;
+  cplx<T> base;
+  if (itp) {
+    base = mk<T>((T)exp(-a), (T)0);
+  } else {
+    base = cis_neg<T>(a);
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) s[c] = s[c] * (base * zf[c]);
+}
+
+// observables.jl:68-88 spin density: Fz = sum m|psi|^2,
+// Fx + iFy = sum_{c>=1} f+(m_c) conj(psi_{c-1}) psi_c.
+template <typename T, int D>
+__device__ __forceinline__ void spin_vec(const cplx<T>* s, T& fx, T& fy, T& fz) {
+  constexpr int F = (D - 1) / 2;
+  fz = 0;
+  fx = 0;
+  fy = 0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) fz += (T)(F - c) * abs2(s[c]);
+#pragma unroll
+  for (int c = 1; c < D; ++c) {
+    const T m = (T)(F - c);
+    const T fp = (T)sqrt((double)(F * (F + 1)) - (double)m * ((double)m + 1.0));
+    cplx<T> pr = conjc(s[c - 1]) * s[c];
+    fx += fp * pr.x;
+    fy += fp * pr.y;
+  }
+}
+
+// spin_mixing.jl:71-98: U = I - i sin(th) nF + (cos th - 1) nF^2 (ITP: sinh/cosh, no i)
+template <typename T>
+__device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, bool itp) {
+  T fx, fy, fz;
+  spin_vec<T, 3>(s, fx, fy, fz);
+  T fmag = sqrt(fx * fx + fy * fy + fz * fz);
+  if (!(fmag >= (T)1e-30)) return;
+  const T is2 = (T)0.70710678118654752440;
+  T a = fx / fmag, b = fy / fmag, c = fz / fmag;
+  // nF = [[c, u*, 0], [u, 0, u*], [0, u, -c]] with u = (a + i b)/sqrt2
+  cplx<T> u = mk(a * is2, b * is2), uc = conjc(u);
+  cplx<T> nF[3][3] = {{mk(c, (T)0), uc, mk((T)0, (T)0)},
+                      {u, mk((T)0, (T)0), uc},
+                      {mk((T)0, (T)0), u, mk(-c, (T)0)}};
+  cplx<T> nF2[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cplx<T> acc = nF[i][0] * nF[0][j];
+      acc = acc + nF[i][1] * nF[1][j];
+      acc = acc + nF[i][2] * nF[2][j];
+      nF2[i][j] = acc;
+    }
+  double th = c1 * (double)fmag * dtf;
+  T s1, c1m;
+  if (itp) {
+    s1 = (T)sinh(th);
+    c1m = (T)(cosh(th) - 1.0);
+  } else {
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    s1 = (T)sn;
+    c1m = (T)(cs - 1.0);
+  }
+  cplx<T> r[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    cplx<T> acc = mk((T)0, (T)0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      // U_ij = delta_ij - i s1 nF_ij + c1m nF2_ij   (ITP: - s1 nF_ij)
+      cplx<T> U = nF2[i][j] * c1m;
+      if (itp) U = U - nF[i][j] * s1;
+      else U = U + mk(nF[i][j].y * s1, -nF[i][j].x * s1);
+      if (i == j) U.x += (T)1;
+      acc = acc + U * s[j];
+    }
+    r[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s[i] = r[i];
+}
+
+// spinor_utils.jl:180-264: exp(-i dt phi.F) via Rz(a) Ry(b) Dz(th) Ry(-b) Rz(-a), with
+// Ry(+-b) = V_Fy diag(cis(-+b lambda)) V_Fy^dagger, lambda = -F..F ascending, and the
+// reference's cis recurrences.  ITP: Dz = exp(-m th).  vfy: D*D complex row-major V_Fy.
+template <typename T, int D>
+__device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, double phz, double dt,
+                                         bool itp, const cplx<double>* __restrict__ vfy) {
+  constexpr int F = (D - 1) / 2;
+  double pm = sqrt(phx * phx + phy * phy + phz * phz);
+  if (pm * fabs(dt) < 1e-14) return;
+  double cz = phz / pm;
+  cz = cz < -1.0 ? -1.0 : (cz > 1.0 ? 1.0 : cz);
+  double beta = acos(cz);
+  double alpha = atan2(phy, phx);
+  double theta = pm * dt;
+  double sa, ca, sb, cb, sFa, cFa, sFb, cFb;
+  sincos(alpha, &sa, &ca);
+  sincos(beta, &sb, &cb);
+  sincos((double)F * alpha, &sFa, &cFa);
+  sincos((double)F * beta, &sFb, &cFb);
+  cplx<T> z_neg_alpha = mk((T)ca, (T)(-sa));
+  cplx<T> z_beta = mk((T)cb, (T)sb);
+  cplx<T> rz_phase = mk((T)cFa, (T)sFa);
+  cplx<T> ry_phase = mk((T)cFb, (T)(-sFb));
+  cplx<T> v[D], w[D];
+  cplx<T> ph = rz_phase;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    v[c] = ph * s[c];
+    ph = ph * z_neg_alpha;
+  }
+  ph = ry_phase;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    cplx<T> acc = mk((T)0, (T)0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      cplx<double> q = vfy[j * D + i];  // Vt[i][j] = conj(V[j][i])
+      acc = acc + mk((T)q.x, (T)(-q.y)) * v[j];
+    }
+    w[i] = ph * acc;
+    ph = ph * z_beta;
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    cplx<T> acc = mk((T)0, (T)0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      cplx<double> q = vfy[i * D + j];
+      acc = acc + mk((T)q.x, (T)q.y) * w[j];
+    }
+    v[i] = acc;
+  }
+  if (itp) {
+    T dz = (T)exp(-(double)F * theta);
+    T st = (T)exp(theta);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      v[c] = v[c] * dz;
+      dz *= st;
+    }
+  } else {
+    double sF, cF, st, ct;
+    sincos((double)F * theta, &sF, &cF);
+    sincos(theta, &st, &ct);
+    cplx<T> dz = mk((T)cF, (T)(-sF));
+    cplx<T> zt = mk((T)ct, (T)st);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      v[c] = v[c] * dz;
+      dz = dz * zt;
+    }
+  }
+  ph = conjc(ry_phase);
+  cplx<T> z_neg_beta = conjc(z_beta);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    cplx<T> acc = mk((T)0, (T)0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      cplx<double> q = vfy[j * D + i];
+      acc = acc + mk((T)q.x, (T)(-q.y)) * v[j];
+    }
+    w[i] = ph * acc;
+    ph = ph * z_neg_beta;
+  }
+  ph = conjc(rz_phase);
+  cplx<T> z_alpha = conjc(z_neg_alpha);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    cplx<T> acc = mk((T)0, (T)0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      cplx<double> q = vfy[i * D + j];
+      acc = acc + mk((T)q.x, (T)q.y) * w[j];
+    }
+    s[i] = ph * acc;
+    ph = ph * z_alpha;
+  }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void op_spin_mixing(cplx<T>* s, const InstParams& P, double dtf, bool itp,
+                                               const cplx<double>* __restrict__ vfy) {
+  if (fabs(P.c1) < 1e-30) return;  // spin_mixing.jl:9
+  if constexpr (D == 3) {
+    op_rodrigues<T>(s, P.c1, dtf, itp);
+  } else {
+    T fx, fy, fz;
+    spin_vec<T, D>(s, fx, fy, fz);
+    op_euler<T, D>(s, P.c1 * (double)fx, P.c1 * (double)fy, P.c1 * (double)fz, dtf, itp, vfy);
+  }
+}
+
+}  // namespace bec

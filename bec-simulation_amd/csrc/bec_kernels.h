@@ -1,0 +1,87 @@
+// Host-visible launch interface of the split-step kernels (implemented in
+// bec_kernels.hip).  Plain pointers and sizes; T is double (FP64 path) or float
+// (FP32-bulk path, FP64 reductions and phase arguments).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bec_types.h"
+
+namespace bec {
+
+// Geometry shared by all instances of a workspace (instances differ only in
+// their InstParams, trap tables and psi).
+struct Geom {
+  int ndim;
+  int n[3];            // valid sizes, 1 for absent axes
+  int D;               // 2F+1
+  int nrows;           // n[1]*n[2]
+  long long nsites;    // n[0]*n[1]*n[2]
+  long long inst_stride;  // D*nsites (complex elements)
+  double inv_ntot;     // 1/nsites
+  double dV;
+  int nmax;            // max(n)
+  int nxh, nxh_pitch;  // rFFT half-row length and row pitch of the DDI spectra
+  long long fhat_field_stride;  // nxh_pitch*n[1]*n[2]
+  long long fhat_inst_stride;   // 3*fhat_field_stride
+  int ddi_secular;
+  int pad_;
+};
+
+struct DevTables {
+  const double* ktab;     // [3][nmax] fftfreq k values per axis
+  const double* axtab;    // [B][3][nmax] omega_d^2 x_i^2 (trap), per instance
+  const double* Varr;     // [B][nsites] explicit potential or null
+  const InstParams* params;  // [B]
+  const int* active;      // active instance list
+  const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), T-typed
+  const double* vfy;      // D*D complex (row-major) eigenvectors of Fy
+  const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
+  double* partials;       // [RS_COUNT][B][max_blocks]
+  int max_blocks;
+  int maxB;
+};
+
+enum StridedMode : int {
+  SM_FWD = 0,
+  SM_INV = 1,
+  SM_KIN = 2,   // forward, kinetic phase/N, inverse
+  SM_DDI = 3,   // 3 fields: forward, C_dd Q(k)/N contraction, inverse
+  SM_KE = 4,    // forward then sum k^2 |v|^2 (no store)
+};
+
+// x-axis pass: tiles of whole x rows, all components; executes `prog`.
+template <typename T>
+hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
+                        const void* src, void* dst, void* fhat, int n_active);
+
+// y or z axis pass (axis = 1 or 2) over psi (fields = D) or over the DDI
+// spectra (fields = 3, DDI mode).  SM_KIN uses dt = (dt_override > 0 ? dt_override :
+// instance dt) * kin_frac.  nblocks_out (optional): partials per instance (SM_KE).
+template <typename T>
+hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, int axis, int mode,
+                          void* data, int n_active, double kin_frac, double dt_override, int itp,
+                          int ddi_data, int* nblocks_out);
+
+// Tile shapes (shared by the launchers and the workspace sizing).
+constexpr int strided_tb_c(int N, int fields, int csize) {
+  return (fields * N * 16 * csize <= 65536) ? 16 : (fields * N * 8 * csize <= 65536) ? 8 : 4;
+}
+int strided_tb(int N, int fields, int csize);
+int xpass_rows(int D, int NX, int nrows);
+
+// Sum (or max) the per-block partials of `slots` into red[inst][slot].
+hipError_t launch_reduce(hipStream_t st, const DevTables& tb, double* red, int n_active,
+                         int nblocks, uint32_t slot_mask, uint32_t max_mask);
+
+// psi *= 1/sqrt(red[inst][RS_NORM] * dV)
+template <typename T>
+hipError_t launch_scale_norm(hipStream_t st, const Geom& g, const DevTables& tb, void* psi,
+                             int n_active);
+
+// partials: RS_NORM <- sum|a-b|^2 (or max|a-b| when want_max), RS_KIN <- sum|b|^2 (or max|a|)
+template <typename T>
+hipError_t launch_diff(hipStream_t st, const Geom& g, const DevTables& tb, const void* a,
+                       const void* b, int n_active, int want_max, int nblocks);
+
+}  // namespace bec

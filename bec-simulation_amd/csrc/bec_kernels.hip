@@ -1,0 +1,732 @@
+// Split-step kernels for gfx950.
+//
+// The step is organised in HBM passes over psi (layout [instance][component][z][y][x],
+// x fastest -- the reference's column-major (x, y, z, component) array):
+//   x pass  : whole x rows of ALL components in LDS -> inverse x FFT, every per-site
+//             operator of the potential half step (diagonal, spin mixing, DDI rotation,
+//             ITP scale), spin density + R2C for the dipolar convolution, norm/energy
+//             partials, forward x FFT.  One read and one write of psi.
+//   y/z pass: tiles of TB contiguous x columns by the whole line length, Stockham FFT
+//             in LDS; the last axis does forward FFT -> k-space multiply -> inverse FFT
+//             in the same launch (kinetic phase, or C_dd Q_ab(k) for the DDI spectra).
+// Reductions are per-block partials (fixed order) summed by a separate reduce launch,
+// so results do not depend on what else runs on the device.
+#include <hip/hip_runtime.h>
+
+#include "bec_device.h"
+#include "bec_kernels.h"
+
+namespace bec {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Deterministic block sum (result on thread 0).  sh: >= 16 doubles of LDS.
+__device__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += sh[i];
+  }
+  return r;
+}
+__device__ double block_max(double v, double* sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r = fmax(r, sh[i]);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double* partial_ptr(const DevTables& tb, int slot, int inst, int blk) {
+  return tb.partials + ((size_t)slot * tb.maxB + inst) * tb.max_blocks + blk;
+}
+
+template <int N>
+__host__ __device__ constexpr int ilog2c() {
+  int l = 0, v = N;
+  while (v > 1) { v >>= 1; ++l; }
+  return l;
+}
+
+// ---------------------------------------------------------------------------
+// x pass
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int D>
+__global__ void __launch_bounds__(1024)
+xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
+             cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, int ROWS) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int TEAM = NX / 8;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int inst = tb.active[blockIdx.y];
+  const InstParams& P = tb.params[inst];
+  const double dt = prog.dt_override > 0.0 ? prog.dt_override : P.dt;
+  const bool itp = prog.itp != 0;
+  const int st = prog.stages;
+  const bool has_aux = (st & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
+  const int TS = ROWS * NX;  // sites in this tile
+
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
+  cplx<T>* aux = tile + (size_t)D * TS;
+  cplx<T>* zf = aux + (has_aux ? 2 * TS : 0);
+  double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
+
+  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(tb.tw[ilog2c<NX>()]);
+  const cplx<double>* vfy = reinterpret_cast<const cplx<double>*>(tb.vfy);
+  const int row0 = blockIdx.x * ROWS;
+  const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
+
+  // Zeeman factors of every diagonal op (uniform over the block).
+  {
+    const int nz = (prog.nA + prog.nB) * D;
+    for (int idx = tid; idx < nz; idx += nthr) {
+      const int i = idx / D, c = idx % D;
+      const Op op = i < prog.nA ? prog.A[i] : prog.B[i - prog.nA];
+      if (op.kind == OP_DIAG) {
+        const double dtf = op.frac * dt;
+        if (itp) zf[idx] = mk<T>((T)exp(-(P.zee[c] - P.zee_shift) * dtf), (T)0);
+        else zf[idx] = cis_neg<T>(P.zee[c] * dtf);
+      }
+    }
+  }
+  // Load the tile: D contiguous chunks of ROWS*NX elements.
+  for (int e = tid; e < D * TS; e += nthr) {
+    const int c = e / TS, r = e - c * TS;
+    tile[e] = src[ibase + (size_t)c * g.nsites + r];
+  }
+  __syncthreads();
+
+  const int line = tid / TEAM, tt = tid - (tid / TEAM) * TEAM;
+  if (st & XS_INV_FFT) fft_team<T, NX, true>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+
+  T sc = (T)1;
+  {
+    // ITP renormalisation factor of the instance (only read when an OP_SCALE is present)
+    const double n2 = tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV;
+    sc = (T)(1.0 / sqrt(n2));
+  }
+
+  double phimax = 0.0;
+  if (st & XS_LOAD_PHI) {
+    // C2R along x of the three DDI potential spectra (reference irfft semantics: bins
+    // 0..NX/2 with the imaginary parts of DC and Nyquist dropped, Hermitian extension).
+    const size_t fb = (size_t)inst * g.fhat_inst_stride;
+    for (int e = tid; e < TS; e += nthr) {
+      const int r = e / NX, x = e - r * NX;
+      const size_t rowoff = (size_t)(row0 + r) * g.nxh_pitch;
+      const int kk = x <= NX / 2 ? x : NX - x;
+      cplx<T> yx = fhat[fb + rowoff + kk];
+      cplx<T> yy = fhat[fb + g.fhat_field_stride + rowoff + kk];
+      cplx<T> yz = fhat[fb + 2 * g.fhat_field_stride + rowoff + kk];
+      if (kk == 0 || kk == NX / 2) {
+        yx.y = 0; yy.y = 0; yz.y = 0;
+      } else if (x > NX / 2) {
+        yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
+      }
+      aux[(2 * r) * NX + x] = mk(yx.x - yy.y, yx.y + yy.x);
+      aux[(2 * r + 1) * NX + x] = yz;
+    }
+    __syncthreads();
+    fft_team<T, NX, true>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
+    if (st & XS_ENERGY) {
+      // nothing extra: energy stage reads aux directly
+    }
+  }
+
+  auto site_coords = [&](int s, int& x, int& y, int& z) {
+    const int r = s / NX;
+    x = s - r * NX;
+    const int row = row0 + r;
+    y = row % g.n[1];
+    z = row / g.n[1];
+  };
+  auto trap_at = [&](int s) -> double {
+    int x, y, z;
+    site_coords(s, x, y, z);
+    if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + (size_t)row0 * NX + s];
+    const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
+    double v = ax[x];
+    if (g.ndim >= 2) v += ax[g.nmax + y];
+    if (g.ndim >= 3) v += ax[2 * g.nmax + z];
+    return 0.5 * v;
+  };
+
+  auto run_ops = [&](const Op* ops, int nops, int zbase) {
+    for (int s = tid; s < TS; s += nthr) {
+      cplx<T> sp[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
+      double Vt = 0.0;
+      bool haveV = false;
+      for (int i = 0; i < nops; ++i) {
+        const Op op = ops[i];
+        const double dtf = op.frac * dt;
+        if (op.kind == OP_DIAG) {
+          if (!haveV) { Vt = trap_at(s); haveV = true; }
+          op_diag<T, D>(sp, Vt, P, dtf, itp, &zf[(zbase + i) * D]);
+        } else if (op.kind == OP_SPINMIX) {
+          op_spin_mixing<T, D>(sp, P, dtf, itp, vfy);
+        } else if (op.kind == OP_DDI_ROT) {
+          const int r = s / NX, x = s - r * NX;
+          const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
+          op_euler<T, D>(sp, (double)a.x, (double)a.y, (double)b.x, dtf, itp, vfy);
+        } else if (op.kind == OP_SCALE) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) sp[c] = sp[c] * sc;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
+    }
+    __syncthreads();
+  };
+
+  if ((st & XS_LOAD_PHI) && (st & XS_ENERGY) == 0) {
+    // max |Phi| for the imaginary-time overflow guard (ground_state.jl:4-29)
+    for (int s = tid; s < TS; s += nthr) {
+      const int r = s / NX, x = s - r * NX;
+      const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
+      phimax = fmax(phimax, fmax(fabs((double)a.x), fmax(fabs((double)a.y), fabs((double)b.x))));
+    }
+    const double m = block_max(phimax, rsh);
+    if (tid == 0) *partial_ptr(tb, RS_PHIMAX, inst, blockIdx.x) = m;
+  }
+
+  if (st & XS_OPS_A) run_ops(prog.A, prog.nA, 0);
+
+  if (st & XS_KIN_1D) {
+    fft_team<T, NX, false>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+    const double kdt = prog.kin_frac * dt;
+    for (int e = tid; e < D * TS; e += nthr) {
+      const int x = e % NX;
+      const double k = tb.ktab[x];
+      const double a = 0.5 * (k * k) * kdt;
+      cplx<T> ph = itp ? mk<T>((T)exp(-a), (T)0) : cis_neg<T>(a);
+      ph = ph * (T)g.inv_ntot;
+      tile[e] = tile[e] * ph;
+    }
+    __syncthreads();
+    fft_team<T, NX, true>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+  }
+
+  if (st & XS_OPS_B) run_ops(prog.B, prog.nB, prog.nA);
+
+  if (st & XS_ENERGY) {
+    double e_norm = 0, e_trap = 0, e_zee = 0, e_c0 = 0, e_c1 = 0, e_lhy = 0, e_ddi = 0, e_mz = 0;
+    constexpr int F = (D - 1) / 2;
+    for (int s = tid; s < TS; s += nthr) {
+      cplx<T> sp[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
+      double n = 0, zz = 0, mz = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double a2 = (double)abs2(sp[c]);
+        n += a2;
+        zz += P.zee[c] * a2;
+        mz += (double)(F - c) * a2;
+      }
+      T fx, fy, fz;
+      spin_vec<T, D>(sp, fx, fy, fz);
+      e_norm += n;
+      e_trap += trap_at(s) * n;
+      e_zee += zz;
+      e_mz += mz;
+      e_c0 += n * n;
+      e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
+      e_lhy += n * n * sqrt(n);
+      if (st & XS_LOAD_PHI) {
+        const int r = s / NX, x = s - r * NX;
+        const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
+        e_ddi += (double)a.x * fx + (double)a.y * fy + (double)b.x * fz;
+      }
+    }
+    double v;
+    v = block_sum(e_norm, rsh); if (tid == 0) *partial_ptr(tb, RS_NORM, inst, blockIdx.x) = v;
+    v = block_sum(e_trap, rsh); if (tid == 0) *partial_ptr(tb, RS_TRAP, inst, blockIdx.x) = v;
+    v = block_sum(e_zee, rsh); if (tid == 0) *partial_ptr(tb, RS_ZEE, inst, blockIdx.x) = v;
+    v = block_sum(e_c0, rsh); if (tid == 0) *partial_ptr(tb, RS_C0, inst, blockIdx.x) = v;
+    v = block_sum(e_c1, rsh); if (tid == 0) *partial_ptr(tb, RS_C1, inst, blockIdx.x) = v;
+    v = block_sum(e_lhy, rsh); if (tid == 0) *partial_ptr(tb, RS_LHY, inst, blockIdx.x) = v;
+    v = block_sum(e_ddi, rsh); if (tid == 0) *partial_ptr(tb, RS_DDI, inst, blockIdx.x) = v;
+    v = block_sum(e_mz, rsh); if (tid == 0) *partial_ptr(tb, RS_MZ, inst, blockIdx.x) = v;
+  }
+
+  if (st & XS_WRITE_FHAT) {
+    for (int s = tid; s < TS; s += nthr) {
+      cplx<T> sp[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
+      T fx, fy, fz;
+      spin_vec<T, D>(sp, fx, fy, fz);
+      const int r = s / NX, x = s - r * NX;
+      aux[(2 * r) * NX + x] = mk(fx, fy);
+      aux[(2 * r + 1) * NX + x] = mk(fz, (T)0);
+    }
+    __syncthreads();
+    fft_team<T, NX, false>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
+    const size_t fb = (size_t)inst * g.fhat_inst_stride;
+    constexpr int NH = NX / 2 + 1;
+    for (int e = tid; e < ROWS * NH; e += nthr) {
+      const int r = e / NH, k = e - r * NH;
+      const cplx<T> z1 = aux[(2 * r) * NX + k];
+      const cplx<T> z1n = conjc(aux[(2 * r) * NX + ((NX - k) & (NX - 1))]);
+      const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
+      const size_t o = fb + (size_t)(row0 + r) * g.nxh_pitch + k;
+      fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
+      fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
+      fhat[o + 2 * g.fhat_field_stride] = aux[(2 * r + 1) * NX + k];
+    }
+  }
+
+  if (st & XS_NORM) {
+    double acc = 0.0;
+    for (int s = tid; s < TS; s += nthr) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc += (double)abs2(tile[c * TS + s]);
+    }
+    const double v = block_sum(acc, rsh);
+    if (tid == 0) *partial_ptr(tb, RS_NORM, inst, blockIdx.x) = v;
+    __syncthreads();
+  }
+
+  if (st & XS_FWD_FFT) fft_team<T, NX, false>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+
+  if (st & XS_KE_1D) {
+    double acc = 0.0;
+    for (int e = tid; e < D * TS; e += nthr) {
+      const double k = tb.ktab[e % NX];
+      acc += (k * k) * (double)abs2(tile[e]);
+    }
+    const double v = block_sum(acc, rsh);
+    if (tid == 0) *partial_ptr(tb, RS_KIN, inst, blockIdx.x) = v;
+  }
+
+  if (!(st & XS_NO_STORE)) {
+    for (int e = tid; e < D * TS; e += nthr) {
+      const int c = e / TS, r = e - c * TS;
+      dst[ibase + (size_t)c * g.nsites + r] = tile[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y / z pass
+// ---------------------------------------------------------------------------
+template <typename T, int N, int MODE, int TB, int FIELDS>
+__global__ void __launch_bounds__(1024)
+strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, double kin_frac,
+               double dt_override, int itp, int ddi_data) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int TEAM = N / 8;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int inst = tb.active[blockIdx.z];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
+  double* tabs = reinterpret_cast<double*>(tile + FIELDS * N * TB);  // [2*(N+TB+1)] (re,im) pairs
+  double* rsh = tabs + 2 * (N + TB + 1);
+  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(tb.tw[ilog2c<N>()]);
+
+  const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
+  const int valid = ddi_data ? g.nxh : g.n[0];
+  const long long fstride = ddi_data ? g.fhat_field_stride : g.nsites;
+  const long long istride = ddi_data ? g.fhat_inst_stride : g.inst_stride;
+  const long long estride = axis == 1 ? (long long)pitch : (long long)pitch * g.n[1];
+  const long long ostride = axis == 1 ? (long long)pitch * g.n[1] : (long long)pitch;
+  const int nxch = (valid + TB - 1) / TB;
+  const int xch = blockIdx.x % nxch;
+  const int other = blockIdx.x / nxch;
+  const int x0 = xch * TB;
+  const size_t base = (size_t)inst * istride + (size_t)blockIdx.y * FIELDS * fstride +
+                      (size_t)other * ostride + x0;
+
+  for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
+    const int xi = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
+    cplx<T> v = mk((T)0, (T)0);
+    if (x0 + xi < valid) v = data[base + (size_t)f * fstride + (size_t)e * estride + xi];
+    tile[idx] = v;
+  }
+  // per-axis phase tables for the kinetic multiply
+  if (MODE == SM_KIN) {
+    const InstParams& P = tb.params[inst];
+    const double dt = (dt_override > 0.0 ? dt_override : P.dt) * kin_frac;
+    for (int i = tid; i < N + TB + 1; i += nthr) {
+      double k;
+      if (i < N) k = tb.ktab[axis * g.nmax + i];
+      else if (i < N + TB) k = (x0 + (i - N) < g.n[0]) ? tb.ktab[x0 + (i - N)] : 0.0;
+      else k = (g.ndim == 3 && axis == 2) ? tb.ktab[g.nmax + other] : 0.0;  // y for the z pass
+      const double a = 0.5 * (k * k) * dt;
+      double re, im;
+      if (itp) { re = exp(-a); im = 0.0; }
+      else { double s, c; sincos(a, &s, &c); re = c; im = -s; }
+      tabs[2 * i] = re;
+      tabs[2 * i + 1] = im;
+    }
+  }
+  __syncthreads();
+
+  const int xi = tid % TB, rest = tid / TB, tt = rest % TEAM, f = rest / TEAM;
+  const bool act = f < FIELDS;
+  cplx<T>* lbase = tile + (act ? f : 0) * N * TB + xi;
+  fft_team<T, N, false>(lbase, TB, tt, act, tw);
+
+  if (MODE == SM_KIN) {
+    for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
+      const int xi2 = idx % TB, e = (idx / TB) % N;
+      const cplx<double> pa = mk(tabs[2 * e], tabs[2 * e + 1]);
+      const cplx<double> px = mk(tabs[2 * (N + xi2)], tabs[2 * (N + xi2) + 1]);
+      const cplx<double> po = mk(tabs[2 * (N + TB)], tabs[2 * (N + TB) + 1]);
+      cplx<double> ph = (px * pa) * po;
+      ph = ph * g.inv_ntot;
+      tile[idx] = tile[idx] * mk((T)ph.x, (T)ph.y);
+    }
+    __syncthreads();
+  } else if (MODE == SM_DDI) {
+    const InstParams& P = tb.params[inst];
+    const double C = P.cdd;
+    for (int idx = tid; idx < N * TB; idx += nthr) {
+      const int xi2 = idx % TB, e = idx / TB;
+      const int x = x0 + xi2;
+      // rfftfreq along x (Nyquist +), fftfreq along y and z (ddi.jl:66-68)
+      double kx = 0.0;
+      if (x < valid) kx = (x == g.n[0] / 2) ? -tb.ktab[g.n[0] / 2] : tb.ktab[x];
+      double ky, kz;
+      if (axis == 2) { ky = tb.ktab[g.nmax + other]; kz = tb.ktab[2 * g.nmax + e]; }
+      else { ky = tb.ktab[g.nmax + e]; kz = 0.0; }
+      double k2 = kx * kx;
+      if (g.ndim >= 2) k2 += ky * ky;
+      if (g.ndim >= 3) k2 += kz * kz;
+      double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
+      if (k2 != 0.0) {
+        const double ik = 1.0 / k2;
+        if (g.ddi_secular) {
+          const double q = kz * kz * ik - 1.0 / 3.0;
+          qzz = q; qxx = -q / 2.0; qyy = -q / 2.0;
+        } else {
+          qxx = kx * kx * ik - 1.0 / 3.0;
+          qyy = ky * ky * ik - 1.0 / 3.0;
+          qzz = kz * kz * ik - 1.0 / 3.0;
+          qxy = kx * ky * ik;
+          qxz = kx * kz * ik;
+          qyz = ky * kz * ik;
+        }
+      }
+      const cplx<T> fx = tile[(0 * N + e) * TB + xi2];
+      const cplx<T> fy = tile[(1 * N + e) * TB + xi2];
+      const cplx<T> fz = tile[(2 * N + e) * TB + xi2];
+      const double s = C * g.inv_ntot;
+      auto mix = [&](double a, double b, double c) -> cplx<T> {
+        const double re = a * (double)fx.x + b * (double)fy.x + c * (double)fz.x;
+        const double im = a * (double)fx.y + b * (double)fy.y + c * (double)fz.y;
+        return mk((T)(re * s), (T)(im * s));
+      };
+      const cplx<T> px = mix(qxx, qxy, qxz);
+      const cplx<T> py = mix(qxy, qyy, qyz);
+      const cplx<T> pz = mix(qxz, qyz, qzz);
+      tile[(0 * N + e) * TB + xi2] = px;
+      tile[(1 * N + e) * TB + xi2] = py;
+      tile[(2 * N + e) * TB + xi2] = pz;
+    }
+    __syncthreads();
+  } else if (MODE == SM_KE) {
+    double acc = 0.0;
+    for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
+      const int xi2 = idx % TB, e = (idx / TB) % N;
+      const int x = x0 + xi2;
+      if (x >= valid) continue;
+      const double kx = tb.ktab[x];
+      double k2 = kx * kx;
+      if (axis == 2) {
+        const double ky = tb.ktab[g.nmax + other], kz = tb.ktab[2 * g.nmax + e];
+        k2 += ky * ky;
+        k2 += kz * kz;
+      } else {
+        const double ky = tb.ktab[g.nmax + e];
+        k2 += ky * ky;
+      }
+      acc += k2 * (double)abs2(tile[idx]);
+    }
+    const double v = block_sum(acc, rsh);
+    if (tid == 0)
+      *partial_ptr(tb, RS_KIN, inst, blockIdx.x + gridDim.x * blockIdx.y) = v;
+    return;
+  }
+
+  if (MODE == SM_INV || MODE == SM_KIN || MODE == SM_DDI)
+    fft_team<T, N, true>(lbase, TB, tt, act, tw);
+
+  for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
+    const int xi2 = idx % TB, rest2 = idx / TB, e = rest2 % N, f2 = rest2 / N;
+    if (x0 + xi2 < valid) data[base + (size_t)f2 * fstride + (size_t)e * estride + xi2] = tile[idx];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reductions and small passes
+// ---------------------------------------------------------------------------
+__global__ void reduce_kernel(DevTables tb, double* red, int nblocks, uint32_t slot_mask,
+                              uint32_t max_mask) {
+  __shared__ double sh[16];
+  // blockIdx.x enumerates the set bits of slot_mask
+  int slot = -1, cnt = 0;
+  for (int s = 0; s < RS_COUNT; ++s)
+    if (slot_mask & (1u << s)) {
+      if (cnt == (int)blockIdx.x) { slot = s; break; }
+      ++cnt;
+    }
+  if (slot < 0) return;
+  const int inst = tb.active[blockIdx.y];
+  const double* p = tb.partials + ((size_t)slot * tb.maxB + inst) * tb.max_blocks;
+  const bool is_max = (max_mask >> slot) & 1u;
+  double acc = 0.0, comp = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    const double v = p[i];
+    if (is_max) {
+      acc = fmax(acc, v);
+    } else {  // Neumaier compensated summation
+      const double t = acc + v;
+      comp += fabs(acc) >= fabs(v) ? (acc - t) + v : (v - t) + acc;
+      acc = t;
+    }
+  }
+  double r;
+  if (is_max) {
+    r = block_max(acc, sh);
+  } else {
+    const double a = block_sum(acc, sh);
+    __syncthreads();
+    const double c = block_sum(comp, sh);
+    r = a + c;
+  }
+  if (threadIdx.x == 0) red[(size_t)inst * RS_COUNT + slot] = r;
+}
+
+template <typename T>
+__global__ void scale_norm_kernel(Geom g, DevTables tb, cplx<T>* psi) {
+  const int inst = tb.active[blockIdx.y];
+  const double n2 = tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV;
+  const T s = (T)(1.0 / sqrt(n2));
+  cplx<T>* p = psi + (size_t)inst * g.inst_stride;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.inst_stride;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = p[i] * s;
+}
+
+template <typename T>
+__global__ void diff_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ a,
+                            const cplx<T>* __restrict__ b, int want_max) {
+  __shared__ double sh[16];
+  const int inst = tb.active[blockIdx.y];
+  const cplx<T>* pa = a + (size_t)inst * g.inst_stride;
+  const cplx<T>* pb = b + (size_t)inst * g.inst_stride;
+  double s0 = 0.0, s1 = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.inst_stride;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cplx<T> va = pa[i], vb = pb[i];
+    const double dr = (double)va.x - (double)vb.x, di = (double)va.y - (double)vb.y;
+    const double d2 = dr * dr + di * di;
+    if (want_max) {
+      s0 = fmax(s0, sqrt(d2));
+      s1 = fmax(s1, sqrt((double)abs2(va)));
+    } else {
+      s0 += d2;
+      s1 += (double)abs2(vb);
+    }
+  }
+  double r0, r1;
+  if (want_max) {
+    r0 = block_max(s0, sh);
+    __syncthreads();
+    r1 = block_max(s1, sh);
+  } else {
+    r0 = block_sum(s0, sh);
+    __syncthreads();
+    r1 = block_sum(s1, sh);
+  }
+  if (threadIdx.x == 0) {
+    *partial_ptr(tb, RS_NORM, inst, blockIdx.x) = r0;
+    *partial_ptr(tb, RS_KIN, inst, blockIdx.x) = r1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int xpass_rows(int D, int NX, int nrows) {
+  int target = 3072 / (D * NX);
+  if (target < 1) target = 1;
+  int r = 1;
+  while (r * 2 <= target) r *= 2;
+  while (r > 1 && (nrows % r) != 0) r /= 2;
+  if (r > nrows) r = nrows;
+  return r;
+}
+
+template <typename T, int NX, int D>
+static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
+                          const void* src, void* dst, void* fhat, int n_active) {
+  const int rows = xpass_rows(D, NX, g.nrows);
+  const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
+  const size_t lds = sizeof(cplx<T>) * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0) +
+                                        2 * kMaxOps * D) +
+                     16 * sizeof(double);
+  const int threads = D * rows * (NX / 8);
+  dim3 grid(g.nrows / rows, n_active);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)xpass_kernel<T, NX, D>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((xpass_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,
+                     (const cplx<T>*)src, (cplx<T>*)dst, (cplx<T>*)fhat, rows);
+  return hipGetLastError();
+}
+
+template <typename T, int D>
+static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
+                          const void* src, void* dst, void* fhat, int n_active) {
+  switch (g.n[0]) {
+    case 16: return xpass_n<T, 16, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 32: return xpass_n<T, 32, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 64: return xpass_n<T, 64, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 128: return xpass_n<T, 128, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 256: return xpass_n<T, 256, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 512: return xpass_n<T, 512, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
+                        const void* src, void* dst, void* fhat, int n_active) {
+  switch (g.D) {
+    case 3: return xpass_d<T, 3>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 5: return xpass_d<T, 5>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 13: return xpass_d<T, 13>(st, g, tb, prog, src, dst, fhat, n_active);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int strided_tb(int N, int fields, int csize) { return strided_tb_c(N, fields, csize); }
+
+template <typename T, int N, int MODE, int FIELDS>
+static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, int axis,
+                            void* data, int n_active, double kin_frac, double dt_override,
+                            int itp, int ddi_data) {
+  // TB columns per tile: keep the tile at <= 64 KiB of LDS
+  constexpr int TB = strided_tb_c(N, FIELDS, (int)sizeof(cplx<T>));
+  const int valid = ddi_data ? g.nxh : g.n[0];
+  const int nxch = (valid + TB - 1) / TB;
+  const int n_other = axis == 1 ? g.n[2] : g.n[1];
+  const int ngroups = ddi_data ? 1 : g.D / FIELDS;
+  dim3 grid(nxch * n_other, ngroups, n_active);
+  const int threads = FIELDS * TB * (N / 8);
+  const size_t lds = sizeof(cplx<T>) * FIELDS * N * TB + sizeof(double) * (2 * (N + TB + 1) + 16);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)strided_kernel<T, N, MODE, TB, FIELDS>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((strided_kernel<T, N, MODE, TB, FIELDS>), grid, dim3(threads), lds, st, g, tb,
+                     (cplx<T>*)data, axis, kin_frac, dt_override, itp, ddi_data);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+static hipError_t strided_mode(hipStream_t st, const Geom& g, const DevTables& tb, int axis,
+                               int mode, void* data, int n_active, double kin_frac,
+                               double dt_override, int itp, int ddi_data) {
+  if (ddi_data) {
+    switch (mode) {
+      case SM_FWD: return strided_n<T, N, SM_FWD, 3>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 1);
+      case SM_INV: return strided_n<T, N, SM_INV, 3>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 1);
+      case SM_DDI: return strided_n<T, N, SM_DDI, 3>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 1);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (mode) {
+    case SM_FWD: return strided_n<T, N, SM_FWD, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
+    case SM_INV: return strided_n<T, N, SM_INV, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
+    case SM_KIN: return strided_n<T, N, SM_KIN, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
+    case SM_KE: return strided_n<T, N, SM_KE, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, int axis, int mode,
+                          void* data, int n_active, double kin_frac, double dt_override, int itp,
+                          int ddi_data, int* nblocks_out) {
+  if (nblocks_out) {
+    const int tbw = strided_tb(g.n[axis], ddi_data ? 3 : 1, (int)sizeof(cplx<T>));
+    const int valid = ddi_data ? g.nxh : g.n[0];
+    const int n_other = axis == 1 ? g.n[2] : g.n[1];
+    *nblocks_out = ((valid + tbw - 1) / tbw) * n_other * (ddi_data ? 1 : g.D);
+  }
+  switch (g.n[axis]) {
+    case 16: return strided_mode<T, 16>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 32: return strided_mode<T, 32>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 64: return strided_mode<T, 64>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 128: return strided_mode<T, 128>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 256: return strided_mode<T, 256>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 512: return strided_mode<T, 512>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_reduce(hipStream_t st, const DevTables& tb, double* red, int n_active,
+                         int nblocks, uint32_t slot_mask, uint32_t max_mask) {
+  int nslots = __builtin_popcount(slot_mask);
+  if (nslots == 0 || n_active == 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_kernel, dim3(nslots, n_active), dim3(256), 0, st, tb, red, nblocks,
+                     slot_mask, max_mask);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale_norm(hipStream_t st, const Geom& g, const DevTables& tb, void* psi,
+                             int n_active) {
+  long long nb = (g.inst_stride + 255) / 256;
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL((scale_norm_kernel<T>), dim3((int)nb, n_active), dim3(256), 0, st, g, tb,
+                     (cplx<T>*)psi);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_diff(hipStream_t st, const Geom& g, const DevTables& tb, const void* a,
+                       const void* b, int n_active, int want_max, int nblocks) {
+  hipLaunchKernelGGL((diff_kernel<T>), dim3(nblocks, n_active), dim3(256), 0, st, g, tb,
+                     (const cplx<T>*)a, (const cplx<T>*)b, want_max);
+  return hipGetLastError();
+}
+
+#define BEC_INST(T)                                                                           \
+  template hipError_t launch_xpass<T>(hipStream_t, const Geom&, const DevTables&,              \
+                                      const XProgram&, const void*, void*, void*, int);        \
+  template hipError_t launch_strided<T>(hipStream_t, const Geom&, const DevTables&, int, int,  \
+                                        void*, int, double, double, int, int, int*);           \
+  template hipError_t launch_scale_norm<T>(hipStream_t, const Geom&, const DevTables&, void*,  \
+                                           int);                                               \
+  template hipError_t launch_diff<T>(hipStream_t, const Geom&, const DevTables&, const void*,  \
+                                     const void*, int, int, int);
+BEC_INST(double)
+BEC_INST(float)
+
+}  // namespace bec

@@ -1,0 +1,206 @@
+"""Benchmark of the MI355X split-step path.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Prints ONE JSON line.  A step is one accepted time step of the whole grid
+(all 2F+1 components); value = site-steps/s with psi resident in HBM.
+Workloads (sizes from the reference benchmark configurations):
+  rt3d_spin1_128 : 3-D spin-1 real-time leapfrog (run_simulation!), 128^3, box 20,
+                   c0=10, c1=+0.5, omega=(1,1,1), q=0.1, dt=1e-3, saves every 10 steps
+                   (energy/norm/Mz on the device); TF polar + 1e-3 seeded noise in m=+-1
+  itp1d_spin1_64 : 1-D spin-1 imaginary-time split_step!, 64 points, box 20, c0=10,
+                   c1=-0.5, omega=1, dt=0.005 (ITP renormalisation every step)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bec-simulation_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def _cfg2_inputs(G, n=128, precision="fp64"):
+    gc = G.GridConfig((n, n, n), (20.0, 20.0, 20.0))
+    grid = G.make_grid(gc)
+    sys_ = G.SpinSystem(1)
+    trap = G.HarmonicTrap(1.0, 1.0, 1.0)
+    psi = G.init_psi_thomas_fermi(grid, sys_, trap, 10.0)
+    psi = G.add_noise(psi, 1e-3, 42, components=[0, 2])
+    return grid, trap, psi
+
+
+def _kernel_bytes(name, D, nsites, csz, nxh_ratio=0.5):
+    """Algorithmic HBM bytes of one launch of a kernel class: one read and one
+    write of every array the pass must touch (psi: D complex per site)."""
+    if name in ("xpass", "fwd", "kin", "inv"):
+        return 2.0 * D * nsites * csz
+    if name in ("ddi_fft", "ddi_conv"):
+        return 2.0 * 3 * nsites * nxh_ratio * csz
+    if name == "ke":
+        return 1.0 * D * nsites * csz
+    return 0.0
+
+
+def run_rt3d(G, A, steps, warmup, precision):
+    grid, trap, psi = _cfg2_inputs(G, precision=precision)
+    save_every = 10
+    sp = G.SimParams(dt=1e-3, n_steps=max(steps, 1), imaginary_time=False, save_every=save_every)
+    ws = G.make_workspace(grid=grid, atom=G.Rb87, interactions=G.InteractionParams(10.0, 0.5),
+                          zeeman=G.ZeemanParams(0.0, 0.1), potential=trap, sim_params=sp,
+                          psi_init=psi, precision=precision)
+    lib = G.api.L.lib()
+
+    def segment(k):
+        G.api.L.check(lib.bec_leapfrog_segment(ws.handle, k), "leapfrog")
+        ws.energy_terms()  # save point: energy, norm, Mz reductions on the device
+
+    def run(nsteps):
+        done = 0
+        while done < nsteps:
+            k = min(save_every, nsteps - done)
+            segment(k)
+            done += k
+
+    run(warmup)
+    ws.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    ws.synchronize()
+    t1 = time.perf_counter()
+    # profiled pass (HIP events around every launch) for per-kernel durations
+    ws.profile(True)
+    run(steps)
+    prof = ws.profile_read()
+    ws.profile(False)
+    return ws, grid, (t1 - t0), prof, psi
+
+
+def run_itp1d(G, A, steps, warmup, precision):
+    gc = G.GridConfig((64,), (20.0,))
+    grid = G.make_grid(gc)
+    psi = G.init_psi(grid, G.SpinSystem(1), "polar")
+    sp = G.SimParams(dt=0.005, n_steps=max(steps, 1), imaginary_time=True)
+    ws = G.make_workspace(grid=grid, atom=G.Rb87, interactions=G.InteractionParams(10.0, -0.5),
+                          potential=G.HarmonicTrap(1.0), sim_params=sp, psi_init=psi,
+                          precision=precision)
+    A.split_step(ws, warmup)
+    ws.synchronize()
+    t0 = time.perf_counter()
+    A.split_step(ws, steps)
+    ws.synchronize()
+    t1 = time.perf_counter()
+    ws.profile(True)
+    A.split_step(ws, steps)
+    prof = ws.profile_read()
+    ws.profile(False)
+    return ws, grid, (t1 - t0), prof, psi
+
+
+def cpu_baseline_rt3d(psi, grid_n=128, budget_s=15.0):
+    """FP64 oracle (numpy/pocketfft, single thread) on a bounded sample of the same
+    workload: leapfrog steps on the same 128^3 input until ~budget_s of CPU time."""
+    from oracle import spinor_oracle as O
+    og = O.make_grid((grid_n,) * 3, (20.0,) * 3)
+    ows = O.make_workspace(og, 1, psi, 1e-3, False, omega=(1, 1, 1), q=0.1, c0=10.0, c1=0.5,
+                           n_steps=1000, save_every=10)
+    O.half_potential_step(ows, 0.5e-3, False)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        ows.psi = O.kinetic_step(ows.psi, ows.kphase)
+        O.half_potential_step(ows, 1e-3, False)
+        n += 1
+    dt = time.perf_counter() - t0
+    sites = grid_n ** 3
+    return dict(value=n * sites / dt, unit="site-steps/s", cores=1, kind="port",
+                sample=f"{n} leapfrog steps (K + merged V(dt)) of the 128^3 spin-1 workload, "
+                       f"FP64 numpy oracle, {dt:.1f} s")
+
+
+def cpu_baseline_itp1d(budget_s=10.0):
+    from oracle import spinor_oracle as O
+    og = O.make_grid((64,), (20.0,))
+    psi = O.init_psi(og, 1, "polar")
+    ows = O.make_workspace(og, 1, psi, 0.005, True, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=5000)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        O.split_step(ows)
+        n += 1
+    dt = time.perf_counter() - t0
+    return dict(value=n * 64 / dt, unit="site-steps/s", cores=1, kind="port",
+                sample=f"{n} ITP split_step! of the 64-point spin-1 workload, FP64 numpy oracle")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="rt3d_spin1_128",
+                    choices=["rt3d_spin1_128", "itp1d_spin1_64"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design "
+                                   "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import becgpu as G
+    from becgpu import api as A
+
+    csz = 16 if args.precision == "fp64" else 8
+    if args.workload == "rt3d_spin1_128":
+        ws, grid, secs, prof, psi = run_rt3d(G, A, args.steps, args.warmup, args.precision)
+        b_alg = 288.0 if args.precision == "fp64" else 144.0
+        cfg = {"workload": "3D spin-1 real-time leapfrog 128^3 (run_simulation!, save_every=10)",
+               "grid": [128, 128, 128], "F": 1, "dt": 1e-3, "c0": 10.0, "c1": 0.5, "q": 0.1}
+    else:
+        ws, grid, secs, prof, psi = run_itp1d(G, A, args.steps, args.warmup, args.precision)
+        b_alg = 384.0 if args.precision == "fp64" else 192.0
+        cfg = {"workload": "1D spin-1 ITP split_step! 64 points", "grid": [64], "F": 1,
+               "dt": 0.005, "c0": 10.0, "c1": -0.5}
+    nsites = int(np.prod(grid.config.n_points))
+    D = ws.D
+    value = args.steps * nsites / secs
+    # dominant kernel by total time in the profiled pass
+    dom = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
+    tot_ms, cnt = prof[dom]
+    avg_s = tot_ms / cnt * 1e-3
+    kb = _kernel_bytes(dom, D, nsites, csz)
+    achieved = kb / avg_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": dom,
+            "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
+            "alg_bytes_per_launch": kb}
+    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
+                   "share": round(v[0] / max(1e-9, sum(x[0] for x in prof.values())), 3)}
+               for k, v in prof.items() if v[1] > 0}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_rt3d(psi) if args.workload == "rt3d_spin1_128" else cpu_baseline_itp1d()
+    line = {
+        "metric": "spinor site-steps/s & %HBM peak",
+        "value": value, "unit": "site-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 reductions)",
+        "data": "synthetic", "config": cfg,
+        "step_hbm_frac": round(value * b_alg / HBM_PEAK, 4), "b_alg_per_site_step": b_alg,
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

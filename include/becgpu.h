@@ -1,0 +1,148 @@
+/* becgpu -- C ABI of the MI355X (gfx950) split-step engine for spinor BECs.
+ *
+ * Drop-in boundary for the hot path of SpinorBEC.jl (anko9801/BEC-simulation).
+ * The reference is pure Julia with no FFI; each entry point below replaces the
+ * Julia function cited next to it, and a host driver (the `becgpu` Python
+ * package here, or a Julia `ccall` module) keeps the reference's public entry
+ * points (make_workspace, split_step!, run_simulation!, find_ground_state,
+ * find_ground_state_multistart, scan_continuation, run_simulation_yoshida!,
+ * total_energy) on top of it.
+ *
+ * Conventions
+ *  - psi is exchanged as interleaved complex double (re, im), in the reference's
+ *    memory order: column-major (x, y, z, component) -- x fastest, component
+ *    slowest; component c <-> m = F - c (types.jl:239-244).
+ *  - A workspace holds B >= 1 independent instances of the same shape (grid,
+ *    spin F, precision); each instance has its own physical parameters, trap
+ *    and psi.  Operations act on the active instances (bec_set_active).
+ *  - Every call returns 0 on success or a negative BEC_E* code; the message of
+ *    the last failure on the calling thread is available from bec_last_error().
+ *  - A workspace is not thread-safe; distinct workspaces may be driven from
+ *    distinct host threads (each owns its HIP stream).
+ */
+#ifndef BECGPU_H
+#define BECGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEC_OK 0
+#define BEC_EINVAL (-1)   /* invalid argument / configuration (ArgumentError)    */
+#define BEC_EDEVICE (-2)  /* HIP runtime error                                   */
+#define BEC_ENOMEM (-3)   /* device allocation failed                            */
+#define BEC_EUNSUP (-4)   /* configuration valid in the reference, not built here */
+
+#define BEC_FP64 0
+#define BEC_FP32 1 /* FP32 psi/FFT/pointwise; FP64 reductions and phase arguments */
+
+/* Energy terms returned per instance by bec_energy (energy.jl:6-54). */
+enum {
+  BEC_E_KIN = 0, BEC_E_TRAP, BEC_E_ZEEMAN, BEC_E_C0, BEC_E_C1, BEC_E_DDI, BEC_E_LHY,
+  BEC_E_TENSOR, BEC_E_NORM, BEC_E_MZ, BEC_E_TOTAL, BEC_E_COUNT
+};
+
+typedef struct bec_ws bec_ws;
+
+/* Shape and fixed options of a workspace (Workspace, types.jl:387-407;
+ * GridConfig types.jl:1-23; SimParams types.jl:217-237). */
+typedef struct bec_config {
+  int ndim;            /* 1, 2 or 3 */
+  int n[3];            /* grid points per axis (x, y, z); even; unused axes 1 */
+  double L[3];         /* box size per axis */
+  int F;               /* spin (2F+1 components); F in {1, 2, 6} */
+  int n_instances;     /* batch size B */
+  int precision;       /* BEC_FP64 or BEC_FP32 */
+  int imaginary_time;  /* SimParams.imaginary_time */
+  int normalize_every; /* SimParams.normalize_every (ITP renormalisation cadence) */
+  int enable_ddi;      /* make_workspace(enable_ddi=...) */
+  int ddi_secular;     /* make_ddi_params(secular=...) */
+  int ddi_padding;     /* make_workspace(ddi_padding=...) (not built: BEC_EUNSUP) */
+  /* Fy eigenvectors, D*D complex interleaved, row-major, columns ordered by the
+   * ascending eigenvalues -F..F (spin_matrices.jl:38-43). */
+  const double* fy_vecs;
+  /* Cell-centred coordinates and fftfreq wavenumbers of each axis, concatenated
+   * x then y then z (grid.jl:1-23). */
+  const double* x_tab;
+  const double* k_tab;
+} bec_config;
+
+/* Per-instance physics (InteractionParams, ZeemanParams, HarmonicTrap, dt). */
+typedef struct bec_inst_params {
+  double dt;
+  double c0, c1, c_lhy; /* interactions.jl / types.jl:336-346 */
+  double c_dd;          /* DDIParams.C_dd */
+  double p, q;          /* ZeemanParams (zeeman.jl:15-19) */
+  double omega[3];      /* HarmonicTrap omega per axis (potentials.jl:5-15) */
+} bec_inst_params;
+
+/* Workspace lifetime -- replaces make_workspace (initialization.jl:86-191). */
+int bec_ws_create(const bec_config* cfg, bec_ws** out);
+void bec_ws_destroy(bec_ws* ws);
+int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p);
+/* Explicit potential array (evaluate_potential, potentials.jl:1-32), n sites, or
+ * NULL to return to the harmonic trap of the instance parameters. */
+int bec_set_potential(bec_ws* ws, int inst, const double* V);
+/* psi upload/download (SimState.psi). */
+int bec_set_psi(bec_ws* ws, int inst, const double* psi);
+int bec_get_psi(bec_ws* ws, int inst, double* psi);
+/* Restrict subsequent operations to the listed instances. */
+int bec_set_active(bec_ws* ws, const int* insts, int n);
+
+/* split_step! x n_steps (split_step.jl:10-35): V(dt/2) K(dt) V(dt/2), ITP
+ * renormalisation every normalize_every steps.  t/step bookkeeping is host-side. */
+int bec_split_step(bec_ws* ws, int n_steps);
+/* One leapfrog segment of _run_simulation_leapfrog! (simulation.jl:49-94):
+ * V(dt/2) [K(dt) V(dt)]^(n-1) K(dt) V(dt/2). */
+int bec_leapfrog_segment(bec_ws* ws, int n_steps);
+/* _half_potential_step! (split_step.jl:48-133) with explicit dt_half. */
+int bec_half_potential(bec_ws* ws, double dt_half, int imaginary_time);
+/* apply_kinetic_step_batched! with the phase for dt (propagators.jl:130-145). */
+int bec_kinetic(bec_ws* ws, double dt, int imaginary_time);
+/* _strang_core! / _yoshida_core! (split_step.jl:146-183), real time. */
+int bec_strang_core(bec_ws* ws, double dt);
+int bec_yoshida_core(bec_ws* ws, double dt);
+/* _normalize_psi! (split_step.jl:185-195). */
+int bec_normalize(bec_ws* ws);
+
+/* total_energy terms (energy.jl:6-121) + norm and Mz (observables.jl:1-33),
+ * out[B][BEC_E_COUNT] for every instance (inactive rows untouched). */
+int bec_energy(bec_ws* ws, double* out);
+/* Copies between psi slots: slot 0 is psi, slots 1..3 are device-resident
+ * copies (psi_prev / psi_old / psi_strang of the drivers). */
+int bec_copy_slot(bec_ws* ws, int dst_slot, int src_slot);
+/* mode 0: out[B][2] = {sum |a-b|^2, sum |b|^2}  (_wavefunction_l2_change, adaptive.jl:42-50)
+ * mode 1: out[B][2] = {max |a-b|, max |a|}      (dpsi test, ground_state.jl:148-149) */
+int bec_diff(bec_ws* ws, int slot_a, int slot_b, int mode, double* out);
+/* ITP guard (ground_state.jl:4-29): out[B][2] = {sum |psi|^2 (NaN if any NaN),
+ * max |Phi| of the last DDI convolution (0 without DDI)}. */
+int bec_itp_guard(bec_ws* ws, double* out);
+
+/* Per-launch HIP-event timing on the workspace stream, by kernel class:
+ *  BEC_K_XPASS      x-axis pass (FFT along x + fused per-site operators)
+ *  BEC_K_FWD        y/z forward FFT pass over psi
+ *  BEC_K_KIN        last-axis pass: forward FFT, kinetic phase, inverse FFT
+ *  BEC_K_INV        y/z inverse FFT pass over psi
+ *  BEC_K_KE         last-axis forward FFT + kinetic-energy partial sums
+ *  BEC_K_DDI_FFT    y forward/inverse FFT of the DDI spectra
+ *  BEC_K_DDI_CONV   last-axis DDI pass: forward FFT, C_dd Q_ab(k) contraction, inverse FFT
+ *  BEC_K_REDUCE     partial-sum reduction
+ * No counterpart in the reference (which uses TimerOutputs, SpinorBEC.jl:12). */
+enum {
+  BEC_K_XPASS = 0, BEC_K_FWD, BEC_K_KIN, BEC_K_INV, BEC_K_KE, BEC_K_DDI_FFT, BEC_K_DDI_CONV,
+  BEC_K_REDUCE, BEC_K_COUNT
+};
+int bec_profile_enable(bec_ws* ws, int on);
+/* ms[BEC_K_COUNT] total milliseconds and count[BEC_K_COUNT] launches since enable;
+ * resets the record. */
+int bec_profile_read(bec_ws* ws, double* ms, long long* count);
+
+int bec_synchronize(bec_ws* ws);
+const char* bec_last_error(void);
+/* Bytes of device memory held by a workspace of this configuration (admission). */
+long long bec_workspace_bytes(const bec_config* cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BECGPU_H */

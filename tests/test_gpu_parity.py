@@ -1,0 +1,178 @@
+"""GPU (libbecgpu.so, FP64) vs the FP64 oracle on identical seeded inputs.
+
+Bar: per-component relative L2 <= 1e-10 for psi and
+relative energy <= 1e-12 for the plain FP64 path.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import spinor_oracle as O
+import becgpu as G
+from becgpu import api as A
+from helpers import grids, pair, random_psi, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+PSI_TOL = 1e-10
+E_TOL = 1e-12
+
+
+def _gs_inputs(n, L, F, state="polar"):
+    og, gg = grids(n, L)
+    return O.init_psi(og, F, state)
+
+
+def test_cfg1_ground_state_1d_spin1():
+    """1D spin-1 ITP: 64 points, box 20, c0=10, c1=-0.5, omega=1, dt=0.005, 5000 steps,
+    tol 1e-10, polar start (the CPU-runnable configuration)."""
+    og, gg = grids((64,), (20.0,))
+    psi0 = O.init_psi(og, 1, "polar")
+    ows = O.make_workspace(og, 1, psi0, 0.005, True, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=5000)
+    ro = O.find_ground_state(ows, 5000, 1e-10)
+    rg = G.find_ground_state(grid=gg, atom=G.Rb87, interactions=G.InteractionParams(10.0, -0.5),
+                             potential=G.HarmonicTrap(1.0), dt=0.005, n_steps=5000, tol=1e-10,
+                             psi_init=psi0)
+    print("oracle", ro, "gpu", {k: v for k, v in rg.items() if k != "workspace"})
+    assert rg["converged"] == ro["converged"]
+    assert rg["steps"] == ro["steps"]
+    assert abs(rg["energy"] - ro["energy"]) <= E_TOL * abs(ro["energy"])
+    assert rel_l2(rg["workspace"].get_psi(), ows.psi) <= PSI_TOL
+
+
+@pytest.mark.parametrize("itp", [True, False])
+def test_split_step_2d_spin1(itp):
+    n, L = (64, 32), (16.0, 12.0)
+    og, _ = grids(n, L)
+    psi = O.init_psi(og, 1, "uniform") * (1 + 0.1 * random_psi(og.shape, 1, 3)[0])
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 1, psi, 0.004, itp, omega=(1.0, 1.3), p=0.3, q=0.2, c0=12.0, c1=-0.7,
+                    c_lhy=0.5)
+    for _ in range(40):
+        O.split_step(ows)
+    A.split_step(gws, 40)
+    err = rel_l2(gws.get_psi(), ows.psi)
+    print("err", err)
+    assert err <= PSI_TOL
+
+
+def test_leapfrog_3d_spin1_snapshots():
+    n, L = (32, 32, 32), (20.0, 20.0, 20.0)
+    og, _ = grids(n, L)
+    psi = O.thomas_fermi_psi(og, 1, O.harmonic_potential(og, (1, 1, 1)), 10.0)
+    noise = 1e-3 * random_psi(og.shape, 3, 7)
+    psi[0] += noise[0]
+    psi[2] += noise[2]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 1, psi, 0.001, False, omega=(1, 1, 1), q=0.1, c0=10.0, c1=0.5,
+                    n_steps=20, save_every=5)
+    ro = O.run_simulation(ows)
+    rg = A.run_simulation(gws, keep_snapshots=False)
+    assert len(rg.energies) == len(ro["energies"]) == 5
+    for eg, eo in zip(rg.energies, ro["energies"]):
+        assert abs(eg - eo) <= E_TOL * abs(eo)
+    for ng, no in zip(rg.norms, ro["norms"]):
+        assert abs(ng - no) <= 1e-12
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
+@pytest.mark.parametrize("F", [2, 6])
+def test_spin_mixing_euler_3d(F):
+    n, L = (16, 16, 16), (10.0, 10.0, 10.0)
+    og, _ = grids(n, L)
+    D = 2 * F + 1
+    psi = random_psi(og.shape, D, 11 + F) * O.gaussian(og, (2.0, 2.0, 2.0))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, F, psi, 0.002, False, omega=(1, 1, 1), p=0.2, q=0.05, c0=8.0, c1=3.0)
+    for _ in range(5):
+        O.split_step(ows)
+    A.split_step(gws, 5)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
+@pytest.mark.parametrize("itp", [False, True])
+@pytest.mark.parametrize("secular", [False, True])
+def test_ddi_3d_split_step(itp, secular):
+    n, L = (16, 32, 16), (12.0, 14.0, 12.0)
+    og, _ = grids(n, L)
+    F = 1
+    psi = random_psi(og.shape, 3, 21) * O.gaussian(og, (2.5, 2.5, 2.5))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, F, psi, 0.002, itp, omega=(1, 1, 1), c0=6.0, c1=-0.4, ddi=True, cdd=2.0,
+                    secular=secular)
+    for _ in range(4):
+        O.split_step(ows)
+    A.split_step(gws, 4)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+    eo = O.energy_terms(ows)
+    eg = A.energy_terms(gws)
+    for k in ("kin", "trap", "c0", "c1", "ddi"):
+        assert abs(eg[k] - eo[k]) <= 1e-12 * max(1.0, abs(eo[k])), k
+
+
+def test_ddi_2d_spin6_energy():
+    n, L = (32, 16), (10.0, 8.0)
+    og, _ = grids(n, L)
+    F = 6
+    psi = random_psi(og.shape, 13, 5)
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, F, psi, 0.001, False, omega=(1, 1), p=0.5, q=0.1, c0=5.0, c1=0.3,
+                    ddi=True, cdd=1.5)
+    eo = O.energy_terms(ows)
+    eg = A.energy_terms(gws)
+    for k in eo:
+        assert abs(eg[k] - eo[k]) <= 1e-12 * max(1.0, abs(eo[k])), (k, eg[k], eo[k])
+    O.split_step(ows)
+    A.split_step(gws, 1)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
+def test_strang_and_yoshida_core():
+    n, L = (16, 16, 32), (10.0, 10.0, 12.0)
+    og, _ = grids(n, L)
+    psi = random_psi(og.shape, 5, 9) * O.gaussian(og, (2.0, 2.0, 2.5))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 2, psi, 0.001, False, omega=(1, 1, 1), c0=7.0, c1=0.8, ddi=True, cdd=1.0)
+    O.strang_core(ows, 0.003)
+    A.strang_core(gws, 0.003)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+    O.yoshida_core(ows, 0.004)
+    A.yoshida_core(gws, 0.004)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
+def test_adaptive_yoshida_accept_reject():
+    n, L = (16, 16, 16), (10.0, 10.0, 10.0)
+    og, _ = grids(n, L)
+    psi = random_psi(og.shape, 3, 4) * O.gaussian(og, (2.0, 2.0, 2.0))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 1, psi, 0.001, False, omega=(1, 1, 1), c0=10.0, c1=0.5, ddi=True, cdd=1.0)
+    ro = O.run_simulation_yoshida(ows, dt_init=0.01, tol=1e-6, t_end=0.05, save_interval=0.02)
+    rg = A.run_simulation_yoshida(gws, adaptive=G.AdaptiveDtParams(dt_init=0.01, tol=1e-6),
+                                  t_end=0.05, save_interval=0.02, keep_snapshots=False)
+    assert (rg["n_accepted"], rg["n_rejected"]) == (ro["n_accepted"], ro["n_rejected"])
+    assert abs(rg["final_dt"] - ro["final_dt"]) <= 1e-9 * ro["final_dt"]
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
+def test_batched_instances_match_single():
+    """Three instances with different c1/p/omega in one workspace == three separate runs."""
+    n, L = (32, 32), (12.0, 12.0)
+    og, gg = grids(n, L)
+    sp = G.SimParams(dt=0.003, n_steps=30, imaginary_time=True)
+    ws = G.Workspace(grid=gg, F=1, n_instances=3, sim_params=sp)
+    cfgs = [(-0.5, 0.1, 1.0), (0.3, 0.0, 1.2), (0.9, 0.4, 0.8)]
+    psis = []
+    for i, (c1, p, w) in enumerate(cfgs):
+        ws.set_instance(i, interactions=G.InteractionParams(9.0, c1), zeeman=G.ZeemanParams(p, 0.05),
+                        potential=G.HarmonicTrap(w, w))
+        psi = O.init_psi(og, 1, ["polar", "ferromagnetic", "uniform"][i])
+        ws.set_psi(psi, i)
+        psis.append(psi)
+    A.split_step(ws, 30)
+    for i, (c1, p, w) in enumerate(cfgs):
+        ows = O.make_workspace(og, 1, psis[i], 0.003, True, omega=(w, w), p=p, q=0.05, c0=9.0, c1=c1)
+        for _ in range(30):
+            O.split_step(ows)
+        assert rel_l2(ws.get_psi(i), ows.psi) <= PSI_TOL
