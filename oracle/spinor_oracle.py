@@ -447,7 +447,8 @@ def ddi_potential(fx, fy, fz, Q, cdd):
     px = cdd * (Q["xx"] * Fx + Q["xy"] * Fy + Q["xz"] * Fz)
     py = cdd * (Q["xy"] * Fx + Q["yy"] * Fy + Q["yz"] * Fz)
     pz = cdd * (Q["xz"] * Fx + Q["yz"] * Fy + Q["zz"] * Fz)
-    return tuple(np.fft.irfftn(a, s=shp) for a in (px, py, pz))
+    ax = tuple(range(len(shp)))
+    return tuple(np.fft.irfftn(a, s=shp, axes=ax) for a in (px, py, pz))
 
 
 def ddi_potential_padded(fx, fy, fz, Qp, cdd):

@@ -1,0 +1,388 @@
+"""Pins of the FP64 oracle against closed forms, brute force and the reference's
+own known-answer tests (CPU only).  Each test names the reference test it
+restates; none of them re-types the oracle's own formula."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from sympy import Rational, sqrt as ssqrt
+from sympy.physics.wigner import clebsch_gordan as s_cg
+
+from oracle import spinor_oracle as O
+
+
+def rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def rspinor(D, n, seed):
+    r = rng(seed)
+    return r.normal(size=(D, n)) + 1j * r.normal(size=(D, n))
+
+
+# --- grid (grid.jl) --------------------------------------------------------------
+def test_grid_cell_centred_and_fftfreq():
+    g = O.make_grid((8, 6), (4.0, 3.0))
+    assert np.allclose(g.x[0], -2 + 0.25 + 0.5 * np.arange(8), atol=1e-15)
+    assert np.allclose(g.k[0], 2 * np.pi / 4 * np.array([0, 1, 2, 3, -4, -3, -2, -1]), atol=1e-14)
+    assert np.allclose(g.k[1], 2 * np.pi / 3 * np.array([0, 1, 2, -3, -2, -1]), atol=1e-14)
+    assert g.k2.shape == (6, 8)
+    assert g.k2[2, 3] == pytest.approx(g.k[0][3] ** 2 + g.k[1][2] ** 2, rel=1e-15)
+    with pytest.raises(ValueError):
+        O.make_grid((7,), (1.0,))
+
+
+# --- spin matrices (test_spin_matrices.jl) ---------------------------------------
+@pytest.mark.parametrize("F", [1, 2, 3, 6])
+def test_spin_algebra(F):
+    s = O.spin_matrices(F)
+    Fx, Fy, Fz = s["Fx"], s["Fy"], s["Fz"]
+    assert np.allclose(Fx @ Fy - Fy @ Fx, 1j * Fz, atol=1e-12)
+    assert np.allclose(Fy @ Fz - Fz @ Fy, 1j * Fx, atol=1e-12)
+    cas = Fx @ Fx + Fy @ Fy + Fz @ Fz
+    assert np.allclose(cas, F * (F + 1) * np.eye(2 * F + 1), atol=1e-11)
+    assert np.allclose(s["fy_vals"], np.arange(-F, F + 1), atol=1e-12)
+
+
+# --- Clebsch-Gordan / 6j (test_clebsch_gordan.jl) --------------------------------
+def test_cg_known_values():
+    assert O.cg(1, 1, -1, 0, 0) == pytest.approx(1 / math.sqrt(3), abs=1e-12)
+    assert O.cg(1, 0, 0, 0, 0) == pytest.approx(-1 / math.sqrt(3), abs=1e-12)
+    assert O.cg(1, 1, 1, 0, 0) == 0.0
+    assert O.cg(2, 2, 2, 4, 4) == pytest.approx(1.0, abs=1e-12)
+    for F in (1, 2, 3):
+        for S in range(0, 2 * F + 1):
+            for M in range(-S, S + 1):
+                s = sum(O.cg(F, m1, M - m1, S, M) ** 2 for m1 in range(-F, F + 1)
+                        if abs(M - m1) <= F)
+                assert s == pytest.approx(1.0, abs=1e-10)
+
+
+def test_6j_roundtrip_cn_gS():
+    """c_k -> g_S -> back (test_clebsch_gordan.jl round-trip) and the F=1
+    physical relation g0 = c0 - 2c1, g2 = c0 + c1 (interactions.jl:123-128)."""
+    F = 2
+    c = {0: 1.3, 2: -0.4, 4: 0.7}
+    g = O.cn_to_gS(F, c)
+    ev = [0, 2, 4]
+    W = np.array([[(2 * k + 1) * O.wigner6j(F, F, k, F, F, S) for k in ev] for S in ev])
+    back = np.linalg.solve(W, [g[S] for S in ev])
+    assert np.allclose(back, [c[k] for k in ev], atol=1e-12)
+    g1 = O.c0c1_to_gS(1, 3.0, 0.5)
+    assert g1[0] == pytest.approx(3.0 - 2 * 0.5) and g1[2] == pytest.approx(3.5)
+
+
+@pytest.mark.parametrize("F", [1, 2, 6])
+def test_channel_identity(F):
+    """sum_S g_S sum_M |A_SM|^2 = c0 n^2 + c1 |F|^2 for g_S = c0 + c1 (S(S+1)-2F(F+1))/2
+    (test_interactions_constraint.jl:129-171) -- pins tensor_energy and the CG table."""
+    c0, c1 = 2.7, -0.9
+    ch = O.c0c1_to_gS(F, c0, c1)
+    psi = rspinor(2 * F + 1, 5, F)
+    E = O.tensor_energy(psi, F, ch, 1.0)           # = 1/2 sum_S g_S |A|^2
+    n = np.sum(np.abs(psi) ** 2, axis=0)
+    fx, fy, fz = O.spin_density(psi, F)
+    rhs = 0.5 * np.sum(c0 * n ** 2 + c1 * (fx ** 2 + fy ** 2 + fz ** 2))
+    assert E == pytest.approx(rhs, rel=1e-10)
+
+
+# --- spin density (observables.jl:61-88) -----------------------------------------
+@pytest.mark.parametrize("F", [1, 2, 6])
+def test_spin_density_equals_expectation(F):
+    s = O.spin_matrices(F)
+    psi = rspinor(2 * F + 1, 4, 10 + F)
+    fx, fy, fz = O.spin_density(psi, F)
+    for i in range(4):
+        v = psi[:, i]
+        assert fx[i] == pytest.approx(np.real(np.vdot(v, s["Fx"] @ v)), rel=1e-12)
+        assert fy[i] == pytest.approx(np.real(np.vdot(v, s["Fy"] @ v)), rel=1e-12)
+        assert fz[i] == pytest.approx(np.real(np.vdot(v, s["Fz"] @ v)), rel=1e-12)
+
+
+# --- Rodrigues / Euler rotations vs brute-force expm (test_spin_mixing.jl) --------
+@pytest.mark.parametrize("itp", [False, True])
+def test_rodrigues_vs_expm(itp):
+    s = O.spin_matrices(1)
+    psi = rspinor(3, 6, 3)
+    c1, dt = 0.7, 0.13
+    out = O.rodrigues_step(psi.copy(), c1, dt, itp)
+    fx, fy, fz = O.spin_density(psi, 1)
+    for i in range(6):
+        H = c1 * (fx[i] * s["Fx"] + fy[i] * s["Fy"] + fz[i] * s["Fz"])
+        U = sla.expm(-H * dt) if itp else sla.expm(-1j * H * dt)
+        assert np.allclose(out[:, i], U @ psi[:, i], atol=1e-12)
+
+
+@pytest.mark.parametrize("F", [2, 6])
+@pytest.mark.parametrize("itp", [False, True])
+def test_euler_vs_expm(F, itp):
+    s = O.spin_matrices(F)
+    D = 2 * F + 1
+    psi = rspinor(D, 5, 7)
+    r = rng(5)
+    ph = r.normal(size=(3, 5))
+    dt = 0.21
+    out = O.euler_rotation(psi.copy(), ph[0], ph[1], ph[2], dt, F, itp)
+    for i in range(5):
+        H = ph[0, i] * s["Fx"] + ph[1, i] * s["Fy"] + ph[2, i] * s["Fz"]
+        U = sla.expm(-H * dt) if itp else sla.expm(-1j * H * dt)
+        assert np.allclose(out[:, i], U @ psi[:, i], atol=1e-11)
+    # identity below the |phi||dt| < 1e-14 threshold (test_spin_mixing.jl Euler identity)
+    z = np.zeros(5)
+    same = O.euler_rotation(psi.copy(), z + 1e-16, z, z, 1.0, F, itp)
+    assert np.array_equal(same, psi)
+
+
+# --- tensor step (tensor_interaction.jl) -----------------------------------------
+def _brute_h(F, ch, v):
+    D = 2 * F + 1
+    h = np.zeros((D, D), complex)
+    for S, g in ch.items():
+        for M in range(-S, S + 1):
+            for m in range(-F, F + 1):
+                mu = M - m
+                if abs(mu) > F:
+                    continue
+                a = float(s_cg(F, F, S, m, mu, M))
+                for mp in range(-F, F + 1):
+                    nu = M - mp
+                    if abs(nu) > F:
+                        continue
+                    b = float(s_cg(F, F, S, mp, nu, M))
+                    h[F - m, F - mp] += g * a * b * np.conj(v[F - mu]) * v[F - nu]
+    return h
+
+
+@pytest.mark.parametrize("itp", [False, True])
+def test_tensor_step_vs_expm(itp):
+    F = 2
+    ch = {0: 1.1, 2: -0.6, 4: 0.9}
+    T = O.tensor_fold(F, ch)
+    psi = rspinor(5, 3, 2)
+    out = O.tensor_step(psi.copy(), T, 0.17, itp)
+    for i in range(3):
+        h = _brute_h(F, ch, psi[:, i])
+        assert np.allclose(h, h.conj().T, atol=1e-12)
+        U = sla.expm(-h * 0.17) if itp else sla.expm(-1j * h * 0.17)
+        assert np.allclose(out[:, i], U @ psi[:, i], atol=1e-11)
+
+
+def test_tensor_f1_mean_field_equals_contact_plus_spin():
+    """F=1 tensor with g from (c0, c1): the mean-field action h psi equals
+    (c0 n + c1 f.F) psi exactly, and the propagation agrees with diagonal(c0) +
+    Rodrigues(c1) to 1e-4 at small dt (test_analytical_validation.jl:197-240)."""
+    c0, c1 = 4.0, -1.3
+    s = O.spin_matrices(1)
+    T = O.tensor_fold(1, O.c0c1_to_gS(1, c0, c1))
+    psi = rspinor(3, 7, 9)
+    fx, fy, fz = O.spin_density(psi, 1)
+    n = np.sum(np.abs(psi) ** 2, axis=0)
+    for i in range(7):
+        v = psi[:, i]
+        h = np.einsum("abcd,c,d->ab", T, np.conj(v), v)
+        rhs = c0 * n[i] * v + c1 * (fx[i] * s["Fx"] + fy[i] * s["Fy"] + fz[i] * s["Fz"]) @ v
+        assert np.allclose(h @ v, rhs, atol=1e-12)
+    psi = psi / np.sqrt(n.max())
+    dt = 1e-3
+    a = O.tensor_step(psi.copy(), T, dt, False)
+    b = O.diagonal_step(psi.copy(), np.zeros(7), np.zeros(3), c0, 0.0, dt, False)
+    b = O.rodrigues_step(b, c1, dt, False)
+    assert np.max(np.abs(a - b)) < 1e-4
+
+
+# --- kinetic step / diagonal step closed forms (test_physics_level0.jl) ----------
+def test_kinetic_plane_wave_exact():
+    g = O.make_grid((16, 8), (5.0, 3.0))
+    kx, ky = g.k[0][3], g.k[1][-2]
+    X = g.x[0][None, :]
+    Y = g.x[1][:, None]
+    pw = np.exp(1j * (kx * X + ky * Y))
+    psi = np.stack([pw, 2 * pw, 0 * pw])
+    dt = 0.37
+    out = O.kinetic_step(psi, O.kinetic_phase(g, dt, False))
+    assert np.allclose(out, psi * np.exp(-0.5j * (kx ** 2 + ky ** 2) * dt), atol=1e-12)
+    out = O.kinetic_step(psi, O.kinetic_phase(g, dt, True))
+    assert np.allclose(out, psi * np.exp(-0.5 * (kx ** 2 + ky ** 2) * dt), atol=1e-12)
+
+
+def test_free_gaussian_spreading():
+    """<x^2>(T) = s0^2/2 + T^2/(2 s0^2) (test_physics_level0.jl:7-48)."""
+    s0, T, dt = 2.0, 3.0, 0.01
+    g = O.make_grid((256,), (40.0,))
+    psi = np.zeros((3, 256), complex)
+    psi[1] = np.exp(-g.x[0] ** 2 / (2 * s0 ** 2))
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * g.dV)
+    ws = O.make_workspace(g, 1, psi, dt, False, n_steps=300)
+    for _ in range(300):
+        O.split_step(ws)
+    n = np.abs(ws.psi[1]) ** 2
+    x = g.x[0]
+    var = np.sum(x ** 2 * n) / np.sum(n)
+    exact = s0 ** 2 / 2 + T ** 2 / (2 * s0 ** 2)
+    assert abs(var - exact) / exact < 1e-3
+
+
+def test_larmor_precession():
+    """Linear Zeeman p: <Fx>(t) = cos(p t), <Fy> = -sin(p t) (test_physics_level3.jl:7-56)."""
+    p, dt, nst = 1.0, 0.005, 200
+    g = O.make_grid((32,), (10.0,))
+    psi = np.zeros((3, 32), complex)
+    psi[0], psi[1], psi[2] = 0.5, 1 / math.sqrt(2), 0.5
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * g.dV)
+    ws = O.make_workspace(g, 1, psi, dt, False, p=p, n_steps=nst)
+    for _ in range(nst):
+        O.split_step(ws)
+    fx, fy, fz = O.spin_density(ws.psi, 1)
+    Nn = O.total_norm(ws.psi, g.dV)
+    t = nst * dt
+    assert np.sum(fx) * g.dV / Nn == pytest.approx(math.cos(p * t), abs=1e-9)
+    assert np.sum(fy) * g.dV / Nn == pytest.approx(-math.sin(p * t), abs=1e-9)
+
+
+def test_diagonal_step_closed_form():
+    g = O.make_grid((8,), (4.0,))
+    psi = rspinor(3, 8, 4)
+    V = O.harmonic_potential(g, (1.3,))
+    zee = O.zeeman_diag(1, 0.4, 0.2)
+    c0, cl, dtf = 3.0, 0.5, 0.07
+    n = np.sum(np.abs(psi) ** 2, axis=0)
+    out = O.diagonal_step(psi.copy(), V, zee, c0, cl, dtf, False)
+    for c in range(3):
+        ph = np.exp(-1j * (V + c0 * n + cl * n ** 1.5 + zee[c]) * dtf)
+        assert np.allclose(out[c], psi[c] * ph, atol=1e-13)
+    out = O.diagonal_step(psi.copy(), V, zee, c0, cl, dtf, True)
+    for c in range(3):
+        f = np.exp(-(V + c0 * n + cl * n ** 1.5 + zee[c] - zee.min()) * dtf)
+        assert np.allclose(out[c], psi[c] * f, atol=1e-13)
+
+
+# --- energies (test_analytical_validation.jl:9-73) -------------------------------
+@pytest.mark.parametrize("state,ec1", [("ferro", 1.0), ("polar", 0.0), ("afm", 0.0)])
+def test_uniform_state_energies(state, ec1):
+    g = O.make_grid((64,), (10.0,))
+    n0 = 1.0 / 10.0
+    psi = np.zeros((3, 64), complex)
+    if state == "ferro":
+        psi[0] = math.sqrt(n0)
+    elif state == "polar":
+        psi[1] = math.sqrt(n0)
+    else:
+        psi[0], psi[2] = math.sqrt(n0 / 2), -math.sqrt(n0 / 2)
+    ws = O.make_workspace(g, 1, psi, 0.01, False, c0=100.0, c1=10.0)
+    E = O.energy_terms(ws)
+    assert E["c0"] == pytest.approx(0.5 * 100 * n0 ** 2 * 10, rel=1e-10)
+    assert E["c1"] == pytest.approx(ec1 * 0.5 * 10 * n0 ** 2 * 10, abs=1e-12)
+    assert E["kin"] == pytest.approx(0.0, abs=1e-20)
+
+
+def test_ddi_energy_zero_uniform_polarised():
+    g = O.make_grid((16, 16, 16), (10.0, 10.0, 10.0))
+    n0 = 1.0 / 1000.0
+    psi = np.zeros((3,) + g.shape, complex)
+    psi[0] = math.sqrt(n0)
+    ws = O.make_workspace(g, 1, psi, 0.01, False, ddi=True, cdd=1.0)
+    assert abs(O.energy_terms(ws)["ddi"]) < 1e-12
+
+
+def test_ddi_q_traceless_and_secular():
+    """test_ddi.jl:2-18 (Q traceless off k=0) and :104-129 (secular identities)."""
+    Q = O.ddi_q((8, 6, 4), (0.5, 0.6, 0.7))
+    tr = Q["xx"] + Q["yy"] + Q["zz"]
+    assert np.allclose(tr.ravel()[1:], 0.0, atol=1e-14) and tr.ravel()[0] == 0.0
+    Qs = O.ddi_q((8, 6, 4), (0.5, 0.6, 0.7), secular=True)
+    assert np.allclose(Qs["xx"], -Qs["zz"] / 2) and np.allclose(Qs["xy"], 0)
+
+
+def test_ddi_potential_brute_force():
+    """Phi from the explicit definition: Phi_a(r) = Re (1/N) sum_k w(kx) e^{i k.r}
+    C sum_b Q_ab(k) F_b(k), the sum over the half spectrum kx = 0..nx/2 with
+    w = 1 at kx = 0, nx/2 and 2 otherwise, and the imaginary part of the kx = 0
+    and kx = nx/2 planes dropped after the y, z inverse sums (c2r semantics)."""
+    n, dx = (4, 6, 4), (0.9, 0.7, 1.1)
+    r = rng(8)
+    Fs = [r.normal(size=(n[2], n[1], n[0])) for _ in range(3)]
+    Q = O.ddi_q(n, dx)
+    cdd = 1.7
+    phi = O.ddi_potential(*Fs, Q, cdd)
+    nx, ny, nz = n
+    N = nx * ny * nz
+    X = np.arange(nx)
+    Yv = np.arange(ny)
+    Zv = np.arange(nz)
+    Fk = []
+    for F in Fs:  # brute-force forward DFT onto the half spectrum
+        out = np.zeros((nz, ny, nx // 2 + 1), complex)
+        for kz in range(nz):
+            for ky in range(ny):
+                for kx in range(nx // 2 + 1):
+                    e = np.exp(-2j * np.pi * (kz * Zv[:, None, None] / nz + ky * Yv[None, :, None] / ny
+                                              + kx * X[None, None, :] / nx))
+                    out[kz, ky, kx] = np.sum(F * e)
+        Fk.append(out)
+    names = [["xx", "xy", "xz"], ["xy", "yy", "yz"], ["xz", "yz", "zz"]]
+    for a in range(3):
+        Pk = cdd * sum(Q[names[a][b]] * Fk[b] for b in range(3))
+        res = np.zeros((nz, ny, nx))
+        for kx in range(nx // 2 + 1):
+            # inverse over z, y (complex), then the c2r x-sum
+            Zp = np.zeros((nz, ny), complex)
+            for kz in range(nz):
+                for ky in range(ny):
+                    Zp += Pk[kz, ky, kx] * np.exp(2j * np.pi * (kz * Zv[:, None] / nz + ky * Yv[None, :] / ny))
+            if kx == 0 or kx == nx // 2:
+                res += np.real(Zp)[:, :, None] * np.cos(2 * np.pi * kx * X / nx)[None, None, :]
+            else:
+                res += 2 * np.real(Zp[:, :, None] * np.exp(2j * np.pi * kx * X / nx)[None, None, :])
+        assert np.allclose(phi[a], res / N, atol=1e-12)
+
+
+# --- ground state / integrators ---------------------------------------------------
+def test_harmonic_ground_state_energy():
+    """c0 = c1 = 0 ITP converges to E = 1/2 per dimension (textbook)."""
+    g = O.make_grid((64, 64), (16.0, 16.0))
+    psi = O.init_psi(g, 1, "polar")
+    ws = O.make_workspace(g, 1, psi, 0.002, True, omega=(1.0, 1.0), n_steps=4000)
+    r = O.find_ground_state(ws, 4000, 1e-12)
+    assert r["energy"] == pytest.approx(1.0, abs=2e-6)
+
+
+def test_strang_second_and_yoshida_fourth_order():
+    """Halving dt: Strang error ratio > 3 (test_split_step.jl:49-78), Yoshida > 6
+    (:159-188).  Reference solution = Yoshida at dt/16."""
+    g = O.make_grid((32,), (12.0,))
+    psi = O.init_psi(g, 1, "uniform") * np.exp(0.3j * g.x[0])[None]
+    T = 0.4
+
+    def run(core, dt):
+        ws = O.make_workspace(g, 1, psi, dt, False, omega=(1.0,), c0=20.0, c1=-2.0, p=0.3, q=0.5)
+        for _ in range(int(round(T / dt))):
+            core(ws, dt)
+        return ws.psi
+
+    ref = run(O.yoshida_core, 0.4 / 128)
+    es = [np.max(np.abs(run(O.strang_core, dt) - ref)) for dt in (0.04, 0.02)]
+    ey = [np.max(np.abs(run(O.yoshida_core, dt) - ref)) for dt in (0.04, 0.02)]
+    assert es[0] / es[1] > 3.0
+    assert ey[0] / ey[1] > 6.0
+
+
+def test_l2_change_phase_identity():
+    """sum|dpsi|^2/sum|psi|^2 = 2(1 - cos phi) for a global phase (test_adaptive_dt.jl:103-106)."""
+    psi = rspinor(3, 50, 1)
+    phi = 0.3
+    assert O.l2_change(psi * np.exp(1j * phi), psi) == pytest.approx(2 * (1 - math.cos(phi)), rel=1e-12)
+
+
+def test_leapfrog_equals_strang_to_roundoff():
+    """The merged V(dt) leapfrog reproduces split_step! up to the non-additivity of the
+    symmetric inner splitting (simulation.jl:40-48): with c1 = 0 the V steps commute and
+    the two schedules agree to roundoff."""
+    g = O.make_grid((32, 16), (8.0, 6.0))
+    psi = O.init_psi(g, 1, "uniform") * (1 + 0.1 * rng(3).normal(size=g.shape))[None]
+    a = O.make_workspace(g, 1, psi, 0.01, False, omega=(1, 1), c0=5.0, n_steps=20, save_every=5)
+    b = O.make_workspace(g, 1, psi, 0.01, False, omega=(1, 1), c0=5.0, n_steps=20, save_every=5)
+    O.run_simulation(a)
+    for _ in range(20):
+        O.split_step(b)
+    assert np.max(np.abs(a.psi - b.psi)) < 1e-12
