@@ -24,7 +24,7 @@ class BecConfig(C.Structure):
         ("ndim", C.c_int), ("n", C.c_int * 3), ("L", C.c_double * 3), ("F", C.c_int),
         ("n_instances", C.c_int), ("precision", C.c_int), ("imaginary_time", C.c_int),
         ("normalize_every", C.c_int), ("enable_ddi", C.c_int), ("ddi_secular", C.c_int),
-        ("ddi_padding", C.c_int), ("fy_vecs", C.POINTER(C.c_double)),
+        ("ddi_padding", C.c_int), ("tensor", C.c_int), ("fy_vecs", C.POINTER(C.c_double)),
         ("x_tab", C.POINTER(C.c_double)), ("k_tab", C.POINTER(C.c_double)),
     ]
 
@@ -47,6 +47,7 @@ SIGNATURES = {
     "bec_ws_create": ([C.POINTER(BecConfig), C.POINTER(C.c_void_p)], C.c_int),
     "bec_ws_destroy": ([C.c_void_p], None),
     "bec_set_params": ([C.c_void_p, C.c_int, C.POINTER(BecInstParams)], C.c_int),
+    "bec_set_tensor": ([C.c_void_p, C.c_int, C.c_void_p, C.c_int], C.c_int),
     "bec_set_potential": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_set_psi": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_get_psi": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),

@@ -51,7 +51,8 @@ class Workspace:
     """Device-resident workspace of B instances of one shape (types.jl:387-407)."""
 
     def __init__(self, *, grid: Grid, F: int, n_instances: int, sim_params: SimParams,
-                 precision="fp64", enable_ddi=False, secular_ddi=False, ddi_padding=False):
+                 precision="fp64", enable_ddi=False, secular_ddi=False, ddi_padding=False,
+                 tensor=False):
         self.grid = grid
         self.F = F
         self.sys = SpinSystem(F)
@@ -60,6 +61,7 @@ class Workspace:
         self.sim_params = sim_params
         self.precision = _precision_code(precision)
         self.enable_ddi = bool(enable_ddi)
+        self.tensor = bool(tensor)
         sm = spin_matrices(F)
         cfg = L.BecConfig()
         nd = grid.ndim
@@ -75,6 +77,7 @@ class Workspace:
         cfg.enable_ddi = 1 if enable_ddi else 0
         cfg.ddi_secular = 1 if secular_ddi else 0
         cfg.ddi_padding = 1 if ddi_padding else 0
+        cfg.tensor = 1 if tensor else 0
         vecs = np.ascontiguousarray(sm["fy_vecs"], dtype=np.complex128)
         xt = np.ascontiguousarray(np.concatenate(grid.x), dtype=np.float64)
         kt = np.ascontiguousarray(np.concatenate(grid.k), dtype=np.float64)
@@ -145,6 +148,11 @@ class Workspace:
         self.interactions[inst] = interactions
         self.zeeman[inst] = zeeman
 
+    def set_tensor(self, inst, channels: dict):
+        """g_S couplings {S: g} of the general-F tensor step for one instance."""
+        g = np.array([channels.get(S, 0.0) for S in range(0, 2 * self.F + 1, 2)], dtype=np.float64)
+        L.check(L.lib().bec_set_tensor(self.handle, inst, g.ctypes.data, len(g)), "bec_set_tensor")
+
     def set_psi(self, psi, inst=0):
         a = np.ascontiguousarray(psi, dtype=np.complex128)
         if a.shape != (self.D,) + self.grid.shape:
@@ -213,6 +221,15 @@ def _tensor_active(F, interactions: InteractionParams):
                for i, v in enumerate(interactions.c_extra, start=1))
 
 
+def tensor_channels(F, interactions: InteractionParams):
+    """g_S = _c0c1_to_gS(F, c0, c1) + _c_extra_to_delta_gS(F, c_extra), channels with
+    |g_S| <= 1e-30 dropped (initialization.jl:169-174, tensor_interaction.jl:106-119)."""
+    gb = c0c1_to_gS(F, interactions.c0, interactions.c1)
+    gd = c_extra_to_delta_gS(F, interactions.c_extra)
+    g = {S: gb.get(S, 0.0) + gd.get(S, 0.0) for S in range(0, 2 * F + 1, 2)}
+    return {S: v for S, v in g.items() if abs(v) > 1e-30}
+
+
 def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
                    zeeman=ZeemanParams(), potential=NoPotential(), sim_params: SimParams,
                    psi_init=None, enable_ddi=False, c_dd=float("nan"), secular_ddi=False,
@@ -222,8 +239,12 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
         raise NotImplementedError("Raman coupling is outside the GPU hot path")
     if loss is not None:
         raise NotImplementedError("loss step is not built on the GPU path yet")
-    if _tensor_active(F, interactions):
-        raise NotImplementedError("general-F tensor interaction step is not built yet")
+    tensor = _tensor_active(F, interactions)
+    channels = None
+    if tensor:
+        channels = tensor_channels(F, interactions)
+        # the tensor step carries every contact channel; c0 = c1 = 0 elsewhere
+        interactions = InteractionParams(0.0, 0.0, interactions.c_lhy, ())
     c_dd_val = 0.0
     if enable_ddi:
         if math.isnan(c_dd):
@@ -234,9 +255,12 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
         else:
             c_dd_val = float(c_dd)
     ws = Workspace(grid=grid, F=F, n_instances=1, sim_params=sim_params, precision=precision,
-                   enable_ddi=enable_ddi, secular_ddi=secular_ddi, ddi_padding=ddi_padding)
+                   enable_ddi=enable_ddi, secular_ddi=secular_ddi, ddi_padding=ddi_padding,
+                   tensor=tensor)
     ws.set_instance(0, interactions=interactions, zeeman=zeeman, potential=potential,
                     c_dd=c_dd_val)
+    if tensor:
+        ws.set_tensor(0, channels)
     psi = init_psi(grid, SpinSystem(F)) if psi_init is None else np.array(psi_init, dtype=complex)
     ws.set_psi(psi, 0)
     ws.atom = atom

@@ -31,7 +31,8 @@ static int fail(int code, const std::string& msg) {
 
 struct bec_ws {
   Geom g{};
-  int F = 0, D = 0, B = 0, prec = 0, itp = 0, normalize_every = 1, ddi = 0;
+  int F = 0, D = 0, B = 0, prec = 0, itp = 0, normalize_every = 1, ddi = 0, tensor = 0;
+  double* d_tens_cg = nullptr;
   size_t csz = 16;
   std::vector<InstParams> hp;
   std::vector<bec_inst_params> user;
@@ -101,6 +102,7 @@ static void refresh_tables(bec_ws* ws) {
   t.active = ws->d_active;
   for (int i = 0; i < 10; ++i) t.tw[i] = ws->d_tw[i];
   t.vfy = ws->d_vfy;
+  t.tens_cg = ws->tensor ? ws->d_tens_cg : nullptr;
   t.red = ws->d_red;
   t.partials = ws->d_partials;
   t.max_blocks = ws->max_blocks;
@@ -214,8 +216,9 @@ struct Sched {
   }
   // _half_potential_step! (split_step.jl:48-133) for dt_half = f*dt
   int V(double f) {
+    const int mix = ws->tensor ? OP_TENSOR : OP_SPINMIX;
     CHK(add_op(OP_DIAG, f / 2));
-    CHK(add_op(OP_SPINMIX, f / 2));
+    CHK(add_op(mix, f / 2));
     if (ws->ddi) {
       if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
       p.stages |= XS_WRITE_FHAT;
@@ -224,7 +227,7 @@ struct Sched {
       p.stages |= XS_LOAD_PHI;
       CHK(add_op(OP_DDI_ROT, f));
     }
-    CHK(add_op(OP_SPINMIX, f / 2));
+    CHK(add_op(mix, f / 2));
     CHK(add_op(OP_DIAG, f / 2));
     return BEC_OK;
   }
@@ -292,6 +295,51 @@ static int run_items(bec_ws* ws, const std::vector<Item>& items, int itp, double
 }
 
 // ---------------------------------------------------------------------------
+// Clebsch-Gordan coefficients, Racah formula in log space (clebsch_gordan.jl:1-73)
+// ---------------------------------------------------------------------------
+static double lfact(int n) { return n < 0 ? -INFINITY : std::lgamma((double)n + 1.0); }
+static double ltri(int a, int b, int c) {
+  const int ab = a + b - c, ac = a - b + c, bc = -a + b + c;
+  if (ab < 0 || ac < 0 || bc < 0) return -INFINITY;
+  return lfact(ab) + lfact(ac) + lfact(bc) - lfact(a + b + c + 1);
+}
+static double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (m1 + m2 + m3 != 0) return 0.0;
+  const double lt = ltri(j1, j2, j3);
+  if (std::isinf(lt)) return 0.0;
+  const double ln = lfact(j1 + m1) + lfact(j1 - m1) + lfact(j2 + m2) + lfact(j2 - m2) +
+                    lfact(j3 + m3) + lfact(j3 - m3);
+  const int tmin = std::max({0, j2 - j3 - m1, j1 - j3 + m2});
+  const int tmax = std::min({j1 + j2 - j3, j1 - m1, j2 + m2});
+  if (tmin > tmax) return 0.0;
+  double s = 0.0;
+  for (int t = tmin; t <= tmax; ++t) {
+    const double ld = lfact(t) + lfact(j1 + j2 - j3 - t) + lfact(j1 - m1 - t) + lfact(j2 + m2 - t) +
+                      lfact(j3 - j2 + m1 + t) + lfact(j3 - j1 - m2 + t);
+    s += ((t & 1) ? -1.0 : 1.0) * std::exp(-ld);
+  }
+  const double ph = ((j1 - j2 - m3) % 2 == 0) ? 1.0 : -1.0;
+  return ph * std::exp(0.5 * (lt + ln)) * s;
+}
+static double clebsch(int j1, int m1, int j2, int m2, int J, int M) {
+  if (m1 + m2 != M) return 0.0;
+  const double ph = ((j1 - j2 + M) % 2 == 0) ? 1.0 : -1.0;
+  return ph * std::sqrt(2.0 * J + 1.0) * wigner3j(j1, j2, J, m1, m2, -M);
+}
+// CG table in the tens:: enumeration order used by the device tensor step.
+static std::vector<double> tensor_cg_table(int F) {
+  std::vector<double> t(tens::size(F), 0.0);
+  for (int M = -2 * F; M <= 2 * F; ++M)
+    for (int m1 = tens::m1lo(F, M); m1 <= tens::m1hi(M); ++m1)
+      for (int S = tens::s0(M); S <= 2 * F; S += 2) {
+        double v = clebsch(F, m1, F, M - m1, S, M);
+        if (std::fabs(v) < 1e-15) v = 0.0;  // clebsch_gordan.jl:174-175
+        t[tens::off(F, M, m1, S)] = v;
+      }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -347,6 +395,7 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
       return fail(BEC_EUNSUP, "grid sides must be powers of two in [16, 512] on this build");
   }
   if (!(c->F == 1 || c->F == 2 || c->F == 6)) return fail(BEC_EUNSUP, "F must be 1, 2 or 6");
+  if (c->F == 6 && c->n[0] > 256) return fail(BEC_EUNSUP, "F=6 supports x sides up to 256");
   if (c->n_instances < 1) return fail(BEC_EINVAL, "n_instances must be >= 1");
   if (c->precision != BEC_FP64 && c->precision != BEC_FP32)
     return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
@@ -362,6 +411,11 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   ws->itp = c->imaginary_time;
   ws->normalize_every = c->normalize_every;
   ws->ddi = c->enable_ddi;
+  ws->tensor = c->tensor;
+  if (ws->tensor && ws->F != 6) {
+    delete ws;
+    return fail(BEC_EUNSUP, "the general-F tensor step is built for F=6");
+  }
   ws->csz = ws->prec == BEC_FP32 ? 8 : 16;
   Geom& g = ws->g;
   g.ndim = c->ndim;
@@ -440,6 +494,12 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMalloc(&ws->d_vfy, vb) != hipSuccess) return cleanup(BEC_ENOMEM, "vfy");
   if (hipMemcpy(ws->d_vfy, c->fy_vecs, vb, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(BEC_EDEVICE, "vfy copy");
+  {
+    const std::vector<double> cgt = tensor_cg_table(ws->F);
+    if (hipMalloc(&ws->d_tens_cg, cgt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "cg");
+    if (hipMemcpy(ws->d_tens_cg, cgt.data(), cgt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "cg copy");
+  }
   // twiddles exp(-2 pi i m / N), computed in long double
   for (int l = 3; l <= 9; ++l) {
     const int N = 1 << l;
@@ -472,7 +532,8 @@ void bec_ws_destroy(bec_ws* ws) {
     if (ws->slot[i]) (void)hipFree(ws->slot[i]);
   if (ws->scratch) (void)hipFree(ws->scratch);
   if (ws->fhat) (void)hipFree(ws->fhat);
-  for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials, ws->d_vfy})
+  for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials, ws->d_vfy,
+                    ws->d_tens_cg})
     if (p) (void)hipFree(p);
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
@@ -490,7 +551,10 @@ int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p) {
   if (!(p->dt > 0)) return fail(BEC_EINVAL, "dt must be positive");
   ws->user[inst] = *p;
   InstParams& q = ws->hp[inst];
+  double keep_g[7];
+  std::memcpy(keep_g, q.gS, sizeof(keep_g));
   q = InstParams{};
+  std::memcpy(q.gS, keep_g, sizeof(keep_g));
   q.dt = p->dt;
   q.c0 = p->c0;
   q.c1 = p->c1;
@@ -521,6 +585,16 @@ int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p) {
   HIPCHK(hipMemcpyAsync(ws->d_axtab + (size_t)inst * 3 * g.nmax, ax.data(), ax.size() * 8,
                         hipMemcpyHostToDevice, ws->stream));
   HIPCHK(hipStreamSynchronize(ws->stream));
+  return BEC_OK;
+}
+
+int bec_set_tensor(bec_ws* ws, int inst, const double* gS, int n) {
+  if (!ws || !gS) return fail(BEC_EINVAL, "null argument");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  if (!ws->tensor) return fail(BEC_EINVAL, "workspace was created without the tensor step");
+  if (n != ws->F + 1) return fail(BEC_EINVAL, "expected F+1 channel couplings (S = 0, 2, ..., 2F)");
+  for (int k = 0; k < 7; ++k) ws->hp[inst].gS[k] = k < n ? gS[k] : 0.0;
+  ws->params_dirty = true;
   return BEC_OK;
 }
 
@@ -678,9 +752,11 @@ int bec_energy(bec_ws* ws, double* out) {
   } else {
     p.stages = XS_ENERGY | XS_NO_STORE;
   }
+  if (ws->tensor) p.stages |= XS_TENSOR_E;
   CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
   const uint32_t emask = (1u << RS_NORM) | (1u << RS_TRAP) | (1u << RS_ZEE) | (1u << RS_C0) |
-                         (1u << RS_C1) | (1u << RS_LHY) | (1u << RS_DDI) | (1u << RS_MZ);
+                         (1u << RS_C1) | (1u << RS_LHY) | (1u << RS_DDI) | (1u << RS_MZ) |
+                         (1u << RS_TENSOR);
   CHK(do_reduce(ws, ws->n_xblocks, emask));
   // kinetic energy: forward FFT of every component, sum k^2 |psi_hat|^2 (energy.jl:56-65)
   if (g.ndim == 1) {
@@ -708,7 +784,7 @@ int bec_energy(bec_ws* ws, double* out) {
     o[BEC_E_C1] = 0.5 * q.c1 * r[RS_C1] * dV;
     o[BEC_E_DDI] = ws->ddi ? 0.5 * r[RS_DDI] * dV : 0.0;
     o[BEC_E_LHY] = q.c_lhy != 0.0 ? (2.0 / 5.0) * q.c_lhy * r[RS_LHY] * dV : 0.0;
-    o[BEC_E_TENSOR] = 0.0;
+    o[BEC_E_TENSOR] = ws->tensor ? r[RS_TENSOR] * dV : 0.0;
     o[BEC_E_NORM] = r[RS_NORM] * dV;
     o[BEC_E_MZ] = r[RS_MZ] * dV;
     o[BEC_E_TOTAL] = o[BEC_E_KIN] + o[BEC_E_TRAP] + o[BEC_E_ZEEMAN] + o[BEC_E_C0] + o[BEC_E_C1] +

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "bec_types.h"
 
 namespace bec {
@@ -101,7 +103,7 @@ __device__ __forceinline__ void dftR(cplx<T>* v) {
 }
 
 // Radix plan for a length-N line: a list of radices (each 2, 4 or 8) whose
-// product is N, every pass processed by N/8 threads holding 8 values each.
+// product is N, every pass processed by N/E threads holding E (8 or 16) values each.
 template <int N>
 struct RadixPlan;
 template <> struct RadixPlan<8>   { static constexpr int np = 1; static constexpr int r[3] = {8, 0, 0}; };
@@ -112,15 +114,15 @@ template <> struct RadixPlan<128> { static constexpr int np = 3; static constexp
 template <> struct RadixPlan<256> { static constexpr int np = 3; static constexpr int r[3] = {8, 8, 4}; };
 template <> struct RadixPlan<512> { static constexpr int np = 3; static constexpr int r[3] = {8, 8, 8}; };
 
-// One Stockham pass: butterflies j = t + b*(N/8), b < 8/R.
+// One Stockham pass: butterflies j = t + b*(N/E), b < E/R.
 // in[j + r*N/R] -> twiddle W_N^{(j mod Ns) r N/(Ns R)} -> DFT_R -> out[(j/Ns)Ns R + j mod Ns + r Ns]
 // The line lives at base[e*ES]; tw[m] = exp(-2 pi i m / N).
-template <typename T, int N, int R, bool INV>
+template <typename T, int N, int R, bool INV, int E = 8>
 __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool active, int Ns,
                                               const cplx<T>* __restrict__ tw) {
-  constexpr int NB = 8 / R;      // butterflies per thread
-  constexpr int TEAM = N / 8;
-  cplx<T> v[8];
+  constexpr int NB = E / R;      // butterflies per thread
+  constexpr int TEAM = N / E;
+  cplx<T> v[E];
   if (active) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -156,19 +158,20 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
 // Full length-N FFT (unnormalised) of the line at base[e*ES], in place in LDS.
 // Every thread of the block must call it (it synchronises); `active` marks the
 // threads that own a (line, team index) pair.
-template <typename T, int N, bool INV>
+template <typename T, int N, bool INV, int E = 8>
 __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool active,
                                          const cplx<T>* __restrict__ tw) {
+  static_assert(N % E == 0, "line shorter than the per-thread register block");
   using P = RadixPlan<N>;
   int Ns = 1;
-  stockham_pass<T, N, P::r[0], INV>(base, ES, t, active, Ns, tw);
+  stockham_pass<T, N, P::r[0], INV, E>(base, ES, t, active, Ns, tw);
   Ns *= P::r[0];
   if constexpr (P::np > 1) {
-    stockham_pass<T, N, P::r[1], INV>(base, ES, t, active, Ns, tw);
+    stockham_pass<T, N, P::r[1], INV, E>(base, ES, t, active, Ns, tw);
     Ns *= P::r[1];
   }
   if constexpr (P::np > 2) {
-    stockham_pass<T, N, P::r[2], INV>(base, ES, t, active, Ns, tw);
+    stockham_pass<T, N, P::r[2], INV, E>(base, ES, t, active, Ns, tw);
   }
 }
 
@@ -276,8 +279,9 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
 // reference's cis recurrences.  ITP: Dz = exp(-m th).  vfy: D*D complex row-major V_Fy.
 template <typename T, int D>
 __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, double phz, double dt,
-                                         bool itp, const cplx<double>* __restrict__ vfy) {
+                                         bool itp, const cplx<double>* __restrict__ vfy_in) {
   constexpr int F = (D - 1) / 2;
+  const cplx<double>* vfy = vfy_in;
   double pm = sqrt(phx * phx + phy * phy + phz * phz);
   if (pm * fabs(dt) < 1e-14) return;
   double cz = phz / pm;
@@ -304,10 +308,11 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   ph = ry_phase;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
+    const cplx<double>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      cplx<double> q = vfy[j * D + i];  // Vt[i][j] = conj(V[j][i])
+      const cplx<double> q = mk((double)m[j * D + i].x, (double)m[j * D + i].y);  // Vt[i][j] = conj(V[j][i])
       acc = acc + mk((T)q.x, (T)(-q.y)) * v[j];
     }
     w[i] = ph * acc;
@@ -315,10 +320,11 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   }
 #pragma unroll
   for (int i = 0; i < D; ++i) {
+    const cplx<double>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      cplx<double> q = vfy[i * D + j];
+      const cplx<double> q = mk((double)m[i * D + j].x, (double)m[i * D + j].y);
       acc = acc + mk((T)q.x, (T)q.y) * w[j];
     }
     v[i] = acc;
@@ -347,10 +353,11 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   cplx<T> z_neg_beta = conjc(z_beta);
 #pragma unroll
   for (int i = 0; i < D; ++i) {
+    const cplx<double>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      cplx<double> q = vfy[j * D + i];
+      const cplx<double> q = mk((double)m[j * D + i].x, (double)m[j * D + i].y);
       acc = acc + mk((T)q.x, (T)(-q.y)) * v[j];
     }
     w[i] = ph * acc;
@@ -360,10 +367,11 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   cplx<T> z_alpha = conjc(z_neg_alpha);
 #pragma unroll
   for (int i = 0; i < D; ++i) {
+    const cplx<double>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      cplx<double> q = vfy[i * D + j];
+      const cplx<double> q = mk((double)m[i * D + j].x, (double)m[i * D + j].y);
       acc = acc + mk((T)q.x, (T)q.y) * w[j];
     }
     s[i] = ph * acc;
@@ -382,6 +390,152 @@ __device__ __forceinline__ void op_spin_mixing(cplx<T>* s, const InstParams& P, 
     spin_vec<T, D>(s, fx, fy, fz);
     op_euler<T, D>(s, P.c1 * (double)fx, P.c1 * (double)fy, P.c1 * (double)fz, dtf, itp, vfy);
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// General-F tensor interaction (tensor_interaction.jl:130-240)
+// ---------------------------------------------------------------------------
+template <int B, int E, int S = 1, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  if constexpr (B < E) {
+    fn(std::integral_constant<int, B>{});
+    static_for<B + S, E, S>(fn);
+  }
+}
+
+// w = h v with the mean-field matrix of the spinor psi,
+//   h_{m m'} = sum_S g_S sum_M CG(m, M-m | S M) CG(m', M-m' | S M) conj(psi_{M-m}) psi_{M-m'},
+// evaluated in factorised form (no D x D matrix is formed):
+//   B_SM = sum_{m'} CG(m', M-m') psi_{M-m'} v_{m'},  (h v)_m = sum_{S,M} g_S CG(m, M-m) conj(psi_{M-m}) B_SM,
+// using CG(m1, m2) = CG(m2, m1) for even S to visit each unordered pair once.
+// cgt: table in tens:: order; every index below is a compile-time constant.
+template <typename T, int D>
+__device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>* v, cplx<T>* w,
+                                              const volatile double* cgt, const T* g) {
+  constexpr int F = (D - 1) / 2;
+#pragma unroll
+  for (int c = 0; c < D; ++c) w[c] = mk((T)0, (T)0);
+  static_for<-2 * F, 2 * F + 1>([&](auto Mc) __attribute__((always_inline)) {
+    constexpr int M = decltype(Mc)::value;
+    constexpr int S0 = tens::s0(M);
+    constexpr int NS = tens::ns(F, M);
+    constexpr int LO = tens::m1lo(F, M);
+    constexpr int HI = tens::m1hi(M);
+    cplx<T> Bs[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) Bs[k] = mk((T)0, (T)0);
+    static_for<LO, HI + 1>([&](auto m1c) __attribute__((always_inline)) {
+      constexpr int m1 = decltype(m1c)::value;
+      constexpr int m2 = M - m1;
+      cplx<T> p = psi[F - m2] * v[F - m1];
+      if constexpr (m1 != m2) p = p + psi[F - m1] * v[F - m2];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] + p * (T)cgt[tens::off(F, M, m1, S0) + k];
+    });
+#pragma unroll
+    for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] * g[S0 / 2 + k];
+    static_for<LO, HI + 1>([&](auto m1c) __attribute__((always_inline)) {
+      constexpr int m1 = decltype(m1c)::value;
+      constexpr int m2 = M - m1;
+      cplx<T> C = mk((T)0, (T)0);
+#pragma unroll
+      for (int k = 0; k < NS; ++k) C = C + Bs[k] * (T)cgt[tens::off(F, M, m1, S0) + k];
+      w[F - m1] = w[F - m1] + conjc(psi[F - m2]) * C;
+      if constexpr (m1 != m2) w[F - m2] = w[F - m2] + conjc(psi[F - m1]) * C;
+    });
+  });
+}
+
+template <typename T>
+__device__ __forceinline__ T cabs1(cplx<T> a) { return fabs(a.x) + fabs(a.y); }
+
+// psi <- exp(-i h dt) psi (real time) or exp(-h dt) psi (imaginary time), h built from the
+// incoming spinor.  The reference diagonalises h (LAPACK eigen); here the action of the
+// exponential is summed as a Taylor series to working precision, with the step split into
+// nsub pieces when the norm bound theta = 2 |dt| n sum_S |g_S| exceeds 1/2.  Sites with
+// n < 1e-20 are left unchanged (tensor_interaction.jl:212).
+template <typename T, int D>
+__device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp,
+                                          const volatile double* cgt) {
+  constexpr int F = (D - 1) / 2;
+  double n = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) n += (double)abs2(s[c]);
+  if (n < 1e-20) return;
+  T g[F + 1];
+  double gsum = 0.0;
+#pragma unroll
+  for (int k = 0; k <= F; ++k) {
+    g[k] = (T)P.gS[k];
+    gsum += fabs(P.gS[k]);
+  }
+  const double theta = 2.0 * fabs(dtf) * n * gsum;
+  int nsub = 1;
+  if (theta > 0.5) nsub = (int)ceil(theta / 0.5);
+  const double hstep = dtf / nsub;
+  const T eps = sizeof(T) == 8 ? (T)1e-17 : (T)3e-9;
+  cplx<T> psi0[D], v[D], w[D], acc[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) psi0[c] = s[c];
+  for (int sub = 0; sub < nsub; ++sub) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      v[c] = s[c];
+      acc[c] = s[c];
+    }
+    for (int k = 1; k <= 60; ++k) {
+      tensor_matvec<T, D>(psi0, v, w, cgt, g);
+      const T f = (T)(hstep / k);
+      T nv = 0, na = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        v[c] = itp ? mk(-w[c].x * f, -w[c].y * f) : mk(w[c].y * f, -w[c].x * f);
+        acc[c] = acc[c] + v[c];
+        nv += cabs1(v[c]);
+        na += cabs1(acc[c]);
+      }
+      if (nv <= eps * na) break;
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) s[c] = acc[c];
+  }
+}
+
+// Tensor interaction energy density 1/2 sum_S g_S sum_M |A_SM|^2 with the pair amplitude
+// A_SM = sum_{m1+m2=M} CG(m1, m2 | S M) psi_m1 psi_m2 (tensor_interaction.jl:247-276).
+template <typename T, int D>
+__device__ __forceinline__ double tensor_energy_site(const cplx<T>* s, const InstParams& P,
+                                                     const volatile double* cgt) {
+  constexpr int F = (D - 1) / 2;
+  double e = 0.0;
+  static_for<-2 * F, 2 * F + 1>([&](auto Mc) __attribute__((always_inline)) {
+    constexpr int M = decltype(Mc)::value;
+    constexpr int S0 = tens::s0(M);
+    constexpr int NS = tens::ns(F, M);
+    constexpr int LO = tens::m1lo(F, M);
+    constexpr int HI = tens::m1hi(M);
+    double ar[NS], ai[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) ar[k] = ai[k] = 0.0;
+    static_for<LO, HI + 1>([&](auto m1c) __attribute__((always_inline)) {
+      constexpr int m1 = decltype(m1c)::value;
+      constexpr int m2 = M - m1;
+      const cplx<T> a = s[F - m1], b = s[F - m2];
+      double pr = (double)a.x * b.x - (double)a.y * b.y;
+      double pi = (double)a.x * b.y + (double)a.y * b.x;
+      if constexpr (m1 != m2) { pr *= 2.0; pi *= 2.0; }
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const double c = cgt[tens::off(F, M, m1, S0) + k];
+        ar[k] += c * pr;
+        ai[k] += c * pi;
+      }
+    });
+#pragma unroll
+    for (int k = 0; k < NS; ++k) e += 0.5 * P.gS[S0 / 2 + k] * (ar[k] * ar[k] + ai[k] * ai[k]);
+  });
+  return e;
 }
 
 }  // namespace bec

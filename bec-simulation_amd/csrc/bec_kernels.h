@@ -36,6 +36,7 @@ struct DevTables {
   const int* active;      // active instance list
   const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), T-typed
   const double* vfy;      // D*D complex (row-major) eigenvectors of Fy
+  const double* tens_cg;  // Clebsch-Gordan table in tens:: order (tensor step)
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
   int max_blocks;

@@ -71,12 +71,21 @@ __host__ __device__ constexpr int ilog2c() {
 // ---------------------------------------------------------------------------
 // x pass
 // ---------------------------------------------------------------------------
+// Threads per x-pass block are capped so that the per-site operators of large spins
+// (13-component Euler rotations and the tensor step) keep their spinors in VGPRs.
+template <int D>
+constexpr int xpass_max_threads() { return D >= 13 ? 256 : 512; }
+// Values per FFT thread along x: 16 for long rows of 13-component spinors, else 8.
+template <int NX, int D>
+constexpr int xpass_e() { return (D >= 13 && NX >= 256) ? 16 : 8; }
+
 template <typename T, int NX, int D>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(xpass_max_threads<D>())
 xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
              cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, int ROWS) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int TEAM = NX / 8;
+  constexpr int EX = xpass_e<NX, D>();
+  constexpr int TEAM = NX / EX;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
   const int inst = tb.active[blockIdx.y];
@@ -94,20 +103,34 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 
   const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(tb.tw[ilog2c<NX>()]);
   const cplx<double>* vfy = reinterpret_cast<const cplx<double>*>(tb.vfy);
+  // (F=6) Clebsch-Gordan table of the tensor step in LDS, read through a volatile
+  // pointer so its ~400 entries are fetched where used rather than hoisted into VGPRs.
+  constexpr int NCG = D == 13 ? tens::size(6) : 1;
+  __shared__ double s_cg[NCG];
+  if (D == 13 && tb.tens_cg)
+    for (int i = tid; i < NCG; i += nthr) s_cg[i] = tb.tens_cg[i];
+  const volatile double* cgt = s_cg;
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
 
+  // Op lists in LDS (A at [0, kMaxOps), B at [kMaxOps, 2 kMaxOps)), copied with
+  // compile-time indices so the by-value kernel argument is never indexed at run time.
+  __shared__ Op s_ops[2 * kMaxOps];
+#pragma unroll
+  for (int i = 0; i < kMaxOps; ++i) {
+    if (tid == i) s_ops[i] = prog.A[i];
+    if (tid == kMaxOps + i) s_ops[kMaxOps + i] = prog.B[i];
+  }
+  __syncthreads();
   // Zeeman factors of every diagonal op (uniform over the block).
-  {
-    const int nz = (prog.nA + prog.nB) * D;
-    for (int idx = tid; idx < nz; idx += nthr) {
-      const int i = idx / D, c = idx % D;
-      const Op op = i < prog.nA ? prog.A[i] : prog.B[i - prog.nA];
-      if (op.kind == OP_DIAG) {
-        const double dtf = op.frac * dt;
-        if (itp) zf[idx] = mk<T>((T)exp(-(P.zee[c] - P.zee_shift) * dtf), (T)0);
-        else zf[idx] = cis_neg<T>(P.zee[c] * dtf);
-      }
+  for (int idx = tid; idx < 2 * kMaxOps * D; idx += nthr) {
+    const int i = idx / D, c = idx % D;
+    const bool used = i < kMaxOps ? i < prog.nA : (i - kMaxOps) < prog.nB;
+    const Op op = s_ops[i];
+    if (used && op.kind == OP_DIAG) {
+      const double dtf = op.frac * dt;
+      if (itp) zf[idx] = mk<T>((T)exp(-(P.zee[c] - P.zee_shift) * dtf), (T)0);
+      else zf[idx] = cis_neg<T>(P.zee[c] * dtf);
     }
   }
   // Load the tile: D contiguous chunks of ROWS*NX elements.
@@ -118,7 +141,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   __syncthreads();
 
   const int line = tid / TEAM, tt = tid - (tid / TEAM) * TEAM;
-  if (st & XS_INV_FFT) fft_team<T, NX, true>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+  if (st & XS_INV_FFT) fft_team<T, NX, true, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
 
   T sc = (T)1;
   {
@@ -148,7 +171,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       aux[(2 * r + 1) * NX + x] = yz;
     }
     __syncthreads();
-    fft_team<T, NX, true>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
+    fft_team<T, NX, true, EX>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
     if (st & XS_ENERGY) {
       // nothing extra: energy stage reads aux directly
     }
@@ -191,6 +214,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
           const int r = s / NX, x = s - r * NX;
           const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
           op_euler<T, D>(sp, (double)a.x, (double)a.y, (double)b.x, dtf, itp, vfy);
+        } else if (op.kind == OP_TENSOR) {
+          if constexpr (D == 13) op_tensor<T, D>(sp, P, dtf, itp, cgt);
         } else if (op.kind == OP_SCALE) {
 #pragma unroll
           for (int c = 0; c < D; ++c) sp[c] = sp[c] * sc;
@@ -213,10 +238,10 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     if (tid == 0) *partial_ptr(tb, RS_PHIMAX, inst, blockIdx.x) = m;
   }
 
-  if (st & XS_OPS_A) run_ops(prog.A, prog.nA, 0);
+  if (st & XS_OPS_A) run_ops(s_ops, prog.nA, 0);
 
   if (st & XS_KIN_1D) {
-    fft_team<T, NX, false>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+    fft_team<T, NX, false, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
     const double kdt = prog.kin_frac * dt;
     for (int e = tid; e < D * TS; e += nthr) {
       const int x = e % NX;
@@ -227,13 +252,15 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       tile[e] = tile[e] * ph;
     }
     __syncthreads();
-    fft_team<T, NX, true>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+    fft_team<T, NX, true, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
   }
 
-  if (st & XS_OPS_B) run_ops(prog.B, prog.nB, prog.nA);
+  if (st & XS_OPS_B) run_ops(s_ops + kMaxOps, prog.nB, kMaxOps);
 
   if (st & XS_ENERGY) {
     double e_norm = 0, e_trap = 0, e_zee = 0, e_c0 = 0, e_c1 = 0, e_lhy = 0, e_ddi = 0, e_mz = 0;
+    double e_ten = 0;
+    const bool with_tensor = tb.tens_cg != nullptr && (st & XS_TENSOR_E);
     constexpr int F = (D - 1) / 2;
     for (int s = tid; s < TS; s += nthr) {
       cplx<T> sp[D];
@@ -256,6 +283,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       e_c0 += n * n;
       e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
       e_lhy += n * n * sqrt(n);
+      if constexpr (D == 13)
+        if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P, cgt);
       if (st & XS_LOAD_PHI) {
         const int r = s / NX, x = s - r * NX;
         const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
@@ -271,6 +300,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     v = block_sum(e_lhy, rsh); if (tid == 0) *partial_ptr(tb, RS_LHY, inst, blockIdx.x) = v;
     v = block_sum(e_ddi, rsh); if (tid == 0) *partial_ptr(tb, RS_DDI, inst, blockIdx.x) = v;
     v = block_sum(e_mz, rsh); if (tid == 0) *partial_ptr(tb, RS_MZ, inst, blockIdx.x) = v;
+    v = block_sum(e_ten, rsh); if (tid == 0) *partial_ptr(tb, RS_TENSOR, inst, blockIdx.x) = v;
   }
 
   if (st & XS_WRITE_FHAT) {
@@ -285,7 +315,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       aux[(2 * r + 1) * NX + x] = mk(fz, (T)0);
     }
     __syncthreads();
-    fft_team<T, NX, false>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
+    fft_team<T, NX, false, EX>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
     const size_t fb = (size_t)inst * g.fhat_inst_stride;
     constexpr int NH = NX / 2 + 1;
     for (int e = tid; e < ROWS * NH; e += nthr) {
@@ -311,7 +341,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     __syncthreads();
   }
 
-  if (st & XS_FWD_FFT) fft_team<T, NX, false>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+  if (st & XS_FWD_FFT) fft_team<T, NX, false, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
 
   if (st & XS_KE_1D) {
     double acc = 0.0;
@@ -574,6 +604,9 @@ __global__ void diff_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ a,
 // ---------------------------------------------------------------------------
 int xpass_rows(int D, int NX, int nrows) {
   int target = 3072 / (D * NX);
+  const int e = (D >= 13 && NX >= 256) ? 16 : 8;
+  const int maxthr = D >= 13 ? 256 : 512;
+  while (target > 1 && D * target * (NX / e) > maxthr) --target;
   if (target < 1) target = 1;
   int r = 1;
   while (r * 2 <= target) r *= 2;
@@ -590,9 +623,10 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
   const size_t lds = sizeof(cplx<T>) * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0) +
                                         2 * kMaxOps * D) +
                      16 * sizeof(double);
-  const int threads = D * rows * (NX / 8);
+  const int threads = D * rows * (NX / xpass_e<NX, D>());
+  if (threads > xpass_max_threads<D>()) return hipErrorInvalidConfiguration;
   dim3 grid(g.nrows / rows, n_active);
-  if (lds > 65536)
+  if (lds > 48 * 1024)  // + static LDS (op lists, Fy eigenvectors, CG table)
     (void)hipFuncSetAttribute((const void*)xpass_kernel<T, NX, D>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((xpass_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,

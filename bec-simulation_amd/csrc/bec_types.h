@@ -15,6 +15,7 @@ struct InstParams {
   double cdd;        // dipolar strength C_dd
   double zee[kMaxD]; // Zeeman energies E_c = -p m + q m^2, m = F - c
   double zee_shift;  // min_c E_c (subtracted in imaginary time)
+  double gS[7];      // channel couplings g_S, S = 0, 2, ..., 2F (general-F tensor step)
   int active;
   int pad_;
 };
@@ -24,6 +25,7 @@ enum OpKind : int {
   OP_SPINMIX = 2,   // apply_spin_mixing_step! (dt*frac)
   OP_DDI_ROT = 3,   // Euler rotation by the DDI potential (dt*frac)
   OP_SCALE = 4,     // multiply by 1/sqrt(norm2) of the instance (ITP renormalisation)
+  OP_TENSOR = 5,    // apply_tensor_interaction_step! (dt*frac)
 };
 
 struct Op {
@@ -46,6 +48,7 @@ enum XStage : int {
   XS_NO_STORE = 1 << 8,    // do not write psi back
   XS_ENERGY = 1 << 9,      // pointwise energy partials (trap, Zeeman, c0, c1, LHY, DDI)
   XS_KE_1D = 1 << 10,      // 1-D kinetic energy partial sum k^2 |psi_hat|^2 after forward FFT
+  XS_TENSOR_E = 1 << 11,   // include the tensor interaction energy in XS_ENERGY
 };
 
 struct XProgram {
@@ -70,7 +73,29 @@ enum RedSlot : int {
   RS_DDI = 7,
   RS_MZ = 8,
   RS_PHIMAX = 9,
-  RS_COUNT = 10,
+  RS_TENSOR = 10,
+  RS_COUNT = 11,
 };
 
 }  // namespace bec
+
+// Enumeration of the Clebsch-Gordan table used by the tensor step: for every
+// M = -2F..2F, every unordered pair m1 <= m2 with m1 + m2 = M, every even
+// S >= |M| up to 2F: <F m1; F m2 | S M>.  Shared by host (table build) and
+// device (compile-time offsets).
+namespace tens {
+__host__ __device__ constexpr int m1lo(int F, int M) { return M - F > -F ? M - F : -F; }
+__host__ __device__ constexpr int m1hi(int M) { return M >= 0 ? M / 2 : -((-M + 1) / 2); }
+__host__ __device__ constexpr int npairs(int F, int M) { return m1hi(M) - m1lo(F, M) + 1; }
+__host__ __device__ constexpr int s0(int M) { return ((M < 0 ? -M : M) + 1) / 2 * 2; }
+__host__ __device__ constexpr int ns(int F, int M) { return (2 * F - s0(M)) / 2 + 1; }
+__host__ __device__ constexpr int base(int F, int M) {
+  int b = 0;
+  for (int q = -2 * F; q < M; ++q) b += npairs(F, q) * ns(F, q);
+  return b;
+}
+__host__ __device__ constexpr int size(int F) { return base(F, 2 * F + 1); }
+__host__ __device__ constexpr int off(int F, int M, int m1, int S) {
+  return base(F, M) + (m1 - m1lo(F, M)) * ns(F, M) + (S - s0(M)) / 2;
+}
+}  // namespace tens

@@ -58,6 +58,8 @@ typedef struct bec_config {
   int enable_ddi;      /* make_workspace(enable_ddi=...) */
   int ddi_secular;     /* make_ddi_params(secular=...) */
   int ddi_padding;     /* make_workspace(ddi_padding=...) (not built: BEC_EUNSUP) */
+  int tensor;          /* general-F tensor step replaces spin mixing (TensorInteractionCache
+                        * active, initialization.jl:164-185); couplings via bec_set_tensor */
   /* Fy eigenvectors, D*D complex interleaved, row-major, columns ordered by the
    * ascending eigenvalues -F..F (spin_matrices.jl:38-43). */
   const double* fy_vecs;
@@ -83,6 +85,10 @@ int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p);
 /* Explicit potential array (evaluate_potential, potentials.jl:1-32), n sites, or
  * NULL to return to the harmonic trap of the instance parameters. */
 int bec_set_potential(bec_ws* ws, int inst, const double* V);
+/* Channel couplings g_S for S = 0, 2, ..., 2F (n = F+1) of the general-F tensor step
+ * (TensorInteractionCache.g_values, tensor_interaction.jl:106-119; the caller forms
+ * g_S = _c0c1_to_gS + _c_extra_to_delta_gS and zeroes c0, c1 as make_workspace does). */
+int bec_set_tensor(bec_ws* ws, int inst, const double* gS, int n);
 /* psi upload/download (SimState.psi). */
 int bec_set_psi(bec_ws* ws, int inst, const double* psi);
 int bec_get_psi(bec_ws* ws, int inst, double* psi);

@@ -176,3 +176,48 @@ def test_batched_instances_match_single():
         for _ in range(30):
             O.split_step(ows)
         assert rel_l2(ws.get_psi(i), ows.psi) <= PSI_TOL
+
+
+def _cfg3_extra(seed=42):
+    """c_extra with even-rank couplings c2..c12 ~ U[-1, 1] (odd entries 0)."""
+    r = np.random.default_rng(seed)
+    ce = [0.0] * 11
+    for k in range(2, 13, 2):
+        ce[k - 2] = float(r.uniform(-1, 1))
+    return ce
+
+
+@pytest.mark.parametrize("itp", [False, True])
+def test_tensor_f6_3d_split_step(itp):
+    n, L = (16, 16, 16), (12.0, 12.0, 12.0)
+    og, _ = grids(n, L)
+    psi = random_psi(og.shape, 13, 31) * O.gaussian(og, (2.5, 2.5, 2.5))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 6, psi, 0.002, itp, omega=(1, 1, 1), p=0.1, c0=10.0,
+                    c_extra=_cfg3_extra())
+    assert gws.tensor and ows.T is not None
+    eo = O.energy_terms(ows)
+    eg = A.energy_terms(gws)
+    for k in eo:
+        assert abs(eg[k] - eo[k]) <= 1e-12 * max(1.0, abs(eo[k])), (k, eg[k], eo[k])
+    for _ in range(2):
+        O.split_step(ows)
+    A.split_step(gws, 2)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
+def test_tensor_f6_ddi_yoshida_core():
+    """One Yoshida attempt of the F=6 dipolar configuration (tensor + DDI) on 16^3."""
+    n, L = (16, 16, 16), (24.0, 24.0, 24.0)
+    og, _ = grids(n, L)
+    psi = np.zeros((13,) + og.shape, complex)
+    psi[0] = O.gaussian(og, (3.0, 3.0, 3.0))
+    psi += 1e-3 * random_psi(og.shape, 13, 42)
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 6, psi, 1e-3, False, omega=(1, 1, 1), c0=10.0, c_extra=_cfg3_extra(),
+                    ddi=True, cdd=1.0)
+    O.yoshida_core(ows, 1e-3)
+    A.yoshida_core(gws, 1e-3)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+    eo, eg = O.total_energy(ows), A.total_energy(gws)
+    assert abs(eg - eo) <= E_TOL * abs(eo)
