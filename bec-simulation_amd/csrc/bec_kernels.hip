@@ -418,7 +418,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   const int xi = tid % TB, rest = tid / TB, tt = rest % TEAM, f = rest / TEAM;
   const bool act = f < FIELDS;
   cplx<T>* lbase = tile + (act ? f : 0) * N * TB + xi;
-  fft_team<T, N, false>(lbase, TB, tt, act, tw);
+  if (MODE != SM_INV) fft_team<T, N, false>(lbase, TB, tt, act, tw);
 
   if (MODE == SM_KIN) {
     for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
