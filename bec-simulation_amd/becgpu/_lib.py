@@ -16,7 +16,7 @@ BEC_FP64, BEC_FP32 = 0, 1
 (E_KIN, E_TRAP, E_ZEEMAN, E_C0, E_C1, E_DDI, E_LHY, E_TENSOR, E_NORM, E_MZ,
  E_TOTAL, E_COUNT) = range(12)
 E_NAMES = ["kin", "trap", "zeeman", "c0", "c1", "ddi", "lhy", "tensor", "norm", "mz", "total"]
-K_NAMES = ["xpass", "fwd", "kin", "inv", "ke", "ddi_fft", "ddi_conv", "reduce"]
+K_NAMES = ["xpass", "fwd", "kin", "inv", "ke", "ddi_fft", "ddi_conv", "reduce", "sites", "energy"]
 
 
 class BecConfig(C.Structure):

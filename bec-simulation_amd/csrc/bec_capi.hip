@@ -33,6 +33,7 @@ struct bec_ws {
   Geom g{};
   int F = 0, D = 0, B = 0, prec = 0, itp = 0, normalize_every = 1, ddi = 0, tensor = 0;
   double* d_tens_cg = nullptr;
+  float* d_tens_cg_f = nullptr;
   size_t csz = 16;
   std::vector<InstParams> hp;
   std::vector<bec_inst_params> user;
@@ -42,12 +43,19 @@ struct bec_ws {
   void* slot[4] = {nullptr, nullptr, nullptr, nullptr};
   void* scratch = nullptr;
   void* fhat = nullptr;
+  void* phi = nullptr;  // Phi(x) fields for the lane-per-site DDI rotation (F=6)
+  // FP64 evaluation copies for FP32-bulk workspaces (energies are always FP64)
+  void* psi64 = nullptr;
+  void* scratch64 = nullptr;
+  void* fhat64 = nullptr;
+  void* tw64[10] = {};
   double* d_ktab = nullptr;
   double* d_axtab = nullptr;
   double* d_varr = nullptr;
   double* d_red = nullptr;
   double* d_partials = nullptr;
   double* d_vfy = nullptr;
+  float* d_vfy_f = nullptr;
   InstParams* d_params = nullptr;
   int* d_active = nullptr;
   void* d_tw[10] = {};
@@ -102,7 +110,9 @@ static void refresh_tables(bec_ws* ws) {
   t.active = ws->d_active;
   for (int i = 0; i < 10; ++i) t.tw[i] = ws->d_tw[i];
   t.vfy = ws->d_vfy;
+  t.vfy_f = ws->d_vfy_f;
   t.tens_cg = ws->tensor ? ws->d_tens_cg : nullptr;
+  t.tens_cg_f = ws->tensor ? ws->d_tens_cg_f : nullptr;
   t.red = ws->d_red;
   t.partials = ws->d_partials;
   t.max_blocks = ws->max_blocks;
@@ -120,15 +130,27 @@ static int sync_params(bec_ws* ws) {
 
 static int n_active(bec_ws* ws) { return (int)ws->active.size(); }
 
-template <typename T>
-static int xlaunch(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
+static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
   const int pe = prof_begin(ws, 0);
-  HIPCHK(launch_xpass<T>(ws->stream, ws->g, ws->tb, p, src, dst, ws->fhat, n_active(ws)));
+  hipError_t e = ws->prec == BEC_FP32
+                     ? launch_xpass<float>(ws->stream, ws->g, ws->tb, p, src, dst, ws->fhat,
+                                           n_active(ws), ws->phi)
+                     : launch_xpass<double>(ws->stream, ws->g, ws->tb, p, src, dst, ws->fhat,
+                                            n_active(ws), ws->phi);
   prof_end(ws, pe);
+  HIPCHK(e);
   return BEC_OK;
 }
-static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
-  return ws->prec == BEC_FP32 ? xlaunch<float>(ws, p, src, dst) : xlaunch<double>(ws, p, src, dst);
+static int do_sites(bec_ws* ws, const SProgram& p) {
+  const int pe = prof_begin(ws, 8);
+  hipError_t e = ws->prec == BEC_FP32
+                     ? launch_sites<float>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->phi,
+                                           n_active(ws))
+                     : launch_sites<double>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->phi,
+                                            n_active(ws));
+  prof_end(ws, pe);
+  HIPCHK(e);
+  return BEC_OK;
 }
 static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_frac, double dt_ovr,
                       int itp, int ddi_data, int* nb = nullptr) {
@@ -171,6 +193,7 @@ struct Sched {
   int itp;
   double dt_ovr;
   XProgram p{};
+  SProgram sp{};  // open lane-per-site program (13-component operators)
   bool scale_pending = false;
 
   void reset() {
@@ -178,21 +201,53 @@ struct Sched {
     p.itp = itp;
     p.dt_override = dt_ovr;
   }
+  void reset_s() {
+    std::memset(&sp, 0, sizeof(sp));
+    sp.itp = itp;
+    sp.dt_override = dt_ovr;
+  }
   bool empty() const { return p.stages == 0 && p.nA == 0 && p.nB == 0; }
+  // 13-component spin rotations and the tensor step run in the lane-per-site kernel
+  bool site_op(int kind) const {
+    return ws->D >= 13 && (kind == OP_TENSOR || kind == OP_DDI_ROT || kind == OP_SPINMIX);
+  }
+  int flush_s() {
+    if (sp.n == 0) return BEC_OK;
+    int r = do_sites(ws, sp);
+    reset_s();
+    return r;
+  }
   int flush() {
+    CHK(flush_s());
     if (empty()) return BEC_OK;
     if (p.nA) p.stages |= XS_OPS_A;
     if (p.nB) p.stages |= XS_OPS_B;
+    // passes that only read psi (norm, spin density) do not write it back
+    const int ro = XS_NORM | XS_WRITE_FHAT;
+    if (p.nA == 0 && p.nB == 0 && (p.stages & ~ro) == 0) p.stages |= XS_NO_STORE;
     int r = do_x(ws, p, ws->slot[0], ws->slot[0]);
     reset();
     return r;
   }
   int add_op(int kind, double frac) {
-    if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
     if (scale_pending) {
       scale_pending = false;
       CHK(add_op(OP_SCALE, 0.0));
     }
+    if (sp.n > 0 || site_op(kind)) {
+      if (!empty()) {  // the x pass before the site kernel must store psi first
+        if (p.nA) p.stages |= XS_OPS_A;
+        if (p.nB) p.stages |= XS_OPS_B;
+        CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+        reset();
+      }
+      if (sp.n >= kMaxOps) CHK(flush_s());
+      sp.ops[sp.n].kind = kind;
+      sp.ops[sp.n].frac = frac;
+      ++sp.n;
+      return BEC_OK;
+    }
+    if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
     if (((p.stages & XS_KIN_1D) ? p.nB : p.nA) >= kMaxOps) CHK(flush());
     const bool toB = (p.stages & XS_KIN_1D) != 0;
     Op* ops = toB ? p.B : p.A;
@@ -220,11 +275,17 @@ struct Sched {
     CHK(add_op(OP_DIAG, f / 2));
     CHK(add_op(mix, f / 2));
     if (ws->ddi) {
+      CHK(flush_s());
       if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
       p.stages |= XS_WRITE_FHAT;
       CHK(flush());
       CHK(ddi_conv());
-      p.stages |= XS_LOAD_PHI;
+      if (ws->D >= 13) {  // C2R of Phi into its real-space buffer, rotation per site
+        p.stages |= XS_LOAD_PHI | XS_STORE_PHI | XS_NO_LOAD | XS_NO_STORE;
+        CHK(flush());
+      } else {
+        p.stages |= XS_LOAD_PHI;
+      }
       CHK(add_op(OP_DDI_ROT, f));
     }
     CHK(add_op(mix, f / 2));
@@ -236,6 +297,7 @@ struct Sched {
       scale_pending = false;
       CHK(add_op(OP_SCALE, 0.0));
     }
+    CHK(flush_s());
     if (ws->g.ndim == 1) {
       if (p.stages & (XS_KIN_1D | XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
       p.stages |= XS_KIN_1D;
@@ -259,6 +321,7 @@ struct Sched {
       scale_pending = false;
       CHK(add_op(OP_SCALE, 0.0));
     }
+    CHK(flush_s());
     p.stages |= XS_NORM;
     CHK(flush());
     CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_NORM));
@@ -286,6 +349,7 @@ static int run_items(bec_ws* ws, const std::vector<Item>& items, int itp, double
   s.itp = itp;
   s.dt_ovr = dt_ovr;
   s.reset();
+  s.reset_s();
   for (const Item& it : items) {
     if (it.kind == IT_V) CHK(s.V(it.frac));
     else if (it.kind == IT_K) CHK(s.K(it.frac));
@@ -372,7 +436,10 @@ static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out
     const int nxh = nx / 2 + 1;
     const int pitch = (nxh + 15) / 16 * 16;
     bytes += 3LL * B * pitch * ny * nz * csz;
+    if (D >= 13) bytes += 3LL * B * ns * (csz / 2);
+    if (c->precision == BEC_FP32) bytes += 3LL * B * pitch * ny * nz * 16;
   }
+  if (c->precision == BEC_FP32) bytes += 2LL * B * D * ns * 16;  // FP64 energy copies
   bytes += (long long)RS_COUNT * B * maxb * 8 + (long long)B * RS_COUNT * 8;
   bytes += (long long)B * 3 * std::max(nx, std::max(ny, nz)) * 8;
   return bytes;
@@ -462,6 +529,10 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     const size_t fb = ws->csz * (size_t)g.fhat_inst_stride * ws->B;
     if (hipMalloc(&ws->fhat, fb) != hipSuccess) return cleanup(BEC_ENOMEM, "fhat alloc");
     if (hipMemset(ws->fhat, 0, fb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
+    if (ws->D >= 13) {
+      const size_t pb = (ws->csz / 2) * 3 * (size_t)g.nsites * ws->B;
+      if (hipMalloc(&ws->phi, pb) != hipSuccess) return cleanup(BEC_ENOMEM, "phi alloc");
+    }
   }
   // k table [3][nmax]
   std::vector<double> kt(3 * g.nmax, 0.0);
@@ -495,9 +566,19 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMemcpy(ws->d_vfy, c->fy_vecs, vb, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(BEC_EDEVICE, "vfy copy");
   {
+    std::vector<float> vf(c->fy_vecs, c->fy_vecs + 2 * ws->D * ws->D);
+    if (hipMalloc(&ws->d_vfy_f, vf.size() * 4) != hipSuccess) return cleanup(BEC_ENOMEM, "vfy");
+    if (hipMemcpy(ws->d_vfy_f, vf.data(), vf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "vfy copy");
+  }
+  {
     const std::vector<double> cgt = tensor_cg_table(ws->F);
     if (hipMalloc(&ws->d_tens_cg, cgt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "cg");
     if (hipMemcpy(ws->d_tens_cg, cgt.data(), cgt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "cg copy");
+    std::vector<float> cgf(cgt.begin(), cgt.end());
+    if (hipMalloc(&ws->d_tens_cg_f, cgf.size() * 4) != hipSuccess) return cleanup(BEC_ENOMEM, "cg");
+    if (hipMemcpy(ws->d_tens_cg_f, cgf.data(), cgf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(BEC_EDEVICE, "cg copy");
   }
   // twiddles exp(-2 pi i m / N), computed in long double
@@ -532,9 +613,16 @@ void bec_ws_destroy(bec_ws* ws) {
     if (ws->slot[i]) (void)hipFree(ws->slot[i]);
   if (ws->scratch) (void)hipFree(ws->scratch);
   if (ws->fhat) (void)hipFree(ws->fhat);
+  if (ws->phi) (void)hipFree(ws->phi);
+  for (void* q : {ws->psi64, ws->scratch64, ws->fhat64})
+    if (q) (void)hipFree(q);
+  for (int i = 0; i < 10; ++i)
+    if (ws->tw64[i]) (void)hipFree(ws->tw64[i]);
   for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials, ws->d_vfy,
                     ws->d_tens_cg})
     if (p) (void)hipFree(p);
+  if (ws->d_tens_cg_f) (void)hipFree(ws->d_tens_cg_f);
+  if (ws->d_vfy_f) (void)hipFree(ws->d_vfy_f);
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
   for (int i = 0; i < 10; ++i)
@@ -733,44 +821,107 @@ static int fetch_red(bec_ws* ws) {
   return BEC_OK;
 }
 
-int bec_energy(bec_ws* ws, double* out) {
-  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
-  if (n_active(ws) == 0) return BEC_OK;
-  CHK(sync_params(ws));
+static int make_twiddles(int prec, void** tw) {
+  for (int l = 3; l <= 9; ++l) {
+    const int N = 1 << l;
+    const size_t esz = prec == BEC_FP32 ? 8 : 16;
+    std::vector<unsigned char> buf(esz * N);
+    for (int m = 0; m < N; ++m) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+      const long double cr = cosl(ang), ci = sinl(ang);
+      if (prec == BEC_FP32) {
+        float v[2] = {(float)cr, (float)ci};
+        std::memcpy(&buf[esz * m], v, 8);
+      } else {
+        double v[2] = {(double)cr, (double)ci};
+        std::memcpy(&buf[esz * m], v, 16);
+      }
+    }
+    if (hipMalloc(&tw[l], buf.size()) != hipSuccess) return fail(BEC_ENOMEM, "twiddles");
+    HIPCHK(hipMemcpy(tw[l], buf.data(), buf.size(), hipMemcpyHostToDevice));
+  }
+  return BEC_OK;
+}
+
+// Energy partial sums of psi (FP64 arithmetic on FP64 data), reduced into d_red.
+static int energy_partials_fp64(bec_ws* ws, const DevTables& tb, void* psi, void* scratch,
+                                void* fhat) {
   const Geom& g = ws->g;
+  const int na = n_active(ws);
   XProgram p{};
   p.itp = ws->itp;
+  auto X = [&](const XProgram& q, const void* src, void* dst) -> int {
+    HIPCHK(launch_xpass<double>(ws->stream, g, tb, q, src, dst, fhat, na, nullptr));
+    return BEC_OK;
+  };
+  auto S = [&](int axis, int mode, void* data, int ddi, int* nb) -> int {
+    HIPCHK(launch_strided<double>(ws->stream, g, tb, axis, mode, data, na, 0, 0, 0, ddi, nb));
+    return BEC_OK;
+  };
   if (ws->ddi) {
     p.stages = XS_WRITE_FHAT | XS_NO_STORE;
-    CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
-    Sched s;
-    s.ws = ws;
-    s.itp = ws->itp;
-    s.dt_ovr = 0;
-    CHK(s.ddi_conv());
+    CHK(X(p, psi, psi));
+    if (g.ndim == 3) {
+      CHK(S(1, SM_FWD, fhat, 1, nullptr));
+      CHK(S(2, SM_DDI, fhat, 1, nullptr));
+      CHK(S(1, SM_INV, fhat, 1, nullptr));
+    } else {
+      CHK(S(1, SM_DDI, fhat, 1, nullptr));
+    }
     p.stages = XS_LOAD_PHI | XS_ENERGY | XS_NO_STORE;
   } else {
     p.stages = XS_ENERGY | XS_NO_STORE;
   }
   if (ws->tensor) p.stages |= XS_TENSOR_E;
-  CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+  CHK(X(p, psi, psi));
   const uint32_t emask = (1u << RS_NORM) | (1u << RS_TRAP) | (1u << RS_ZEE) | (1u << RS_C0) |
                          (1u << RS_C1) | (1u << RS_LHY) | (1u << RS_DDI) | (1u << RS_MZ) |
                          (1u << RS_TENSOR);
-  CHK(do_reduce(ws, ws->n_xblocks, emask));
+  HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, ws->n_xblocks, emask, 0));
   // kinetic energy: forward FFT of every component, sum k^2 |psi_hat|^2 (energy.jl:56-65)
   if (g.ndim == 1) {
     p.stages = XS_FWD_FFT | XS_KE_1D | XS_NO_STORE;
-    CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
-    CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_KIN));
+    CHK(X(p, psi, psi));
+    HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, ws->n_xblocks, 1u << RS_KIN, 0));
   } else {
     p.stages = XS_FWD_FFT;
-    CHK(do_x(ws, p, ws->slot[0], ws->scratch));
+    CHK(X(p, psi, scratch));
     int nb = 0;
-    if (g.ndim == 3) CHK(do_strided(ws, 1, SM_FWD, ws->scratch, 0, 0, 0, 0));
-    CHK(do_strided(ws, g.ndim - 1, SM_KE, ws->scratch, 0, 0, 0, 0, &nb));
-    CHK(do_reduce(ws, nb, 1u << RS_KIN));
+    if (g.ndim == 3) CHK(S(1, SM_FWD, scratch, 0, nullptr));
+    CHK(S(g.ndim - 1, SM_KE, scratch, 0, &nb));
+    HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, nb, 1u << RS_KIN, 0));
   }
+  return BEC_OK;
+}
+
+int bec_energy(bec_ws* ws, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  if (n_active(ws) == 0) return BEC_OK;
+  CHK(sync_params(ws));
+  const Geom& g = ws->g;
+  const int pe = prof_begin(ws, 9);
+  if (ws->prec == BEC_FP64) {
+    CHK(energy_partials_fp64(ws, ws->tb, ws->slot[0], ws->scratch, ws->fhat));
+  } else {
+    // FP32-bulk: the state is widened to FP64 and every energy term (FFT, DDI
+    // convolution, reductions) is evaluated in FP64
+    const size_t n = (size_t)g.inst_stride * ws->B;
+    if (!ws->psi64) {
+      if (hipMalloc(&ws->psi64, n * 16) != hipSuccess) return fail(BEC_ENOMEM, "psi64");
+      if (hipMalloc(&ws->scratch64, n * 16) != hipSuccess) return fail(BEC_ENOMEM, "scratch64");
+      if (ws->ddi) {
+        const size_t fb = 16 * (size_t)g.fhat_inst_stride * ws->B;
+        if (hipMalloc(&ws->fhat64, fb) != hipSuccess) return fail(BEC_ENOMEM, "fhat64");
+        HIPCHK(hipMemset(ws->fhat64, 0, fb));
+      }
+      CHK(make_twiddles(BEC_FP64, ws->tw64));
+    }
+    HIPCHK(launch_widen(ws->stream, ws->slot[0], ws->psi64, (long long)(2 * n)));
+    DevTables tb = ws->tb;
+    for (int i = 0; i < 10; ++i) tb.tw[i] = ws->tw64[i];
+    CHK(energy_partials_fp64(ws, tb, ws->psi64, ws->scratch64, ws->fhat64));
+  }
+  prof_end(ws, pe);
   CHK(fetch_red(ws));
   const double dV = g.dV;
   for (int inst : ws->active) {

@@ -279,9 +279,9 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
 // reference's cis recurrences.  ITP: Dz = exp(-m th).  vfy: D*D complex row-major V_Fy.
 template <typename T, int D>
 __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, double phz, double dt,
-                                         bool itp, const cplx<double>* __restrict__ vfy_in) {
+                                         bool itp, const cplx<T>* __restrict__ vfy_in) {
   constexpr int F = (D - 1) / 2;
-  const cplx<double>* vfy = vfy_in;
+  const cplx<T>* vfy = vfy_in;
   double pm = sqrt(phx * phx + phy * phy + phz * phz);
   if (pm * fabs(dt) < 1e-14) return;
   double cz = phz / pm;
@@ -308,24 +308,24 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   ph = ry_phase;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const cplx<double>* m = vfy;
+    const cplx<T>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      const cplx<double> q = mk((double)m[j * D + i].x, (double)m[j * D + i].y);  // Vt[i][j] = conj(V[j][i])
-      acc = acc + mk((T)q.x, (T)(-q.y)) * v[j];
+      const cplx<T> q = m[j * D + i];  // Vt[i][j] = conj(V[j][i])
+      acc = acc + mk(q.x, -q.y) * v[j];
     }
     w[i] = ph * acc;
     ph = ph * z_beta;
   }
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const cplx<double>* m = vfy;
+    const cplx<T>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      const cplx<double> q = mk((double)m[i * D + j].x, (double)m[i * D + j].y);
-      acc = acc + mk((T)q.x, (T)q.y) * w[j];
+      const cplx<T> q = m[i * D + j];
+      acc = acc + q * w[j];
     }
     v[i] = acc;
   }
@@ -353,12 +353,12 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   cplx<T> z_neg_beta = conjc(z_beta);
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const cplx<double>* m = vfy;
+    const cplx<T>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      const cplx<double> q = mk((double)m[j * D + i].x, (double)m[j * D + i].y);
-      acc = acc + mk((T)q.x, (T)(-q.y)) * v[j];
+      const cplx<T> q = m[j * D + i];
+      acc = acc + mk(q.x, -q.y) * v[j];
     }
     w[i] = ph * acc;
     ph = ph * z_neg_beta;
@@ -367,12 +367,12 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   cplx<T> z_alpha = conjc(z_neg_alpha);
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const cplx<double>* m = vfy;
+    const cplx<T>* m = vfy;
     cplx<T> acc = mk((T)0, (T)0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      const cplx<double> q = mk((double)m[i * D + j].x, (double)m[i * D + j].y);
-      acc = acc + mk((T)q.x, (T)q.y) * w[j];
+      const cplx<T> q = m[i * D + j];
+      acc = acc + q * w[j];
     }
     s[i] = ph * acc;
     ph = ph * z_alpha;
@@ -381,7 +381,7 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
 
 template <typename T, int D>
 __device__ __forceinline__ void op_spin_mixing(cplx<T>* s, const InstParams& P, double dtf, bool itp,
-                                               const cplx<double>* __restrict__ vfy) {
+                                               const cplx<T>* __restrict__ vfy) {
   if (fabs(P.c1) < 1e-30) return;  // spin_mixing.jl:9
   if constexpr (D == 3) {
     op_rodrigues<T>(s, P.c1, dtf, itp);
@@ -412,7 +412,7 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
 // cgt: table in tens:: order; every index below is a compile-time constant.
 template <typename T, int D>
 __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>* v, cplx<T>* w,
-                                              const volatile double* cgt, const T* g) {
+                                              const T* __restrict__ cgt, const T* g) {
   constexpr int F = (D - 1) / 2;
 #pragma unroll
   for (int c = 0; c < D; ++c) w[c] = mk((T)0, (T)0);
@@ -431,7 +431,7 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       cplx<T> p = psi[F - m2] * v[F - m1];
       if constexpr (m1 != m2) p = p + psi[F - m1] * v[F - m2];
 #pragma unroll
-      for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] + p * (T)cgt[tens::off(F, M, m1, S0) + k];
+      for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] + p * cgt[tens::off(F, M, m1, S0) + k];
     });
 #pragma unroll
     for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] * g[S0 / 2 + k];
@@ -440,7 +440,7 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       constexpr int m2 = M - m1;
       cplx<T> C = mk((T)0, (T)0);
 #pragma unroll
-      for (int k = 0; k < NS; ++k) C = C + Bs[k] * (T)cgt[tens::off(F, M, m1, S0) + k];
+      for (int k = 0; k < NS; ++k) C = C + Bs[k] * cgt[tens::off(F, M, m1, S0) + k];
       w[F - m1] = w[F - m1] + conjc(psi[F - m2]) * C;
       if constexpr (m1 != m2) w[F - m2] = w[F - m2] + conjc(psi[F - m1]) * C;
     });
@@ -457,7 +457,7 @@ __device__ __forceinline__ T cabs1(cplx<T> a) { return fabs(a.x) + fabs(a.y); }
 // n < 1e-20 are left unchanged (tensor_interaction.jl:212).
 template <typename T, int D>
 __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp,
-                                          const volatile double* cgt) {
+                                          const T* __restrict__ cgt) {
   constexpr int F = (D - 1) / 2;
   double n = 0.0;
 #pragma unroll
@@ -506,7 +506,7 @@ __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, doubl
 // A_SM = sum_{m1+m2=M} CG(m1, m2 | S M) psi_m1 psi_m2 (tensor_interaction.jl:247-276).
 template <typename T, int D>
 __device__ __forceinline__ double tensor_energy_site(const cplx<T>* s, const InstParams& P,
-                                                     const volatile double* cgt) {
+                                                     const double* __restrict__ cgt) {
   constexpr int F = (D - 1) / 2;
   double e = 0.0;
   static_for<-2 * F, 2 * F + 1>([&](auto Mc) __attribute__((always_inline)) {

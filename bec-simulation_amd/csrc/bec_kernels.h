@@ -36,7 +36,9 @@ struct DevTables {
   const int* active;      // active instance list
   const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), T-typed
   const double* vfy;      // D*D complex (row-major) eigenvectors of Fy
-  const double* tens_cg;  // Clebsch-Gordan table in tens:: order (tensor step)
+  const float* vfy_f;     // the same matrix rounded to FP32
+  const double* tens_cg;  // Clebsch-Gordan table in tens:: order (tensor step), FP64
+  const float* tens_cg_f; // the same table rounded to FP32 (FP32-bulk tensor step)
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
   int max_blocks;
@@ -54,7 +56,13 @@ enum StridedMode : int {
 // x-axis pass: tiles of whole x rows, all components; executes `prog`.
 template <typename T>
 hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                        const void* src, void* dst, void* fhat, int n_active);
+                        const void* src, void* dst, void* fhat, int n_active, void* phi = nullptr);
+
+// Lane-per-site pass: one thread per grid site applies `prog` to its spinor in VGPRs
+// (phi: Phi(x) fields [inst][3][nsites] written by an XS_STORE_PHI x pass, or null).
+template <typename T>
+hipError_t launch_sites(hipStream_t st, const Geom& g, const DevTables& tb, const SProgram& prog,
+                        void* psi, const void* phi, int n_active);
 
 // y or z axis pass (axis = 1 or 2) over psi (fields = D) or over the DDI
 // spectra (fields = 3, DDI mode).  SM_KIN uses dt = (dt_override > 0 ? dt_override :
@@ -70,6 +78,9 @@ constexpr int strided_tb_c(int N, int fields, int csize) {
 }
 int strided_tb(int N, int fields, int csize);
 int xpass_rows(int D, int NX, int nrows);
+
+// dst[i] = (double)src[i], i < n (FP32 psi widened for FP64 energy evaluation)
+hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n);
 
 // Sum (or max) the per-block partials of `slots` into red[inst][slot].
 hipError_t launch_reduce(hipStream_t st, const DevTables& tb, double* red, int n_active,

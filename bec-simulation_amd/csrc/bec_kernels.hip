@@ -74,15 +74,15 @@ __host__ __device__ constexpr int ilog2c() {
 // Threads per x-pass block are capped so that the per-site operators of large spins
 // (13-component Euler rotations and the tensor step) keep their spinors in VGPRs.
 template <int D>
-constexpr int xpass_max_threads() { return D >= 13 ? 256 : 512; }
-// Values per FFT thread along x: 16 for long rows of 13-component spinors, else 8.
+constexpr int xpass_max_threads() { return 512; }
+// Values per FFT thread along x.  With E <= D every tile has at most one site per thread.
 template <int NX, int D>
-constexpr int xpass_e() { return (D >= 13 && NX >= 256) ? 16 : 8; }
+constexpr int xpass_e() { return 8; }
 
 template <typename T, int NX, int D>
 __global__ void __launch_bounds__(xpass_max_threads<D>())
 xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
-             cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, int ROWS) {
+             cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, T* __restrict__ phi, int ROWS) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int EX = xpass_e<NX, D>();
   constexpr int TEAM = NX / EX;
@@ -102,14 +102,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
 
   const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(tb.tw[ilog2c<NX>()]);
-  const cplx<double>* vfy = reinterpret_cast<const cplx<double>*>(tb.vfy);
-  // (F=6) Clebsch-Gordan table of the tensor step in LDS, read through a volatile
-  // pointer so its ~400 entries are fetched where used rather than hoisted into VGPRs.
-  constexpr int NCG = D == 13 ? tens::size(6) : 1;
-  __shared__ double s_cg[NCG];
-  if (D == 13 && tb.tens_cg)
-    for (int i = tid; i < NCG; i += nthr) s_cg[i] = tb.tens_cg[i];
-  const volatile double* cgt = s_cg;
+  const cplx<T>* vfy = sizeof(T) == 8 ? reinterpret_cast<const cplx<T>*>(tb.vfy)
+                                       : reinterpret_cast<const cplx<T>*>(tb.vfy_f);
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
 
@@ -134,9 +128,11 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     }
   }
   // Load the tile: D contiguous chunks of ROWS*NX elements.
-  for (int e = tid; e < D * TS; e += nthr) {
-    const int c = e / TS, r = e - c * TS;
-    tile[e] = src[ibase + (size_t)c * g.nsites + r];
+  if (!(st & XS_NO_LOAD)) {
+    for (int e = tid; e < D * TS; e += nthr) {
+      const int c = e / TS, r = e - c * TS;
+      tile[e] = src[ibase + (size_t)c * g.nsites + r];
+    }
   }
   __syncthreads();
 
@@ -195,34 +191,56 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     return 0.5 * v;
   };
 
+  // Per-site operators, one site loop per operator: each loop body holds a single
+  // operator, so the register allocation of the kernel is the largest single operator
+  // (not their sum), and every site stays owned by the same thread across operators.
+  // When a tile has no more sites than threads (E <= D) the "loop" is a single guarded
+  // iteration, so no uniform table load can be hoisted out of it into VGPRs.
+  auto for_sites = [&](auto&& body) __attribute__((always_inline)) {
+    if constexpr (EX <= D) {
+      if (tid < TS) body(tid);
+    } else {
+      for (int s = tid; s < TS; s += nthr) body(s);
+    }
+  };
   auto run_ops = [&](const Op* ops, int nops, int zbase) {
-    for (int s = tid; s < TS; s += nthr) {
-      cplx<T> sp[D];
+    for (int i = 0; i < nops; ++i) {
+      const Op op = ops[i];
+      const double dtf = op.frac * dt;
+      if (op.kind == OP_DIAG) {
+        const cplx<T>* zfi = &zf[(zbase + i) * D];
+        for_sites([&](int s) __attribute__((always_inline)) {
+          cplx<T> sp[D];
 #pragma unroll
-      for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
-      double Vt = 0.0;
-      bool haveV = false;
-      for (int i = 0; i < nops; ++i) {
-        const Op op = ops[i];
-        const double dtf = op.frac * dt;
-        if (op.kind == OP_DIAG) {
-          if (!haveV) { Vt = trap_at(s); haveV = true; }
-          op_diag<T, D>(sp, Vt, P, dtf, itp, &zf[(zbase + i) * D]);
-        } else if (op.kind == OP_SPINMIX) {
+          for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
+          op_diag<T, D>(sp, trap_at(s), P, dtf, itp, zfi);
+#pragma unroll
+          for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
+        });
+      } else if (op.kind == OP_SPINMIX && D < 13) {
+        for_sites([&](int s) __attribute__((always_inline)) {
+          cplx<T> sp[D];
+#pragma unroll
+          for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
           op_spin_mixing<T, D>(sp, P, dtf, itp, vfy);
-        } else if (op.kind == OP_DDI_ROT) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
+        });
+      } else if (op.kind == OP_DDI_ROT && D < 13) {
+        for_sites([&](int s) __attribute__((always_inline)) {
+          cplx<T> sp[D];
+#pragma unroll
+          for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
           const int r = s / NX, x = s - r * NX;
           const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
           op_euler<T, D>(sp, (double)a.x, (double)a.y, (double)b.x, dtf, itp, vfy);
-        } else if (op.kind == OP_TENSOR) {
-          if constexpr (D == 13) op_tensor<T, D>(sp, P, dtf, itp, cgt);
-        } else if (op.kind == OP_SCALE) {
 #pragma unroll
-          for (int c = 0; c < D; ++c) sp[c] = sp[c] * sc;
-        }
+          for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
+        });
+      } else if (op.kind == OP_SCALE) {
+        for (int e = tid; e < D * TS; e += nthr) tile[e] = tile[e] * sc;
+        __syncthreads();  // the scale loop's element ownership differs from the site loops
       }
-#pragma unroll
-      for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
     }
     __syncthreads();
   };
@@ -236,6 +254,17 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     }
     const double m = block_max(phimax, rsh);
     if (tid == 0) *partial_ptr(tb, RS_PHIMAX, inst, blockIdx.x) = m;
+  }
+
+  if (st & XS_STORE_PHI) {
+    T* ph = phi + (size_t)inst * 3 * g.nsites + (size_t)row0 * NX;
+    for (int s2 = tid; s2 < TS; s2 += nthr) {
+      const int r = s2 / NX, x = s2 - r * NX;
+      const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
+      ph[s2] = a.x;
+      ph[g.nsites + s2] = a.y;
+      ph[2 * g.nsites + s2] = b.x;
+    }
   }
 
   if (st & XS_OPS_A) run_ops(s_ops, prog.nA, 0);
@@ -284,7 +313,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
       e_lhy += n * n * sqrt(n);
       if constexpr (D == 13)
-        if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P, cgt);
+        if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P, tb.tens_cg);
       if (st & XS_LOAD_PHI) {
         const int r = s / NX, x = s - r * NX;
         const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
@@ -359,6 +388,80 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       dst[ibase + (size_t)c * g.nsites + r] = tile[e];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// lane-per-site pass (13-component operators: tensor expm-action, Euler rotations)
+// ---------------------------------------------------------------------------
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
+            const T* __restrict__ phi) {
+  __shared__ cplx<T> zf[kMaxOps * D];
+  __shared__ Op s_ops[kMaxOps];
+  const int tid = threadIdx.x;
+  const int inst = tb.active[blockIdx.y];
+  const InstParams& P = tb.params[inst];
+  const double dt = prog.dt_override > 0.0 ? prog.dt_override : P.dt;
+  const bool itp = prog.itp != 0;
+#pragma unroll
+  for (int i = 0; i < kMaxOps; ++i)
+    if (tid == i) s_ops[i] = prog.ops[i];
+  __syncthreads();
+  for (int idx = tid; idx < kMaxOps * D; idx += blockDim.x) {
+    const int i = idx / D, c = idx % D;
+    const Op op = s_ops[i];
+    if (i < prog.n && op.kind == OP_DIAG) {
+      const double dtf = op.frac * dt;
+      if (itp) zf[idx] = mk<T>((T)exp(-(P.zee[c] - P.zee_shift) * dtf), (T)0);
+      else zf[idx] = cis_neg<T>(P.zee[c] * dtf);
+    }
+  }
+  __syncthreads();
+  const long long site = (long long)blockIdx.x * blockDim.x + tid;
+  if (site >= g.nsites) return;
+  cplx<T>* p = psi + (size_t)inst * g.inst_stride + site;
+  cplx<T> sp[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) sp[c] = p[(size_t)c * g.nsites];
+  const cplx<T>* vfy = sizeof(T) == 8 ? reinterpret_cast<const cplx<T>*>(tb.vfy)
+                                       : reinterpret_cast<const cplx<T>*>(tb.vfy_f);
+  const T* cgt = sizeof(T) == 8 ? reinterpret_cast<const T*>(tb.tens_cg)
+                                : reinterpret_cast<const T*>(tb.tens_cg_f);
+  for (int i = 0; i < prog.n; ++i) {
+    const Op op = s_ops[i];
+    const double dtf = op.frac * dt;
+    if (op.kind == OP_DIAG) {
+      const int x = (int)(site % g.n[0]);
+      const long long row = site / g.n[0];
+      const int y = (int)(row % g.n[1]), z = (int)(row / g.n[1]);
+      double Vt;
+      if (tb.Varr) {
+        Vt = tb.Varr[(size_t)inst * g.nsites + site];
+      } else {
+        const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
+        double v = ax[x];
+        if (g.ndim >= 2) v += ax[g.nmax + y];
+        if (g.ndim >= 3) v += ax[2 * g.nmax + z];
+        Vt = 0.5 * v;
+      }
+      op_diag<T, D>(sp, Vt, P, dtf, itp, &zf[i * D]);
+    } else if (op.kind == OP_SPINMIX) {
+      op_spin_mixing<T, D>(sp, P, dtf, itp, vfy);
+    } else if (op.kind == OP_DDI_ROT) {
+      const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
+      op_euler<T, D>(sp, (double)ph[0], (double)ph[g.nsites], (double)ph[2 * g.nsites], dtf, itp,
+                     vfy);
+    } else if (op.kind == OP_TENSOR) {
+      if constexpr (D == 13) op_tensor<T, D>(sp, P, dtf, itp, cgt);
+    } else if (op.kind == OP_SCALE) {
+      const T sc = (T)(1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV));
+#pragma unroll
+      for (int c = 0; c < D; ++c) sp[c] = sp[c] * sc;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) p[(size_t)c * g.nsites] = sp[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -604,8 +707,8 @@ __global__ void diff_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ a,
 // ---------------------------------------------------------------------------
 int xpass_rows(int D, int NX, int nrows) {
   int target = 3072 / (D * NX);
-  const int e = (D >= 13 && NX >= 256) ? 16 : 8;
-  const int maxthr = D >= 13 ? 256 : 512;
+  const int e = 8;
+  const int maxthr = 512;
   while (target > 1 && D * target * (NX / e) > maxthr) --target;
   if (target < 1) target = 1;
   int r = 1;
@@ -617,7 +720,7 @@ int xpass_rows(int D, int NX, int nrows) {
 
 template <typename T, int NX, int D>
 static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                          const void* src, void* dst, void* fhat, int n_active) {
+                          const void* src, void* dst, void* fhat, int n_active, void* phi) {
   const int rows = xpass_rows(D, NX, g.nrows);
   const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
   const size_t lds = sizeof(cplx<T>) * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0) +
@@ -630,31 +733,52 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
     (void)hipFuncSetAttribute((const void*)xpass_kernel<T, NX, D>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((xpass_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,
-                     (const cplx<T>*)src, (cplx<T>*)dst, (cplx<T>*)fhat, rows);
+                     (const cplx<T>*)src, (cplx<T>*)dst, (cplx<T>*)fhat, (T*)phi, rows);
   return hipGetLastError();
 }
 
 template <typename T, int D>
 static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                          const void* src, void* dst, void* fhat, int n_active) {
+                          const void* src, void* dst, void* fhat, int n_active, void* phi) {
   switch (g.n[0]) {
-    case 16: return xpass_n<T, 16, D>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 32: return xpass_n<T, 32, D>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 64: return xpass_n<T, 64, D>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 128: return xpass_n<T, 128, D>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 256: return xpass_n<T, 256, D>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 512: return xpass_n<T, 512, D>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 16: return xpass_n<T, 16, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 32: return xpass_n<T, 32, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 64: return xpass_n<T, 64, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 128: return xpass_n<T, 128, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 256: return xpass_n<T, 256, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 512: return xpass_n<T, 512, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename T>
 hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                        const void* src, void* dst, void* fhat, int n_active) {
+                        const void* src, void* dst, void* fhat, int n_active, void* phi) {
   switch (g.D) {
-    case 3: return xpass_d<T, 3>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 5: return xpass_d<T, 5>(st, g, tb, prog, src, dst, fhat, n_active);
-    case 13: return xpass_d<T, 13>(st, g, tb, prog, src, dst, fhat, n_active);
+    case 3: return xpass_d<T, 3>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 5: return xpass_d<T, 5>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 13: return xpass_d<T, 13>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_sites(hipStream_t st, const Geom& g, const DevTables& tb, const SProgram& prog,
+                        void* psi, const void* phi, int n_active) {
+  const dim3 grid((unsigned)((g.nsites + 255) / 256), n_active);
+  switch (g.D) {
+    case 13:
+      hipLaunchKernelGGL((site_kernel<T, 13>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
+                         (const T*)phi);
+      return hipGetLastError();
+    case 5:
+      hipLaunchKernelGGL((site_kernel<T, 5>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
+                         (const T*)phi);
+      return hipGetLastError();
+    case 3:
+      hipLaunchKernelGGL((site_kernel<T, 3>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
+                         (const T*)phi);
+      return hipGetLastError();
     default: return hipErrorInvalidValue;
   }
 }
@@ -751,9 +875,25 @@ hipError_t launch_diff(hipStream_t st, const Geom& g, const DevTables& tb, const
   return hipGetLastError();
 }
 
+__global__ void widen_kernel(const float* __restrict__ src, double* __restrict__ dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (double)src[i];
+}
+
+hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n) {
+  long long nb = (n + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(widen_kernel, dim3((unsigned)nb), dim3(256), 0, st, (const float*)src,
+                     (double*)dst, n);
+  return hipGetLastError();
+}
+
 #define BEC_INST(T)                                                                           \
   template hipError_t launch_xpass<T>(hipStream_t, const Geom&, const DevTables&,              \
-                                      const XProgram&, const void*, void*, void*, int);        \
+                                      const XProgram&, const void*, void*, void*, int, void*); \
+  template hipError_t launch_sites<T>(hipStream_t, const Geom&, const DevTables&,              \
+                                      const SProgram&, void*, const void*, int);               \
   template hipError_t launch_strided<T>(hipStream_t, const Geom&, const DevTables&, int, int,  \
                                         void*, int, double, double, int, int, int*);           \
   template hipError_t launch_scale_norm<T>(hipStream_t, const Geom&, const DevTables&, void*,  \

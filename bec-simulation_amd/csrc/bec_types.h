@@ -49,6 +49,8 @@ enum XStage : int {
   XS_ENERGY = 1 << 9,      // pointwise energy partials (trap, Zeeman, c0, c1, LHY, DDI)
   XS_KE_1D = 1 << 10,      // 1-D kinetic energy partial sum k^2 |psi_hat|^2 after forward FFT
   XS_TENSOR_E = 1 << 11,   // include the tensor interaction energy in XS_ENERGY
+  XS_STORE_PHI = 1 << 12,  // after XS_LOAD_PHI: write Phi(x) (3 real fields) for the site kernel
+  XS_NO_LOAD = 1 << 13,    // psi is not read (with XS_NO_STORE: a pass over the DDI spectra only)
 };
 
 struct XProgram {
@@ -59,6 +61,14 @@ struct XProgram {
   double kin_frac;    // 1-D kinetic step dt fraction
   Op A[kMaxOps];
   Op B[kMaxOps];
+};
+
+// Program of the lane-per-site kernel (heavy per-site operators of large spins).
+struct SProgram {
+  int n;
+  int itp;
+  double dt_override;
+  Op ops[kMaxOps];
 };
 
 // Reduction slots written per block into the partials buffer.

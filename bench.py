@@ -10,6 +10,11 @@ Workloads (sizes from the reference benchmark configurations):
                    (energy/norm/Mz on the device); TF polar + 1e-3 seeded noise in m=+-1
   itp1d_spin1_64 : 1-D spin-1 imaginary-time split_step!, 64 points, box 20, c0=10,
                    c1=-0.5, omega=1, dt=0.005 (ITP renormalisation every step)
+  yoshida3d_spin6_ddi_128 : 3-D spin-6 dipolar (13 components) 128^3, box 24, c0=10,
+                   tensor channels from c2..c12 ~ U[-1,1] (seed 42), c_dd=1 along z (full
+                   Q_ab), omega=(1,1,1), run_simulation_yoshida! adaptive (dt_init 1e-3, PI
+                   controller, tol 1e-3); psi0 = m=+6 Gaussian + 1e-3 seeded noise.
+                   A step = one ACCEPTED Yoshida step (rejected attempts included in time).
 """
 from __future__ import annotations
 
@@ -106,6 +111,73 @@ def run_itp1d(G, A, steps, warmup, precision):
     return ws, grid, (t1 - t0), prof, psi
 
 
+def cfg3_inputs(G, n=128):
+    L = 24.0
+    grid = G.make_grid(G.GridConfig((n, n, n), (L, L, L)))
+    gen = G.physics.SplitMix64(42)
+    u = gen.uniform(6)
+    c_extra = [0.0] * 11
+    for i, k in enumerate(range(2, 13, 2)):
+        c_extra[k - 2] = float(2 * u[i] - 1)          # c_k, k = 2..12, U[-1, 1]
+    psi = G.init_psi(grid, G.SpinSystem(6), "ferromagnetic")
+    psi = G.add_noise(psi, 1e-3, 42)
+    inter = G.InteractionParams(10.0, 0.0, 0.0, tuple(c_extra))
+    return grid, inter, psi, c_extra
+
+
+def run_yoshida3d(G, A, steps, warmup, precision):
+    grid, inter, psi, _ = cfg3_inputs(G)
+    sp = G.SimParams(dt=1e-3, n_steps=500)
+    ws = G.make_workspace(grid=grid, atom=G.Eu151, interactions=inter,
+                          potential=G.HarmonicTrap(1.0, 1.0, 1.0), sim_params=sp, psi_init=psi,
+                          enable_ddi=True, c_dd=1.0, precision=precision)
+    ad = G.AdaptiveDtParams(dt_init=1e-3)
+    r = A.run_simulation_yoshida(ws, adaptive=ad, t_end=1e9, save_interval=0.05,
+                                 keep_snapshots=False, max_accepted=warmup)
+    ws.synchronize()
+    ad2 = G.AdaptiveDtParams(dt_init=r["final_dt"])
+    t0 = time.perf_counter()
+    r2 = A.run_simulation_yoshida(ws, adaptive=ad2, t_end=1e9, save_interval=0.05,
+                                  keep_snapshots=False, max_accepted=steps)
+    ws.synchronize()
+    t1 = time.perf_counter()
+    ws.profile(True)
+    A.run_simulation_yoshida(ws, adaptive=G.AdaptiveDtParams(dt_init=r2["final_dt"]), t_end=1e9,
+                             save_interval=0.05, keep_snapshots=False,
+                             max_accepted=max(2, steps // 4))
+    prof = ws.profile_read()
+    ws.profile(False)
+    return ws, grid, (t1 - t0), prof, psi, r2
+
+
+def cpu_baseline_yoshida3d(budget_s=20.0):
+    """FP64 oracle on a bounded sample of the spin-6 dipolar workload: Yoshida
+    attempts (Strang core + Yoshida core + L2 error, yoshida.jl:50-56) on the same
+    configuration at 32^3 (same box and couplings), scaled to site-steps/s."""
+    import becgpu as G
+    from oracle import spinor_oracle as O
+    n = 32
+    grid, inter, psi, c_extra = cfg3_inputs(G, n)
+    og = O.make_grid((n,) * 3, (24.0,) * 3)
+    ows = O.make_workspace(og, 6, psi, 1e-3, False, omega=(1, 1, 1), c0=10.0, c_extra=c_extra,
+                           ddi=True, cdd=1.0)
+    dt = 1e-3
+    att = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        old = ows.psi.copy()
+        O.strang_core(ows, dt)
+        ps = ows.psi.copy()
+        ows.psi = old
+        O.yoshida_core(ows, dt)
+        O.l2_change(ows.psi, ps)
+        att += 1
+    el = time.perf_counter() - t0
+    return dict(value=att * n ** 3 / el, unit="site-steps/s", cores=1, kind="port",
+                sample=f"{att} Yoshida attempts (dt=1e-3) of the spin-6 dipolar workload at 32^3, "
+                       f"FP64 numpy oracle (LAPACK eigh per site), {el:.1f} s")
+
+
 def cpu_baseline_rt3d(psi, grid_n=128, budget_s=15.0):
     """FP64 oracle (numpy/pocketfft, single thread) on a bounded sample of the same
     workload: leapfrog steps on the same 128^3 input until ~budget_s of CPU time."""
@@ -148,8 +220,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="rt3d_spin1_128",
-                    choices=["rt3d_spin1_128", "itp1d_spin1_64"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+                    choices=["rt3d_spin1_128", "itp1d_spin1_64", "yoshida3d_spin6_ddi_128"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -159,8 +231,21 @@ def main():
     import becgpu as G
     from becgpu import api as A
 
+    if args.precision is None:
+        args.precision = "fp32" if args.workload.startswith("yoshida") else "fp64"
     csz = 16 if args.precision == "fp64" else 8
-    if args.workload == "rt3d_spin1_128":
+    extra = {}
+    if args.workload == "yoshida3d_spin6_ddi_128":
+        ws, grid, secs, prof, psi, r2 = run_yoshida3d(G, A, args.steps, args.warmup,
+                                                      args.precision)
+        # Yoshida attempt = 6V + 4K + 3 psi copies + L2 reduction (B_alg per site-step)
+        b_alg = 10852.5 if args.precision == "fp64" else 5426.3
+        cfg = {"workload": "3D spin-6 dipolar 128^3, adaptive Yoshida-4 (run_simulation_yoshida!)",
+               "grid": [128, 128, 128], "F": 6, "box": 24.0, "c0": 10.0, "c_dd": 1.0,
+               "tensor": "c2..c12 ~ U[-1,1] seed 42", "dt_init": 1e-3}
+        extra = {"accepted": r2["n_accepted"], "rejected": r2["n_rejected"],
+                 "final_dt": r2["final_dt"]}
+    elif args.workload == "rt3d_spin1_128":
         ws, grid, secs, prof, psi = run_rt3d(G, A, args.steps, args.warmup, args.precision)
         b_alg = 288.0 if args.precision == "fp64" else 144.0
         cfg = {"workload": "3D spin-1 real-time leapfrog 128^3 (run_simulation!, save_every=10)",
@@ -188,7 +273,12 @@ def main():
                for k, v in prof.items() if v[1] > 0}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_rt3d(psi) if args.workload == "rt3d_spin1_128" else cpu_baseline_itp1d()
+        if args.workload == "rt3d_spin1_128":
+            cpu = cpu_baseline_rt3d(psi)
+        elif args.workload == "itp1d_spin1_64":
+            cpu = cpu_baseline_itp1d()
+        else:
+            cpu = cpu_baseline_yoshida3d()
     line = {
         "metric": "spinor site-steps/s & %HBM peak",
         "value": value, "unit": "site-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -199,6 +289,7 @@ def main():
         "step_hbm_frac": round(value * b_alg / HBM_PEAK, 4), "b_alg_per_site_step": b_alg,
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

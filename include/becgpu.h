@@ -133,10 +133,12 @@ int bec_itp_guard(bec_ws* ws, double* out);
  *  BEC_K_DDI_FFT    y forward/inverse FFT of the DDI spectra
  *  BEC_K_DDI_CONV   last-axis DDI pass: forward FFT, C_dd Q_ab(k) contraction, inverse FFT
  *  BEC_K_REDUCE     partial-sum reduction
+ *  BEC_K_SITES      lane-per-site pass (13-component tensor step and spin rotations)
+ *  BEC_K_ENERGY     a whole bec_energy evaluation (all of its launches)
  * No counterpart in the reference (which uses TimerOutputs, SpinorBEC.jl:12). */
 enum {
   BEC_K_XPASS = 0, BEC_K_FWD, BEC_K_KIN, BEC_K_INV, BEC_K_KE, BEC_K_DDI_FFT, BEC_K_DDI_CONV,
-  BEC_K_REDUCE, BEC_K_COUNT
+  BEC_K_REDUCE, BEC_K_SITES, BEC_K_ENERGY, BEC_K_COUNT
 };
 int bec_profile_enable(bec_ws* ws, int on);
 /* ms[BEC_K_COUNT] total milliseconds and count[BEC_K_COUNT] launches since enable;

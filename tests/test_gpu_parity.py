@@ -221,3 +221,53 @@ def test_tensor_f6_ddi_yoshida_core():
     assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
     eo, eg = O.total_energy(ows), A.total_energy(gws)
     assert abs(eg - eo) <= E_TOL * abs(eo)
+
+
+# --- FP32-bulk path: FP32 psi/FFT/pointwise, FP64 reductions and phase arguments ------
+def test_fp32_bulk_ground_state_energy_cfg1():
+    """cfg1 ground state in FP32 bulk: relative energy error <= 1e-7 vs the FP64 oracle."""
+    og, gg = grids((64,), (20.0,))
+    psi0 = O.init_psi(og, 1, "polar")
+    ows = O.make_workspace(og, 1, psi0, 0.005, True, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=5000)
+    ro = O.find_ground_state(ows, 5000, 1e-10)
+    rg = G.find_ground_state(grid=gg, atom=G.Rb87, interactions=G.InteractionParams(10.0, -0.5),
+                             potential=G.HarmonicTrap(1.0), dt=0.005, n_steps=5000, tol=1e-10,
+                             psi_init=psi0, precision="fp32")
+    rel = abs(rg["energy"] - ro["energy"]) / abs(ro["energy"])
+    print("fp32 cfg1 rel energy error", rel)
+    assert rel <= 1e-7
+    assert rel_l2(rg["workspace"].get_psi(), ows.psi) <= 1e-4
+
+
+def test_fp32_bulk_tensor_ddi_step():
+    n, L = (16, 16, 16), (24.0, 24.0, 24.0)
+    og, _ = grids(n, L)
+    psi = np.zeros((13,) + og.shape, complex)
+    psi[0] = O.gaussian(og, (3.0, 3.0, 3.0))
+    psi += 1e-3 * random_psi(og.shape, 13, 42)
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 6, psi, 1e-3, False, omega=(1, 1, 1), c0=10.0, c_extra=_cfg3_extra(),
+                    ddi=True, cdd=1.0, precision="fp32")
+    for _ in range(3):
+        O.yoshida_core(ows, 1e-3)
+        A.yoshida_core(gws, 1e-3)
+    err = rel_l2(gws.get_psi(), ows.psi)
+    eo, eg = O.total_energy(ows), A.total_energy(gws)
+    print("fp32 tensor/ddi psi err", err, "energy rel err", abs(eg - eo) / abs(eo))
+    assert err <= 5e-5
+    assert abs(eg - eo) <= 1e-6 * abs(eo)
+
+
+def test_fp32_bulk_leapfrog_3d():
+    n, L = (32, 32, 32), (20.0, 20.0, 20.0)
+    og, _ = grids(n, L)
+    psi = O.thomas_fermi_psi(og, 1, O.harmonic_potential(og, (1, 1, 1)), 10.0)
+    psi[0] += 1e-3 * random_psi(og.shape, 1, 7)[0]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 1, psi, 0.001, False, omega=(1, 1, 1), q=0.1, c0=10.0, c1=0.5,
+                    n_steps=50, save_every=10, precision="fp32")
+    ro = O.run_simulation(ows)
+    rg = A.run_simulation(gws, keep_snapshots=False)
+    for eg, eo in zip(rg.energies, ro["energies"]):
+        assert abs(eg - eo) <= 1e-6 * abs(eo)
+    assert rel_l2(gws.get_psi(), ows.psi) <= 1e-5
