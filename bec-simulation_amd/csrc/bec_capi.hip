@@ -48,6 +48,7 @@ struct bec_ws {
   void* psi64 = nullptr;
   void* scratch64 = nullptr;
   void* fhat64 = nullptr;
+  void* phi64 = nullptr;
   void* tw64[10] = {};
   double* d_ktab = nullptr;
   double* d_axtab = nullptr;
@@ -207,9 +208,10 @@ struct Sched {
     sp.dt_override = dt_ovr;
   }
   bool empty() const { return p.stages == 0 && p.nA == 0 && p.nB == 0; }
-  // 13-component spin rotations and the tensor step run in the lane-per-site kernel
+  // Euler spin rotations (DDI, spin mixing for F >= 2) and the tensor step run in the
+  // lane-per-site kernel; the x pass keeps FFTs and light pointwise work
   bool site_op(int kind) const {
-    return ws->D >= 13 && (kind == OP_TENSOR || kind == OP_DDI_ROT || kind == OP_SPINMIX);
+    return kind == OP_TENSOR || kind == OP_DDI_ROT || (kind == OP_SPINMIX && ws->D >= 5);
   }
   int flush_s() {
     if (sp.n == 0) return BEC_OK;
@@ -280,12 +282,9 @@ struct Sched {
       p.stages |= XS_WRITE_FHAT;
       CHK(flush());
       CHK(ddi_conv());
-      if (ws->D >= 13) {  // C2R of Phi into its real-space buffer, rotation per site
-        p.stages |= XS_LOAD_PHI | XS_STORE_PHI | XS_NO_LOAD | XS_NO_STORE;
-        CHK(flush());
-      } else {
-        p.stages |= XS_LOAD_PHI;
-      }
+      // C2R of Phi into its real-space buffer; the rotation runs per site
+      p.stages |= XS_LOAD_PHI | XS_STORE_PHI | XS_NO_LOAD | XS_NO_STORE;
+      CHK(flush());
       CHK(add_op(OP_DDI_ROT, f));
     }
     CHK(add_op(mix, f / 2));
@@ -436,7 +435,7 @@ static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out
     const int nxh = nx / 2 + 1;
     const int pitch = (nxh + 15) / 16 * 16;
     bytes += 3LL * B * pitch * ny * nz * csz;
-    if (D >= 13) bytes += 3LL * B * ns * (csz / 2);
+    bytes += 3LL * B * ns * (csz / 2);
     if (c->precision == BEC_FP32) bytes += 3LL * B * pitch * ny * nz * 16;
   }
   if (c->precision == BEC_FP32) bytes += 2LL * B * D * ns * 16;  // FP64 energy copies
@@ -529,10 +528,8 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     const size_t fb = ws->csz * (size_t)g.fhat_inst_stride * ws->B;
     if (hipMalloc(&ws->fhat, fb) != hipSuccess) return cleanup(BEC_ENOMEM, "fhat alloc");
     if (hipMemset(ws->fhat, 0, fb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
-    if (ws->D >= 13) {
-      const size_t pb = (ws->csz / 2) * 3 * (size_t)g.nsites * ws->B;
-      if (hipMalloc(&ws->phi, pb) != hipSuccess) return cleanup(BEC_ENOMEM, "phi alloc");
-    }
+    const size_t pb = (ws->csz / 2) * 3 * (size_t)g.nsites * ws->B;
+    if (hipMalloc(&ws->phi, pb) != hipSuccess) return cleanup(BEC_ENOMEM, "phi alloc");
   }
   // k table [3][nmax]
   std::vector<double> kt(3 * g.nmax, 0.0);
@@ -614,7 +611,7 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->scratch) (void)hipFree(ws->scratch);
   if (ws->fhat) (void)hipFree(ws->fhat);
   if (ws->phi) (void)hipFree(ws->phi);
-  for (void* q : {ws->psi64, ws->scratch64, ws->fhat64})
+  for (void* q : {ws->psi64, ws->scratch64, ws->fhat64, ws->phi64})
     if (q) (void)hipFree(q);
   for (int i = 0; i < 10; ++i)
     if (ws->tw64[i]) (void)hipFree(ws->tw64[i]);
@@ -845,13 +842,13 @@ static int make_twiddles(int prec, void** tw) {
 
 // Energy partial sums of psi (FP64 arithmetic on FP64 data), reduced into d_red.
 static int energy_partials_fp64(bec_ws* ws, const DevTables& tb, void* psi, void* scratch,
-                                void* fhat) {
+                                void* fhat, void* phi) {
   const Geom& g = ws->g;
   const int na = n_active(ws);
   XProgram p{};
   p.itp = ws->itp;
   auto X = [&](const XProgram& q, const void* src, void* dst) -> int {
-    HIPCHK(launch_xpass<double>(ws->stream, g, tb, q, src, dst, fhat, na, nullptr));
+    HIPCHK(launch_xpass<double>(ws->stream, g, tb, q, src, dst, fhat, na, phi));
     return BEC_OK;
   };
   auto S = [&](int axis, int mode, void* data, int ddi, int* nb) -> int {
@@ -868,16 +865,16 @@ static int energy_partials_fp64(bec_ws* ws, const DevTables& tb, void* psi, void
     } else {
       CHK(S(1, SM_DDI, fhat, 1, nullptr));
     }
-    p.stages = XS_LOAD_PHI | XS_ENERGY | XS_NO_STORE;
-  } else {
-    p.stages = XS_ENERGY | XS_NO_STORE;
+    p.stages = XS_LOAD_PHI | XS_STORE_PHI | XS_NO_LOAD | XS_NO_STORE;
+    CHK(X(p, psi, psi));
   }
-  if (ws->tensor) p.stages |= XS_TENSOR_E;
-  CHK(X(p, psi, psi));
+  int nbe = 0;
+  HIPCHK(launch_energy<double>(ws->stream, g, tb, psi, ws->ddi ? phi : nullptr, ws->tensor, na,
+                               &nbe));
   const uint32_t emask = (1u << RS_NORM) | (1u << RS_TRAP) | (1u << RS_ZEE) | (1u << RS_C0) |
                          (1u << RS_C1) | (1u << RS_LHY) | (1u << RS_DDI) | (1u << RS_MZ) |
                          (1u << RS_TENSOR);
-  HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, ws->n_xblocks, emask, 0));
+  HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, nbe, emask, 0));
   // kinetic energy: forward FFT of every component, sum k^2 |psi_hat|^2 (energy.jl:56-65)
   if (g.ndim == 1) {
     p.stages = XS_FWD_FFT | XS_KE_1D | XS_NO_STORE;
@@ -901,7 +898,7 @@ int bec_energy(bec_ws* ws, double* out) {
   const Geom& g = ws->g;
   const int pe = prof_begin(ws, 9);
   if (ws->prec == BEC_FP64) {
-    CHK(energy_partials_fp64(ws, ws->tb, ws->slot[0], ws->scratch, ws->fhat));
+    CHK(energy_partials_fp64(ws, ws->tb, ws->slot[0], ws->scratch, ws->fhat, ws->phi));
   } else {
     // FP32-bulk: the state is widened to FP64 and every energy term (FFT, DDI
     // convolution, reductions) is evaluated in FP64
@@ -913,13 +910,15 @@ int bec_energy(bec_ws* ws, double* out) {
         const size_t fb = 16 * (size_t)g.fhat_inst_stride * ws->B;
         if (hipMalloc(&ws->fhat64, fb) != hipSuccess) return fail(BEC_ENOMEM, "fhat64");
         HIPCHK(hipMemset(ws->fhat64, 0, fb));
+        if (hipMalloc(&ws->phi64, 24 * (size_t)g.nsites * ws->B) != hipSuccess)
+          return fail(BEC_ENOMEM, "phi64");
       }
       CHK(make_twiddles(BEC_FP64, ws->tw64));
     }
     HIPCHK(launch_widen(ws->stream, ws->slot[0], ws->psi64, (long long)(2 * n)));
     DevTables tb = ws->tb;
     for (int i = 0; i < 10; ++i) tb.tw[i] = ws->tw64[i];
-    CHK(energy_partials_fp64(ws, tb, ws->psi64, ws->scratch64, ws->fhat64));
+    CHK(energy_partials_fp64(ws, tb, ws->psi64, ws->scratch64, ws->fhat64, ws->phi64));
   }
   prof_end(ws, pe);
   CHK(fetch_red(ws));

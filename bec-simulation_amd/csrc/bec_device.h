@@ -52,6 +52,13 @@ __device__ __forceinline__ cplx<T> cis_neg(double a) {
   return mk<T>((T)c, (T)(-s));
 }
 
+// Multiply by a real FP64 factor with one rounding to T per component (an FP32 copy of
+// the factor would bias every element of a renormalised FP32 state the same way).
+template <typename T>
+__device__ __forceinline__ cplx<T> scale64(cplx<T> a, double f) {
+  return mk((T)((double)a.x * f), (T)((double)a.y * f));
+}
+
 // ---------------------------------------------------------------------------
 // In-register small DFTs
 // ---------------------------------------------------------------------------
@@ -220,7 +227,9 @@ __device__ __forceinline__ void spin_vec(const cplx<T>* s, T& fx, T& fy, T& fz) 
   }
 }
 
-// spin_mixing.jl:71-98: U = I - i sin(th) nF + (cos th - 1) nF^2 (ITP: sinh/cosh, no i)
+// spin_mixing.jl:71-98: U = I - i sin(th) nF + (cos th - 1) nF^2 (ITP: sinh/cosh, no i),
+// applied as U s = s - i sin(th) (nF s) + (cos th - 1) nF (nF s) with the spin-1 matrices
+// written out (nF = [[c, u*, 0], [u, 0, u*], [0, u, -c]], u = (a + i b)/sqrt2).
 template <typename T>
 __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, bool itp) {
   T fx, fy, fz;
@@ -228,23 +237,17 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
   T fmag = sqrt(fx * fx + fy * fy + fz * fz);
   if (!(fmag >= (T)1e-30)) return;
   const T is2 = (T)0.70710678118654752440;
-  T a = fx / fmag, b = fy / fmag, c = fz / fmag;
-  // nF = [[c, u*, 0], [u, 0, u*], [0, u, -c]] with u = (a + i b)/sqrt2
-  cplx<T> u = mk(a * is2, b * is2), uc = conjc(u);
-  cplx<T> nF[3][3] = {{mk(c, (T)0), uc, mk((T)0, (T)0)},
-                      {u, mk((T)0, (T)0), uc},
-                      {mk((T)0, (T)0), u, mk(-c, (T)0)}};
-  cplx<T> nF2[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      cplx<T> acc = nF[i][0] * nF[0][j];
-      acc = acc + nF[i][1] * nF[1][j];
-      acc = acc + nF[i][2] * nF[2][j];
-      nF2[i][j] = acc;
-    }
-  double th = c1 * (double)fmag * dtf;
+  const T c = fz / fmag;
+  const cplx<T> u = mk(fx / fmag * is2, fy / fmag * is2), uc = conjc(u);
+  auto nF = [&](const cplx<T>* v, cplx<T>* o) __attribute__((always_inline)) {
+    o[0] = v[0] * c + uc * v[1];
+    o[1] = u * v[0] + uc * v[2];
+    o[2] = u * v[1] - v[2] * c;
+  };
+  cplx<T> a[3], b[3];
+  nF(s, a);
+  nF(a, b);
+  const double th = c1 * (double)fmag * dtf;
   T s1, c1m;
   if (itp) {
     s1 = (T)sinh(th);
@@ -255,23 +258,12 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
     s1 = (T)sn;
     c1m = (T)(cs - 1.0);
   }
-  cplx<T> r[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    cplx<T> acc = mk((T)0, (T)0);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      // U_ij = delta_ij - i s1 nF_ij + c1m nF2_ij   (ITP: - s1 nF_ij)
-      cplx<T> U = nF2[i][j] * c1m;
-      if (itp) U = U - nF[i][j] * s1;
-      else U = U + mk(nF[i][j].y * s1, -nF[i][j].x * s1);
-      if (i == j) U.x += (T)1;
-      acc = acc + U * s[j];
-    }
-    r[i] = acc;
+    // - i s1 a  (real time)   or   - s1 a  (imaginary time)
+    const cplx<T> t = itp ? mk(-a[i].x * s1, -a[i].y * s1) : mk(a[i].y * s1, -a[i].x * s1);
+    s[i] = s[i] + t + b[i] * c1m;
   }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) s[i] = r[i];
 }
 
 // spinor_utils.jl:180-264: exp(-i dt phi.F) via Rz(a) Ry(b) Dz(th) Ry(-b) Rz(-a), with

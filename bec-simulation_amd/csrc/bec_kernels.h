@@ -64,6 +64,11 @@ template <typename T>
 hipError_t launch_sites(hipStream_t st, const Geom& g, const DevTables& tb, const SProgram& prog,
                         void* psi, const void* phi, int n_active);
 
+// Pointwise energy partials of psi (phi: Phi(x) of an XS_STORE_PHI pass, or null).
+template <typename T>
+hipError_t launch_energy(hipStream_t st, const Geom& g, const DevTables& tb, const void* psi,
+                         const void* phi, int with_tensor, int n_active, int* nblocks_out);
+
 // y or z axis pass (axis = 1 or 2) over psi (fields = D) or over the DDI
 // spectra (fields = 3, DDI mode).  SM_KIN uses dt = (dt_override > 0 ? dt_override :
 // instance dt) * kin_frac.  nblocks_out (optional): partials per instance (SM_KE).

@@ -13,6 +13,8 @@
 // so results do not depend on what else runs on the device.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "bec_device.h"
 #include "bec_kernels.h"
 
@@ -139,12 +141,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   const int line = tid / TEAM, tt = tid - (tid / TEAM) * TEAM;
   if (st & XS_INV_FFT) fft_team<T, NX, true, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
 
-  T sc = (T)1;
-  {
-    // ITP renormalisation factor of the instance (only read when an OP_SCALE is present)
-    const double n2 = tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV;
-    sc = (T)(1.0 / sqrt(n2));
-  }
+  // ITP renormalisation factor of the instance (only used when an OP_SCALE is present)
+  const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 
   double phimax = 0.0;
   if (st & XS_LOAD_PHI) {
@@ -168,9 +166,6 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     }
     __syncthreads();
     fft_team<T, NX, true, EX>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
-    if (st & XS_ENERGY) {
-      // nothing extra: energy stage reads aux directly
-    }
   }
 
   auto site_coords = [&](int s, int& x, int& y, int& z) {
@@ -217,7 +212,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
           for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
         });
-      } else if (op.kind == OP_SPINMIX && D < 13) {
+      } else if (op.kind == OP_SPINMIX && D == 3) {
         for_sites([&](int s) __attribute__((always_inline)) {
           cplx<T> sp[D];
 #pragma unroll
@@ -226,7 +221,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
           for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
         });
-      } else if (op.kind == OP_DDI_ROT && D < 13) {
+      } else if (op.kind == OP_DDI_ROT && D < 0) {  // runs in the lane-per-site kernel
         for_sites([&](int s) __attribute__((always_inline)) {
           cplx<T> sp[D];
 #pragma unroll
@@ -238,14 +233,14 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
           for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
         });
       } else if (op.kind == OP_SCALE) {
-        for (int e = tid; e < D * TS; e += nthr) tile[e] = tile[e] * sc;
+        for (int e = tid; e < D * TS; e += nthr) tile[e] = scale64(tile[e], sc);
         __syncthreads();  // the scale loop's element ownership differs from the site loops
       }
     }
     __syncthreads();
   };
 
-  if ((st & XS_LOAD_PHI) && (st & XS_ENERGY) == 0) {
+  if (st & XS_LOAD_PHI) {
     // max |Phi| for the imaginary-time overflow guard (ground_state.jl:4-29)
     for (int s = tid; s < TS; s += nthr) {
       const int r = s / NX, x = s - r * NX;
@@ -285,52 +280,6 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   }
 
   if (st & XS_OPS_B) run_ops(s_ops + kMaxOps, prog.nB, kMaxOps);
-
-  if (st & XS_ENERGY) {
-    double e_norm = 0, e_trap = 0, e_zee = 0, e_c0 = 0, e_c1 = 0, e_lhy = 0, e_ddi = 0, e_mz = 0;
-    double e_ten = 0;
-    const bool with_tensor = tb.tens_cg != nullptr && (st & XS_TENSOR_E);
-    constexpr int F = (D - 1) / 2;
-    for (int s = tid; s < TS; s += nthr) {
-      cplx<T> sp[D];
-#pragma unroll
-      for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
-      double n = 0, zz = 0, mz = 0;
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        const double a2 = (double)abs2(sp[c]);
-        n += a2;
-        zz += P.zee[c] * a2;
-        mz += (double)(F - c) * a2;
-      }
-      T fx, fy, fz;
-      spin_vec<T, D>(sp, fx, fy, fz);
-      e_norm += n;
-      e_trap += trap_at(s) * n;
-      e_zee += zz;
-      e_mz += mz;
-      e_c0 += n * n;
-      e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
-      e_lhy += n * n * sqrt(n);
-      if constexpr (D == 13)
-        if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P, tb.tens_cg);
-      if (st & XS_LOAD_PHI) {
-        const int r = s / NX, x = s - r * NX;
-        const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
-        e_ddi += (double)a.x * fx + (double)a.y * fy + (double)b.x * fz;
-      }
-    }
-    double v;
-    v = block_sum(e_norm, rsh); if (tid == 0) *partial_ptr(tb, RS_NORM, inst, blockIdx.x) = v;
-    v = block_sum(e_trap, rsh); if (tid == 0) *partial_ptr(tb, RS_TRAP, inst, blockIdx.x) = v;
-    v = block_sum(e_zee, rsh); if (tid == 0) *partial_ptr(tb, RS_ZEE, inst, blockIdx.x) = v;
-    v = block_sum(e_c0, rsh); if (tid == 0) *partial_ptr(tb, RS_C0, inst, blockIdx.x) = v;
-    v = block_sum(e_c1, rsh); if (tid == 0) *partial_ptr(tb, RS_C1, inst, blockIdx.x) = v;
-    v = block_sum(e_lhy, rsh); if (tid == 0) *partial_ptr(tb, RS_LHY, inst, blockIdx.x) = v;
-    v = block_sum(e_ddi, rsh); if (tid == 0) *partial_ptr(tb, RS_DDI, inst, blockIdx.x) = v;
-    v = block_sum(e_mz, rsh); if (tid == 0) *partial_ptr(tb, RS_MZ, inst, blockIdx.x) = v;
-    v = block_sum(e_ten, rsh); if (tid == 0) *partial_ptr(tb, RS_TENSOR, inst, blockIdx.x) = v;
-  }
 
   if (st & XS_WRITE_FHAT) {
     for (int s = tid; s < TS; s += nthr) {
@@ -390,6 +339,79 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   }
 }
 
+// Trap potential at a flat site index (potentials.jl:5-15 via per-axis omega^2 x^2 tables,
+// or an explicit potential array).
+__device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, int inst,
+                                            long long site) {
+  if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + site];
+  const int x = (int)(site % g.n[0]);
+  const long long row = site / g.n[0];
+  const int y = (int)(row % g.n[1]), z = (int)(row / g.n[1]);
+  const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
+  double v = ax[x];
+  if (g.ndim >= 2) v += ax[g.nmax + y];
+  if (g.ndim >= 3) v += ax[2 * g.nmax + z];
+  return 0.5 * v;
+}
+
+// ---------------------------------------------------------------------------
+// pointwise energy terms (energy.jl:6-121, observables.jl:1-33): per-block partials of
+// norm, trap, Zeeman, n^2, |F|^2, n^{5/2}, Phi.F, Mz and the tensor pair-amplitude sum.
+// ---------------------------------------------------------------------------
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __restrict__ phi,
+              int with_tensor) {
+  __shared__ double rsh[16];
+  constexpr int F = (D - 1) / 2;
+  const int inst = tb.active[blockIdx.y];
+  const InstParams& P = tb.params[inst];
+  double e_norm = 0, e_trap = 0, e_zee = 0, e_c0 = 0, e_c1 = 0, e_lhy = 0, e_ddi = 0, e_mz = 0;
+  double e_ten = 0;
+  const cplx<T>* p = psi + (size_t)inst * g.inst_stride;
+  for (long long site = blockIdx.x * (long long)blockDim.x + threadIdx.x; site < g.nsites;
+       site += (long long)gridDim.x * blockDim.x) {
+    cplx<T> sp[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) sp[c] = p[(size_t)c * g.nsites + site];
+    double n = 0, zz = 0, mz = 0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double a2 = (double)abs2(sp[c]);
+      n += a2;
+      zz += P.zee[c] * a2;
+      mz += (double)(F - c) * a2;
+    }
+    T fx, fy, fz;
+    spin_vec<T, D>(sp, fx, fy, fz);
+    e_norm += n;
+    e_trap += trap_site(g, tb, inst, site) * n;
+    e_zee += zz;
+    e_mz += mz;
+    e_c0 += n * n;
+    e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
+    e_lhy += n * n * sqrt(n);
+    if constexpr (D == 13) {
+      if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P, tb.tens_cg);
+    }
+    if (phi) {
+      const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
+      e_ddi += (double)ph[0] * fx + (double)ph[g.nsites] * fy + (double)ph[2 * g.nsites] * fz;
+    }
+  }
+  const int blk = blockIdx.x, tid = threadIdx.x;
+  double v;
+  v = block_sum(e_norm, rsh); if (tid == 0) *partial_ptr(tb, RS_NORM, inst, blk) = v;
+  v = block_sum(e_trap, rsh); if (tid == 0) *partial_ptr(tb, RS_TRAP, inst, blk) = v;
+  v = block_sum(e_zee, rsh); if (tid == 0) *partial_ptr(tb, RS_ZEE, inst, blk) = v;
+  v = block_sum(e_c0, rsh); if (tid == 0) *partial_ptr(tb, RS_C0, inst, blk) = v;
+  v = block_sum(e_c1, rsh); if (tid == 0) *partial_ptr(tb, RS_C1, inst, blk) = v;
+  v = block_sum(e_lhy, rsh); if (tid == 0) *partial_ptr(tb, RS_LHY, inst, blk) = v;
+  v = block_sum(e_ddi, rsh); if (tid == 0) *partial_ptr(tb, RS_DDI, inst, blk) = v;
+  v = block_sum(e_mz, rsh); if (tid == 0) *partial_ptr(tb, RS_MZ, inst, blk) = v;
+  v = block_sum(e_ten, rsh); if (tid == 0) *partial_ptr(tb, RS_TENSOR, inst, blk) = v;
+}
+
 // ---------------------------------------------------------------------------
 // lane-per-site pass (13-component operators: tensor expm-action, Euler rotations)
 // ---------------------------------------------------------------------------
@@ -432,19 +454,7 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
     const Op op = s_ops[i];
     const double dtf = op.frac * dt;
     if (op.kind == OP_DIAG) {
-      const int x = (int)(site % g.n[0]);
-      const long long row = site / g.n[0];
-      const int y = (int)(row % g.n[1]), z = (int)(row / g.n[1]);
-      double Vt;
-      if (tb.Varr) {
-        Vt = tb.Varr[(size_t)inst * g.nsites + site];
-      } else {
-        const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
-        double v = ax[x];
-        if (g.ndim >= 2) v += ax[g.nmax + y];
-        if (g.ndim >= 3) v += ax[2 * g.nmax + z];
-        Vt = 0.5 * v;
-      }
+      const double Vt = trap_site(g, tb, inst, site);
       op_diag<T, D>(sp, Vt, P, dtf, itp, &zf[i * D]);
     } else if (op.kind == OP_SPINMIX) {
       op_spin_mixing<T, D>(sp, P, dtf, itp, vfy);
@@ -455,9 +465,9 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
     } else if (op.kind == OP_TENSOR) {
       if constexpr (D == 13) op_tensor<T, D>(sp, P, dtf, itp, cgt);
     } else if (op.kind == OP_SCALE) {
-      const T sc = (T)(1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV));
+      const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 #pragma unroll
-      for (int c = 0; c < D; ++c) sp[c] = sp[c] * sc;
+      for (int c = 0; c < D; ++c) sp[c] = scale64(sp[c], sc);
     }
   }
 #pragma unroll
@@ -658,11 +668,11 @@ template <typename T>
 __global__ void scale_norm_kernel(Geom g, DevTables tb, cplx<T>* psi) {
   const int inst = tb.active[blockIdx.y];
   const double n2 = tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV;
-  const T s = (T)(1.0 / sqrt(n2));
+  const double s = 1.0 / sqrt(n2);
   cplx<T>* p = psi + (size_t)inst * g.inst_stride;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.inst_stride;
        i += (long long)gridDim.x * blockDim.x)
-    p[i] = p[i] * s;
+    p[i] = scale64(p[i], s);
 }
 
 template <typename T>
@@ -705,8 +715,17 @@ __global__ void diff_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ a,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static int xpass_tile_elems() {
+  static int v = [] {
+    const char* e = getenv("BEC_XPASS_TILE");
+    const int t = e ? atoi(e) : 0;
+    return t >= 256 ? t : 1536;
+  }();
+  return v;
+}
+
 int xpass_rows(int D, int NX, int nrows) {
-  int target = 3072 / (D * NX);
+  int target = xpass_tile_elems() / (D * NX);
   const int e = 8;
   const int maxthr = 512;
   while (target > 1 && D * target * (NX / e) > maxthr) --target;
@@ -758,6 +777,30 @@ hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, cons
     case 3: return xpass_d<T, 3>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 5: return xpass_d<T, 5>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 13: return xpass_d<T, 13>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_energy(hipStream_t st, const Geom& g, const DevTables& tb, const void* psi,
+                         const void* phi, int with_tensor, int n_active, int* nblocks_out) {
+  long long nb = (g.nsites + 255) / 256;
+  if (nb > 1024) nb = 1024;
+  *nblocks_out = (int)nb;
+  const dim3 grid((unsigned)nb, n_active);
+  switch (g.D) {
+    case 13:
+      hipLaunchKernelGGL((energy_kernel<T, 13>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
+                         (const T*)phi, with_tensor);
+      return hipGetLastError();
+    case 5:
+      hipLaunchKernelGGL((energy_kernel<T, 5>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
+                         (const T*)phi, with_tensor);
+      return hipGetLastError();
+    case 3:
+      hipLaunchKernelGGL((energy_kernel<T, 3>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
+                         (const T*)phi, with_tensor);
+      return hipGetLastError();
     default: return hipErrorInvalidValue;
   }
 }
@@ -892,6 +935,8 @@ hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n)
 #define BEC_INST(T)                                                                           \
   template hipError_t launch_xpass<T>(hipStream_t, const Geom&, const DevTables&,              \
                                       const XProgram&, const void*, void*, void*, int, void*); \
+  template hipError_t launch_energy<T>(hipStream_t, const Geom&, const DevTables&, const void*, \
+                                       const void*, int, int, int*);                          \
   template hipError_t launch_sites<T>(hipStream_t, const Geom&, const DevTables&,              \
                                       const SProgram&, void*, const void*, int);               \
   template hipError_t launch_strided<T>(hipStream_t, const Geom&, const DevTables&, int, int,  \

@@ -46,9 +46,7 @@ enum XStage : int {
   XS_NORM = 1 << 6,        // per-block partial of sum |psi|^2 (after ops)
   XS_FWD_FFT = 1 << 7,     // forward x FFT before the store
   XS_NO_STORE = 1 << 8,    // do not write psi back
-  XS_ENERGY = 1 << 9,      // pointwise energy partials (trap, Zeeman, c0, c1, LHY, DDI)
   XS_KE_1D = 1 << 10,      // 1-D kinetic energy partial sum k^2 |psi_hat|^2 after forward FFT
-  XS_TENSOR_E = 1 << 11,   // include the tensor interaction energy in XS_ENERGY
   XS_STORE_PHI = 1 << 12,  // after XS_LOAD_PHI: write Phi(x) (3 real fields) for the site kernel
   XS_NO_LOAD = 1 << 13,    // psi is not read (with XS_NO_STORE: a pass over the DDI spectra only)
 };
