@@ -49,7 +49,6 @@ struct bec_ws {
   void* scratch64 = nullptr;
   void* fhat64 = nullptr;
   void* phi64 = nullptr;
-  void* tw64[10] = {};
   double* d_ktab = nullptr;
   double* d_axtab = nullptr;
   double* d_varr = nullptr;
@@ -578,24 +577,18 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     if (hipMemcpy(ws->d_tens_cg_f, cgf.data(), cgf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(BEC_EDEVICE, "cg copy");
   }
-  // twiddles exp(-2 pi i m / N), computed in long double
+  // twiddles exp(-2 pi i m / N), computed in long double and stored in FP64 for both
+  // precisions (FP32-bulk butterflies multiply by them in FP64 with one rounding)
   for (int l = 3; l <= 9; ++l) {
     const int N = 1 << l;
-    const size_t esz = ws->csz;
-    std::vector<unsigned char> buf(esz * N);
+    std::vector<double> buf(2 * N);
     for (int m = 0; m < N; ++m) {
       const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
-      const long double cr = cosl(ang), ci = sinl(ang);
-      if (ws->prec == BEC_FP32) {
-        float v[2] = {(float)cr, (float)ci};
-        std::memcpy(&buf[esz * m], v, 8);
-      } else {
-        double v[2] = {(double)cr, (double)ci};
-        std::memcpy(&buf[esz * m], v, 16);
-      }
+      buf[2 * m] = (double)cosl(ang);
+      buf[2 * m + 1] = (double)sinl(ang);
     }
-    if (hipMalloc(&ws->d_tw[l], buf.size()) != hipSuccess) return cleanup(BEC_ENOMEM, "twiddles");
-    if (hipMemcpy(ws->d_tw[l], buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMalloc(&ws->d_tw[l], buf.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "twiddles");
+    if (hipMemcpy(ws->d_tw[l], buf.data(), buf.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(BEC_EDEVICE, "twiddle copy");
   }
   refresh_tables(ws);
@@ -613,8 +606,6 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->phi) (void)hipFree(ws->phi);
   for (void* q : {ws->psi64, ws->scratch64, ws->fhat64, ws->phi64})
     if (q) (void)hipFree(q);
-  for (int i = 0; i < 10; ++i)
-    if (ws->tw64[i]) (void)hipFree(ws->tw64[i]);
   for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials, ws->d_vfy,
                     ws->d_tens_cg})
     if (p) (void)hipFree(p);
@@ -818,28 +809,6 @@ static int fetch_red(bec_ws* ws) {
   return BEC_OK;
 }
 
-static int make_twiddles(int prec, void** tw) {
-  for (int l = 3; l <= 9; ++l) {
-    const int N = 1 << l;
-    const size_t esz = prec == BEC_FP32 ? 8 : 16;
-    std::vector<unsigned char> buf(esz * N);
-    for (int m = 0; m < N; ++m) {
-      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
-      const long double cr = cosl(ang), ci = sinl(ang);
-      if (prec == BEC_FP32) {
-        float v[2] = {(float)cr, (float)ci};
-        std::memcpy(&buf[esz * m], v, 8);
-      } else {
-        double v[2] = {(double)cr, (double)ci};
-        std::memcpy(&buf[esz * m], v, 16);
-      }
-    }
-    if (hipMalloc(&tw[l], buf.size()) != hipSuccess) return fail(BEC_ENOMEM, "twiddles");
-    HIPCHK(hipMemcpy(tw[l], buf.data(), buf.size(), hipMemcpyHostToDevice));
-  }
-  return BEC_OK;
-}
-
 // Energy partial sums of psi (FP64 arithmetic on FP64 data), reduced into d_red.
 static int energy_partials_fp64(bec_ws* ws, const DevTables& tb, void* psi, void* scratch,
                                 void* fhat, void* phi) {
@@ -913,11 +882,9 @@ int bec_energy(bec_ws* ws, double* out) {
         if (hipMalloc(&ws->phi64, 24 * (size_t)g.nsites * ws->B) != hipSuccess)
           return fail(BEC_ENOMEM, "phi64");
       }
-      CHK(make_twiddles(BEC_FP64, ws->tw64));
     }
     HIPCHK(launch_widen(ws->stream, ws->slot[0], ws->psi64, (long long)(2 * n)));
     DevTables tb = ws->tb;
-    for (int i = 0; i < 10; ++i) tb.tw[i] = ws->tw64[i];
     CHK(energy_partials_fp64(ws, tb, ws->psi64, ws->scratch64, ws->fhat64, ws->phi64));
   }
   prof_end(ws, pe);

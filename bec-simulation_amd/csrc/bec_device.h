@@ -59,6 +59,21 @@ __device__ __forceinline__ cplx<T> scale64(cplx<T> a, double f) {
   return mk((T)((double)a.x * f), (T)((double)a.y * f));
 }
 
+// Multiply by an FP64 complex factor with one rounding to T per component.  Unit-modulus
+// factors (twiddles, kinetic and potential phases) rounded to FP32 first have |w| != 1
+// with a bias that does not average out: the FP32 norm would drift every step.
+template <typename T>
+__device__ __forceinline__ cplx<T> mul64(cplx<T> a, cplx<double> w) {
+  const double ax = a.x, ay = a.y;
+  return mk((T)(ax * w.x - ay * w.y), (T)(ax * w.y + ay * w.x));
+}
+
+// x / sqrt(2) with a single rounding (the FP32 constant 1/sqrt(2) is 1.7e-8 low).
+template <typename T>
+__device__ __forceinline__ T mul_rsqrt2(T x) {
+  return (T)((double)x * 0.70710678118654752440);
+}
+
 // ---------------------------------------------------------------------------
 // In-register small DFTs
 // ---------------------------------------------------------------------------
@@ -83,14 +98,14 @@ __device__ __forceinline__ void dft8(cplx<T>* v) {
   // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
   dft4<T, INV>(v[0], v[2], v[4], v[6]);
   dft4<T, INV>(v[1], v[3], v[5], v[7]);
-  const T r = (T)0.70710678118654752440;
   // twiddle odd part by W8^k, W8 = exp(-+ i pi/4)
   cplx<T> o1 = v[3], o2 = v[5], o3 = v[7];
+  auto r = [](T x) __attribute__((always_inline)) { return mul_rsqrt2<T>(x); };
   // W8^1 = (1 -+ i)/sqrt2
-  o1 = INV ? mk((o1.x - o1.y) * r, (o1.x + o1.y) * r) : mk((o1.x + o1.y) * r, (o1.y - o1.x) * r);
+  o1 = INV ? mk(r(o1.x - o1.y), r(o1.x + o1.y)) : mk(r(o1.x + o1.y), r(o1.y - o1.x));
   o2 = mul_mi<T, INV>(o2);
   // W8^3 = (-1 -+ i)/sqrt2
-  o3 = INV ? mk((-o3.x - o3.y) * r, (o3.x - o3.y) * r) : mk((o3.y - o3.x) * r, (-o3.x - o3.y) * r);
+  o3 = INV ? mk(r(-o3.x - o3.y), r(o3.x - o3.y)) : mk(r(o3.y - o3.x), r(-o3.x - o3.y));
   cplx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
   v[0] = e0 + o0;
   v[4] = e0 - o0;
@@ -123,10 +138,10 @@ template <> struct RadixPlan<512> { static constexpr int np = 3; static constexp
 
 // One Stockham pass: butterflies j = t + b*(N/E), b < E/R.
 // in[j + r*N/R] -> twiddle W_N^{(j mod Ns) r N/(Ns R)} -> DFT_R -> out[(j/Ns)Ns R + j mod Ns + r Ns]
-// The line lives at base[e*ES]; tw[m] = exp(-2 pi i m / N).
+// The line lives at base[e*ES]; tw[m] = exp(-2 pi i m / N) in FP64 for every T.
 template <typename T, int N, int R, bool INV, int E = 8>
 __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool active, int Ns,
-                                              const cplx<T>* __restrict__ tw) {
+                                              const cplx<double>* __restrict__ tw) {
   constexpr int NB = E / R;      // butterflies per thread
   constexpr int TEAM = N / E;
   cplx<T> v[E];
@@ -148,9 +163,9 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
         int stepm = k * (N / (Ns * R));
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cplx<T> w = tw[r * stepm];
+          cplx<double> w = tw[r * stepm];
           if (INV) w.y = -w.y;
-          v[b * R + r] = v[b * R + r] * w;
+          v[b * R + r] = mul64(v[b * R + r], w);
         }
       }
       dftR<T, R, INV>(&v[b * R]);
@@ -167,7 +182,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
 // threads that own a (line, team index) pair.
 template <typename T, int N, bool INV, int E = 8>
 __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool active,
-                                         const cplx<T>* __restrict__ tw) {
+                                         const cplx<double>* __restrict__ tw) {
   static_assert(N % E == 0, "line shorter than the per-thread register block");
   using P = RadixPlan<N>;
   int Ns = 1;
@@ -190,21 +205,24 @@ __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool acti
 // exponentials with the Zeeman shift min_c E_c, real time uses cis.
 template <typename T, int D>
 __device__ __forceinline__ void op_diag(cplx<T>* s, double Vtrap, const InstParams& P, double dtf,
-                                        bool itp, const cplx<T>* zf) {
+                                        bool itp, const cplx<double>* zf) {
   T n = 0;
 #pragma unroll
   for (int c = 0; c < D; ++c) n += abs2(s[c]);
   double nd = (double)n;
   double Vint = P.c0 * nd + P.c_lhy * nd * sqrt(nd);
   double a = (Vtrap + Vint) * dtf;
-  cplx<T> base;
   if (itp) {
-    base = mk<T>((T)exp(-a), (T)0);
-  } else {
-    base = cis_neg<T>(a);
-  }
+    const double b = exp(-a);
 #pragma unroll
-  for (int c = 0; c < D; ++c) s[c] = s[c] * (base * zf[c]);
+    for (int c = 0; c < D; ++c) s[c] = scale64(s[c], b * zf[c].x);
+  } else {
+    double sn, cs;
+    sincos(a, &sn, &cs);
+    const cplx<double> base = mk(cs, -sn);
+#pragma unroll
+    for (int c = 0; c < D; ++c) s[c] = mul64(s[c], base * zf[c]);
+  }
 }
 
 // observables.jl:68-88 spin density: Fz = sum m|psi|^2,
@@ -248,21 +266,24 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
   nF(s, a);
   nF(a, b);
   const double th = c1 * (double)fmag * dtf;
-  T s1, c1m;
+  double s1, c1m;
   if (itp) {
-    s1 = (T)sinh(th);
-    c1m = (T)(cosh(th) - 1.0);
+    s1 = sinh(th);
+    c1m = cosh(th) - 1.0;
   } else {
     double sn, cs;
     sincos(th, &sn, &cs);
-    s1 = (T)sn;
-    c1m = (T)(cs - 1.0);
+    s1 = sn;
+    c1m = cs - 1.0;
   }
+  // combined in FP64 with one rounding per component (FP32-bulk: no systematic bias)
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     // - i s1 a  (real time)   or   - s1 a  (imaginary time)
-    const cplx<T> t = itp ? mk(-a[i].x * s1, -a[i].y * s1) : mk(a[i].y * s1, -a[i].x * s1);
-    s[i] = s[i] + t + b[i] * c1m;
+    const double tx = itp ? -(double)a[i].x * s1 : (double)a[i].y * s1;
+    const double ty = itp ? -(double)a[i].y * s1 : -(double)a[i].x * s1;
+    s[i] = mk((T)((double)s[i].x + tx + (double)b[i].x * c1m),
+              (T)((double)s[i].y + ty + (double)b[i].y * c1m));
   }
 }
 

@@ -34,7 +34,7 @@ struct DevTables {
   const double* Varr;     // [B][nsites] explicit potential or null
   const InstParams* params;  // [B]
   const int* active;      // active instance list
-  const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), T-typed
+  const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), FP64 for every T
   const double* vfy;      // D*D complex (row-major) eigenvectors of Fy
   const float* vfy_f;     // the same matrix rounded to FP32
   const double* tens_cg;  // Clebsch-Gordan table in tens:: order (tensor step), FP64

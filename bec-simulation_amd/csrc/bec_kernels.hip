@@ -100,10 +100,11 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* aux = tile + (size_t)D * TS;
-  cplx<T>* zf = aux + (has_aux ? 2 * TS : 0);
+  // FP64 Zeeman factors (TS is even, so this stays 16-byte aligned)
+  cplx<double>* zf = reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * TS : 0));
   double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
 
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(tb.tw[ilog2c<NX>()]);
+  const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
   const cplx<T>* vfy = sizeof(T) == 8 ? reinterpret_cast<const cplx<T>*>(tb.vfy)
                                        : reinterpret_cast<const cplx<T>*>(tb.vfy_f);
   const int row0 = blockIdx.x * ROWS;
@@ -125,8 +126,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     const Op op = s_ops[i];
     if (used && op.kind == OP_DIAG) {
       const double dtf = op.frac * dt;
-      if (itp) zf[idx] = mk<T>((T)exp(-(P.zee[c] - P.zee_shift) * dtf), (T)0);
-      else zf[idx] = cis_neg<T>(P.zee[c] * dtf);
+      if (itp) zf[idx] = mk(exp(-(P.zee[c] - P.zee_shift) * dtf), 0.0);
+      else zf[idx] = cis_neg<double>(P.zee[c] * dtf);
     }
   }
   // Load the tile: D contiguous chunks of ROWS*NX elements.
@@ -203,7 +204,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       const Op op = ops[i];
       const double dtf = op.frac * dt;
       if (op.kind == OP_DIAG) {
-        const cplx<T>* zfi = &zf[(zbase + i) * D];
+        const cplx<double>* zfi = &zf[(zbase + i) * D];
         for_sites([&](int s) __attribute__((always_inline)) {
           cplx<T> sp[D];
 #pragma unroll
@@ -271,9 +272,9 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       const int x = e % NX;
       const double k = tb.ktab[x];
       const double a = 0.5 * (k * k) * kdt;
-      cplx<T> ph = itp ? mk<T>((T)exp(-a), (T)0) : cis_neg<T>(a);
-      ph = ph * (T)g.inv_ntot;
-      tile[e] = tile[e] * ph;
+      cplx<double> ph = itp ? mk(exp(-a), 0.0) : cis_neg<double>(a);
+      ph = ph * g.inv_ntot;
+      tile[e] = mul64(tile[e], ph);
     }
     __syncthreads();
     fft_team<T, NX, true, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
@@ -419,7 +420,7 @@ template <typename T, int D>
 __global__ void __launch_bounds__(256)
 site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
             const T* __restrict__ phi) {
-  __shared__ cplx<T> zf[kMaxOps * D];
+  __shared__ cplx<double> zf[kMaxOps * D];
   __shared__ Op s_ops[kMaxOps];
   const int tid = threadIdx.x;
   const int inst = tb.active[blockIdx.y];
@@ -435,8 +436,8 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
     const Op op = s_ops[i];
     if (i < prog.n && op.kind == OP_DIAG) {
       const double dtf = op.frac * dt;
-      if (itp) zf[idx] = mk<T>((T)exp(-(P.zee[c] - P.zee_shift) * dtf), (T)0);
-      else zf[idx] = cis_neg<T>(P.zee[c] * dtf);
+      if (itp) zf[idx] = mk(exp(-(P.zee[c] - P.zee_shift) * dtf), 0.0);
+      else zf[idx] = cis_neg<double>(P.zee[c] * dtf);
     }
   }
   __syncthreads();
@@ -488,7 +489,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
   double* tabs = reinterpret_cast<double*>(tile + FIELDS * N * TB);  // [2*(N+TB+1)] (re,im) pairs
   double* rsh = tabs + 2 * (N + TB + 1);
-  const cplx<T>* tw = reinterpret_cast<const cplx<T>*>(tb.tw[ilog2c<N>()]);
+  const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<N>()]);
 
   const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
   const int valid = ddi_data ? g.nxh : g.n[0];
@@ -541,7 +542,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       const cplx<double> po = mk(tabs[2 * (N + TB)], tabs[2 * (N + TB) + 1]);
       cplx<double> ph = (px * pa) * po;
       ph = ph * g.inv_ntot;
-      tile[idx] = tile[idx] * mk((T)ph.x, (T)ph.y);
+      tile[idx] = mul64(tile[idx], ph);
     }
     __syncthreads();
   } else if (MODE == SM_DDI) {
@@ -742,9 +743,8 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
                           const void* src, void* dst, void* fhat, int n_active, void* phi) {
   const int rows = xpass_rows(D, NX, g.nrows);
   const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
-  const size_t lds = sizeof(cplx<T>) * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0) +
-                                        2 * kMaxOps * D) +
-                     16 * sizeof(double);
+  const size_t lds = sizeof(cplx<T>) * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0)) +
+                     sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double);
   const int threads = D * rows * (NX / xpass_e<NX, D>());
   if (threads > xpass_max_threads<D>()) return hipErrorInvalidConfiguration;
   dim3 grid(g.nrows / rows, n_active);
