@@ -68,6 +68,46 @@ __device__ __forceinline__ cplx<T> mul64(cplx<T> a, cplx<double> w) {
   return mk((T)(ax * w.x - ay * w.y), (T)(ax * w.y + ay * w.x));
 }
 
+// The same product for register-heavy callers: in FP32 the factor is split into
+// hi + lo floats and applied with fused multiply-adds (as unbiased as the FP64 form,
+// without FP64 temporaries per component).
+template <typename T>
+__device__ __forceinline__ cplx<T> mul_split(cplx<T> a, cplx<double> f) {
+  if constexpr (sizeof(T) == 8) {
+    return a * f;
+  } else {
+    const float hx = (float)f.x, hy = (float)f.y;
+    const float lx = (float)(f.x - (double)hx), ly = (float)(f.y - (double)hy);
+    return mk<T>(fmaf(a.x, hx, fmaf(-a.y, hy, fmaf(a.x, lx, -a.y * ly))),
+                 fmaf(a.x, hy, fmaf(a.y, hx, fmaf(a.x, ly, a.y * lx))));
+  }
+}
+
+// Sum |s_c|^2 in FP64.
+template <typename T, int D>
+__device__ __forceinline__ double site_norm2(const cplx<T>* s) {
+  double n = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) n += (double)s[c].x * s[c].x + (double)s[c].y * s[c].y;
+  return n;
+}
+
+// FP32-bulk, real time: a per-site unitary step (spin rotation, tensor step) preserves
+// sum_c |psi_c|^2 exactly, but FP32 copies of its fixed matrices are not exactly
+// unitary, and a state whose spin points the same way at every site picks up the same
+// norm error everywhere.  Restore the per-site norm (one FP64 factor).
+template <typename T, int D>
+__device__ __forceinline__ void restore_norm(cplx<T>* s, double n0) {
+  if constexpr (sizeof(T) == 4) {
+    const double n1 = site_norm2<T, D>(s);
+    if (n1 > 0.0) {
+      const double f = sqrt(n0 / n1);
+#pragma unroll
+      for (int c = 0; c < D; ++c) s[c] = scale64(s[c], f);
+    }
+  }
+}
+
 // x / sqrt(2) with a single rounding (the FP32 constant 1/sqrt(2) is 1.7e-8 low).
 template <typename T>
 __device__ __forceinline__ T mul_rsqrt2(T x) {
@@ -212,17 +252,16 @@ __device__ __forceinline__ void op_diag(cplx<T>* s, double Vtrap, const InstPara
   double nd = (double)n;
   double Vint = P.c0 * nd + P.c_lhy * nd * sqrt(nd);
   double a = (Vtrap + Vint) * dtf;
+  cplx<double> base;
   if (itp) {
-    const double b = exp(-a);
-#pragma unroll
-    for (int c = 0; c < D; ++c) s[c] = scale64(s[c], b * zf[c].x);
+    base = mk(exp(-a), 0.0);
   } else {
     double sn, cs;
     sincos(a, &sn, &cs);
-    const cplx<double> base = mk(cs, -sn);
-#pragma unroll
-    for (int c = 0; c < D; ++c) s[c] = mul64(s[c], base * zf[c]);
+    base = mk(cs, -sn);
   }
+#pragma unroll
+  for (int c = 0; c < D; ++c) s[c] = mul_split<T>(s[c], base * zf[c]);
 }
 
 // observables.jl:68-88 spin density: Fz = sum m|psi|^2,
@@ -297,6 +336,7 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
   const cplx<T>* vfy = vfy_in;
   double pm = sqrt(phx * phx + phy * phy + phz * phz);
   if (pm * fabs(dt) < 1e-14) return;
+  const double n0 = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
   double cz = phz / pm;
   cz = cz < -1.0 ? -1.0 : (cz > 1.0 ? 1.0 : cz);
   double beta = acos(cz);
@@ -390,6 +430,7 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
     s[i] = ph * acc;
     ph = ph * z_alpha;
   }
+  if (!itp) restore_norm<T, D>(s, n0);
 }
 
 template <typename T, int D>
@@ -488,6 +529,7 @@ __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, doubl
   if (theta > 0.5) nsub = (int)ceil(theta / 0.5);
   const double hstep = dtf / nsub;
   const T eps = sizeof(T) == 8 ? (T)1e-17 : (T)3e-9;
+  const double n_in = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
   cplx<T> psi0[D], v[D], w[D], acc[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) psi0[c] = s[c];
@@ -513,6 +555,7 @@ __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, doubl
 #pragma unroll
     for (int c = 0; c < D; ++c) s[c] = acc[c];
   }
+  if (!itp) restore_norm<T, D>(s, n_in);
 }
 
 // Tensor interaction energy density 1/2 sum_S g_S sum_M |A_SM|^2 with the pair amplitude
