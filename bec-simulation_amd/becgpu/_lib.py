@@ -24,7 +24,7 @@ class BecConfig(C.Structure):
         ("ndim", C.c_int), ("n", C.c_int * 3), ("L", C.c_double * 3), ("F", C.c_int),
         ("n_instances", C.c_int), ("precision", C.c_int), ("imaginary_time", C.c_int),
         ("normalize_every", C.c_int), ("enable_ddi", C.c_int), ("ddi_secular", C.c_int),
-        ("ddi_padding", C.c_int), ("tensor", C.c_int), ("fy_vecs", C.POINTER(C.c_double)),
+        ("ddi_padding", C.c_int), ("tensor", C.c_int),
         ("x_tab", C.POINTER(C.c_double)), ("k_tab", C.POINTER(C.c_double)),
     ]
 
@@ -65,6 +65,7 @@ SIGNATURES = {
     "bec_itp_guard": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "bec_profile_read": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "bec_profile_work": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_synchronize": ([C.c_void_p], C.c_int),
     "bec_last_error": ([], C.c_char_p),
     "bec_workspace_bytes": ([C.POINTER(BecConfig)], C.c_longlong),

@@ -62,7 +62,6 @@ class Workspace:
         self.precision = _precision_code(precision)
         self.enable_ddi = bool(enable_ddi)
         self.tensor = bool(tensor)
-        sm = spin_matrices(F)
         cfg = L.BecConfig()
         nd = grid.ndim
         cfg.ndim = nd
@@ -78,11 +77,9 @@ class Workspace:
         cfg.ddi_secular = 1 if secular_ddi else 0
         cfg.ddi_padding = 1 if ddi_padding else 0
         cfg.tensor = 1 if tensor else 0
-        vecs = np.ascontiguousarray(sm["fy_vecs"], dtype=np.complex128)
         xt = np.ascontiguousarray(np.concatenate(grid.x), dtype=np.float64)
         kt = np.ascontiguousarray(np.concatenate(grid.k), dtype=np.float64)
-        self._keep = (vecs, xt, kt)
-        cfg.fy_vecs = vecs.ctypes.data_as(C.POINTER(C.c_double))
+        self._keep = (xt, kt)
         cfg.x_tab = xt.ctypes.data_as(C.POINTER(C.c_double))
         cfg.k_tab = kt.ctypes.data_as(C.POINTER(C.c_double))
         self._cfg = cfg
@@ -206,6 +203,16 @@ class Workspace:
         L.check(L.lib().bec_profile_read(self.handle, ms.ctypes.data, cnt.ctypes.data),
                 "bec_profile_read")
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(L.K_NAMES)}
+
+    def profile_work(self):
+        """Work since profile(True) or the last call: lane-per-site pass counts
+        {"tensor_matvecs", "rotations", "flops_per_matvec", "flops_per_rotation", "flops"}
+        and "bytes": algorithmic HBM bytes issued per kernel class."""
+        out = np.zeros(4 + len(L.K_NAMES))
+        L.check(L.lib().bec_profile_work(self.handle, out.ctypes.data), "bec_profile_work")
+        return {"tensor_matvecs": out[0], "rotations": out[1], "flops_per_matvec": out[2],
+                "flops_per_rotation": out[3], "flops": out[0] * out[2] + out[1] * out[3],
+                "bytes": {n: float(out[4 + i]) for i, n in enumerate(L.K_NAMES)}}
 
     def synchronize(self):
         L.check(L.lib().bec_synchronize(self.handle), "bec_synchronize")

@@ -32,8 +32,6 @@ static int fail(int code, const std::string& msg) {
 struct bec_ws {
   Geom g{};
   int F = 0, D = 0, B = 0, prec = 0, itp = 0, normalize_every = 1, ddi = 0, tensor = 0;
-  double* d_tens_cg = nullptr;
-  float* d_tens_cg_f = nullptr;
   size_t csz = 16;
   std::vector<InstParams> hp;
   std::vector<bec_inst_params> user;
@@ -54,10 +52,9 @@ struct bec_ws {
   double* d_varr = nullptr;
   double* d_red = nullptr;
   double* d_partials = nullptr;
-  double* d_vfy = nullptr;
-  float* d_vfy_f = nullptr;
   InstParams* d_params = nullptr;
   int* d_active = nullptr;
+  unsigned long long* d_work = nullptr;  // profiling counters (tensor matvecs, spin rotations)
   void* d_tw[10] = {};
   double* h_red = nullptr;  // pinned
   int max_blocks = 0;
@@ -70,11 +67,15 @@ struct bec_ws {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<int> prof_cls;
+  double prof_bytes[BEC_K_COUNT] = {};  // algorithmic HBM bytes issued per kernel class
 };
 
 // Kernel classes reported by bec_profile_read (order = BEC_K_* in becgpu.h).
-static int prof_begin(bec_ws* ws, int cls) {
+// `bytes`: algorithmic HBM bytes of the launch (each array the pass must touch read
+// and/or written once), accumulated per class for bec_profile_work.
+static int prof_begin(bec_ws* ws, int cls, double bytes = 0.0) {
   if (!ws->prof_on) return -1;
+  ws->prof_bytes[cls] += bytes;
   if (ws->ev_used + 2 > ws->ev_pool.size()) {
     for (int i = 0; i < 256; ++i) {
       hipEvent_t e;
@@ -109,14 +110,11 @@ static void refresh_tables(bec_ws* ws) {
   t.params = ws->d_params;
   t.active = ws->d_active;
   for (int i = 0; i < 10; ++i) t.tw[i] = ws->d_tw[i];
-  t.vfy = ws->d_vfy;
-  t.vfy_f = ws->d_vfy_f;
-  t.tens_cg = ws->tensor ? ws->d_tens_cg : nullptr;
-  t.tens_cg_f = ws->tensor ? ws->d_tens_cg_f : nullptr;
   t.red = ws->d_red;
   t.partials = ws->d_partials;
   t.max_blocks = ws->max_blocks;
   t.maxB = ws->B;
+  t.work = ws->prof_on ? ws->d_work : nullptr;
 }
 
 static int sync_params(bec_ws* ws) {
@@ -128,10 +126,29 @@ static int sync_params(bec_ws* ws) {
   return BEC_OK;
 }
 
-static int n_active(bec_ws* ws) { return (int)ws->active.size(); }
+static int n_active(const bec_ws* ws) { return (int)ws->active.size(); }
+
+static double psi_bytes(const bec_ws* ws) {
+  return (double)ws->D * (double)ws->g.nsites * (double)ws->csz * n_active(ws);
+}
+static double half_field_bytes(const bec_ws* ws) {  // 3 rFFT half-grid fields
+  return 3.0 * ws->g.nxh * (double)(ws->g.nsites / ws->g.n[0]) * (double)ws->csz * n_active(ws);
+}
+static double phi_bytes(const bec_ws* ws) {  // 3 real fields
+  return 3.0 * (double)ws->g.nsites * (ws->csz / 2) * n_active(ws);
+}
 
 static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
-  const int pe = prof_begin(ws, 0);
+  double b = 0.0;
+  if (ws->prof_on) {
+    const double pb = psi_bytes(ws);
+    if (!(p.stages & XS_NO_LOAD)) b += pb;
+    if (!(p.stages & XS_NO_STORE)) b += pb;
+    if (p.stages & (XS_WRITE_FHAT)) b += half_field_bytes(ws);
+    if (p.stages & (XS_LOAD_PHI)) b += half_field_bytes(ws);
+    if (p.stages & (XS_STORE_PHI)) b += phi_bytes(ws);
+  }
+  const int pe = prof_begin(ws, 0, b);
   hipError_t e = ws->prec == BEC_FP32
                      ? launch_xpass<float>(ws->stream, ws->g, ws->tb, p, src, dst, ws->fhat,
                                            n_active(ws), ws->phi)
@@ -142,7 +159,16 @@ static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
   return BEC_OK;
 }
 static int do_sites(bec_ws* ws, const SProgram& p) {
-  const int pe = prof_begin(ws, 8);
+  double b = 0.0;
+  if (ws->prof_on) {
+    b = 2.0 * psi_bytes(ws);
+    for (int i = 0; i < p.n; ++i)
+      if (p.ops[i].kind == OP_DDI_ROT) {
+        b += phi_bytes(ws);
+        break;
+      }
+  }
+  const int pe = prof_begin(ws, 8, b);
   hipError_t e = ws->prec == BEC_FP32
                      ? launch_sites<float>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->phi,
                                            n_active(ws))
@@ -155,7 +181,12 @@ static int do_sites(bec_ws* ws, const SProgram& p) {
 static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_frac, double dt_ovr,
                       int itp, int ddi_data, int* nb = nullptr) {
   hipError_t e;
-  const int pe = prof_begin(ws, ddi_data ? 5 + (mode == SM_DDI ? 1 : 0) : 1 + (mode == SM_KIN ? 1 : mode == SM_INV ? 2 : mode == SM_KE ? 3 : 0));
+  const int cls = ddi_data ? 5 + (mode == SM_DDI ? 1 : 0)
+                           : 1 + (mode == SM_KIN ? 1 : mode == SM_INV ? 2 : mode == SM_KE ? 3 : 0);
+  double b = 0.0;
+  if (ws->prof_on)
+    b = ddi_data ? 2.0 * half_field_bytes(ws) : (mode == SM_KE ? 1.0 : 2.0) * psi_bytes(ws);
+  const int pe = prof_begin(ws, cls, b);
   if (ws->prec == BEC_FP32)
     e = launch_strided<float>(ws->stream, ws->g, ws->tb, axis, mode, data, n_active(ws), kin_frac,
                               dt_ovr, itp, ddi_data, nb);
@@ -271,10 +302,20 @@ struct Sched {
     return BEC_OK;
   }
   // _half_potential_step! (split_step.jl:48-133) for dt_half = f*dt
+  // spin mixing is the identity when |c1| < 1e-30 (spin_mixing.jl:9): with every active
+  // instance in that case the operator (and, for large spins, its lane-per-site pass) is
+  // left out of the schedule
+  bool mixing_active() const {
+    if (ws->tensor) return true;
+    for (int inst : ws->active)
+      if (std::fabs(ws->hp[inst].c1) >= 1e-30) return true;
+    return false;
+  }
   int V(double f) {
     const int mix = ws->tensor ? OP_TENSOR : OP_SPINMIX;
+    const bool mixing = mixing_active();
     CHK(add_op(OP_DIAG, f / 2));
-    CHK(add_op(mix, f / 2));
+    if (mixing) CHK(add_op(mix, f / 2));
     if (ws->ddi) {
       CHK(flush_s());
       if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
@@ -286,7 +327,7 @@ struct Sched {
       CHK(flush());
       CHK(add_op(OP_DDI_ROT, f));
     }
-    CHK(add_op(mix, f / 2));
+    if (mixing) CHK(add_op(mix, f / 2));
     CHK(add_op(OP_DIAG, f / 2));
     return BEC_OK;
   }
@@ -322,7 +363,7 @@ struct Sched {
     CHK(flush_s());
     p.stages |= XS_NORM;
     CHK(flush());
-    CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_NORM));
+    CHK(do_reduce(ws, xpass_grid(ws->g), 1u << RS_NORM));
     scale_pending = true;
     return BEC_OK;
   }
@@ -356,49 +397,36 @@ static int run_items(bec_ws* ws, const std::vector<Item>& items, int itp, double
   return s.finish();
 }
 
-// ---------------------------------------------------------------------------
-// Clebsch-Gordan coefficients, Racah formula in log space (clebsch_gordan.jl:1-73)
-// ---------------------------------------------------------------------------
-static double lfact(int n) { return n < 0 ? -INFINITY : std::lgamma((double)n + 1.0); }
-static double ltri(int a, int b, int c) {
-  const int ab = a + b - c, ac = a - b + c, bc = -a + b + c;
-  if (ab < 0 || ac < 0 || bc < 0) return -INFINITY;
-  return lfact(ab) + lfact(ac) + lfact(bc) - lfact(a + b + c + 1);
-}
-static double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
-  if (m1 + m2 + m3 != 0) return 0.0;
-  const double lt = ltri(j1, j2, j3);
-  if (std::isinf(lt)) return 0.0;
-  const double ln = lfact(j1 + m1) + lfact(j1 - m1) + lfact(j2 + m2) + lfact(j2 - m2) +
-                    lfact(j3 + m3) + lfact(j3 - m3);
-  const int tmin = std::max({0, j2 - j3 - m1, j1 - j3 + m2});
-  const int tmax = std::min({j1 + j2 - j3, j1 - m1, j2 + m2});
-  if (tmin > tmax) return 0.0;
-  double s = 0.0;
-  for (int t = tmin; t <= tmax; ++t) {
-    const double ld = lfact(t) + lfact(j1 + j2 - j3 - t) + lfact(j1 - m1 - t) + lfact(j2 + m2 - t) +
-                      lfact(j3 - j2 + m1 + t) + lfact(j3 - j1 - m2 + t);
-    s += ((t & 1) ? -1.0 : 1.0) * std::exp(-ld);
+// Floating-point operations of one tensor matvec plus its Taylor update, as written in
+// tensor_matvec / op_tensor (complex*complex = 6, complex fma = 8, complex*real fma = 4;
+// zero Clebsch-Gordan entries are skipped at compile time).
+template <int F>
+static double tensor_matvec_flops() {
+  constexpr tens::CGTab<F> cg{};
+  double f = 0.0;
+  for (int M = -2 * F; M <= 2 * F; ++M) {
+    const int ns = tens::ns(F, M);
+    for (int m1 = tens::m1lo(F, M); m1 <= tens::m1hi(M); ++m1) {
+      const bool off = m1 != M - m1;
+      int nz = 0;
+      for (int k = 0; k < ns; ++k) nz += cg.v[tens::off(F, M, m1, tens::s0(M)) + k] != 0.0;
+      f += off ? 6 + 8 : 6;  // pair product(s) psi * v
+      f += 8.0 * nz;         // B_SM and C accumulations
+      f += off ? 16 : 8;     // w update(s) conj(psi) * C
+    }
+    f += 2.0 * ns;  // B_SM *= g_S
   }
-  const double ph = ((j1 - j2 - m3) % 2 == 0) ? 1.0 : -1.0;
-  return ph * std::exp(0.5 * (lt + ln)) * s;
+  return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
 }
-static double clebsch(int j1, int m1, int j2, int m2, int J, int M) {
-  if (m1 + m2 != M) return 0.0;
-  const double ph = ((j1 - j2 + M) % 2 == 0) ? 1.0 : -1.0;
-  return ph * std::sqrt(2.0 * J + 1.0) * wigner3j(j1, j2, J, m1, m2, -M);
-}
-// CG table in the tens:: enumeration order used by the device tensor step.
-static std::vector<double> tensor_cg_table(int F) {
-  std::vector<double> t(tens::size(F), 0.0);
-  for (int M = -2 * F; M <= 2 * F; ++M)
-    for (int m1 = tens::m1lo(F, M); m1 <= tens::m1hi(M); ++m1)
-      for (int S = tens::s0(M); S <= 2 * F; S += 2) {
-        double v = clebsch(F, m1, F, M - m1, S, M);
-        if (std::fabs(v) < 1e-15) v = 0.0;  // clebsch_gordan.jl:174-175
-        t[tens::off(F, M, m1, S)] = v;
-      }
-  return t;
+
+// One Euler rotation (op_euler): 4 products of the real matrix d^F(pi/2) with a complex
+// spinor (4 flop per nonzero entry) and 5 diagonal phase sweeps (12 flop per component).
+template <int F>
+static double euler_flops() {
+  constexpr tens::DHalf<F> d{};
+  int nz = 0;
+  for (double x : d.v) nz += x != 0.0;
+  return 16.0 * nz + 60.0 * (2 * F + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,7 +494,7 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
   if (c->enable_ddi && c->ndim < 2) return fail(BEC_EUNSUP, "DDI needs a 2-D or 3-D grid here");
   if (c->ddi_padding) return fail(BEC_EUNSUP, "zero-padded DDI is not built yet");
-  if (!c->fy_vecs || !c->x_tab || !c->k_tab) return fail(BEC_EINVAL, "missing tables");
+  if (!c->x_tab || !c->k_tab) return fail(BEC_EINVAL, "missing tables");
 
   bec_ws* ws = new bec_ws();
   ws->F = c->F;
@@ -554,29 +582,10 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMemset(ws->d_red, 0, redb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
   const size_t pb = (size_t)RS_COUNT * ws->B * ws->max_blocks * 8;
   if (hipMalloc(&ws->d_partials, pb) != hipSuccess) return cleanup(BEC_ENOMEM, "partials");
+  if (hipMalloc(&ws->d_work, 2 * sizeof(unsigned long long)) != hipSuccess)
+    return cleanup(BEC_ENOMEM, "work counters");
   if (hipMemset(ws->d_partials, 0, pb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
   if (hipHostMalloc(&ws->h_red, redb) != hipSuccess) return cleanup(BEC_ENOMEM, "pinned red");
-  // Fy eigenvectors
-  const size_t vb = (size_t)ws->D * ws->D * 16;
-  if (hipMalloc(&ws->d_vfy, vb) != hipSuccess) return cleanup(BEC_ENOMEM, "vfy");
-  if (hipMemcpy(ws->d_vfy, c->fy_vecs, vb, hipMemcpyHostToDevice) != hipSuccess)
-    return cleanup(BEC_EDEVICE, "vfy copy");
-  {
-    std::vector<float> vf(c->fy_vecs, c->fy_vecs + 2 * ws->D * ws->D);
-    if (hipMalloc(&ws->d_vfy_f, vf.size() * 4) != hipSuccess) return cleanup(BEC_ENOMEM, "vfy");
-    if (hipMemcpy(ws->d_vfy_f, vf.data(), vf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-      return cleanup(BEC_EDEVICE, "vfy copy");
-  }
-  {
-    const std::vector<double> cgt = tensor_cg_table(ws->F);
-    if (hipMalloc(&ws->d_tens_cg, cgt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "cg");
-    if (hipMemcpy(ws->d_tens_cg, cgt.data(), cgt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
-      return cleanup(BEC_EDEVICE, "cg copy");
-    std::vector<float> cgf(cgt.begin(), cgt.end());
-    if (hipMalloc(&ws->d_tens_cg_f, cgf.size() * 4) != hipSuccess) return cleanup(BEC_ENOMEM, "cg");
-    if (hipMemcpy(ws->d_tens_cg_f, cgf.data(), cgf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-      return cleanup(BEC_EDEVICE, "cg copy");
-  }
   // twiddles exp(-2 pi i m / N), computed in long double and stored in FP64 for both
   // precisions (FP32-bulk butterflies multiply by them in FP64 with one rounding)
   for (int l = 3; l <= 9; ++l) {
@@ -606,13 +615,11 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->phi) (void)hipFree(ws->phi);
   for (void* q : {ws->psi64, ws->scratch64, ws->fhat64, ws->phi64})
     if (q) (void)hipFree(q);
-  for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials, ws->d_vfy,
-                    ws->d_tens_cg})
+  for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials})
     if (p) (void)hipFree(p);
-  if (ws->d_tens_cg_f) (void)hipFree(ws->d_tens_cg_f);
-  if (ws->d_vfy_f) (void)hipFree(ws->d_vfy_f);
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
+  if (ws->d_work) (void)hipFree(ws->d_work);
   for (int i = 0; i < 10; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
   if (ws->h_red) (void)hipHostFree(ws->h_red);
@@ -848,7 +855,7 @@ static int energy_partials_fp64(bec_ws* ws, const DevTables& tb, void* psi, void
   if (g.ndim == 1) {
     p.stages = XS_FWD_FFT | XS_KE_1D | XS_NO_STORE;
     CHK(X(p, psi, psi));
-    HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, ws->n_xblocks, 1u << RS_KIN, 0));
+    HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, xpass_grid(g), 1u << RS_KIN, 0));
   } else {
     p.stages = XS_FWD_FFT;
     CHK(X(p, psi, scratch));
@@ -964,7 +971,7 @@ int bec_itp_guard(bec_ws* ws, double* out) {
                                            n_active(ws), 0, nb);
   HIPCHK(e);
   CHK(do_reduce(ws, nb, 1u << RS_KIN));
-  if (ws->ddi) CHK(do_reduce(ws, ws->n_xblocks, 1u << RS_PHIMAX, 1u << RS_PHIMAX));
+  if (ws->ddi) CHK(do_reduce(ws, xpass_grid(ws->g), 1u << RS_PHIMAX, 1u << RS_PHIMAX));
   CHK(fetch_red(ws));
   for (int inst : ws->active) {
     out[2 * inst] = ws->h_red[(size_t)inst * RS_COUNT + RS_KIN];
@@ -979,6 +986,26 @@ int bec_profile_enable(bec_ws* ws, int on) {
   ws->prof_on = on != 0;
   ws->ev_used = 0;
   ws->prof_cls.clear();
+  HIPCHK(hipMemset(ws->d_work, 0, 2 * sizeof(unsigned long long)));
+  for (double& b : ws->prof_bytes) b = 0.0;
+  refresh_tables(ws);
+  return BEC_OK;
+}
+
+int bec_profile_work(bec_ws* ws, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  unsigned long long h[2] = {0, 0};
+  HIPCHK(hipMemcpy(h, ws->d_work, sizeof(h), hipMemcpyDeviceToHost));
+  out[0] = (double)h[0];
+  out[1] = (double)h[1];
+  out[2] = ws->tensor && ws->F == 6 ? tensor_matvec_flops<6>() : 0.0;
+  out[3] = ws->F == 6 ? euler_flops<6>() : ws->F == 2 ? euler_flops<2>() : euler_flops<1>();
+  for (int k = 0; k < BEC_K_COUNT; ++k) {
+    out[4 + k] = ws->prof_bytes[k];
+    ws->prof_bytes[k] = 0.0;
+  }
+  HIPCHK(hipMemset(ws->d_work, 0, sizeof(h)));
   return BEC_OK;
 }
 

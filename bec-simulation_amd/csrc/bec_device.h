@@ -42,13 +42,57 @@ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) {
   return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
 }
 
-__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
-__device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
+// Out-of-line library sine/cosine for the rare arguments |a| > 1e6 (and NaN), so that
+// its large-argument reduction does not set the register budget of the calling kernels.
+static __device__ __noinline__ void sincos_lib(double a, double* s, double* c) { sincos(a, s, c); }
+
+// sin and cos of a double.  Cody-Waite reduction a = k pi/2 + x with pi/2 split into three
+// doubles (FMA steps, exact enough for |k| < 2^20), then the Taylor series of sin (to x^17)
+// and cos (to x^18) on |x| <= pi/4 in Horner form (truncation < 1e-19); the quadrant k mod 4
+// selects and signs the pair.  Measured against long-double sinl/cosl: <= 1.5 ulp for
+// |a| <= 1e6.  The phase arguments of the split-step operators are far below that bound.
+__device__ __forceinline__ void sincos_d(double a, double* s, double* c) {
+  if (!(fabs(a) <= 1.0e6)) {
+    sincos_lib(a, s, c);
+    return;
+  }
+  const double k = rint(a * 0.6366197723675814);  // 2/pi
+  double x = fma(-k, 1.5707963267948966, a);       // pi/2 = hi + mid + lo
+  x = fma(-k, 6.123233995736766e-17, x);
+  x = fma(-k, -1.4973849048591698e-33, x);
+  const double z = x * x;
+  double ps = 2.8114572543455206e-15;  // 1/17!, alternating signs down to -1/3!
+  ps = fma(ps, z, -7.647163731819816e-13);
+  ps = fma(ps, z, 1.6059043836821613e-10);
+  ps = fma(ps, z, -2.505210838544172e-08);
+  ps = fma(ps, z, 2.7557319223985893e-06);
+  ps = fma(ps, z, -0.0001984126984126984);
+  ps = fma(ps, z, 0.008333333333333333);
+  ps = fma(ps, z, -0.16666666666666666);
+  const double sn = fma(x * z, ps, x);
+  double pc = -1.5619206968586225e-16;  // -1/18!, alternating signs down to -1/2!
+  pc = fma(pc, z, 4.779477332387385e-14);
+  pc = fma(pc, z, -1.1470745597729725e-11);
+  pc = fma(pc, z, 2.08767569878681e-09);
+  pc = fma(pc, z, -2.755731922398589e-07);
+  pc = fma(pc, z, 2.48015873015873e-05);
+  pc = fma(pc, z, -0.001388888888888889);
+  pc = fma(pc, z, 0.041666666666666664);
+  pc = fma(pc, z, -0.5);
+  const double cs = fma(z, pc, 1.0);
+  const int q = (int)(long long)k & 3;
+  double rs = (q & 1) ? cs : sn, rc = (q & 1) ? sn : cs;
+  if (q == 1 || q == 2) rc = -rc;
+  if (q & 2) rs = -rs;
+  *s = rs;
+  *c = rc;
+}
+
 // cis(-a) for a phase argument that is always formed in FP64
 template <typename T>
 __device__ __forceinline__ cplx<T> cis_neg(double a) {
   double s, c;
-  sincos(a, &s, &c);
+  sincos_d(a, &s, &c);
   return mk<T>((T)c, (T)(-s));
 }
 
@@ -179,7 +223,14 @@ template <> struct RadixPlan<512> { static constexpr int np = 3; static constexp
 // One Stockham pass: butterflies j = t + b*(N/E), b < E/R.
 // in[j + r*N/R] -> twiddle W_N^{(j mod Ns) r N/(Ns R)} -> DFT_R -> out[(j/Ns)Ns R + j mod Ns + r Ns]
 // The line lives at base[e*ES]; tw[m] = exp(-2 pi i m / N) in FP64 for every T.
-template <typename T, int N, int R, bool INV, int E = 8>
+// Index of element i of a line: i * ES, or with PAD (unit-stride lines) i + i/8, one pad
+// slot per 8 elements so that the stride-R writes of the first pass hit distinct LDS banks.
+template <bool PAD>
+__device__ __forceinline__ int lds_at(int i, int ES) {
+  return PAD ? i + (i >> 3) : i * ES;
+}
+
+template <typename T, int N, int R, bool INV, int E = 8, bool PAD = false>
 __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool active, int Ns,
                                               const cplx<double>* __restrict__ tw) {
   constexpr int NB = E / R;      // butterflies per thread
@@ -190,7 +241,7 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
     for (int b = 0; b < NB; ++b) {
       int j = t + b * TEAM;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b * R + r] = base[(j + r * (N / R)) * ES];
+      for (int r = 0; r < R; ++r) v[b * R + r] = base[lds_at<PAD>(j + r * (N / R), ES)];
     }
   }
   __syncthreads();
@@ -200,18 +251,21 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
       int j = t + b * TEAM;
       int k = j % Ns;
       if (Ns > 1) {
-        int stepm = k * (N / (Ns * R));
+        // one table load per butterfly; the other twiddles are its powers (FP64), which
+        // keeps the per-element cache traffic of the pass at the data itself
+        cplx<double> w1 = tw[k * (N / (Ns * R))];
+        if (INV) w1.y = -w1.y;
+        cplx<double> wr = w1;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cplx<double> w = tw[r * stepm];
-          if (INV) w.y = -w.y;
-          v[b * R + r] = mul64(v[b * R + r], w);
+          v[b * R + r] = mul64(v[b * R + r], wr);
+          if (r + 1 < R) wr = wr * w1;
         }
       }
       dftR<T, R, INV>(&v[b * R]);
       int o = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[(o + r * Ns) * ES] = v[b * R + r];
+      for (int r = 0; r < R; ++r) base[lds_at<PAD>(o + r * Ns, ES)] = v[b * R + r];
     }
   }
   __syncthreads();
@@ -220,20 +274,20 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
 // Full length-N FFT (unnormalised) of the line at base[e*ES], in place in LDS.
 // Every thread of the block must call it (it synchronises); `active` marks the
 // threads that own a (line, team index) pair.
-template <typename T, int N, bool INV, int E = 8>
+template <typename T, int N, bool INV, int E = 8, bool PAD = false>
 __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool active,
                                          const cplx<double>* __restrict__ tw) {
   static_assert(N % E == 0, "line shorter than the per-thread register block");
   using P = RadixPlan<N>;
   int Ns = 1;
-  stockham_pass<T, N, P::r[0], INV, E>(base, ES, t, active, Ns, tw);
+  stockham_pass<T, N, P::r[0], INV, E, PAD>(base, ES, t, active, Ns, tw);
   Ns *= P::r[0];
   if constexpr (P::np > 1) {
-    stockham_pass<T, N, P::r[1], INV, E>(base, ES, t, active, Ns, tw);
+    stockham_pass<T, N, P::r[1], INV, E, PAD>(base, ES, t, active, Ns, tw);
     Ns *= P::r[1];
   }
   if constexpr (P::np > 2) {
-    stockham_pass<T, N, P::r[2], INV, E>(base, ES, t, active, Ns, tw);
+    stockham_pass<T, N, P::r[2], INV, E, PAD>(base, ES, t, active, Ns, tw);
   }
 }
 
@@ -243,23 +297,23 @@ __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool acti
 
 // propagators.jl:277-314: n from the pre-update spinor; ITP uses real
 // exponentials with the Zeeman shift min_c E_c, real time uses cis.
+// Common factor of all components: cis(-(V + c0 n + c_lhy n^1.5) dt) or its ITP exponential.
+__device__ __forceinline__ cplx<double> diag_base(double nd, double Vtrap, const InstParams& P,
+                                                  double dtf, bool itp) {
+  const double a = (Vtrap + P.c0 * nd + P.c_lhy * nd * sqrt(nd)) * dtf;
+  if (itp) return mk(exp(-a), 0.0);
+  double sn, cs;
+  sincos_d(a, &sn, &cs);
+  return mk(cs, -sn);
+}
+
 template <typename T, int D>
 __device__ __forceinline__ void op_diag(cplx<T>* s, double Vtrap, const InstParams& P, double dtf,
                                         bool itp, const cplx<double>* zf) {
   T n = 0;
 #pragma unroll
   for (int c = 0; c < D; ++c) n += abs2(s[c]);
-  double nd = (double)n;
-  double Vint = P.c0 * nd + P.c_lhy * nd * sqrt(nd);
-  double a = (Vtrap + Vint) * dtf;
-  cplx<double> base;
-  if (itp) {
-    base = mk(exp(-a), 0.0);
-  } else {
-    double sn, cs;
-    sincos(a, &sn, &cs);
-    base = mk(cs, -sn);
-  }
+  const cplx<double> base = diag_base((double)n, Vtrap, P, dtf, itp);
 #pragma unroll
   for (int c = 0; c < D; ++c) s[c] = mul_split<T>(s[c], base * zf[c]);
 }
@@ -311,7 +365,7 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
     c1m = cosh(th) - 1.0;
   } else {
     double sn, cs;
-    sincos(th, &sn, &cs);
+    sincos_d(th, &sn, &cs);
     s1 = sn;
     c1m = cs - 1.0;
   }
@@ -326,65 +380,100 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
   }
 }
 
-// spinor_utils.jl:180-264: exp(-i dt phi.F) via Rz(a) Ry(b) Dz(th) Ry(-b) Rz(-a), with
-// Ry(+-b) = V_Fy diag(cis(-+b lambda)) V_Fy^dagger, lambda = -F..F ascending, and the
-// reference's cis recurrences.  ITP: Dz = exp(-m th).  vfy: D*D complex row-major V_Fy.
-template <typename T, int D>
-__device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, double phz, double dt,
-                                         bool itp, const cplx<T>* __restrict__ vfy_in) {
+template <int F>
+inline constexpr tens::DHalf<F> kDHalf{};
+
+// w = d v (TRANS false) or w = d^T v with d = d^F(pi/2): a real matrix with
+// compile-time entries (zeros skipped), so each entry costs two FMAs on literals.
+template <typename T, int D, bool TRANS>
+__device__ __forceinline__ void dhalf_apply(const cplx<T>* v, cplx<T>* w) {
   constexpr int F = (D - 1) / 2;
-  const cplx<T>* vfy = vfy_in;
-  double pm = sqrt(phx * phx + phy * phy + phz * phz);
-  if (pm * fabs(dt) < 1e-14) return;
-  const double n0 = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
-  double cz = phz / pm;
-  cz = cz < -1.0 ? -1.0 : (cz > 1.0 ? 1.0 : cz);
-  double beta = acos(cz);
-  double alpha = atan2(phy, phx);
-  double theta = pm * dt;
-  double sa, ca, sb, cb, sFa, cFa, sFb, cFb;
-  sincos(alpha, &sa, &ca);
-  sincos(beta, &sb, &cb);
-  sincos((double)F * alpha, &sFa, &cFa);
-  sincos((double)F * beta, &sFb, &cFb);
-  cplx<T> z_neg_alpha = mk((T)ca, (T)(-sa));
-  cplx<T> z_beta = mk((T)cb, (T)sb);
-  cplx<T> rz_phase = mk((T)cFa, (T)sFa);
-  cplx<T> ry_phase = mk((T)cFb, (T)(-sFb));
-  cplx<T> v[D], w[D];
-  cplx<T> ph = rz_phase;
+  static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+    constexpr int i = decltype(ic)::value;
+    cplx<T> acc = mk((T)0, (T)0);
+    static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+      constexpr int j = decltype(jc)::value;
+      constexpr T c = (T)kDHalf<F>.v[TRANS ? j * D + i : i * D + j];
+      if constexpr (c != (T)0) acc = mk(acc.x + c * v[j].x, acc.y + c * v[j].y);
+    });
+    w[i] = acc;
+  });
+}
+
+// v_c *= ph * z^c for c = 0..D-1 (a diagonal phase sweep by recurrence).
+template <typename T, int D>
+__device__ __forceinline__ void phase_sweep(cplx<T>* v, cplx<T> ph, cplx<T> z) {
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    v[c] = ph * s[c];
-    ph = ph * z_neg_alpha;
+    v[c] = ph * v[c];
+    ph = ph * z;
   }
-  ph = ry_phase;
+}
+
+// spinor_utils.jl:82-166: exp(-i dt phi.F) = Rz(alpha) Ry(beta) Dz(theta) Ry(-beta) Rz(-alpha),
+// beta = acos(phi_z/|phi|), alpha = atan2(phi_y, phi_x), theta = |phi| dt; ITP: Dz = exp(-m theta).
+// The reference forms Ry(+-beta) from a numerical eigenbasis V_Fy of F_y; here the eigenbasis
+// is the closed form U = Rz(pi/2) d^F(pi/2), so that, with a = alpha + pi/2,
+//   R = Rz(a) d Rz(beta) d^T Dz(theta) d Rz(-beta) d^T Rz(-a),   Rz(x) = diag(e^{-i x m}),
+// four real D x D matvecs and five phase sweeps (cis recurrences as in the reference).
+// Returns 1 if the rotation was applied (0: |phi| |dt| < 1e-14, identity).
+template <typename T, int D>
+__device__ __forceinline__ int op_euler(cplx<T>* s, double phx, double phy, double phz, double dt,
+                                        bool itp) {
+  constexpr int F = (D - 1) / 2;
+  const double pm = sqrt(phx * phx + phy * phy + phz * phz);
+  if (pm * fabs(dt) < 1e-14) return 0;
+  const double n0 = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
+  // cos/sin of beta and alpha from phi itself (no acos/atan2): cos beta = phi_z/|phi| (clamped),
+  // sin beta = rho/|phi| >= 0, (cos alpha, sin alpha) = (phi_x, phi_y)/rho, with atan2's
+  // signed-zero convention when rho = 0.  cos/sin of F*x by De Moivre powers.
+  const double rho = sqrt(phx * phx + phy * phy);
+  double cb = phz / pm;
+  cb = cb < -1.0 ? -1.0 : (cb > 1.0 ? 1.0 : cb);
+  const double sb = rho / pm;
+  double cal, sal;
+  if (rho > 0.0) {
+    cal = phx / rho;
+    sal = phy / rho;
+  } else {
+    cal = signbit(phx) ? -1.0 : 1.0;
+    sal = 0.0;
+  }
+  const double ca = -sal, sa = cal;  // a = alpha + pi/2
+  auto powF = [](double c, double sn, double& cF, double& sF) __attribute__((always_inline)) {
+    double rc = 1.0, rs = 0.0, bc = c, bs = sn;
 #pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const cplx<T>* m = vfy;
-    cplx<T> acc = mk((T)0, (T)0);
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      const cplx<T> q = m[j * D + i];  // Vt[i][j] = conj(V[j][i])
-      acc = acc + mk(q.x, -q.y) * v[j];
+    for (int e = F; e > 0; e >>= 1) {
+      if (e & 1) {
+        const double t = rc * bc - rs * bs;
+        rs = rc * bs + rs * bc;
+        rc = t;
+      }
+      const double t = bc * bc - bs * bs;
+      bs = 2.0 * bc * bs;
+      bc = t;
     }
-    w[i] = ph * acc;
-    ph = ph * z_beta;
-  }
+    cF = rc;
+    sF = rs;
+  };
+  const double theta = pm * dt;
+  double sFa, cFa, sFb, cFb;
+  powF(ca, sa, cFa, sFa);
+  powF(cb, sb, cFb, sFb);
+  cplx<T> v[D], w[D];
 #pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const cplx<T>* m = vfy;
-    cplx<T> acc = mk((T)0, (T)0);
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      const cplx<T> q = m[i * D + j];
-      acc = acc + q * w[j];
-    }
-    v[i] = acc;
-  }
+  for (int c = 0; c < D; ++c) v[c] = s[c];
+  phase_sweep<T, D>(v, mk((T)cFa, (T)sFa), mk((T)ca, (T)(-sa)));  // Rz(-a): e^{+i a m}
+  dhalf_apply<T, D, true>(v, w);
+  phase_sweep<T, D>(w, mk((T)cFb, (T)sFb), mk((T)cb, (T)(-sb)));  // Rz(-beta)
+  dhalf_apply<T, D, false>(w, v);
   if (itp) {
-    T dz = (T)exp(-(double)F * theta);
-    T st = (T)exp(theta);
+    const double et = exp(theta);
+    double eF = 1.0;
+#pragma unroll
+    for (int k = 0; k < F; ++k) eF *= et;
+    T dz = (T)(1.0 / eF);
+    const T st = (T)et;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       v[c] = v[c] * dz;
@@ -392,57 +481,31 @@ __device__ __forceinline__ void op_euler(cplx<T>* s, double phx, double phy, dou
     }
   } else {
     double sF, cF, st, ct;
-    sincos((double)F * theta, &sF, &cF);
-    sincos(theta, &st, &ct);
-    cplx<T> dz = mk((T)cF, (T)(-sF));
-    cplx<T> zt = mk((T)ct, (T)st);
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      v[c] = v[c] * dz;
-      dz = dz * zt;
-    }
+    sincos_d(theta, &st, &ct);
+    powF(ct, st, cF, sF);
+    phase_sweep<T, D>(v, mk((T)cF, (T)(-sF)), mk((T)ct, (T)st));  // e^{-i theta m}
   }
-  ph = conjc(ry_phase);
-  cplx<T> z_neg_beta = conjc(z_beta);
+  dhalf_apply<T, D, true>(v, w);
+  phase_sweep<T, D>(w, mk((T)cFb, (T)(-sFb)), mk((T)cb, (T)sb));  // Rz(beta)
+  dhalf_apply<T, D, false>(w, v);
+  phase_sweep<T, D>(v, mk((T)cFa, (T)(-sFa)), mk((T)ca, (T)sa));  // Rz(a)
 #pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const cplx<T>* m = vfy;
-    cplx<T> acc = mk((T)0, (T)0);
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      const cplx<T> q = m[j * D + i];
-      acc = acc + mk(q.x, -q.y) * v[j];
-    }
-    w[i] = ph * acc;
-    ph = ph * z_neg_beta;
-  }
-  ph = conjc(rz_phase);
-  cplx<T> z_alpha = conjc(z_neg_alpha);
-#pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const cplx<T>* m = vfy;
-    cplx<T> acc = mk((T)0, (T)0);
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      const cplx<T> q = m[i * D + j];
-      acc = acc + q * w[j];
-    }
-    s[i] = ph * acc;
-    ph = ph * z_alpha;
-  }
+  for (int c = 0; c < D; ++c) s[c] = v[c];
   if (!itp) restore_norm<T, D>(s, n0);
+  return 1;
 }
 
+// Returns the number of Euler rotations applied (0 or 1; Rodrigues steps count 0).
 template <typename T, int D>
-__device__ __forceinline__ void op_spin_mixing(cplx<T>* s, const InstParams& P, double dtf, bool itp,
-                                               const cplx<T>* __restrict__ vfy) {
-  if (fabs(P.c1) < 1e-30) return;  // spin_mixing.jl:9
+__device__ __forceinline__ int op_spin_mixing(cplx<T>* s, const InstParams& P, double dtf, bool itp) {
+  if (fabs(P.c1) < 1e-30) return 0;  // spin_mixing.jl:9
   if constexpr (D == 3) {
     op_rodrigues<T>(s, P.c1, dtf, itp);
+    return 0;
   } else {
     T fx, fy, fz;
     spin_vec<T, D>(s, fx, fy, fz);
-    op_euler<T, D>(s, P.c1 * (double)fx, P.c1 * (double)fy, P.c1 * (double)fz, dtf, itp, vfy);
+    return op_euler<T, D>(s, P.c1 * (double)fx, P.c1 * (double)fy, P.c1 * (double)fz, dtf, itp);
   }
 }
 
@@ -458,6 +521,10 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
   }
 }
 
+// Clebsch-Gordan table of spin F in tens:: order, evaluated by the compiler.
+template <int F>
+inline constexpr tens::CGTab<F> kCG{};
+
 // w = h v with the mean-field matrix of the spinor psi,
 //   h_{m m'} = sum_S g_S sum_M CG(m, M-m | S M) CG(m', M-m' | S M) conj(psi_{M-m}) psi_{M-m'},
 // evaluated in factorised form (no D x D matrix is formed):
@@ -466,7 +533,7 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
 // cgt: table in tens:: order; every index below is a compile-time constant.
 template <typename T, int D>
 __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>* v, cplx<T>* w,
-                                              const T* __restrict__ cgt, const T* g) {
+                                              const T* g) {
   constexpr int F = (D - 1) / 2;
 #pragma unroll
   for (int c = 0; c < D; ++c) w[c] = mk((T)0, (T)0);
@@ -484,8 +551,11 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       constexpr int m2 = M - m1;
       cplx<T> p = psi[F - m2] * v[F - m1];
       if constexpr (m1 != m2) p = p + psi[F - m1] * v[F - m2];
-#pragma unroll
-      for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] + p * cgt[tens::off(F, M, m1, S0) + k];
+      static_for<0, NS>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        constexpr T c = (T)kCG<F>.v[tens::off(F, M, m1, S0) + k];
+        if constexpr (c != (T)0) Bs[k] = Bs[k] + p * c;
+      });
     });
 #pragma unroll
     for (int k = 0; k < NS; ++k) Bs[k] = Bs[k] * g[S0 / 2 + k];
@@ -493,8 +563,11 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       constexpr int m1 = decltype(m1c)::value;
       constexpr int m2 = M - m1;
       cplx<T> C = mk((T)0, (T)0);
-#pragma unroll
-      for (int k = 0; k < NS; ++k) C = C + Bs[k] * cgt[tens::off(F, M, m1, S0) + k];
+      static_for<0, NS>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        constexpr T c = (T)kCG<F>.v[tens::off(F, M, m1, S0) + k];
+        if constexpr (c != (T)0) C = C + Bs[k] * c;
+      });
       w[F - m1] = w[F - m1] + conjc(psi[F - m2]) * C;
       if constexpr (m1 != m2) w[F - m2] = w[F - m2] + conjc(psi[F - m1]) * C;
     });
@@ -510,13 +583,12 @@ __device__ __forceinline__ T cabs1(cplx<T> a) { return fabs(a.x) + fabs(a.y); }
 // nsub pieces when the norm bound theta = 2 |dt| n sum_S |g_S| exceeds 1/2.  Sites with
 // n < 1e-20 are left unchanged (tensor_interaction.jl:212).
 template <typename T, int D>
-__device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp,
-                                          const T* __restrict__ cgt) {
+__device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp) {
   constexpr int F = (D - 1) / 2;
   double n = 0.0;
 #pragma unroll
   for (int c = 0; c < D; ++c) n += (double)abs2(s[c]);
-  if (n < 1e-20) return;
+  if (n < 1e-20) return 0;
   T g[F + 1];
   double gsum = 0.0;
 #pragma unroll
@@ -531,6 +603,7 @@ __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, doubl
   const T eps = sizeof(T) == 8 ? (T)1e-17 : (T)3e-9;
   const double n_in = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
   cplx<T> psi0[D], v[D], w[D], acc[D];
+  int nmv = 0;
 #pragma unroll
   for (int c = 0; c < D; ++c) psi0[c] = s[c];
   for (int sub = 0; sub < nsub; ++sub) {
@@ -540,7 +613,8 @@ __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, doubl
       acc[c] = s[c];
     }
     for (int k = 1; k <= 60; ++k) {
-      tensor_matvec<T, D>(psi0, v, w, cgt, g);
+      tensor_matvec<T, D>(psi0, v, w, g);
+      ++nmv;
       const T f = (T)(hstep / k);
       T nv = 0, na = 0;
 #pragma unroll
@@ -556,13 +630,13 @@ __device__ __forceinline__ void op_tensor(cplx<T>* s, const InstParams& P, doubl
     for (int c = 0; c < D; ++c) s[c] = acc[c];
   }
   if (!itp) restore_norm<T, D>(s, n_in);
+  return nmv;
 }
 
 // Tensor interaction energy density 1/2 sum_S g_S sum_M |A_SM|^2 with the pair amplitude
 // A_SM = sum_{m1+m2=M} CG(m1, m2 | S M) psi_m1 psi_m2 (tensor_interaction.jl:247-276).
 template <typename T, int D>
-__device__ __forceinline__ double tensor_energy_site(const cplx<T>* s, const InstParams& P,
-                                                     const double* __restrict__ cgt) {
+__device__ __forceinline__ double tensor_energy_site(const cplx<T>* s, const InstParams& P) {
   constexpr int F = (D - 1) / 2;
   double e = 0.0;
   static_for<-2 * F, 2 * F + 1>([&](auto Mc) __attribute__((always_inline)) {
@@ -581,12 +655,12 @@ __device__ __forceinline__ double tensor_energy_site(const cplx<T>* s, const Ins
       double pr = (double)a.x * b.x - (double)a.y * b.y;
       double pi = (double)a.x * b.y + (double)a.y * b.x;
       if constexpr (m1 != m2) { pr *= 2.0; pi *= 2.0; }
-#pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        const double c = cgt[tens::off(F, M, m1, S0) + k];
+      static_for<0, NS>([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        constexpr double c = kCG<F>.v[tens::off(F, M, m1, S0) + k];
         ar[k] += c * pr;
         ai[k] += c * pi;
-      }
+      });
     });
 #pragma unroll
     for (int k = 0; k < NS; ++k) e += 0.5 * P.gS[S0 / 2 + k] * (ar[k] * ar[k] + ai[k] * ai[k]);

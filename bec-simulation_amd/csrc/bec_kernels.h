@@ -35,12 +35,9 @@ struct DevTables {
   const InstParams* params;  // [B]
   const int* active;      // active instance list
   const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), FP64 for every T
-  const double* vfy;      // D*D complex (row-major) eigenvectors of Fy
-  const float* vfy_f;     // the same matrix rounded to FP32
-  const double* tens_cg;  // Clebsch-Gordan table in tens:: order (tensor step), FP64
-  const float* tens_cg_f; // the same table rounded to FP32 (FP32-bulk tensor step)
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
+  unsigned long long* work;  // profiling only (else null): [0] tensor matvecs, [1] spin rotations
   int max_blocks;
   int maxB;
 };
@@ -83,6 +80,8 @@ constexpr int strided_tb_c(int N, int fields, int csize) {
 }
 int strided_tb(int N, int fields, int csize);
 int xpass_rows(int D, int NX, int nrows);
+// Blocks per instance of an x-pass launch (= number of per-block partial sums it writes).
+int xpass_grid(const Geom& g);
 
 // dst[i] = (double)src[i], i < n (FP32 psi widened for FP64 energy evaluation)
 hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n);

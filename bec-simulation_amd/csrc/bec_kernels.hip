@@ -13,6 +13,7 @@
 // so results do not depend on what else runs on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 
 #include "bec_device.h"
@@ -83,6 +84,7 @@ constexpr int xpass_e() { return 8; }
 
 template <typename T, int NX, int D>
 __global__ void __launch_bounds__(xpass_max_threads<D>())
+    __attribute__((amdgpu_waves_per_eu(D == 13 && sizeof(T) == 8 ? 3 : 4)))
 xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
              cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, T* __restrict__ phi, int ROWS) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -97,16 +99,18 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   const int st = prog.stages;
   const bool has_aux = (st & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
   const int TS = ROWS * NX;  // sites in this tile
+  // LDS tiles are padded (one slot per 8 elements, lds_at<true>): element e of the tile
+  // lives at P(e) = e + e/8; lines start at multiples of NX, so P(line*NX + i) =
+  // P(line*NX) + P(i).
+  auto P8 = [](int e) __attribute__((always_inline)) { return e + (e >> 3); };
 
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
-  cplx<T>* aux = tile + (size_t)D * TS;
-  // FP64 Zeeman factors (TS is even, so this stays 16-byte aligned)
-  cplx<double>* zf = reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * TS : 0));
+  cplx<T>* aux = tile + (size_t)D * TS / 8 * 9;
+  // FP64 Zeeman factors (TS/8*9 is even, so this stays 16-byte aligned)
+  cplx<double>* zf = reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * TS / 8 * 9 : 0));
   double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
 
   const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
-  const cplx<T>* vfy = sizeof(T) == 8 ? reinterpret_cast<const cplx<T>*>(tb.vfy)
-                                       : reinterpret_cast<const cplx<T>*>(tb.vfy_f);
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
 
@@ -134,18 +138,19 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   if (!(st & XS_NO_LOAD)) {
     for (int e = tid; e < D * TS; e += nthr) {
       const int c = e / TS, r = e - c * TS;
-      tile[e] = src[ibase + (size_t)c * g.nsites + r];
+      tile[P8(e)] = src[ibase + (size_t)c * g.nsites + r];
     }
   }
   __syncthreads();
+  double norm_acc = 0.0, ke_acc = 0.0, phimax = 0.0;
 
   const int line = tid / TEAM, tt = tid - (tid / TEAM) * TEAM;
-  if (st & XS_INV_FFT) fft_team<T, NX, true, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+  if (st & XS_INV_FFT)
+    fft_team<T, NX, true, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
 
   // ITP renormalisation factor of the instance (only used when an OP_SCALE is present)
   const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 
-  double phimax = 0.0;
   if (st & XS_LOAD_PHI) {
     // C2R along x of the three DDI potential spectra (reference irfft semantics: bins
     // 0..NX/2 with the imaginary parts of DC and Nyquist dropped, Hermitian extension).
@@ -162,11 +167,11 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       } else if (x > NX / 2) {
         yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
       }
-      aux[(2 * r) * NX + x] = mk(yx.x - yy.y, yx.y + yy.x);
-      aux[(2 * r + 1) * NX + x] = yz;
+      aux[P8((2 * r) * NX + x)] = mk(yx.x - yy.y, yx.y + yy.x);
+      aux[P8((2 * r + 1) * NX + x)] = yz;
     }
     __syncthreads();
-    fft_team<T, NX, true, EX>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
+    fft_team<T, NX, true, EX, true>(aux + P8(line * NX), 1, tt, line < 2 * ROWS, tw);
   }
 
   auto site_coords = [&](int s, int& x, int& y, int& z) {
@@ -206,35 +211,31 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       if (op.kind == OP_DIAG) {
         const cplx<double>* zfi = &zf[(zbase + i) * D];
         for_sites([&](int s) __attribute__((always_inline)) {
-          cplx<T> sp[D];
+          // op_diag with the components left in LDS: a density sweep, then the phase
+          // products (the barrier makes the second sweep re-read LDS instead of keeping
+          // all D values live in VGPRs)
+          T n = 0;
 #pragma unroll
-          for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
-          op_diag<T, D>(sp, trap_at(s), P, dtf, itp, zfi);
+          for (int c = 0; c < D; ++c) n += abs2(tile[P8(c * TS + s)]);
+          const cplx<double> base = diag_base((double)n, trap_at(s), P, dtf, itp);
+          __asm__ volatile("" ::: "memory");
 #pragma unroll
-          for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
+          for (int c = 0; c < D; ++c) {
+            const int e = P8(c * TS + s);
+            tile[e] = mul_split<T>(tile[e], base * zfi[c]);
+          }
         });
       } else if (op.kind == OP_SPINMIX && D == 3) {
         for_sites([&](int s) __attribute__((always_inline)) {
           cplx<T> sp[D];
 #pragma unroll
-          for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
-          op_spin_mixing<T, D>(sp, P, dtf, itp, vfy);
+          for (int c = 0; c < D; ++c) sp[c] = tile[P8(c * TS + s)];
+          op_spin_mixing<T, D>(sp, P, dtf, itp);
 #pragma unroll
-          for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
-        });
-      } else if (op.kind == OP_DDI_ROT && D < 0) {  // runs in the lane-per-site kernel
-        for_sites([&](int s) __attribute__((always_inline)) {
-          cplx<T> sp[D];
-#pragma unroll
-          for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
-          const int r = s / NX, x = s - r * NX;
-          const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
-          op_euler<T, D>(sp, (double)a.x, (double)a.y, (double)b.x, dtf, itp, vfy);
-#pragma unroll
-          for (int c = 0; c < D; ++c) tile[c * TS + s] = sp[c];
+          for (int c = 0; c < D; ++c) tile[P8(c * TS + s)] = sp[c];
         });
       } else if (op.kind == OP_SCALE) {
-        for (int e = tid; e < D * TS; e += nthr) tile[e] = scale64(tile[e], sc);
+        for (int e = tid; e < D * TS; e += nthr) tile[P8(e)] = scale64(tile[P8(e)], sc);
         __syncthreads();  // the scale loop's element ownership differs from the site loops
       }
     }
@@ -245,18 +246,16 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     // max |Phi| for the imaginary-time overflow guard (ground_state.jl:4-29)
     for (int s = tid; s < TS; s += nthr) {
       const int r = s / NX, x = s - r * NX;
-      const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
+      const cplx<T> a = aux[P8((2 * r) * NX + x)], b = aux[P8((2 * r + 1) * NX + x)];
       phimax = fmax(phimax, fmax(fabs((double)a.x), fmax(fabs((double)a.y), fabs((double)b.x))));
     }
-    const double m = block_max(phimax, rsh);
-    if (tid == 0) *partial_ptr(tb, RS_PHIMAX, inst, blockIdx.x) = m;
   }
 
   if (st & XS_STORE_PHI) {
     T* ph = phi + (size_t)inst * 3 * g.nsites + (size_t)row0 * NX;
     for (int s2 = tid; s2 < TS; s2 += nthr) {
       const int r = s2 / NX, x = s2 - r * NX;
-      const cplx<T> a = aux[(2 * r) * NX + x], b = aux[(2 * r + 1) * NX + x];
+      const cplx<T> a = aux[P8((2 * r) * NX + x)], b = aux[P8((2 * r + 1) * NX + x)];
       ph[s2] = a.x;
       ph[g.nsites + s2] = a.y;
       ph[2 * g.nsites + s2] = b.x;
@@ -266,7 +265,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   if (st & XS_OPS_A) run_ops(s_ops, prog.nA, 0);
 
   if (st & XS_KIN_1D) {
-    fft_team<T, NX, false, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+    fft_team<T, NX, false, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
     const double kdt = prog.kin_frac * dt;
     for (int e = tid; e < D * TS; e += nthr) {
       const int x = e % NX;
@@ -274,10 +273,10 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       const double a = 0.5 * (k * k) * kdt;
       cplx<double> ph = itp ? mk(exp(-a), 0.0) : cis_neg<double>(a);
       ph = ph * g.inv_ntot;
-      tile[e] = mul64(tile[e], ph);
+      tile[P8(e)] = mul64(tile[P8(e)], ph);
     }
     __syncthreads();
-    fft_team<T, NX, true, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+    fft_team<T, NX, true, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
   }
 
   if (st & XS_OPS_B) run_ops(s_ops + kMaxOps, prog.nB, kMaxOps);
@@ -286,57 +285,65 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     for (int s = tid; s < TS; s += nthr) {
       cplx<T> sp[D];
 #pragma unroll
-      for (int c = 0; c < D; ++c) sp[c] = tile[c * TS + s];
+      for (int c = 0; c < D; ++c) sp[c] = tile[P8(c * TS + s)];
       T fx, fy, fz;
       spin_vec<T, D>(sp, fx, fy, fz);
       const int r = s / NX, x = s - r * NX;
-      aux[(2 * r) * NX + x] = mk(fx, fy);
-      aux[(2 * r + 1) * NX + x] = mk(fz, (T)0);
+      aux[P8((2 * r) * NX + x)] = mk(fx, fy);
+      aux[P8((2 * r + 1) * NX + x)] = mk(fz, (T)0);
     }
     __syncthreads();
-    fft_team<T, NX, false, EX>(aux + line * NX, 1, tt, line < 2 * ROWS, tw);
+    fft_team<T, NX, false, EX, true>(aux + P8(line * NX), 1, tt, line < 2 * ROWS, tw);
     const size_t fb = (size_t)inst * g.fhat_inst_stride;
     constexpr int NH = NX / 2 + 1;
     for (int e = tid; e < ROWS * NH; e += nthr) {
       const int r = e / NH, k = e - r * NH;
-      const cplx<T> z1 = aux[(2 * r) * NX + k];
-      const cplx<T> z1n = conjc(aux[(2 * r) * NX + ((NX - k) & (NX - 1))]);
+      const cplx<T> z1 = aux[P8((2 * r) * NX + k)];
+      const cplx<T> z1n = conjc(aux[P8((2 * r) * NX + ((NX - k) & (NX - 1)))]);
       const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
       const size_t o = fb + (size_t)(row0 + r) * g.nxh_pitch + k;
       fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
       fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
-      fhat[o + 2 * g.fhat_field_stride] = aux[(2 * r + 1) * NX + k];
+      fhat[o + 2 * g.fhat_field_stride] = aux[P8((2 * r + 1) * NX + k)];
     }
   }
 
   if (st & XS_NORM) {
-    double acc = 0.0;
     for (int s = tid; s < TS; s += nthr) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) acc += (double)abs2(tile[c * TS + s]);
+      for (int c = 0; c < D; ++c) norm_acc += (double)abs2(tile[P8(c * TS + s)]);
     }
-    const double v = block_sum(acc, rsh);
-    if (tid == 0) *partial_ptr(tb, RS_NORM, inst, blockIdx.x) = v;
-    __syncthreads();
   }
 
-  if (st & XS_FWD_FFT) fft_team<T, NX, false, EX>(tile + line * NX, 1, tt, line < D * ROWS, tw);
+  if (st & XS_FWD_FFT)
+    fft_team<T, NX, false, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
 
   if (st & XS_KE_1D) {
-    double acc = 0.0;
     for (int e = tid; e < D * TS; e += nthr) {
       const double k = tb.ktab[e % NX];
-      acc += (k * k) * (double)abs2(tile[e]);
+      ke_acc += (k * k) * (double)abs2(tile[P8(e)]);
     }
-    const double v = block_sum(acc, rsh);
-    if (tid == 0) *partial_ptr(tb, RS_KIN, inst, blockIdx.x) = v;
   }
 
   if (!(st & XS_NO_STORE)) {
     for (int e = tid; e < D * TS; e += nthr) {
       const int c = e / TS, r = e - c * TS;
-      dst[ibase + (size_t)c * g.nsites + r] = tile[e];
+      dst[ibase + (size_t)c * g.nsites + r] = tile[P8(e)];
     }
+  }
+
+  // per-block partial sums
+  if (st & XS_LOAD_PHI) {
+    const double m = block_max(phimax, rsh);
+    if (tid == 0) *partial_ptr(tb, RS_PHIMAX, inst, blockIdx.x) = m;
+  }
+  if (st & XS_NORM) {
+    const double v = block_sum(norm_acc, rsh);
+    if (tid == 0) *partial_ptr(tb, RS_NORM, inst, blockIdx.x) = v;
+  }
+  if (st & XS_KE_1D) {
+    const double v = block_sum(ke_acc, rsh);
+    if (tid == 0) *partial_ptr(tb, RS_KIN, inst, blockIdx.x) = v;
   }
 }
 
@@ -393,7 +400,7 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
     e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
     e_lhy += n * n * sqrt(n);
     if constexpr (D == 13) {
-      if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P, tb.tens_cg);
+      if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P);
     }
     if (phi) {
       const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
@@ -416,6 +423,40 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
 // ---------------------------------------------------------------------------
 // lane-per-site pass (13-component operators: tensor expm-action, Euler rotations)
 // ---------------------------------------------------------------------------
+template <typename T, int D>
+__device__ __forceinline__ void site_ops(const Geom& g, const DevTables& tb, const SProgram& prog,
+                                         cplx<T>* __restrict__ psi, const T* __restrict__ phi,
+                                         const Op* s_ops, const cplx<double>* zf, int inst,
+                                         long long site, double dt, bool itp, int& nmv, int& nrot) {
+  const InstParams& P = tb.params[inst];
+  cplx<T>* p = psi + (size_t)inst * g.inst_stride + site;
+  cplx<T> sp[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) sp[c] = p[(size_t)c * g.nsites];
+  for (int i = 0; i < prog.n; ++i) {
+    const Op op = s_ops[i];
+    const double dtf = op.frac * dt;
+    if (op.kind == OP_DIAG) {
+      const double Vt = trap_site(g, tb, inst, site);
+      op_diag<T, D>(sp, Vt, P, dtf, itp, &zf[i * D]);
+    } else if (op.kind == OP_SPINMIX) {
+      nrot += op_spin_mixing<T, D>(sp, P, dtf, itp);
+    } else if (op.kind == OP_DDI_ROT) {
+      const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
+      nrot += op_euler<T, D>(sp, (double)ph[0], (double)ph[g.nsites], (double)ph[2 * g.nsites], dtf,
+                             itp);
+    } else if (op.kind == OP_TENSOR) {
+      if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
+    } else if (op.kind == OP_SCALE) {
+      const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
+#pragma unroll
+      for (int c = 0; c < D; ++c) sp[c] = scale64(sp[c], sc);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) p[(size_t)c * g.nsites] = sp[c];
+}
+
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
 site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
@@ -442,38 +483,22 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
   }
   __syncthreads();
   const long long site = (long long)blockIdx.x * blockDim.x + tid;
-  if (site >= g.nsites) return;
-  cplx<T>* p = psi + (size_t)inst * g.inst_stride + site;
-  cplx<T> sp[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) sp[c] = p[(size_t)c * g.nsites];
-  const cplx<T>* vfy = sizeof(T) == 8 ? reinterpret_cast<const cplx<T>*>(tb.vfy)
-                                       : reinterpret_cast<const cplx<T>*>(tb.vfy_f);
-  const T* cgt = sizeof(T) == 8 ? reinterpret_cast<const T*>(tb.tens_cg)
-                                : reinterpret_cast<const T*>(tb.tens_cg_f);
-  for (int i = 0; i < prog.n; ++i) {
-    const Op op = s_ops[i];
-    const double dtf = op.frac * dt;
-    if (op.kind == OP_DIAG) {
-      const double Vt = trap_site(g, tb, inst, site);
-      op_diag<T, D>(sp, Vt, P, dtf, itp, &zf[i * D]);
-    } else if (op.kind == OP_SPINMIX) {
-      op_spin_mixing<T, D>(sp, P, dtf, itp, vfy);
-    } else if (op.kind == OP_DDI_ROT) {
-      const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
-      op_euler<T, D>(sp, (double)ph[0], (double)ph[g.nsites], (double)ph[2 * g.nsites], dtf, itp,
-                     vfy);
-    } else if (op.kind == OP_TENSOR) {
-      if constexpr (D == 13) op_tensor<T, D>(sp, P, dtf, itp, cgt);
-    } else if (op.kind == OP_SCALE) {
-      const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
-#pragma unroll
-      for (int c = 0; c < D; ++c) sp[c] = scale64(sp[c], sc);
+  int nmv = 0, nrot = 0;
+  if (site < g.nsites) site_ops<T, D>(g, tb, prog, psi, phi, s_ops, zf, inst, site, dt, itp, nmv, nrot);
+  if (tb.work) {  // profiling: executed tensor matvecs and spin rotations
+    __shared__ unsigned int s_cnt[2];
+    if (tid == 0) s_cnt[0] = s_cnt[1] = 0u;
+    __syncthreads();
+    if (nmv) atomicAdd(&s_cnt[0], (unsigned int)nmv);
+    if (nrot) atomicAdd(&s_cnt[1], (unsigned int)nrot);
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&tb.work[0], (unsigned long long)s_cnt[0]);
+      atomicAdd(&tb.work[1], (unsigned long long)s_cnt[1]);
     }
   }
-#pragma unroll
-  for (int c = 0; c < D; ++c) p[(size_t)c * g.nsites] = sp[c];
 }
+
 
 // ---------------------------------------------------------------------------
 // y / z pass
@@ -522,7 +547,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       const double a = 0.5 * (k * k) * dt;
       double re, im;
       if (itp) { re = exp(-a); im = 0.0; }
-      else { double s, c; sincos(a, &s, &c); re = c; im = -s; }
+      else { double s, c; sincos_d(a, &s, &c); re = c; im = -s; }
       tabs[2 * i] = re;
       tabs[2 * i + 1] = im;
     }
@@ -738,23 +763,30 @@ int xpass_rows(int D, int NX, int nrows) {
   return r;
 }
 
+static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux) {
+  // padded tiles: 9 slots per 8 elements
+  return csz * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0)) / 8 * 9 +
+         sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double);
+}
+
 template <typename T, int NX, int D>
 static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
                           const void* src, void* dst, void* fhat, int n_active, void* phi) {
   const int rows = xpass_rows(D, NX, g.nrows);
   const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
-  const size_t lds = sizeof(cplx<T>) * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0)) +
-                     sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double);
+  const size_t lds = xpass_lds(D, rows, NX, sizeof(cplx<T>), has_aux);
   const int threads = D * rows * (NX / xpass_e<NX, D>());
   if (threads > xpass_max_threads<D>()) return hipErrorInvalidConfiguration;
-  dim3 grid(g.nrows / rows, n_active);
-  if (lds > 48 * 1024)  // + static LDS (op lists, Fy eigenvectors, CG table)
+  dim3 grid(xpass_grid(g), n_active);
+  if (lds > 48 * 1024)  // + static LDS (op lists)
     (void)hipFuncSetAttribute((const void*)xpass_kernel<T, NX, D>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((xpass_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,
                      (const cplx<T>*)src, (cplx<T>*)dst, (cplx<T>*)fhat, (T*)phi, rows);
   return hipGetLastError();
 }
+
+int xpass_grid(const Geom& g) { return g.nrows / xpass_rows(g.D, g.n[0], g.nrows); }
 
 template <typename T, int D>
 static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,

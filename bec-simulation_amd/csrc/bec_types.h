@@ -106,4 +106,82 @@ __host__ __device__ constexpr int size(int F) { return base(F, 2 * F + 1); }
 __host__ __device__ constexpr int off(int F, int M, int m1, int S) {
   return base(F, M) + (m1 - m1lo(F, M)) * ns(F, M) + (S - s0(M)) / 2;
 }
+
+// Compile-time Clebsch-Gordan coefficients (Racah's closed form, Condon-Shortley phase),
+// so the device tensor step takes them as instruction literals instead of loads.
+__host__ __device__ constexpr double cfact(int n) {
+  double r = 1.0;
+  for (int i = 2; i <= n; ++i) r *= (double)i;
+  return r;
+}
+__host__ __device__ constexpr double csqrt(double x) {  // Newton from above, monotone
+  if (!(x > 0.0)) return 0.0;
+  double r = x < 1.0 ? 1.0 : x;
+  for (int i = 0; i < 4000; ++i) {
+    const double n = 0.5 * (r + x / r);
+    if (!(n < r)) break;
+    r = n;
+  }
+  return r;
+}
+__host__ __device__ constexpr double cg(int j1, int m1, int j2, int m2, int J, int M) {
+  if (m1 + m2 != M || m1 > j1 || -m1 > j1 || m2 > j2 || -m2 > j2 || M > J || -M > J) return 0.0;
+  if (J > j1 + j2 || J < (j1 > j2 ? j1 - j2 : j2 - j1)) return 0.0;
+  const double pre = (2.0 * J + 1.0) * cfact(J + j1 - j2) * cfact(J - j1 + j2) *
+                     cfact(j1 + j2 - J) / cfact(j1 + j2 + J + 1) * cfact(J + M) * cfact(J - M) *
+                     cfact(j1 - m1) * cfact(j1 + m1) * cfact(j2 - m2) * cfact(j2 + m2);
+  int klo = 0;
+  if (j2 - J - m1 > klo) klo = j2 - J - m1;
+  if (j1 - J + m2 > klo) klo = j1 - J + m2;
+  int khi = j1 + j2 - J;
+  if (j1 - m1 < khi) khi = j1 - m1;
+  if (j2 + m2 < khi) khi = j2 + m2;
+  double sum = 0.0;
+  for (int k = klo; k <= khi; ++k) {
+    const double d = cfact(k) * cfact(j1 + j2 - J - k) * cfact(j1 - m1 - k) * cfact(j2 + m2 - k) *
+                     cfact(J - j2 + m1 + k) * cfact(J - j1 - m2 + k);
+    sum += (k % 2 ? -1.0 : 1.0) / d;
+  }
+  return csqrt(pre) * sum;
+}
+// Wigner small-d at beta = pi/2: d^j_{m'm}(pi/2) = <j m'| exp(-i pi/2 F_y) |j m>
+// = 2^-j sqrt((j+m')!(j-m')!(j+m)!(j-m)!) sum_k (-1)^(k-m+m') / ((j+m-k)! k! (j-k-m')! (k-m+m')!).
+__host__ __device__ constexpr double wigner_d_half_pi(int j, int mp, int m) {
+  double s = 0.0;
+  for (int k = 0; k <= 2 * j; ++k) {
+    if (j + m - k < 0 || j - k - mp < 0 || k - m + mp < 0) continue;
+    const double den = cfact(j + m - k) * cfact(k) * cfact(j - k - mp) * cfact(k - m + mp);
+    s += ((k - m + mp) % 2 != 0 ? -1.0 : 1.0) / den;
+  }
+  double p = 1.0;
+  for (int i = 0; i < j; ++i) p *= 0.5;
+  return p * csqrt(cfact(j + mp) * cfact(j - mp) * cfact(j + m) * cfact(j - m)) * s;
+}
+// d^F(pi/2) in component order (c <-> m = F - c), row-major [c'][c].  With
+// U = exp(-i pi/2 F_z) d^F(pi/2), F_y = U F_z U^dagger: U holds eigenvectors of F_y.
+template <int F>
+struct DHalf {
+  double v[(2 * F + 1) * (2 * F + 1)];
+  __host__ __device__ constexpr DHalf() : v() {
+    for (int cp = 0; cp <= 2 * F; ++cp)
+      for (int c = 0; c <= 2 * F; ++c) {
+        const double x = wigner_d_half_pi(F, F - cp, F - c);
+        v[cp * (2 * F + 1) + c] = (x < 1e-15 && x > -1e-15) ? 0.0 : x;
+      }
+  }
+};
+
+// The table in off() order, with |c| < 1e-15 set to 0 (clebsch_gordan.jl:174-175).
+template <int F>
+struct CGTab {
+  double v[size(F)];
+  __host__ __device__ constexpr CGTab() : v() {
+    for (int M = -2 * F; M <= 2 * F; ++M)
+      for (int m1 = m1lo(F, M); m1 <= m1hi(M); ++m1)
+        for (int S = s0(M); S <= 2 * F; S += 2) {
+          const double c = cg(F, m1, F, M - m1, S, M);
+          v[off(F, M, m1, S)] = (c < 1e-15 && c > -1e-15) ? 0.0 : c;
+        }
+  }
+};
 }  // namespace tens

@@ -1,6 +1,7 @@
 """Benchmark of the MI355X split-step path.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+(default workload: yoshida3d_spin6_ddi_128, FP32-bulk)
 
 Prints ONE JSON line.  A step is one accepted time step of the whole grid
 (all 2F+1 components); value = site-steps/s with psi resident in HBM.
@@ -44,16 +45,48 @@ def _cfg2_inputs(G, n=128, precision="fp64"):
     return grid, trap, psi
 
 
-def _kernel_bytes(name, D, nsites, csz, nxh_ratio=0.5):
-    """Algorithmic HBM bytes of one launch of a kernel class: one read and one
-    write of every array the pass must touch (psi: D complex per site)."""
-    if name in ("xpass", "fwd", "kin", "inv"):
-        return 2.0 * D * nsites * csz
-    if name in ("ddi_fft", "ddi_conv"):
-        return 2.0 * 3 * nsites * nxh_ratio * csz
-    if name == "ke":
-        return 1.0 * D * nsites * csz
-    return 0.0
+FP32_PEAK = 157.3e12  # FLOP/s, MI355X FP32 (vector with packed FMA, or matrix)
+FP64_PEAK = 78.6e12   # FLOP/s, MI355X FP64 vector / matrix
+HBM_CLASSES = ("xpass", "fwd", "kin", "inv", "ddi_fft", "ddi_conv")
+
+
+def _traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE passes, scaled by a copy kernel of known size), if a summary for this
+    workload is committed under profiles/ as JSON; otherwise None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def _rooflines(prof, work, precision):
+    """Per-class achieved figures from the profiled pass: HBM classes in GB/s of
+    algorithmic bytes; the lane-per-site pass in TFLOP/s of executed flops."""
+    out = {}
+    for k, (ms, cnt) in prof.items():
+        if cnt == 0 or ms <= 0:
+            continue
+        secs = ms * 1e-3
+        if k in HBM_CLASSES and work["bytes"].get(k, 0) > 0:
+            gbs = work["bytes"][k] / secs / 1e9
+            out[k] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                      "frac": round(gbs / 8000.0, 4), "avg_launch_us": round(ms / cnt * 1e3, 2),
+                      "launches": cnt, "alg_bytes_per_launch": work["bytes"][k] / cnt}
+        elif k == "sites" and work["flops"] > 0:
+            peak = FP32_PEAK if precision == "fp32" else FP64_PEAK
+            tf = work["flops"] / secs / 1e12
+            out[k] = {"bound": "mfma", "achieved": round(tf, 2), "peak": peak / 1e12,
+                      "unit": "TFLOP/s", "frac": round(tf * 1e12 / peak, 4),
+                      "engine": "VALU (no MFMA; non-packed FMA issue caps it at half the peak)",
+                      "avg_launch_us": round(ms / cnt * 1e3, 2), "launches": cnt,
+                      "alg_flops_per_launch": work["flops"] / cnt,
+                      "tensor_matvecs": work["tensor_matvecs"], "rotations": work["rotations"],
+                      "alg_bytes_per_launch": work["bytes"]["sites"] / cnt}
+    return out
 
 
 def run_rt3d(G, A, steps, warmup, precision):
@@ -86,8 +119,9 @@ def run_rt3d(G, A, steps, warmup, precision):
     ws.profile(True)
     run(steps)
     prof = ws.profile_read()
+    work = ws.profile_work()
     ws.profile(False)
-    return ws, grid, (t1 - t0), prof, psi
+    return ws, grid, (t1 - t0), prof, psi, work
 
 
 def run_itp1d(G, A, steps, warmup, precision):
@@ -107,8 +141,9 @@ def run_itp1d(G, A, steps, warmup, precision):
     ws.profile(True)
     A.split_step(ws, steps)
     prof = ws.profile_read()
+    work = ws.profile_work()
     ws.profile(False)
-    return ws, grid, (t1 - t0), prof, psi
+    return ws, grid, (t1 - t0), prof, psi, work
 
 
 def cfg3_inputs(G, n=128):
@@ -146,8 +181,9 @@ def run_yoshida3d(G, A, steps, warmup, precision):
                              save_interval=0.05, keep_snapshots=False,
                              max_accepted=max(2, steps // 4))
     prof = ws.profile_read()
+    work = ws.profile_work()
     ws.profile(False)
-    return ws, grid, (t1 - t0), prof, psi, r2
+    return ws, grid, (t1 - t0), prof, psi, r2, work
 
 
 def cpu_baseline_yoshida3d(budget_s=20.0):
@@ -219,58 +255,59 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="rt3d_spin1_128",
-                    choices=["rt3d_spin1_128", "itp1d_spin1_64", "yoshida3d_spin6_ddi_128"])
+    ap.add_argument("--workload", default="yoshida3d_spin6_ddi_128",
+                    choices=["yoshida3d_spin6_ddi_128", "rt3d_spin1_128", "itp1d_spin1_64"])
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is a design "
-                                   "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build measures one GPU; sharding across GPUs is a "
+                                   "design note only, not built", "n_gpus": args.gpus}))
         sys.exit(2)
     import becgpu as G
     from becgpu import api as A
 
     if args.precision is None:
         args.precision = "fp32" if args.workload.startswith("yoshida") else "fp64"
-    csz = 16 if args.precision == "fp64" else 8
     extra = {}
     if args.workload == "yoshida3d_spin6_ddi_128":
-        ws, grid, secs, prof, psi, r2 = run_yoshida3d(G, A, args.steps, args.warmup,
-                                                      args.precision)
+        ws, grid, secs, prof, psi, r2, work = run_yoshida3d(G, A, args.steps, args.warmup,
+                                                            args.precision)
         # Yoshida attempt = 6V + 4K + 3 psi copies + L2 reduction (B_alg per site-step)
         b_alg = 10852.5 if args.precision == "fp64" else 5426.3
         cfg = {"workload": "3D spin-6 dipolar 128^3, adaptive Yoshida-4 (run_simulation_yoshida!)",
                "grid": [128, 128, 128], "F": 6, "box": 24.0, "c0": 10.0, "c_dd": 1.0,
-               "tensor": "c2..c12 ~ U[-1,1] seed 42", "dt_init": 1e-3}
+               "tensor": "c2..c12 ~ U[-1,1] seed 42", "dt_init": 1e-3, "ddi_padding": False}
         extra = {"accepted": r2["n_accepted"], "rejected": r2["n_rejected"],
                  "final_dt": r2["final_dt"]}
     elif args.workload == "rt3d_spin1_128":
-        ws, grid, secs, prof, psi = run_rt3d(G, A, args.steps, args.warmup, args.precision)
+        ws, grid, secs, prof, psi, work = run_rt3d(G, A, args.steps, args.warmup, args.precision)
         b_alg = 288.0 if args.precision == "fp64" else 144.0
         cfg = {"workload": "3D spin-1 real-time leapfrog 128^3 (run_simulation!, save_every=10)",
                "grid": [128, 128, 128], "F": 1, "dt": 1e-3, "c0": 10.0, "c1": 0.5, "q": 0.1}
     else:
-        ws, grid, secs, prof, psi = run_itp1d(G, A, args.steps, args.warmup, args.precision)
+        ws, grid, secs, prof, psi, work = run_itp1d(G, A, args.steps, args.warmup, args.precision)
         b_alg = 384.0 if args.precision == "fp64" else 192.0
         cfg = {"workload": "1D spin-1 ITP split_step! 64 points", "grid": [64], "F": 1,
                "dt": 0.005, "c0": 10.0, "c1": -0.5}
     nsites = int(np.prod(grid.config.n_points))
-    D = ws.D
     value = args.steps * nsites / secs
-    # dominant kernel by total time in the profiled pass
-    dom = max((k for k in prof if prof[k][1] > 0), key=lambda k: prof[k][0])
-    tot_ms, cnt = prof[dom]
-    avg_s = tot_ms / cnt * 1e-3
-    kb = _kernel_bytes(dom, D, nsites, csz)
-    achieved = kb / avg_s / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": dom,
-            "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
-            "alg_bytes_per_launch": kb}
+    roofs = _rooflines(prof, work, args.precision)
+    # dominant kernel class by total time in the profiled pass
+    dom = max((k for k in prof if prof[k][1] > 0 and k != "energy"), key=lambda k: prof[k][0])
+    roof = dict(roofs.get(dom, {"bound": "hbm", "achieved": None, "peak": 8000.0, "unit": "GB/s",
+                                "frac": None}))
+    roof["kernel"] = dom
+    tr = _traffic(args.workload, dom)
+    roof["traffic"] = tr
+    if tr is not None:
+        roof["traffic_source"] = f"profiles/{args.workload}_traffic.json"
+    total_ms = sum(v[0] for v in prof.values())
     kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
-                   "share": round(v[0] / max(1e-9, sum(x[0] for x in prof.values())), 3)}
+                   "share": round(v[0] / max(1e-9, total_ms), 3)}
                for k, v in prof.items() if v[1] > 0}
+    hbm = {k: {"GB/s": r["achieved"], "frac": r["frac"]} for k, r in roofs.items()
+           if r["bound"] == "hbm"}
     cpu = None
     if not args.no_cpu:
         if args.workload == "rt3d_spin1_128":
@@ -284,10 +321,10 @@ def main():
         "value": value, "unit": "site-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 reductions)",
+        "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 reductions and phases)",
         "data": "synthetic", "config": cfg,
         "step_hbm_frac": round(value * b_alg / HBM_PEAK, 4), "b_alg_per_site_step": b_alg,
-        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roof, "hbm_kernels": hbm, "kernels": kernels, "cpu_baseline": cpu,
     }
     line.update(extra)
     print(json.dumps(line))

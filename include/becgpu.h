@@ -60,9 +60,6 @@ typedef struct bec_config {
   int ddi_padding;     /* make_workspace(ddi_padding=...) (not built: BEC_EUNSUP) */
   int tensor;          /* general-F tensor step replaces spin mixing (TensorInteractionCache
                         * active, initialization.jl:164-185); couplings via bec_set_tensor */
-  /* Fy eigenvectors, D*D complex interleaved, row-major, columns ordered by the
-   * ascending eigenvalues -F..F (spin_matrices.jl:38-43). */
-  const double* fy_vecs;
   /* Cell-centred coordinates and fftfreq wavenumbers of each axis, concatenated
    * x then y then z (grid.jl:1-23). */
   const double* x_tab;
@@ -144,6 +141,13 @@ int bec_profile_enable(bec_ws* ws, int on);
 /* ms[BEC_K_COUNT] total milliseconds and count[BEC_K_COUNT] launches since enable;
  * resets the record. */
 int bec_profile_read(bec_ws* ws, double* ms, long long* count);
+/* Work executed by the lane-per-site pass since bec_profile_enable (or the last call):
+ * out[0] tensor matvecs (Taylor terms of the tensor step), out[1] Euler spin rotations,
+ * out[2] flops per tensor matvec, out[3] flops per rotation, out[4 + k] algorithmic HBM
+ * bytes issued by kernel class k (BEC_K_*; 0 for BEC_K_REDUCE and BEC_K_ENERGY), so
+ * out[4 + k] / time of class k is its achieved algorithmic bandwidth.  Resets the
+ * counters.  No counterpart in the reference. */
+int bec_profile_work(bec_ws* ws, double* out);
 
 int bec_synchronize(bec_ws* ws);
 const char* bec_last_error(void);

@@ -47,7 +47,6 @@ def _cfg(**kw):
     c.normalize_every = 1
     c.enable_ddi = kw.get("ddi", 0)
     buf = (C.c_double * 4096)()
-    c.fy_vecs = C.cast(buf, C.POINTER(C.c_double))
     c.x_tab = C.cast(buf, C.POINTER(C.c_double))
     c.k_tab = C.cast(buf, C.POINTER(C.c_double))
     return c, buf
