@@ -523,8 +523,8 @@ def find_ground_state(*, grid: Grid, atom, interactions: InteractionParams,
 def find_ground_state_multistart(*, initial_states=("polar", "ferromagnetic", "uniform",
                                                     "antiferromagnetic"),
                                  n_random=3, seed=42, grid, atom, interactions, **kw):
-    """ground_state.jl:239-271 (sequential semantics; the batched engine in
-    becgpu.batch runs the same solves in one device launch)."""
+    """ground_state.jl:239-271: the solves run one after another, each on its own
+    workspace; best = first minimum of the energies (argmin)."""
     results = []
     for state in initial_states:
         if state == "random":
