@@ -32,7 +32,7 @@ class BecConfig(C.Structure):
 class BecInstParams(C.Structure):
     _fields_ = [
         ("dt", C.c_double), ("c0", C.c_double), ("c1", C.c_double), ("c_lhy", C.c_double),
-        ("c_dd", C.c_double), ("p", C.c_double), ("q", C.c_double), ("omega", C.c_double * 3),
+        ("c2", C.c_double), ("c_dd", C.c_double), ("p", C.c_double), ("q", C.c_double), ("omega", C.c_double * 3),
     ]
 
 

@@ -129,6 +129,8 @@ class Workspace:
         p = L.BecInstParams()
         p.dt = float(self.sim_params.dt if dt is None else dt)
         p.c0, p.c1, p.c_lhy = interactions.c0, interactions.c1, interactions.c_lhy
+        # get_cn(ip, 2) (types.jl:157-162): the nematic coupling is c_extra[1]
+        p.c2 = float(interactions.c_extra[0]) if len(interactions.c_extra) >= 1 else 0.0
         p.c_dd = float(c_dd)
         p.p, p.q = zeeman.p, zeeman.q
         if isinstance(potential, HarmonicTrap):

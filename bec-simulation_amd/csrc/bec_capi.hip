@@ -241,7 +241,8 @@ struct Sched {
   // Euler spin rotations (DDI, spin mixing for F >= 2) and the tensor step run in the
   // lane-per-site kernel; the x pass keeps FFTs and light pointwise work
   bool site_op(int kind) const {
-    return kind == OP_TENSOR || kind == OP_DDI_ROT || (kind == OP_SPINMIX && ws->D >= 5);
+    return kind == OP_TENSOR || kind == OP_DDI_ROT || (kind == OP_SPINMIX && ws->D >= 5) ||
+           (kind == OP_NEMATIC && ws->D >= 13);
   }
   int flush_s() {
     if (sp.n == 0) return BEC_OK;
@@ -311,11 +312,20 @@ struct Sched {
       if (std::fabs(ws->hp[inst].c1) >= 1e-30) return true;
     return false;
   }
+  // likewise the nematic step with |c2| < 1e-30 (nematic.jl:43); never with the tensor step
+  bool nematic_active() const {
+    if (ws->tensor) return false;
+    for (int inst : ws->active)
+      if (std::fabs(ws->hp[inst].c2) >= 1e-30) return true;
+    return false;
+  }
   int V(double f) {
     const int mix = ws->tensor ? OP_TENSOR : OP_SPINMIX;
     const bool mixing = mixing_active();
+    const bool nematic = nematic_active();
     CHK(add_op(OP_DIAG, f / 2));
     if (mixing) CHK(add_op(mix, f / 2));
+    if (nematic) CHK(add_op(OP_NEMATIC, f / 2));
     if (ws->ddi) {
       CHK(flush_s());
       if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
@@ -327,6 +337,7 @@ struct Sched {
       CHK(flush());
       CHK(add_op(OP_DDI_ROT, f));
     }
+    if (nematic) CHK(add_op(OP_NEMATIC, f / 2));
     if (mixing) CHK(add_op(mix, f / 2));
     CHK(add_op(OP_DIAG, f / 2));
     return BEC_OK;
@@ -642,6 +653,7 @@ int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p) {
   q.c0 = p->c0;
   q.c1 = p->c1;
   q.c_lhy = p->c_lhy;
+  q.c2 = ws->tensor ? 0.0 : p->c2;
   q.cdd = p->c_dd;
   const int F = ws->F;
   double zmin = 0.0;
@@ -908,7 +920,7 @@ int bec_energy(bec_ws* ws, double* out) {
     o[BEC_E_C1] = 0.5 * q.c1 * r[RS_C1] * dV;
     o[BEC_E_DDI] = ws->ddi ? 0.5 * r[RS_DDI] * dV : 0.0;
     o[BEC_E_LHY] = q.c_lhy != 0.0 ? (2.0 / 5.0) * q.c_lhy * r[RS_LHY] * dV : 0.0;
-    o[BEC_E_TENSOR] = ws->tensor ? r[RS_TENSOR] * dV : 0.0;
+    o[BEC_E_TENSOR] = r[RS_TENSOR] * dV;  // tensor channels, or the nematic term without them
     o[BEC_E_NORM] = r[RS_NORM] * dV;
     o[BEC_E_MZ] = r[RS_MZ] * dV;
     o[BEC_E_TOTAL] = o[BEC_E_KIN] + o[BEC_E_TRAP] + o[BEC_E_ZEEMAN] + o[BEC_E_C0] + o[BEC_E_C1] +

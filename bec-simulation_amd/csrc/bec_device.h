@@ -510,6 +510,53 @@ __device__ __forceinline__ int op_spin_mixing(cplx<T>* s, const InstParams& P, d
 }
 
 
+// nematic.jl:34-119: A00 = sum_c (-1)^c psi_c psi_{D-1-c} / sqrt(D) from the pre-update
+// spinor (the sign (-1)^(F-m) is (-1)^c), V = c2 (-1)^c A00 / sqrt(D) per (c, D-1-c) pair:
+// real time psi_c <- cos(|V|dt) psi_c - i (V/|V|) sin(|V|dt) conj(psi_pair),
+// ITP       psi_c <- cosh(|V|dt) psi_c - (V/|V|) sinh(|V|dt) conj(psi_pair); |V| < 1e-30 skips.
+template <typename T, int D>
+__device__ __forceinline__ void op_nematic(cplx<T>* s, double c2, double dtf, bool itp) {
+  if (fabs(c2) < 1e-30) return;  // nematic.jl:43
+  const double isd = 1.0 / sqrt((double)D);
+  double ar = 0.0, ai = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const double sg = (c % 2 == 0) ? 1.0 : -1.0;
+    const double xr = s[c].x, xi = s[c].y, yr = s[D - 1 - c].x, yi = s[D - 1 - c].y;
+    ar += sg * (xr * yr - xi * yi);
+    ai += sg * (xr * yi + xi * yr);
+  }
+  const double vr0 = c2 * ar * isd * isd, vi0 = c2 * ai * isd * isd;
+  const double av = sqrt(vr0 * vr0 + vi0 * vi0);  // |V| is the same for every pair
+  if (av < 1e-30) return;
+  double cc, ss;
+  if (itp) {
+    cc = cosh(av * dtf);
+    ss = sinh(av * dtf);
+  } else {
+    sincos_d(av * dtf, &ss, &cc);
+  }
+#pragma unroll
+  for (int c = 0; c < (D + 1) / 2; ++c) {
+    const int cp = D - 1 - c;
+    const double sg = (c % 2 == 0) ? 1.0 : -1.0;
+    const double pr = sg * vr0 / av, pi = sg * vi0 / av;  // V/|V|
+    // coef = ph * sh (ITP) or i * ph * sin (real time)
+    const double kr = itp ? pr * ss : -pi * ss, ki = itp ? pi * ss : pr * ss;
+    const double mr = s[c].x, mi = s[c].y, nr = s[cp].x, ni = s[cp].y;
+    // x <- cc x - coef conj(y)
+    auto upd = [&](double xr, double xi, double yr, double yi) __attribute__((always_inline)) {
+      return mk((T)(cc * xr - (kr * yr + ki * yi)), (T)(cc * xi - (ki * yr - kr * yi)));
+    };
+    if (c == cp) {
+      s[c] = upd(mr, mi, mr, mi);
+    } else {
+      s[c] = upd(mr, mi, nr, ni);
+      s[cp] = upd(nr, ni, mr, mi);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // General-F tensor interaction (tensor_interaction.jl:130-240)
 // ---------------------------------------------------------------------------

@@ -234,6 +234,15 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
           for (int c = 0; c < D; ++c) tile[P8(c * TS + s)] = sp[c];
         });
+      } else if (op.kind == OP_NEMATIC && D < 13) {
+        for_sites([&](int s) __attribute__((always_inline)) {
+          cplx<T> sp[D];
+#pragma unroll
+          for (int c = 0; c < D; ++c) sp[c] = tile[P8(c * TS + s)];
+          op_nematic<T, D>(sp, P.c2, dtf, itp);
+#pragma unroll
+          for (int c = 0; c < D; ++c) tile[P8(c * TS + s)] = sp[c];
+        });
       } else if (op.kind == OP_SCALE) {
         for (int e = tid; e < D * TS; e += nthr) tile[P8(e)] = scale64(tile[P8(e)], sc);
         __syncthreads();  // the scale loop's element ownership differs from the site loops
@@ -399,8 +408,18 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
     e_c0 += n * n;
     e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
     e_lhy += n * n * sqrt(n);
-    if constexpr (D == 13) {
-      if (with_tensor) e_ten += tensor_energy_site<T, D>(sp, P);
+    if (with_tensor) {
+      if constexpr (D == 13) e_ten += tensor_energy_site<T, D>(sp, P);
+    } else if (P.c2 != 0.0) {  // energy.jl:43-46,106-109: 1/2 c2 |A00|^2, tensor slot
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double sg = (c % 2 == 0) ? 1.0 : -1.0;
+        const double xr = sp[c].x, xi = sp[c].y, yr = sp[D - 1 - c].x, yi = sp[D - 1 - c].y;
+        ar += sg * (xr * yr - xi * yi);
+        ai += sg * (xr * yi + xi * yr);
+      }
+      e_ten += 0.5 * P.c2 * (ar * ar + ai * ai) / (double)D;
     }
     if (phi) {
       const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
@@ -445,6 +464,8 @@ __device__ __forceinline__ void site_ops(const Geom& g, const DevTables& tb, con
       const T* ph = phi + (size_t)inst * 3 * g.nsites + site;
       nrot += op_euler<T, D>(sp, (double)ph[0], (double)ph[g.nsites], (double)ph[2 * g.nsites], dtf,
                              itp);
+    } else if (op.kind == OP_NEMATIC) {
+      op_nematic<T, D>(sp, P.c2, dtf, itp);
     } else if (op.kind == OP_TENSOR) {
       if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
     } else if (op.kind == OP_SCALE) {

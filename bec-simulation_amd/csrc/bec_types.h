@@ -12,6 +12,7 @@ constexpr int kMaxOps = 12;
 struct InstParams {
   double dt;         // time step of this instance
   double c0, c1, c_lhy;
+  double c2;         // nematic (singlet-pair) coupling, 0 when the tensor step is active
   double cdd;        // dipolar strength C_dd
   double zee[kMaxD]; // Zeeman energies E_c = -p m + q m^2, m = F - c
   double zee_shift;  // min_c E_c (subtracted in imaginary time)
@@ -26,6 +27,7 @@ enum OpKind : int {
   OP_DDI_ROT = 3,   // Euler rotation by the DDI potential (dt*frac)
   OP_SCALE = 4,     // multiply by 1/sqrt(norm2) of the instance (ITP renormalisation)
   OP_TENSOR = 5,    // apply_tensor_interaction_step! (dt*frac)
+  OP_NEMATIC = 6,   // apply_nematic_step! (dt*frac)
 };
 
 struct Op {

@@ -70,6 +70,8 @@ typedef struct bec_config {
 typedef struct bec_inst_params {
   double dt;
   double c0, c1, c_lhy; /* interactions.jl / types.jl:336-346 */
+  double c2;            /* nematic coupling get_cn(ip, 2) = c_extra[1] (types.jl:157-162); set
+                         * 0 when the tensor step carries the contact channels */
   double c_dd;          /* DDIParams.C_dd */
   double p, q;          /* ZeemanParams (zeeman.jl:15-19) */
   double omega[3];      /* HarmonicTrap omega per axis (potentials.jl:5-15) */

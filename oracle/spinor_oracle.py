@@ -21,6 +21,8 @@ a retyped copy of its own formula):
   Rodrigues / Euler ....... scipy.linalg.expm(-i dt phi.F) brute force
   tensor step ............. scipy.linalg.expm of brute-force h; F=1 == spin mixing;
                             sum_S g_S |A_SM|^2 identity
+  nematic step ............ scipy.linalg.expm of the real-linear pair ODE with A00 frozen;
+                            per-pair norm conservation in real time
   CG / 6j ................. sympy exact + the reference test's known values
   DDI ..................... direct O(N^2) k-sum convolution on small grids;
                             uniform polarised E_ddi = 0
@@ -377,6 +379,44 @@ def spin_mixing_step(psi, F, c1, dtf, itp):
     return euler_rotation(psi, c1 * fx, c1 * fy, c1 * fz, dtf, F, itp)
 
 
+def singlet_amplitude(psi, F):
+    """observables.jl:94-111: A00 = sum_c (-1)^(F-m) psi_c psi_{D-1-c} / sqrt(D)."""
+    D = 2 * F + 1
+    A = np.zeros(psi.shape[1:], dtype=complex)
+    for c in range(D):
+        A += (1.0 if c % 2 == 0 else -1.0) * psi[c] * psi[D - 1 - c]
+    return A / math.sqrt(D)
+
+
+def nematic_step(psi, F, c2, dtf, itp):
+    """nematic.jl:34-119: singlet-pair step with A00 from the pre-update spinor;
+    each (m, -m) pair mixes with the conjugate of its partner."""
+    if abs(c2) < 1e-30:
+        return psi
+    D = 2 * F + 1
+    out = psi.copy()
+    Vb = c2 * singlet_amplitude(psi, F) / math.sqrt(D)
+    for c in range((D + 1) // 2):
+        cp = D - 1 - c
+        V = Vb * (1.0 if c % 2 == 0 else -1.0)
+        a = np.abs(V)
+        ok = a >= 1e-30
+        ph = np.where(ok, V / np.where(ok, a, 1.0), 0.0)
+        if itp:
+            cc, ss = np.cosh(a * dtf), np.sinh(a * dtf)
+            coef = ph * ss
+        else:
+            cc, ss = np.cos(a * dtf), np.sin(a * dtf)
+            coef = 1j * ph * ss
+        pm, pn = psi[c], psi[cp]
+        if c == cp:
+            out[c] = np.where(ok, cc * pm - coef * np.conj(pm), pm)
+        else:
+            out[c] = np.where(ok, cc * pm - coef * np.conj(pn), pm)
+            out[cp] = np.where(ok, cc * pn - coef * np.conj(pm), pn)
+    return out
+
+
 def tensor_step(psi, T, dtf, itp):
     """tensor_interaction.jl:195-240: h = sum T conj(psi_mu) psi_nu;
     psi <- V exp(-lambda dt) V^dagger psi (ITP) / V cis(-lambda dt) V^dagger psi;
@@ -483,6 +523,7 @@ class OracleWS:
     c0: float = 0.0
     c1: float = 0.0
     c_lhy: float = 0.0
+    c2: float = 0.0                # nematic coupling (c_extra[1]) when the tensor path is off
     channels: dict = None          # tensor path (g_S), or None
     T: np.ndarray = None
     cdd: float = 0.0
@@ -510,15 +551,17 @@ def make_workspace(grid, F, psi, dt, itp, *, omega=None, V=None, p=0.0, q=0.0,
         V = harmonic_potential(grid, omega) if omega is not None else np.zeros(grid.shape)
     ch = tensor_channels(F, c0, c1, list(c_extra))
     T = None
+    c2 = float(c_extra[0]) if len(c_extra) >= 1 else 0.0   # get_cn(ip, 2), types.jl:157-162
     if ch is not None:
         T = tensor_fold(F, ch)
         c0 = c1 = 0.0
+        c2 = 0.0
     if normalize_every is None:
         normalize_every = 1 if itp else 0
     if save_every is None:
         save_every = max(1, n_steps // 100)           # types.jl:230
     ws = OracleWS(grid=grid, F=F, psi=np.array(psi, dtype=complex), dt=dt, itp=itp, V=V,
-                  p=p, q=q, c0=c0, c1=c1, c_lhy=c_lhy, channels=ch, T=T, cdd=cdd, ddi=ddi,
+                  p=p, q=q, c0=c0, c1=c1, c_lhy=c_lhy, c2=c2, channels=ch, T=T, cdd=cdd, ddi=ddi,
                   normalize_every=normalize_every, save_every=save_every, n_steps=n_steps)
     if ddi:
         ws.Q = ddi_q(grid.n, grid.dx, secular)
@@ -559,11 +602,13 @@ def half_potential_step(ws: OracleWS, dth, itp):
         ws.psi = tensor_step(ws.psi, ws.T, dth / 2, itp)
     else:
         ws.psi = spin_mixing_step(ws.psi, ws.F, ws.c1, dth / 2, itp)
+        ws.psi = nematic_step(ws.psi, ws.F, ws.c2, dth / 2, itp)
     if ws.ddi:
         ddi_step(ws, dth, itp)
     if ws.T is not None:
         ws.psi = tensor_step(ws.psi, ws.T, dth / 2, itp)
     else:
+        ws.psi = nematic_step(ws.psi, ws.F, ws.c2, dth / 2, itp)
         ws.psi = spin_mixing_step(ws.psi, ws.F, ws.c1, dth / 2, itp)
     ws.psi = diagonal_step(ws.psi, ws.V, zee, ws.c0, ws.c_lhy, dth / 2, itp)
 
@@ -635,7 +680,9 @@ def energy_terms(ws: OracleWS):
         E["c0"] = 0.5 * ws.c0 * float(np.sum(n ** 2)) * dV
         fx, fy, fz = spin_density(psi, ws.F)
         E["c1"] = 0.5 * ws.c1 * float(np.sum(fx ** 2 + fy ** 2 + fz ** 2)) * dV
-        E["tensor"] = 0.0
+        # energy.jl:43-46,106-109: the nematic term is reported in the tensor slot
+        E["tensor"] = (0.5 * ws.c2 * float(np.sum(np.abs(singlet_amplitude(psi, ws.F)) ** 2)) * dV
+                       if ws.c2 != 0.0 else 0.0)
     else:
         E["c0"] = 0.0
         E["c1"] = 0.0

@@ -57,6 +57,48 @@ def test_split_step_2d_spin1(itp):
     assert err <= PSI_TOL
 
 
+@pytest.mark.parametrize("F,itp", [(1, False), (1, True), (2, False), (2, True)])
+def test_nematic_split_step_2d(F, itp):
+    """Nematic singlet-pair step (c2 = c_extra[1], nematic.jl:34-119) with spin mixing and
+    Zeeman, 2-D: psi after 30 steps and every energy term (the nematic energy is reported
+    in the tensor slot, energy.jl:43-46)."""
+    n, L = (32, 32), (12.0, 12.0)
+    og, _ = grids(n, L)
+    D = 2 * F + 1
+    psi = O.gaussian(og, (2.5, 2.5)) * (0.6 + 0.3 * random_psi(og.shape, D, 17 + F))
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, F, psi, 0.003, itp, omega=(1.0, 1.1), p=0.2, q=0.1, c0=8.0, c1=-0.4,
+                    c_extra=(1.7,))
+    for _ in range(30):
+        O.split_step(ows)
+    A.split_step(gws, 30)
+    err = rel_l2(gws.get_psi(), ows.psi)
+    eo, eg = O.energy_terms(ows), A.energy_terms(gws)
+    print("F", F, "itp", itp, "err", err, "tensor slot", eo["tensor"], eg["tensor"])
+    assert err <= PSI_TOL
+    assert eo["tensor"] != 0.0
+    for k in ("kin", "trap", "zeeman", "c0", "c1", "tensor"):
+        assert abs(eg[k] - eo[k]) <= E_TOL * max(abs(eo[k]), 1e-300), k
+
+
+def test_nematic_f6_site_kernel():
+    """F=6 with only c2 (no rank >= 4 channel: the tensor step stays off), so the nematic
+    step runs in the lane-per-site pass."""
+    n, L = (16, 16, 16), (10.0, 10.0, 10.0)
+    og, _ = grids(n, L)
+    psi = O.gaussian(og, (2.0, 2.0, 2.0)) * (0.5 + 0.3 * random_psi(og.shape, 13, 41))
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 6, psi, 0.002, False, omega=(1.0, 1.0, 1.0), c0=5.0, c1=0.3,
+                    c_extra=(2.2,))
+    assert ows.T is None
+    for _ in range(6):
+        O.split_step(ows)
+    A.split_step(gws, 6)
+    err = rel_l2(gws.get_psi(), ows.psi)
+    print("err", err)
+    assert err <= PSI_TOL
+
+
 def test_leapfrog_3d_spin1_snapshots():
     n, L = (32, 32, 32), (20.0, 20.0, 20.0)
     og, _ = grids(n, L)

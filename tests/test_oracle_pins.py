@@ -191,6 +191,114 @@ def test_tensor_f1_mean_field_equals_contact_plus_spin():
     assert np.max(np.abs(a - b)) < 1e-4
 
 
+# --- nematic step (nematic.jl; restates test_nematic.jl) --------------------------
+@pytest.mark.parametrize("F", [1, 2])
+def test_nematic_itp_step_vs_expm(F):
+    """ITP: each (m, -m) pair follows da/dtau = -V conj(b), db/dtau = -V conj(a) with V
+    frozen at the pre-update A00; the oracle's cosh/sinh form must equal expm of that
+    real-linear generator (nematic.jl:77-98)."""
+    D = 2 * F + 1
+    c2, dt = 0.8, 0.37
+    psi = rspinor(D, 5, 11 + F)
+    out = O.nematic_step(psi.copy(), F, c2, dt, True)
+    for i in range(5):
+        A = sum((1 if c % 2 == 0 else -1) * psi[c, i] * psi[D - 1 - c, i] for c in range(D))
+        A /= math.sqrt(D)
+        for c in range((D + 1) // 2):
+            cp = D - 1 - c
+            V = c2 * A / math.sqrt(D) * (1 if c % 2 == 0 else -1)
+            B = np.array([[-V.real, -V.imag], [-V.imag, V.real]])   # -V conj(.) on (re, im)
+            if c == cp:
+                z = sla.expm(B * dt) @ np.array([psi[c, i].real, psi[c, i].imag])
+                assert abs(out[c, i] - (z[0] + 1j * z[1])) < 1e-12
+            else:
+                M = np.zeros((4, 4))
+                M[0:2, 2:4] = B
+                M[2:4, 0:2] = B
+                z = sla.expm(M * dt) @ np.array([psi[c, i].real, psi[c, i].imag,
+                                                  psi[cp, i].real, psi[cp, i].imag])
+                assert abs(out[c, i] - (z[0] + 1j * z[1])) < 1e-12
+                assert abs(out[cp, i] - (z[2] + 1j * z[3])) < 1e-12
+
+
+def test_nematic_singlet_amplitude_states():
+    """test_nematic.jl:2-55: |A00|^2 = n^2/3 (F=1 polar), 0 (F=1 ferromagnetic),
+    n^2/5 (F=2 polar)."""
+    g = O.make_grid((64,), (20.0,))
+    for F, state, den in [(1, "polar", 3), (1, "ferromagnetic", None), (2, "polar", 5)]:
+        psi = O.init_psi(g, F, state)
+        A2 = float(np.sum(np.abs(O.singlet_amplitude(psi, F)) ** 2))
+        n = np.sum(np.abs(psi) ** 2, axis=0)
+        if den is None:
+            assert A2 < 1e-28
+        else:
+            assert abs(A2 - float(np.sum(n ** 2)) / den) < 1e-12 * A2
+
+
+def test_nematic_step_reference_invariants():
+    """test_nematic.jl:79-137,206-246: c2 = 0 and ferromagnetic states are left
+    unchanged; one real-time step conserves the norm of uniform states (F=1, c2=50;
+    F=2, c2=30) to 1e-12; m <-> -m exchange symmetry in real and imaginary time; an
+    ITP step on the polar state keeps 0 < N < 2 N0."""
+    g = O.make_grid((64,), (20.0,))
+    psi = O.init_psi(g, 1, "uniform")
+    assert np.array_equal(O.nematic_step(psi.copy(), 1, 0.0, 0.01, False), psi)
+    fer = O.init_psi(g, 1, "ferromagnetic")
+    assert np.max(np.abs(O.nematic_step(fer.copy(), 1, 50.0, 0.01, False) - fer)) < 1e-14
+    for F, c2 in [(1, 50.0), (2, 30.0)]:
+        psi = O.init_psi(g, F, "uniform")
+        out = O.nematic_step(psi.copy(), F, c2, 0.01, False)
+        n0, n1 = np.sum(np.abs(psi) ** 2), np.sum(np.abs(out) ** 2)
+        assert abs(n1 - n0) / n0 < 1e-12
+    v = np.array([0.5 * np.exp(0.7j), 0.3 + 0.2j, 0.4 * np.exp(-0.3j)]).reshape(3, 1)
+    for itp in (False, True):
+        a = O.nematic_step(v.copy(), 1, 50.0, 0.05, itp)
+        b = O.nematic_step(v[::-1].copy(), 1, 50.0, 0.05, itp)
+        assert abs(a[0, 0] - b[2, 0]) < 1e-14 and abs(a[2, 0] - b[0, 0]) < 1e-14
+    pol = O.init_psi(g, 1, "polar")
+    out = O.nematic_step(pol.copy(), 1, 50.0, 0.001, True)
+    assert 0 < np.sum(np.abs(out) ** 2) < 2 * np.sum(np.abs(pol) ** 2)
+
+
+@pytest.mark.parametrize("F", [1, 2])
+def test_nematic_complex_state_norm_and_small_dt(F):
+    """test_nematic.jl:138-204: 50 real-time steps of a complex single-site spinor keep
+    the norm to 1e-6; at dt = 1e-6 the step equals psi_m - i V_m dt conj(psi_-m) to
+    rtol 1e-4 (the GP equation, V_m = c2 (-1)^(F-m) A00 / sqrt(D))."""
+    if F == 1:
+        v = np.array([0.5 * np.exp(0.7j), 0.3 + 0.2j, 0.4 * np.exp(-0.3j)])
+        c2 = 50.0
+    else:
+        v = np.array([0.4 * np.exp(0.5j), 0.3 * np.exp(-0.8j), 0.2 + 0.1j, 0.35 * np.exp(1.2j),
+                      0.25 * np.exp(-0.3j)])
+        c2 = 30.0
+    D = 2 * F + 1
+    psi = v.reshape(D, 1).copy()
+    for _ in range(50):
+        psi = O.nematic_step(psi, F, c2, 0.01, False)
+    assert abs(np.sum(np.abs(psi) ** 2) - np.sum(np.abs(v) ** 2)) < 1e-6 * np.sum(np.abs(v) ** 2)
+    dt = 1e-6
+    out = O.nematic_step(v.reshape(D, 1).copy(), F, c2, dt, False)[:, 0]
+    A = sum((1 if c % 2 == 0 else -1) * v[c] * v[D - 1 - c] for c in range(D)) / math.sqrt(D)
+    for c in range(D):
+        V = c2 * (1 if c % 2 == 0 else -1) * A / math.sqrt(D)
+        exp = v[c] - 1j * V * dt * np.conj(v[D - 1 - c])
+        assert abs(out[c] - exp) <= 1e-4 * abs(exp)
+
+
+def test_nematic_energy_of_polar_and_ferro():
+    """|A00|^2 = n^2/D for the F=2 polar state and 0 for the ferromagnetic one, so
+    E_nematic = c2/(2D) sum n^2 dV and 0 (energy.jl:106-109, test_nematic.jl:57-77)."""
+    g = O.make_grid((16,), (10.0,))
+    for state, expect in [("polar", True), ("ferromagnetic", False)]:
+        psi = O.init_psi(g, 2, state)
+        ws = O.make_workspace(g, 2, psi, 0.01, True, c0=1.0, c_extra=(0.9,))
+        E = O.energy_terms(ws)["tensor"]
+        n = np.sum(np.abs(psi) ** 2, axis=0)
+        ref = 0.9 / (2 * 5) * float(np.sum(n ** 2)) * g.dV if expect else 0.0
+        assert abs(E - ref) < 1e-14
+
+
 # --- kinetic step / diagonal step closed forms (test_physics_level0.jl) ----------
 def test_kinetic_plane_wave_exact():
     g = O.make_grid((16, 8), (5.0, 3.0))
