@@ -22,8 +22,8 @@ import numpy as np
 from . import _lib as L
 from .physics import (AdaptiveDtParams, Grid, HarmonicTrap, InteractionParams, NoPotential,
                       SimParams, SpinSystem, ZeemanParams, c0c1_to_gS, c_extra_to_delta_gS,
-                      cell_volume, classify_phase, evaluate_potential, init_psi, spin_matrices,
-                      zeeman_energies)
+                      cell_volume, classify_phase, dipolar_relaxation_rates, evaluate_potential,
+                      init_psi, spin_matrices, zeeman_energies)
 
 ITP_EXPONENT_LIMIT = 50.0          # ground_state.jl:1
 
@@ -91,6 +91,7 @@ class Workspace:
         self.params = [None] * n_instances
         self.interactions = [None] * n_instances
         self.zeeman = [ZeemanParams()] * n_instances
+        self._loss = [False] * n_instances
         self._active = list(range(n_instances))
 
     # -- handle ------------------------------------------------------------
@@ -146,6 +147,22 @@ class Workspace:
         self.params[inst] = p
         self.interactions[inst] = interactions
         self.zeeman[inst] = zeeman
+
+    def set_loss(self, inst, loss):
+        """LossParams(gamma_dr, L3) of one instance, or None (types.jl:291-296). The
+        per-component rates are formed here (losses.jl:60-82) and applied on the device."""
+        if loss is None or (loss.gamma_dr < 1e-30 and loss.L3 < 1e-30):   # losses.jl:31
+            L.check(L.lib().bec_set_loss(self.handle, inst, None, 0), "bec_set_loss")
+            self._loss[inst] = False
+            return
+        rates = np.ascontiguousarray(
+            [g + loss.L3 for g in dipolar_relaxation_rates(self.F, loss.gamma_dr)], dtype=np.float64)
+        L.check(L.lib().bec_set_loss(self.handle, inst, rates.ctypes.data, self.D), "bec_set_loss")
+        self._loss[inst] = True
+
+    @property
+    def has_loss(self):
+        return any(self._loss[i] for i in self.active)
 
     def set_tensor(self, inst, channels: dict):
         """g_S couplings {S: g} of the general-F tensor step for one instance."""
@@ -246,8 +263,6 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
     F = atom.F
     if raman is not None:
         raise NotImplementedError("Raman coupling is outside the GPU hot path")
-    if loss is not None:
-        raise NotImplementedError("loss step is not built on the GPU path yet")
     tensor = _tensor_active(F, interactions)
     channels = None
     if tensor:
@@ -270,6 +285,8 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
                     c_dd=c_dd_val)
     if tensor:
         ws.set_tensor(0, channels)
+    if loss is not None:
+        ws.set_loss(0, loss)
     psi = init_psi(grid, SpinSystem(F)) if psi_init is None else np.array(psi_init, dtype=complex)
     ws.set_psi(psi, 0)
     ws.atom = atom
@@ -394,6 +411,8 @@ def run_simulation_yoshida(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end:
             dt = max(dt_step * max(0.3, 0.9 * (adaptive.tol / err) ** 0.2), adaptive.dt_min)
             n_rej += 1
             continue
+        if ws.has_loss:                    # yoshida.jl:64-66
+            L.check(L.lib().bec_loss_step(ws.handle, dt_step), "bec_loss_step")
         ws.t[0] += dt_step
         ws.step[0] += 1
         n_acc += 1

@@ -241,6 +241,35 @@ def precompute_cg_table(F: int):
     return t
 
 
+@dataclass(frozen=True)
+class LossParams:
+    """types.jl:291-296: dipolar relaxation rate and three-body loss."""
+    gamma_dr: float
+    L3: float = 0.0
+
+
+def dipolar_relaxation_rates(F: int, gamma_dr: float):
+    """losses.jl:60-82: gamma_m = gamma_dr * sum_{q=-1,-2} CG(F,m;2,q|F,m+q)^2 / Z, Z the
+    mean raw sum (so the mean rate is gamma_dr; m = -F is stable)."""
+    D = 2 * F + 1
+    raw = []
+    for c in range(D):
+        m = F - c
+        acc = 0.0
+        for q in (-1, -2):
+            mp = m + q
+            if abs(mp) > F:
+                continue
+            v = clebsch_gordan(F, m, 2, q, F, mp)
+            acc += v * v
+        raw.append(acc)
+    tot = sum(raw)
+    if tot < 1e-30:
+        return [0.0] * D
+    Z = tot / D
+    return [gamma_dr * r / Z for r in raw]
+
+
 def cn_to_gS(F, c_dict):
     """clebsch_gordan.jl:117-141."""
     ev = list(range(0, 2 * F + 1, 2))

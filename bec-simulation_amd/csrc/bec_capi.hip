@@ -213,7 +213,7 @@ static int do_reduce(bec_ws* ws, int nblocks, uint32_t slots, uint32_t maxmask =
 // ---------------------------------------------------------------------------
 // Schedule compiler
 // ---------------------------------------------------------------------------
-enum ItemKind { IT_V, IT_K, IT_NORM };
+enum ItemKind { IT_V, IT_K, IT_NORM, IT_LOSS };
 struct Item {
   ItemKind kind;
   double frac;  // V: dt_half = frac*dt ; K: kinetic dt = frac*dt
@@ -366,6 +366,8 @@ struct Sched {
     p.stages |= XS_INV_FFT;
     return BEC_OK;
   }
+  // apply_loss_step! (losses.jl:25-49) for dt = f*dt; fuses into the next x pass
+  int LOSS(double f) { return add_op(OP_LOSS, f); }
   int NORM() {
     if (scale_pending) {
       scale_pending = false;
@@ -403,6 +405,7 @@ static int run_items(bec_ws* ws, const std::vector<Item>& items, int itp, double
   for (const Item& it : items) {
     if (it.kind == IT_V) CHK(s.V(it.frac));
     else if (it.kind == IT_K) CHK(s.K(it.frac));
+    else if (it.kind == IT_LOSS) CHK(s.LOSS(it.frac));
     else CHK(s.NORM());
   }
   return s.finish();
@@ -645,10 +648,11 @@ int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p) {
   if (!(p->dt > 0)) return fail(BEC_EINVAL, "dt must be positive");
   ws->user[inst] = *p;
   InstParams& q = ws->hp[inst];
-  double keep_g[7];
-  std::memcpy(keep_g, q.gS, sizeof(keep_g));
+  const InstParams keep = q;  // tensor couplings and loss rates have their own setters
   q = InstParams{};
-  std::memcpy(q.gS, keep_g, sizeof(keep_g));
+  std::memcpy(q.gS, keep.gS, sizeof(q.gS));
+  std::memcpy(q.loss, keep.loss, sizeof(q.loss));
+  q.has_loss = keep.has_loss;
   q.dt = p->dt;
   q.c0 = p->c0;
   q.c1 = p->c1;
@@ -757,15 +761,23 @@ int bec_set_active(bec_ws* ws, const int* insts, int n) {
   return BEC_OK;
 }
 
+static bool any_loss(const bec_ws* ws) {
+  for (int inst : ws->active)
+    if (ws->hp[inst].has_loss) return true;
+  return false;
+}
+
 int bec_split_step(bec_ws* ws, int n_steps) {
   if (!ws) return fail(BEC_EINVAL, "null workspace");
   if (n_steps < 0) return fail(BEC_EINVAL, "n_steps < 0");
+  const bool loss = !ws->itp && any_loss(ws);  // split_step.jl:21-23: real time only
   std::vector<Item> items;
-  items.reserve(4 * (size_t)n_steps);
+  items.reserve(5 * (size_t)n_steps);
   for (int s = 1; s <= n_steps; ++s) {
     items.push_back({IT_V, 0.5});
     items.push_back({IT_K, 1.0});
     items.push_back({IT_V, 0.5});
+    if (loss) items.push_back({IT_LOSS, 1.0});
     if (ws->itp && ws->normalize_every > 0 && (s % ws->normalize_every) == 0)
       items.push_back({IT_NORM, 0.0});
   }
@@ -775,13 +787,40 @@ int bec_split_step(bec_ws* ws, int n_steps) {
 int bec_leapfrog_segment(bec_ws* ws, int n_steps) {
   if (!ws) return fail(BEC_EINVAL, "null workspace");
   if (n_steps < 1) return fail(BEC_EINVAL, "n_steps < 1");
+  const bool loss = any_loss(ws);
   std::vector<Item> items;
   items.push_back({IT_V, 0.5});
   for (int s = 1; s <= n_steps; ++s) {
     items.push_back({IT_K, 1.0});
     items.push_back({IT_V, s == n_steps ? 0.5 : 1.0});
+    if (loss) items.push_back({IT_LOSS, 1.0});  // simulation.jl:74-76
   }
   return run_items(ws, items, 0, 0.0);
+}
+
+int bec_loss_step(bec_ws* ws, double dt) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (!any_loss(ws)) return BEC_OK;
+  return run_items(ws, {{IT_LOSS, 1.0}}, 0, dt);
+}
+
+int bec_set_loss(bec_ws* ws, int inst, const double* rates, int n) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  InstParams& q = ws->hp[inst];
+  if (!rates) {
+    q.has_loss = 0;
+    for (double& r : q.loss) r = 0.0;
+  } else {
+    if (n != ws->D) return fail(BEC_EINVAL, "loss rates: expected 2F+1 values");
+    for (int c = 0; c < ws->D; ++c) {
+      if (!(rates[c] >= 0.0)) return fail(BEC_EINVAL, "loss rates must be >= 0");
+      q.loss[c] = rates[c];
+    }
+    q.has_loss = 1;
+  }
+  ws->params_dirty = true;
+  return BEC_OK;
 }
 
 int bec_half_potential(bec_ws* ws, double dt_half, int imaginary_time) {

@@ -510,6 +510,22 @@ __device__ __forceinline__ int op_spin_mixing(cplx<T>* s, const InstParams& P, d
 }
 
 
+// losses.jl:25-49: psi_c *= exp(-rate_c n dt / 2) with n from the pre-update spinor;
+// components with rate_c < 1e-30 are left unchanged.
+__device__ __forceinline__ double loss_factor(double rate, double n, double dtf) {
+  return exp(-rate * n * dtf / 2.0);
+}
+template <typename T, int D>
+__device__ __forceinline__ void op_loss(cplx<T>* s, const InstParams& P, double dtf) {
+  if (!P.has_loss) return;
+  double n = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) n += (double)abs2(s[c]);
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+    if (!(P.loss[c] < 1e-30)) s[c] = scale64(s[c], loss_factor(P.loss[c], n, dtf));
+}
+
 // nematic.jl:34-119: A00 = sum_c (-1)^c psi_c psi_{D-1-c} / sqrt(D) from the pre-update
 // spinor (the sign (-1)^(F-m) is (-1)^c), V = c2 (-1)^c A00 / sqrt(D) per (c, D-1-c) pair:
 // real time psi_c <- cos(|V|dt) psi_c - i (V/|V|) sin(|V|dt) conj(psi_pair),

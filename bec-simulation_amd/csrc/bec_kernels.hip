@@ -234,6 +234,22 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
           for (int c = 0; c < D; ++c) tile[P8(c * TS + s)] = sp[c];
         });
+      } else if (op.kind == OP_LOSS) {
+        if (P.has_loss) {
+          for_sites([&](int s) __attribute__((always_inline)) {
+            // components stay in LDS (density sweep, then the per-component factors)
+            double n = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) n += (double)abs2(tile[P8(c * TS + s)]);
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+              if (P.loss[c] < 1e-30) continue;
+              const int e = P8(c * TS + s);
+              tile[e] = scale64(tile[e], loss_factor(P.loss[c], n, dtf));
+            }
+          });
+        }
       } else if (op.kind == OP_NEMATIC && D < 13) {
         for_sites([&](int s) __attribute__((always_inline)) {
           cplx<T> sp[D];
@@ -466,6 +482,8 @@ __device__ __forceinline__ void site_ops(const Geom& g, const DevTables& tb, con
                              itp);
     } else if (op.kind == OP_NEMATIC) {
       op_nematic<T, D>(sp, P.c2, dtf, itp);
+    } else if (op.kind == OP_LOSS) {
+      op_loss<T, D>(sp, P, dtf);
     } else if (op.kind == OP_TENSOR) {
       if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
     } else if (op.kind == OP_SCALE) {

@@ -17,8 +17,9 @@ struct InstParams {
   double zee[kMaxD]; // Zeeman energies E_c = -p m + q m^2, m = F - c
   double zee_shift;  // min_c E_c (subtracted in imaginary time)
   double gS[7];      // channel couplings g_S, S = 0, 2, ..., 2F (general-F tensor step)
+  double loss[kMaxD]; // loss rate gamma_c + L3 per component (real time only)
   int active;
-  int pad_;
+  int has_loss;      // 0: no loss step (LossParams absent, or gamma_dr and L3 < 1e-30)
 };
 
 enum OpKind : int {
@@ -28,6 +29,7 @@ enum OpKind : int {
   OP_SCALE = 4,     // multiply by 1/sqrt(norm2) of the instance (ITP renormalisation)
   OP_TENSOR = 5,    // apply_tensor_interaction_step! (dt*frac)
   OP_NEMATIC = 6,   // apply_nematic_step! (dt*frac)
+  OP_LOSS = 7,      // apply_loss_step! (dt*frac), real time
 };
 
 struct Op {

@@ -88,6 +88,13 @@ int bec_set_potential(bec_ws* ws, int inst, const double* V);
  * (TensorInteractionCache.g_values, tensor_interaction.jl:106-119; the caller forms
  * g_S = _c0c1_to_gS + _c_extra_to_delta_gS and zeroes c0, c1 as make_workspace does). */
 int bec_set_tensor(bec_ws* ws, int inst, const double* gS, int n);
+/* Loss rates gamma_c + L3 per component (n = 2F+1) of apply_loss_step! (losses.jl:25-82;
+ * the caller forms gamma_c = _dipolar_relaxation_rates(F, gamma_dr)), or rates = NULL for
+ * no loss (LossParams absent).  Real-time split_step and leapfrog segments then apply it. */
+int bec_set_loss(bec_ws* ws, int inst, const double* rates, int n);
+/* apply_loss_step! with step dt (losses.jl:25-49), as run_simulation_yoshida! applies it
+ * after each accepted step (yoshida.jl:64-66). */
+int bec_loss_step(bec_ws* ws, double dt);
 /* psi upload/download (SimState.psi). */
 int bec_set_psi(bec_ws* ws, int inst, const double* psi);
 int bec_get_psi(bec_ws* ws, int inst, double* psi);

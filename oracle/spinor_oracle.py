@@ -21,8 +21,10 @@ a retyped copy of its own formula):
   Rodrigues / Euler ....... scipy.linalg.expm(-i dt phi.F) brute force
   tensor step ............. scipy.linalg.expm of brute-force h; F=1 == spin mixing;
                             sum_S g_S |A_SM|^2 identity
-  nematic step ............ scipy.linalg.expm of the real-linear pair ODE with A00 frozen;
-                            per-pair norm conservation in real time
+  nematic step ............ ITP: scipy.linalg.expm of the real-linear pair ODE with A00
+                            frozen; the reference's nematic tests (invariants, small dt)
+  loss step ............... the reference's loss tests (rates: m=-F stable, mean gamma_dr;
+                            m-ordering of the decay; density dependence; L3 uniform)
   CG / 6j ................. sympy exact + the reference test's known values
   DDI ..................... direct O(N^2) k-sum convolution on small grids;
                             uniform polarised E_ddi = 0
@@ -417,6 +419,42 @@ def nematic_step(psi, F, c2, dtf, itp):
     return out
 
 
+def dipolar_relaxation_rates(F, gamma_dr):
+    """losses.jl:60-82: gamma_m = gamma_dr sum_{q=-1,-2} CG(F,m;2,q|F,m+q)^2 / Z with Z the
+    mean of the raw sums (sympy CG)."""
+    from sympy.physics.wigner import clebsch_gordan as scg
+    D = 2 * F + 1
+    raw = []
+    for c in range(D):
+        m = F - c
+        sacc = 0.0
+        for q in (-1, -2):
+            mp = m + q
+            if abs(mp) > F:
+                continue
+            v = float(scg(F, 2, F, m, q, mp))
+            sacc += v * v
+        raw.append(sacc)
+    tot = sum(raw)
+    if tot < 1e-30:
+        return np.zeros(D)
+    Z = tot / D
+    return np.array([gamma_dr * r / Z for r in raw])
+
+
+def loss_step(psi, rates, L3, dt):
+    """losses.jl:25-49: psi_c *= exp(-(gamma_c + L3) n dt / 2), n before the update;
+    components with rate < 1e-30 are left alone."""
+    out = psi.copy()
+    n = density(psi)
+    for c in range(psi.shape[0]):
+        rate = rates[c] + L3
+        if rate < 1e-30:
+            continue
+        out[c] = psi[c] * np.exp(-rate * n * dt / 2)
+    return out
+
+
 def tensor_step(psi, T, dtf, itp):
     """tensor_interaction.jl:195-240: h = sum T conj(psi_mu) psi_nu;
     psi <- V exp(-lambda dt) V^dagger psi (ITP) / V cis(-lambda dt) V^dagger psi;
@@ -524,6 +562,7 @@ class OracleWS:
     c1: float = 0.0
     c_lhy: float = 0.0
     c2: float = 0.0                # nematic coupling (c_extra[1]) when the tensor path is off
+    loss: tuple = None             # (gamma_dr, L3) or None (LossParams)
     channels: dict = None          # tensor path (g_S), or None
     T: np.ndarray = None
     cdd: float = 0.0
@@ -545,7 +584,7 @@ class OracleWS:
 def make_workspace(grid, F, psi, dt, itp, *, omega=None, V=None, p=0.0, q=0.0,
                    c0=0.0, c1=0.0, c_lhy=0.0, c_extra=(), ddi=False, cdd=0.0,
                    secular=False, padding=False, n_steps=1, normalize_every=None,
-                   save_every=None):
+                   save_every=None, loss=None):
     """initialization.jl:86-191 (the subset on the hot path)."""
     if V is None:
         V = harmonic_potential(grid, omega) if omega is not None else np.zeros(grid.shape)
@@ -562,7 +601,8 @@ def make_workspace(grid, F, psi, dt, itp, *, omega=None, V=None, p=0.0, q=0.0,
         save_every = max(1, n_steps // 100)           # types.jl:230
     ws = OracleWS(grid=grid, F=F, psi=np.array(psi, dtype=complex), dt=dt, itp=itp, V=V,
                   p=p, q=q, c0=c0, c1=c1, c_lhy=c_lhy, c2=c2, channels=ch, T=T, cdd=cdd, ddi=ddi,
-                  normalize_every=normalize_every, save_every=save_every, n_steps=n_steps)
+                  normalize_every=normalize_every, save_every=save_every, n_steps=n_steps,
+                  loss=None if loss is None else (float(loss[0]), float(loss[1])))
     if ddi:
         ws.Q = ddi_q(grid.n, grid.dx, secular)
         if padding:
@@ -619,11 +659,24 @@ def normalize(psi, dV):
     return psi / math.sqrt(nrm2)
 
 
+def apply_loss(ws: OracleWS, dt):
+    """apply_loss_step! with the workspace's LossParams (no-op when absent or when both
+    rates are below 1e-30, losses.jl:31)."""
+    if ws.loss is None:
+        return
+    g, l3 = ws.loss
+    if g < 1e-30 and l3 < 1e-30:
+        return
+    ws.psi = loss_step(ws.psi, dipolar_relaxation_rates(ws.F, g), l3, dt)
+
+
 def split_step(ws: OracleWS):
     """split_step.jl:10-35."""
     half_potential_step(ws, ws.dt / 2, ws.itp)
     ws.psi = kinetic_step(ws.psi, ws.kphase)
     half_potential_step(ws, ws.dt / 2, ws.itp)
+    if not ws.itp:
+        apply_loss(ws, ws.dt)
     if not ws.itp:
         ws.t += ws.dt
     ws.step += 1
@@ -771,6 +824,7 @@ def run_simulation(ws: OracleWS, keep_psi=False):
         is_last = step == ws.n_steps
         split = is_save or is_last
         half_potential_step(ws, dt / 2 if split else dt, False)
+        apply_loss(ws, dt)                              # simulation.jl:74-76
         ws.t += dt
         ws.step += 1
         if is_save:
@@ -810,6 +864,7 @@ def run_simulation_yoshida(ws: OracleWS, *, dt_init=1e-3, dt_min=1e-5, dt_max=1e
             dt = max(dt_step * max(0.3, 0.9 * (tol / err) ** 0.2), dt_min)
             n_rej += 1
             continue
+        apply_loss(ws, dt_step)                         # yoshida.jl:64-66
         ws.t += dt_step
         ws.step += 1
         n_acc += 1

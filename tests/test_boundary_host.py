@@ -148,3 +148,11 @@ def test_classify_phase_host():
     polar = G.init_psi(g, G.SpinSystem(1), "polar")
     assert P.classify_phase(ferro, 1, g)["phase"] == "ferromagnetic"
     assert P.classify_phase(polar, 1, g)["phase"] == "polar"
+
+
+@pytest.mark.parametrize("F", [1, 2, 6])
+def test_host_loss_rates_match_oracle(F):
+    """Host dipolar relaxation rates (log-factorial Racah CG) vs the oracle's sympy CG."""
+    from becgpu import physics as P
+    assert np.allclose(P.dipolar_relaxation_rates(F, 0.7), O.dipolar_relaxation_rates(F, 0.7),
+                       rtol=1e-12, atol=1e-15)

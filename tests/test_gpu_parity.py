@@ -99,6 +99,58 @@ def test_nematic_f6_site_kernel():
     assert err <= PSI_TOL
 
 
+@pytest.mark.parametrize("F", [1, 2])
+def test_loss_split_step_and_leapfrog_2d(F):
+    """Dipolar relaxation + three-body loss (losses.jl:25-82): applied after V K V in real-time
+    split_step! and after the merged potential step of the leapfrog (simulation.jl:74-76);
+    skipped in imaginary time (split_step.jl:21-23)."""
+    n, L = (32, 32), (12.0, 12.0)
+    og, _ = grids(n, L)
+    D = 2 * F + 1
+    psi = O.gaussian(og, (2.0, 2.5)) * (0.6 + 0.3 * random_psi(og.shape, D, 5 + F))
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    loss = (0.8, 0.3)
+    ows, gws = pair(n, L, F, psi, 0.004, False, omega=(1.0, 1.2), q=0.1, c0=6.0, c1=0.5,
+                    loss=loss)
+    for _ in range(25):
+        O.split_step(ows)
+    A.split_step(gws, 25)
+    err = rel_l2(gws.get_psi(), ows.psi)
+    print("split_step err", err, "norm", O.total_norm(ows.psi, og.dV))
+    assert err <= PSI_TOL
+    assert O.total_norm(ows.psi, og.dV) < 0.999
+    ows, gws = pair(n, L, F, psi, 0.004, False, omega=(1.0, 1.2), q=0.1, c0=6.0, c1=0.5,
+                    loss=loss, n_steps=30, save_every=10)
+    ro = O.run_simulation(ows)
+    rg = A.run_simulation(gws, keep_snapshots=False)
+    print("leapfrog norms", ro["norms"], rg.norms)
+    assert np.allclose(rg.norms, ro["norms"], rtol=1e-12, atol=0)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+    ows, gws = pair(n, L, F, psi, 0.004, True, omega=(1.0, 1.2), c0=6.0, c1=0.5, loss=(100.0, 100.0))
+    for _ in range(10):
+        O.split_step(ows)
+    A.split_step(gws, 10)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+    assert abs(O.total_norm(ows.psi, og.dV) - 1.0) < 1e-12
+
+
+def test_loss_yoshida_driver():
+    """run_simulation_yoshida! applies the loss after each accepted step (yoshida.jl:64-66)."""
+    n, L = (32, 32), (12.0, 12.0)
+    og, gg = grids(n, L)
+    psi = O.gaussian(og, (2.0, 2.0)) * (0.6 + 0.3 * random_psi(og.shape, 3, 8))
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 1, psi, 1e-3, False, omega=(1.0, 1.0), c0=6.0, c1=-0.5,
+                    loss=(0.6, 0.1))
+    ro = O.run_simulation_yoshida(ows, dt_init=1e-3, t_end=0.05, save_interval=0.02)
+    rg = A.run_simulation_yoshida(gws, adaptive=G.AdaptiveDtParams(dt_init=1e-3), t_end=0.05,
+                                  save_interval=0.02, keep_snapshots=False)
+    print("acc", ro["n_accepted"], rg["n_accepted"], "norms", ro["result"]["norms"], rg["result"].norms)
+    assert rg["n_accepted"] == ro["n_accepted"] and rg["n_rejected"] == ro["n_rejected"]
+    assert np.allclose(rg["result"].norms, ro["result"]["norms"], rtol=1e-12, atol=0)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+
+
 def test_leapfrog_3d_spin1_snapshots():
     n, L = (32, 32, 32), (20.0, 20.0, 20.0)
     og, _ = grids(n, L)

@@ -299,6 +299,73 @@ def test_nematic_energy_of_polar_and_ferro():
         assert abs(E - ref) < 1e-14
 
 
+# --- loss step (losses.jl; restates test_losses.jl) -------------------------------
+@pytest.mark.parametrize("F", [1, 2, 3, 6])
+def test_dipolar_relaxation_rate_properties(F):
+    """test_losses.jl:89-108: m = -F stable, mean rate = gamma_dr, rates >= 0; F=6: every
+    other component has a rate > 0.01 at gamma_dr = 0.5."""
+    r = O.dipolar_relaxation_rates(F, 1.0)
+    assert len(r) == 2 * F + 1 and abs(r[-1]) < 1e-15
+    assert abs(np.mean(r) - 1.0) < 1e-12 and np.all(r >= -1e-15)
+    if F == 6:
+        assert np.all(O.dipolar_relaxation_rates(6, 0.5)[:12] > 0.01)
+
+
+def test_loss_step_reference_behaviour():
+    """test_losses.jl:11-87,110-145,270-291: LossParams(0,0) keeps the norm; spin-1 uniform
+    state: m=-1 untouched, m=+1 decays faster than m=0; spin-2: m=-2 untouched, m=+2
+    fastest; the denser half loses a larger fraction; L3 alone is m-independent; 2-D step."""
+    g = O.make_grid((64,), (20.0,))
+    psi = O.init_psi(g, 1, "uniform")
+    assert np.allclose(O.loss_step(psi, O.dipolar_relaxation_rates(1, 0.0), 0.0, 0.01), psi,
+                       rtol=1e-14, atol=0)
+    r1 = O.dipolar_relaxation_rates(1, 0.1)
+    p1 = psi.copy()
+    for _ in range(100):
+        p1 = O.loss_step(p1, r1, 0.0, 0.01)
+    b, a = [np.sum(np.abs(psi[c]) ** 2) for c in range(3)], [np.sum(np.abs(p1[c]) ** 2) for c in range(3)]
+    assert abs(a[2] - b[2]) <= 1e-14 * b[2] and a[0] < b[0] and a[1] < b[1] and a[0] < a[1]
+    psi2 = O.init_psi(g, 2, "uniform")
+    r2 = O.dipolar_relaxation_rates(2, 0.1)
+    p2 = psi2.copy()
+    for _ in range(50):
+        p2 = O.loss_step(p2, r2, 0.0, 0.01)
+    b2 = [np.sum(np.abs(psi2[c]) ** 2) for c in range(5)]
+    a2 = [np.sum(np.abs(p2[c]) ** 2) for c in range(5)]
+    assert abs(a2[4] - b2[4]) <= 1e-14 * b2[4]
+    assert all(a2[c] < b2[c] for c in range(4)) and a2[0] < a2[1] and a2[0] < a2[2]
+    dens = np.zeros((3, 64), dtype=complex)
+    dens[1, :32], dens[1, 32:] = 2.0, 0.5
+    out = O.loss_step(dens, O.dipolar_relaxation_rates(1, 0.05), 0.0, 0.01)
+    fl = 1 - np.sum(np.abs(out[1, :32]) ** 2) / np.sum(np.abs(dens[1, :32]) ** 2)
+    fr = 1 - np.sum(np.abs(out[1, 32:]) ** 2) / np.sum(np.abs(dens[1, 32:]) ** 2)
+    assert fl > fr
+    p3 = psi.copy()
+    for _ in range(50):
+        p3 = O.loss_step(p3, np.zeros(3), 0.1, 0.01)
+    pops = [np.sum(np.abs(p3[c]) ** 2) for c in range(3)]
+    assert abs(pops[0] - pops[1]) < 1e-10 * pops[0] and abs(pops[1] - pops[2]) < 1e-10 * pops[1]
+    q = np.full((3, 32, 32), 1 / math.sqrt(3 * 32 * 32), dtype=complex)
+    qo = O.loss_step(q, r1, 0.0, 0.01)
+    assert abs(np.sum(np.abs(qo[2]) ** 2) - np.sum(np.abs(q[2]) ** 2)) < 1e-14
+    assert np.sum(np.abs(qo[0]) ** 2) < np.sum(np.abs(q[0]) ** 2)
+
+
+def test_loss_step_single_site_closed_form():
+    """One site, one step: psi_c * exp(-(gamma_c + L3) n dt / 2) with n the pre-update
+    density; gamma_c of spin 1 from the CG values by hand: raw = (1/2 + 1/2, 1/2, 0)
+    (|<1 1;2 -1|1 0>|^2 = 1/2, |<1 1;2 -2|1 -1>|^2 = 1/2... normalised to mean gamma)."""
+    raw = np.array([float(s_cg(1, 2, 1, 1, -1, 0) ** 2 + s_cg(1, 2, 1, 1, -2, -1) ** 2),
+                    float(s_cg(1, 2, 1, 0, -1, -1) ** 2), 0.0])
+    gam = 0.3 * raw / raw.mean()
+    assert np.allclose(O.dipolar_relaxation_rates(1, 0.3), gam, rtol=1e-14)
+    v = np.array([0.6 + 0.1j, -0.3j, 0.5]).reshape(3, 1)
+    n = float(np.sum(np.abs(v) ** 2))
+    out = O.loss_step(v, gam, 0.02, 0.7)
+    for c in range(3):
+        assert abs(out[c, 0] - v[c, 0] * math.exp(-(gam[c] + 0.02) * n * 0.7 / 2)) < 1e-15
+
+
 # --- kinetic step / diagonal step closed forms (test_physics_level0.jl) ----------
 def test_kinetic_plane_wave_exact():
     g = O.make_grid((16, 8), (5.0, 3.0))
