@@ -164,6 +164,21 @@ class Workspace:
     def has_loss(self):
         return any(self._loss[i] for i in self.active)
 
+    def set_dt(self, inst, dt):
+        """Change one instance's step (the dt of _rebuild_workspace_with_dt,
+        ground_state.jl:380-392); every other parameter is kept."""
+        p = self.params[inst]
+        p.dt = float(dt)
+        L.check(L.lib().bec_set_params(self.handle, inst, C.byref(p)), "bec_set_params")
+
+    def normalize_constrained(self, target_mz):
+        """_normalize_psi_constrained! (ground_state.jl:279-320) of the active instances;
+        target_mz: one value for all, or a sequence of B values."""
+        t = np.full(self.B, float(target_mz)) if np.isscalar(target_mz) else \
+            np.ascontiguousarray(target_mz, dtype=np.float64)
+        L.check(L.lib().bec_normalize_constrained(self.handle, t.ctypes.data),
+                "bec_normalize_constrained")
+
     def set_tensor(self, inst, channels: dict):
         """g_S couplings {S: g} of the general-F tensor step for one instance."""
         g = np.array([channels.get(S, 0.0) for S in range(0, 2 * self.F + 1, 2)], dtype=np.float64)
@@ -467,17 +482,21 @@ def _check_itp_overflow(ws: Workspace, step: int, insts=None):
                                  f"{ITP_EXPONENT_LIMIT}. Reduce dt (current={ws.params[i].dt}).")
 
 
-def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int):
+def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int, target_mz=None):
     """The ITP convergence loop of find_ground_state for every active instance of ws
-    (ground_state.jl:132-162).  Instances retire independently when converged."""
+    (ground_state.jl:132-162).  Instances retire independently when converged.  With
+    target_mz the workspace has normalize_every = 0 and the constrained normalisation runs
+    first and after every step (ground_state.jl:124-127,143-146)."""
     insts = ws.active
+    if target_mz is not None:
+        ws.normalize_constrained(target_mz)
     E_prev = {i: float(ws.energy_terms()[i, L.E_TOTAL]) for i in insts}
     ws.copy_slot(1, 0)
     out = {i: dict(converged=False, dE=float("nan"), dpsi=float("nan"), steps=0) for i in insts}
     running = list(insts)
     step = 0
     while step < n_steps and running:
-        if step < 10:
+        if step < 10 or target_mz is not None:
             k = 1
         else:
             k = min(save_every - step % save_every, n_steps - step)
@@ -485,6 +504,8 @@ def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int):
         step += k
         if step <= 10:
             _check_itp_overflow(ws, step, running)
+        if target_mz is not None:
+            ws.normalize_constrained(target_mz)
         if step % save_every == 0:
             et = ws.energy_terms()
             dd = ws.diff(0, 1, 1)
@@ -528,17 +549,65 @@ def find_ground_state(*, grid: Grid, atom, interactions: InteractionParams,
     eff_cdd = c_dd if (enable_ddi and not math.isnan(c_dd)) else 0.0
     _validate_itp_interactions(interactions, F, dt, psi=psi0, c_dd=eff_cdd)
     if adaptive_dt:
-        raise NotImplementedError("adaptive-dt ITP is not built on the GPU path yet")
-    if target_magnetization is not None:
-        raise NotImplementedError("constrained-Mz ITP is not built on the GPU path yet")
-    sp = SimParams(dt=dt, n_steps=n_steps, imaginary_time=True, normalize_every=1,
-                   save_every=max(1, n_steps // 10))
+        return _find_ground_state_adaptive(
+            grid=grid, atom=atom, interactions=interactions, zeeman=zeeman, potential=potential,
+            dt=dt, n_steps=n_steps, tol=tol, psi0=psi0, enable_ddi=enable_ddi, c_dd=c_dd,
+            secular_ddi=secular_ddi, dt_max=10.0 * dt if dt_max is None else dt_max,
+            precision=precision)
+    constrained = target_magnetization is not None
+    sp = SimParams(dt=dt, n_steps=n_steps, imaginary_time=True,
+                   normalize_every=0 if constrained else 1, save_every=max(1, n_steps // 10))
     ws = make_workspace(grid=grid, atom=atom, interactions=interactions, zeeman=zeeman,
                         potential=potential, sim_params=sp, psi_init=psi0, enable_ddi=enable_ddi,
                         c_dd=c_dd, secular_ddi=secular_ddi, precision=precision)
-    r = _itp_loop(ws, n_steps, tol, sp.save_every)[0]
+    r = _itp_loop(ws, n_steps, tol, sp.save_every,
+                  target_mz=float(target_magnetization) if constrained else None)[0]
     return dict(workspace=ws, converged=r["converged"], energy=r["energy"], dE=r["dE"],
                 dpsi=r["dpsi"], steps=r["steps"])
+
+
+def _find_ground_state_adaptive(*, grid, atom, interactions, zeeman, potential, dt, n_steps, tol,
+                                psi0, enable_ddi, c_dd, secular_ddi, dt_max, precision):
+    """ground_state.jl:172-228: every max(1, n_steps // 100) steps compare the energy; a
+    rise (or NaN) restores psi and halves dt (floor 1e-8), otherwise test convergence and
+    grow dt by 10 % up to dt_max.  psi backup stays on the device (slot 1)."""
+    check_every = max(1, n_steps // 100)
+    sp = SimParams(dt=dt, n_steps=check_every, imaginary_time=True, normalize_every=1,
+                   save_every=check_every)
+    ws = make_workspace(grid=grid, atom=atom, interactions=interactions, zeeman=zeeman,
+                        potential=potential, sim_params=sp, psi_init=psi0, enable_ddi=enable_ddi,
+                        c_dd=c_dd, secular_ddi=secular_ddi, precision=precision)
+    cur = float(dt)
+    E_prev = total_energy(ws)
+    converged = False
+    total = 0
+    dE = dpsi = float("nan")
+    while total < n_steps:
+        ws.copy_slot(1, 0)
+        for i in range(check_every):
+            split_step(ws, 1)
+            if i == 0 and total == 0:
+                _check_itp_overflow(ws, 1)
+        total += check_every
+        E = total_energy(ws)
+        if math.isnan(E) or E > E_prev:
+            ws.copy_slot(0, 1)
+            cur = max(cur * 0.5, 1e-8)
+            ws.set_dt(0, cur)
+        else:
+            dE = abs(E - E_prev)
+            d = ws.diff(0, 1, 1)[0]
+            dpsi = float(d[0] / d[1]) if d[1] > 0 else 0.0
+            if dE < tol and dpsi < tol:
+                converged = True
+                break
+            E_prev = E
+            new_dt = min(cur * 1.1, dt_max)
+            if new_dt != cur:
+                cur = new_dt
+                ws.set_dt(0, cur)
+    return dict(workspace=ws, converged=converged, energy=total_energy(ws), dE=dE, dpsi=dpsi,
+                steps=total, final_dt=cur)
 
 
 def find_ground_state_multistart(*, initial_states=("polar", "ferromagnetic", "uniform",

@@ -55,6 +55,9 @@ struct bec_ws {
   InstParams* d_params = nullptr;
   int* d_active = nullptr;
   unsigned long long* d_work = nullptr;  // profiling counters (tensor matvecs, spin rotations)
+  double* d_cpart = nullptr;   // constrained normalisation: per-component partials, targets,
+  double* d_ctarget = nullptr; // factors (allocated on first use)
+  double* d_cfac = nullptr;
   void* d_tw[10] = {};
   double* h_red = nullptr;  // pinned
   int max_blocks = 0;
@@ -634,6 +637,8 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
   if (ws->d_work) (void)hipFree(ws->d_work);
+  for (double* q : {ws->d_cpart, ws->d_ctarget, ws->d_cfac})
+    if (q) (void)hipFree(q);
   for (int i = 0; i < 10; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
   if (ws->h_red) (void)hipHostFree(ws->h_red);
@@ -796,6 +801,32 @@ int bec_leapfrog_segment(bec_ws* ws, int n_steps) {
     if (loss) items.push_back({IT_LOSS, 1.0});  // simulation.jl:74-76
   }
   return run_items(ws, items, 0, 0.0);
+}
+
+int bec_normalize_constrained(bec_ws* ws, const double* target_mz) {
+  if (!ws || !target_mz) return fail(BEC_EINVAL, "null argument");
+  if (n_active(ws) == 0) return BEC_OK;
+  constexpr int kBlk = 256;
+  if (!ws->d_cpart) {
+    if (hipMalloc(&ws->d_cpart, sizeof(double) * ws->B * ws->D * kBlk) != hipSuccess ||
+        hipMalloc(&ws->d_ctarget, sizeof(double) * ws->B) != hipSuccess ||
+        hipMalloc(&ws->d_cfac, sizeof(double) * ws->B * kMaxD) != hipSuccess)
+      return fail(BEC_ENOMEM, "constrained normalisation buffers");
+  }
+  CHK(sync_params(ws));
+  HIPCHK(hipMemcpyAsync(ws->d_ctarget, target_mz, sizeof(double) * ws->B, hipMemcpyHostToDevice,
+                        ws->stream));
+  hipError_t e = ws->prec == BEC_FP32
+                     ? launch_constrained_norm<float>(ws->stream, ws->g, ws->tb, ws->slot[0],
+                                                      n_active(ws), ws->d_cpart, kBlk,
+                                                      ws->d_ctarget, ws->d_cfac)
+                     : launch_constrained_norm<double>(ws->stream, ws->g, ws->tb, ws->slot[0],
+                                                       n_active(ws), ws->d_cpart, kBlk,
+                                                       ws->d_ctarget, ws->d_cfac);
+  HIPCHK(e);
+  // the host array must stay valid until the copy has been consumed
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  return BEC_OK;
 }
 
 int bec_loss_step(bec_ws* ws, double dt) {

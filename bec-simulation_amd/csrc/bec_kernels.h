@@ -95,6 +95,13 @@ template <typename T>
 hipError_t launch_scale_norm(hipStream_t st, const Geom& g, const DevTables& tb, void* psi,
                              int n_active);
 
+// Magnetisation-constrained normalisation of every active instance (target[inst] = Mz):
+// per-component norms into part[B][D][nblk], factors into fac[B][kMaxD], then the scaling.
+template <typename T>
+hipError_t launch_constrained_norm(hipStream_t st, const Geom& g, const DevTables& tb, void* psi,
+                                   int n_active, double* part, int nblk, const double* target,
+                                   double* fac);
+
 // partials: RS_NORM <- sum|a-b|^2 (or max|a-b| when want_max), RS_KIN <- sum|b|^2 (or max|a|)
 template <typename T>
 hipError_t launch_diff(hipStream_t st, const Geom& g, const DevTables& tb, const void* a,

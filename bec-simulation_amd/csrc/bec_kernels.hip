@@ -740,6 +740,91 @@ __global__ void scale_norm_kernel(Geom g, DevTables tb, cplx<T>* psi) {
     p[i] = scale64(p[i], s);
 }
 
+// Per-block, per-component partial norms: part[(inst*D + c)*nblk + blk] = sum |psi_c|^2.
+template <typename T>
+__global__ void comp_norm_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi,
+                                 double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int inst = tb.active[blockIdx.y];
+  const cplx<T>* p = psi + (size_t)inst * g.inst_stride;
+  for (int c = 0; c < g.D; ++c) {
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.nsites;
+         i += (long long)gridDim.x * blockDim.x)
+      acc += (double)abs2(p[(size_t)c * g.nsites + i]);
+    const double v = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[((size_t)inst * g.D + c) * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// ground_state.jl:279-320 for one instance per block (thread 0): component norms N_c
+// (fixed-order compensated sums of the partials, times dV), then the factors of
+// normalise -> Newton on lambda for Mz(lambda) = target (<= 20 iterations, lambda clamped
+// to [-10, 10], weights exp(lambda m)) -> normalise, composed into one factor per component.
+__global__ void constrained_factor_kernel(Geom g, DevTables tb, const double* __restrict__ part,
+                                          int nblk, const double* __restrict__ target,
+                                          double* __restrict__ fac) {
+  if (threadIdx.x != 0) return;
+  const int inst = tb.active[blockIdx.x];
+  const int D = g.D, F = (D - 1) / 2;
+  double nc[kMaxD];
+  double tot = 0.0;
+  for (int c = 0; c < D; ++c) {
+    double s = 0.0, comp = 0.0;
+    const double* q = part + ((size_t)inst * D + c) * nblk;
+    for (int b = 0; b < nblk; ++b) {  // Neumaier
+      const double t = s + q[b];
+      comp += fabs(s) >= fabs(q[b]) ? (s - t) + q[b] : (q[b] - t) + s;
+      s = t;
+    }
+    nc[c] = (s + comp) * g.dV;
+    tot += nc[c];
+  }
+  const double s0 = tot > 0.0 ? 1.0 / sqrt(tot) : 1.0;  // first _normalize_psi!
+  for (int c = 0; c < D; ++c) nc[c] *= s0 * s0;
+  const double tz = target[inst];
+  double lambda = 0.0;
+  for (int it = 0; it < 20; ++it) {
+    double w2n[kMaxD], total = 0.0;
+    for (int c = 0; c < D; ++c) {
+      const double w = exp(lambda * (double)(F - c));
+      w2n[c] = nc[c] * w * w;
+      total += w2n[c];
+    }
+    if (total < 1e-30) break;
+    double mz = 0.0;
+    for (int c = 0; c < D; ++c) mz += (double)(F - c) * w2n[c];
+    mz /= total;
+    if (fabs(mz - tz) < 1e-12) break;
+    double dmz = 0.0;
+    for (int c = 0; c < D; ++c) {
+      const double m = (double)(F - c);
+      dmz += 2.0 * m * (m - mz) * w2n[c] / total;
+    }
+    if (fabs(dmz) < 1e-30) break;
+    lambda -= (mz - tz) / dmz;
+    lambda = lambda < -10.0 ? -10.0 : (lambda > 10.0 ? 10.0 : lambda);
+  }
+  double tw = 0.0;
+  double w[kMaxD];
+  for (int c = 0; c < D; ++c) {
+    w[c] = exp(lambda * (double)(F - c));
+    tw += nc[c] * w[c] * w[c];
+  }
+  const double s1 = tw > 0.0 ? 1.0 / sqrt(tw) : 1.0;  // final _normalize_psi!
+  for (int c = 0; c < D; ++c) fac[(size_t)inst * kMaxD + c] = s0 * w[c] * s1;
+}
+
+template <typename T>
+__global__ void comp_scale_kernel(Geom g, DevTables tb, cplx<T>* psi,
+                                  const double* __restrict__ fac) {
+  const int inst = tb.active[blockIdx.y];
+  cplx<T>* p = psi + (size_t)inst * g.inst_stride;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < g.inst_stride;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = scale64(p[i], fac[(size_t)inst * kMaxD + i / g.nsites]);
+}
+
 template <typename T>
 __global__ void diff_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ a,
                             const cplx<T>* __restrict__ b, int want_max) {
@@ -982,6 +1067,21 @@ hipError_t launch_scale_norm(hipStream_t st, const Geom& g, const DevTables& tb,
 }
 
 template <typename T>
+hipError_t launch_constrained_norm(hipStream_t st, const Geom& g, const DevTables& tb, void* psi,
+                                   int n_active, double* part, int nblk, const double* target,
+                                   double* fac) {
+  hipLaunchKernelGGL((comp_norm_kernel<T>), dim3(nblk, n_active), dim3(256), 0, st, g, tb,
+                     (const cplx<T>*)psi, part);
+  hipLaunchKernelGGL(constrained_factor_kernel, dim3(n_active), dim3(64), 0, st, g, tb, part,
+                     nblk, target, fac);
+  long long nb = (g.inst_stride + 255) / 256;
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL((comp_scale_kernel<T>), dim3((int)nb, n_active), dim3(256), 0, st, g, tb,
+                     (cplx<T>*)psi, (const double*)fac);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_diff(hipStream_t st, const Geom& g, const DevTables& tb, const void* a,
                        const void* b, int n_active, int want_max, int nblocks) {
   hipLaunchKernelGGL((diff_kernel<T>), dim3(nblocks, n_active), dim3(256), 0, st, g, tb,
@@ -1015,7 +1115,10 @@ hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n)
   template hipError_t launch_scale_norm<T>(hipStream_t, const Geom&, const DevTables&, void*,  \
                                            int);                                               \
   template hipError_t launch_diff<T>(hipStream_t, const Geom&, const DevTables&, const void*,  \
-                                     const void*, int, int, int);
+                                     const void*, int, int, int);                             \
+  template hipError_t launch_constrained_norm<T>(hipStream_t, const Geom&, const DevTables&,   \
+                                                 void*, int, double*, int, const double*,     \
+                                                 double*);
 BEC_INST(double)
 BEC_INST(float)
 

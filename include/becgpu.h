@@ -116,6 +116,9 @@ int bec_strang_core(bec_ws* ws, double dt);
 int bec_yoshida_core(bec_ws* ws, double dt);
 /* _normalize_psi! (split_step.jl:185-195). */
 int bec_normalize(bec_ws* ws);
+/* _normalize_psi_constrained! (ground_state.jl:279-320) of every active instance, with
+ * target <F_z> target_mz[inst] (array of B values). */
+int bec_normalize_constrained(bec_ws* ws, const double* target_mz);
 
 /* total_energy terms (energy.jl:6-121) + norm and Mz (observables.jl:1-33),
  * out[B][BEC_E_COUNT] for every instance (inactive rows untouched). */

@@ -769,10 +769,41 @@ def magnetization(psi, F, dV):
 # ---------------------------------------------------------------------------
 
 
-def find_ground_state(ws: OracleWS, n_steps, tol):
+def normalize_constrained(psi, F, dV, target_mz):
+    """ground_state.jl:279-320: normalise, Newton on lambda (<= 20 iterations, clamp to
+    [-10, 10]) so that the exp(lambda m)-weighted component norms have <F_z> = target,
+    apply the weights, normalise again."""
+    psi = normalize(psi, dV)
+    D = 2 * F + 1
+    lam = 0.0
+    for _ in range(20):
+        norms = np.array([float(np.sum(np.abs(psi[c]) ** 2)) * dV * math.exp(lam * (F - c)) ** 2
+                          for c in range(D)])
+        tot = float(norms.sum())
+        if tot < 1e-30:
+            break
+        mz = sum((F - c) * norms[c] for c in range(D)) / tot
+        if abs(mz - target_mz) < 1e-12:
+            break
+        dmz = sum(2 * (F - c) * ((F - c) - mz) * norms[c] / tot for c in range(D))
+        if abs(dmz) < 1e-30:
+            break
+        lam -= (mz - target_mz) / dmz
+        lam = min(max(lam, -10.0), 10.0)
+    out = psi.copy()
+    for c in range(D):
+        out[c] = psi[c] * math.exp(lam * (F - c))
+    return normalize(out, dV)
+
+
+def find_ground_state(ws: OracleWS, n_steps, tol, target_mz=None):
     """ground_state.jl:80-163 after make_workspace: convergence every
-    max(1, n_steps // 10) steps on |dE| and max|dpsi|/max|psi|."""
+    max(1, n_steps // 10) steps on |dE| and max|dpsi|/max|psi|.  With target_mz the
+    workspace must have normalize_every = 0; the constrained normalisation runs at the
+    start and after every step."""
     save_every = max(1, n_steps // 10)
+    if target_mz is not None:
+        ws.psi = normalize_constrained(ws.psi, ws.F, ws.grid.dV, target_mz)
     E_prev = total_energy(ws)
     psi_prev = ws.psi.copy()
     converged = False
@@ -783,6 +814,8 @@ def find_ground_state(ws: OracleWS, n_steps, tol):
         steps = step
         if step <= 10 and np.any(np.isnan(ws.psi)):
             raise ValueError(f"NaN detected in ITP at step {step}")
+        if target_mz is not None:
+            ws.psi = normalize_constrained(ws.psi, ws.F, ws.grid.dV, target_mz)
         if step % save_every == 0:
             E = total_energy(ws)
             dE = abs(E - E_prev)
@@ -794,6 +827,41 @@ def find_ground_state(ws: OracleWS, n_steps, tol):
             E_prev = E
             psi_prev = ws.psi.copy()
     return dict(converged=converged, energy=total_energy(ws), dE=dE, dpsi=dpsi, steps=steps)
+
+
+def find_ground_state_adaptive(ws: OracleWS, n_steps, tol, dt_max):
+    """ground_state.jl:172-228: check every max(1, n_steps // 100) steps; an energy rise
+    (or NaN) restores psi and halves dt (floor 1e-8), otherwise convergence test and
+    dt *= 1.1 up to dt_max.  ws: ITP workspace with normalize_every = 1."""
+    check_every = max(1, n_steps // 100)
+    E_prev = total_energy(ws)
+    converged = False
+    total = 0
+    dE = dpsi = float("nan")
+    while total < n_steps:
+        backup = ws.psi.copy()
+        for _ in range(check_every):
+            split_step(ws)
+        total += check_every
+        E = total_energy(ws)
+        if math.isnan(E) or E > E_prev:
+            ws.psi = backup
+            ws.dt = max(ws.dt * 0.5, 1e-8)
+            ws.kphase = kinetic_phase(ws.grid, ws.dt, True)
+        else:
+            dE = abs(E - E_prev)
+            pmax = np.max(np.abs(ws.psi))
+            dpsi = float(np.max(np.abs(ws.psi - backup)) / pmax) if pmax > 0 else 0.0
+            if dE < tol and dpsi < tol:
+                converged = True
+                break
+            E_prev = E
+            new_dt = min(ws.dt * 1.1, dt_max)
+            if new_dt != ws.dt:
+                ws.dt = new_dt
+                ws.kphase = kinetic_phase(ws.grid, ws.dt, True)
+    return dict(converged=converged, energy=total_energy(ws), dE=dE, dpsi=dpsi, steps=total,
+                final_dt=ws.dt)
 
 
 def _snapshot(ws, rec):

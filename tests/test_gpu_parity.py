@@ -151,6 +151,42 @@ def test_loss_yoshida_driver():
     assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
 
 
+@pytest.mark.parametrize("F,target", [(1, 0.3), (2, -0.5)])
+def test_constrained_mz_ground_state_1d(F, target):
+    """find_ground_state(target_magnetization=...) (ground_state.jl:118-146,279-320)."""
+    og, gg = grids((64,), (20.0,))
+    psi0 = O.init_psi(og, F, "uniform")
+    ows = O.make_workspace(og, F, psi0, 0.005, True, omega=(1.0,), c0=10.0, c1=-0.5, q=0.05,
+                           n_steps=400, normalize_every=0)
+    ro = O.find_ground_state(ows, 400, 1e-10, target_mz=target)
+    rg = G.find_ground_state(grid=gg, atom=G.AtomSpecies("test", 1.0, F),
+                             interactions=G.InteractionParams(10.0, -0.5),
+                             zeeman=G.ZeemanParams(0.0, 0.05), potential=G.HarmonicTrap(1.0),
+                             dt=0.005, n_steps=400, tol=1e-10, psi_init=psi0,
+                             target_magnetization=target)
+    print("oracle", ro, "gpu", {k: v for k, v in rg.items() if k != "workspace"})
+    assert rg["steps"] == ro["steps"]
+    assert abs(rg["energy"] - ro["energy"]) <= E_TOL * abs(ro["energy"])
+    assert rel_l2(rg["workspace"].get_psi(), ows.psi) <= PSI_TOL
+    assert abs(A.magnetization(rg["workspace"]) - target) < 1e-10
+
+
+def test_adaptive_dt_ground_state_1d():
+    """find_ground_state(adaptive_dt=true, dt_max) (ground_state.jl:172-228)."""
+    og, gg = grids((64,), (15.0,))
+    psi0 = O.init_psi(og, 1, "polar")
+    ows = O.make_workspace(og, 1, psi0, 0.001, True, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=2000)
+    ro = O.find_ground_state_adaptive(ows, 2000, 1e-8, 0.01)
+    rg = G.find_ground_state(grid=gg, atom=G.Rb87, interactions=G.InteractionParams(10.0, -0.5),
+                             potential=G.HarmonicTrap(1.0), dt=0.001, n_steps=2000, tol=1e-8,
+                             psi_init=psi0, adaptive_dt=True, dt_max=0.01)
+    print("oracle", ro, "gpu", {k: v for k, v in rg.items() if k != "workspace"})
+    assert rg["steps"] == ro["steps"] and rg["converged"] == ro["converged"]
+    assert abs(rg["final_dt"] - ro["final_dt"]) <= 1e-15 * ro["final_dt"]
+    assert abs(rg["energy"] - ro["energy"]) <= E_TOL * abs(ro["energy"])
+    assert rel_l2(rg["workspace"].get_psi(), ows.psi) <= PSI_TOL
+
+
 def test_leapfrog_3d_spin1_snapshots():
     n, L = (32, 32, 32), (20.0, 20.0, 20.0)
     og, _ = grids(n, L)

@@ -366,6 +366,42 @@ def test_loss_step_single_site_closed_form():
         assert abs(out[c, 0] - v[c, 0] * math.exp(-(gam[c] + 0.02) * n * 0.7 / 2)) < 1e-15
 
 
+# --- constrained-Mz and adaptive-dt ITP (ground_state.jl) ------------------------
+def test_constrained_normalisation_closed_form():
+    """Spin-1 with an empty m=0 component: Mz(lambda) = tanh(2 lambda + ln(N+/N-)/2), so the
+    Newton solution is lambda = (atanh(target) - ln(N+/N-)/2)/2; the result has unit norm and
+    <F_z> = target, and target = current Mz leaves the state normalised but unchanged."""
+    g = O.make_grid((32,), (10.0,))
+    base = O.gaussian(g, (2.0,))
+    psi = np.stack([0.8 * base, 0 * base, 0.5 * base]).astype(complex)
+    Np, Nm = (float(np.sum(np.abs(psi[c]) ** 2)) for c in (0, 2))
+    for target in (-0.4, 0.1, 0.7):
+        out = O.normalize_constrained(psi, 1, g.dV, target)
+        lam = (math.atanh(target) - 0.5 * math.log(Np / Nm)) / 2
+        exp = O.normalize(np.stack([psi[0] * math.exp(lam), psi[1], psi[2] * math.exp(-lam)]), g.dV)
+        assert np.max(np.abs(out - exp)) < 1e-12 * np.max(np.abs(exp))
+        assert abs(O.total_norm(out, g.dV) - 1) < 1e-13
+        assert abs(O.magnetization(out, 1, g.dV) - target) < 1e-12
+    m0 = O.magnetization(O.normalize(psi, g.dV), 1, g.dV)
+    out = O.normalize_constrained(psi, 1, g.dV, m0)
+    assert np.max(np.abs(out - O.normalize(psi, g.dV))) < 1e-14
+
+
+def test_adaptive_itp_reaches_fixed_dt_ground_state():
+    """test_adaptive_dt.jl:6-25 (c0 = 10, c1 = -0.5 here): adaptive-dt ITP ends within 1 %
+    of the fixed-dt energy; with c0 = c1 = 0 it reaches the harmonic E = 1/2."""
+    g = O.make_grid((64,), (15.0,))
+    psi0 = O.init_psi(g, 1, "polar")
+    wf = O.make_workspace(g, 1, psi0, 0.001, True, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=5000)
+    rf = O.find_ground_state(wf, 5000, 1e-8)
+    wa = O.make_workspace(g, 1, psi0, 0.001, True, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=5000)
+    ra = O.find_ground_state_adaptive(wa, 5000, 1e-8, 0.01)
+    assert abs(rf["energy"] - ra["energy"]) / abs(rf["energy"]) < 0.01
+    w0 = O.make_workspace(g, 1, psi0, 0.001, True, omega=(1.0,), n_steps=5000)
+    r0 = O.find_ground_state_adaptive(w0, 5000, 1e-10, 0.01)
+    assert abs(r0["energy"] - 0.5) < 1e-6
+
+
 # --- kinetic step / diagonal step closed forms (test_physics_level0.jl) ----------
 def test_kinetic_plane_wave_exact():
     g = O.make_grid((16, 8), (5.0, 3.0))
