@@ -48,6 +48,7 @@ struct bec_ws {
   void* fhat64 = nullptr;
   void* phi64 = nullptr;
   double* d_ktab = nullptr;
+  double* d_sktab = nullptr;  // k of the zero-padded DDI grid (null: the grid's own k)
   double* d_axtab = nullptr;
   double* d_varr = nullptr;
   double* d_red = nullptr;
@@ -106,6 +107,7 @@ static bool is_pow2_supported(int n) {
 static void refresh_tables(bec_ws* ws) {
   DevTables& t = ws->tb;
   t.ktab = ws->d_ktab;
+  t.sktab = ws->d_sktab ? ws->d_sktab : ws->d_ktab;
   t.axtab = ws->d_axtab;
   t.Varr = ws->d_varr && std::any_of(ws->has_varr.begin(), ws->has_varr.end(), [](int v) { return v; })
                ? ws->d_varr
@@ -136,6 +138,16 @@ static double psi_bytes(const bec_ws* ws) {
 }
 static double half_field_bytes(const bec_ws* ws) {  // 3 rFFT half-grid fields
   return 3.0 * ws->g.nxh * (double)(ws->g.nsites / ws->g.n[0]) * (double)ws->csz * n_active(ws);
+}
+// DDI spectra pass along `axis`: lines of the other transverse axis it visits, entries of
+// each line it reads and writes (zero padding: the first n entries only on one side)
+static double strided_ddi_bytes(const bec_ws* ws, int axis, int mode) {
+  const Geom& g = ws->g;
+  const int N = g.sn[axis];
+  const int n_in = (g.spad && mode != SM_INV) ? g.n[axis] : N;
+  const int n_out = (g.spad && mode != SM_FWD) ? g.n[axis] : N;
+  const int n_other = axis == 1 ? g.n[2] : g.sn[1];
+  return 3.0 * g.nxh * (double)n_other * (n_in + n_out) * (double)ws->csz * n_active(ws);
 }
 static double phi_bytes(const bec_ws* ws) {  // 3 real fields
   return 3.0 * (double)ws->g.nsites * (ws->csz / 2) * n_active(ws);
@@ -188,7 +200,7 @@ static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_fra
                            : 1 + (mode == SM_KIN ? 1 : mode == SM_INV ? 2 : mode == SM_KE ? 3 : 0);
   double b = 0.0;
   if (ws->prof_on)
-    b = ddi_data ? 2.0 * half_field_bytes(ws) : (mode == SM_KE ? 1.0 : 2.0) * psi_bytes(ws);
+    b = ddi_data ? strided_ddi_bytes(ws, axis, mode) : (mode == SM_KE ? 1.0 : 2.0) * psi_bytes(ws);
   const int pe = prof_begin(ws, cls, b);
   if (ws->prec == BEC_FP32)
     e = launch_strided<float>(ws->stream, ws->g, ws->tb, axis, mode, data, n_active(ws), kin_frac,
@@ -478,7 +490,9 @@ static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out
   if (c->enable_ddi) {
     const int nxh = nx / 2 + 1;
     const int pitch = (nxh + 15) / 16 * 16;
-    bytes += 3LL * B * pitch * ny * nz * csz;
+    const int pf = c->ddi_padding ? 2 : 1;  // padded spectra: (2nx/2+1) x 2ny x 2nz
+    const int ppitch = (pf * nx / 2 + 1 + 15) / 16 * 16;
+    bytes += 3LL * B * ppitch * (pf * ny) * (c->ndim >= 3 ? pf * nz : nz) * csz;
     bytes += 3LL * B * ns * (csz / 2);
     if (c->precision == BEC_FP32) bytes += 3LL * B * pitch * ny * nz * 16;
   }
@@ -510,7 +524,11 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (c->precision != BEC_FP64 && c->precision != BEC_FP32)
     return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
   if (c->enable_ddi && c->ndim < 2) return fail(BEC_EUNSUP, "DDI needs a 2-D or 3-D grid here");
-  if (c->ddi_padding) return fail(BEC_EUNSUP, "zero-padded DDI is not built yet");
+  if (c->ddi_padding && !c->enable_ddi) return fail(BEC_EINVAL, "ddi_padding needs enable_ddi");
+  if (c->ddi_padding)
+    for (int d = 0; d < c->ndim; ++d)
+      if (c->n[d] > 256)
+        return fail(BEC_EUNSUP, "zero-padded DDI supports sides up to 256 (padded FFTs <= 512)");
   if (!c->x_tab || !c->k_tab) return fail(BEC_EINVAL, "missing tables");
 
   bec_ws* ws = new bec_ws();
@@ -539,9 +557,12 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   for (int d = 0; d < c->ndim; ++d) dV *= c->L[d] / c->n[d];
   g.dV = dV;
   g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
-  g.nxh = g.n[0] / 2 + 1;
+  g.spad = c->enable_ddi && c->ddi_padding ? 1 : 0;
+  for (int d = 0; d < 3; ++d) g.sn[d] = d < c->ndim ? g.n[d] * (g.spad ? 2 : 1) : 1;
+  g.smax = std::max(g.sn[0], std::max(g.sn[1], g.sn[2]));
+  g.nxh = g.sn[0] / 2 + 1;
   g.nxh_pitch = (g.nxh + 15) / 16 * 16;
-  g.fhat_field_stride = (long long)g.nxh_pitch * g.n[1] * g.n[2];
+  g.fhat_field_stride = (long long)g.nxh_pitch * g.sn[1] * g.sn[2];
   g.fhat_inst_stride = 3 * g.fhat_field_stride;
   g.ddi_secular = c->ddi_secular;
   ws_bytes(c, &ws->max_blocks, &ws->n_xblocks);
@@ -585,6 +606,18 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMalloc(&ws->d_ktab, kt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "ktab");
   if (hipMemcpy(ws->d_ktab, kt.data(), kt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(BEC_EDEVICE, "ktab copy");
+  if (g.spad) {
+    // k of the padded grid (same dx, twice the points): fftfreq(2n) * 2 pi / (2n dx)
+    std::vector<double> sk(3 * g.smax, 0.0);
+    for (int d = 0; d < c->ndim; ++d) {
+      const int ns = g.sn[d];
+      const double dk = 2.0 * M_PI / (ns * (c->L[d] / c->n[d]));
+      for (int m = 0; m < ns; ++m) sk[d * g.smax + m] = (m < ns / 2 ? m : m - ns) * dk;
+    }
+    if (hipMalloc(&ws->d_sktab, sk.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "sktab");
+    if (hipMemcpy(ws->d_sktab, sk.data(), sk.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "sktab copy");
+  }
   const size_t axb = (size_t)ws->B * 3 * g.nmax * 8;
   if (hipMalloc(&ws->d_axtab, axb) != hipSuccess) return cleanup(BEC_ENOMEM, "axtab");
   if (hipMemset(ws->d_axtab, 0, axb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
@@ -632,7 +665,7 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->phi) (void)hipFree(ws->phi);
   for (void* q : {ws->psi64, ws->scratch64, ws->fhat64, ws->phi64})
     if (q) (void)hipFree(q);
-  for (double* p : {ws->d_ktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials})
+  for (double* p : {ws->d_ktab, ws->d_sktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials})
     if (p) (void)hipFree(p);
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
@@ -899,9 +932,25 @@ static int fetch_red(bec_ws* ws) {
 }
 
 // Energy partial sums of psi (FP64 arithmetic on FP64 data), reduced into d_red.
-static int energy_partials_fp64(bec_ws* ws, const DevTables& tb, void* psi, void* scratch,
+// The DDI energy always uses the unpadded convolution (energy.jl:111-121), also in a
+// workspace whose steps use the zero-padded one: same grid, spectra of the unpadded shape.
+static Geom unpadded_geom(const Geom& g0) {
+  Geom g = g0;
+  g.spad = 0;
+  for (int d = 0; d < 3; ++d) g.sn[d] = g.n[d];
+  g.smax = g.nmax;
+  g.nxh = g.n[0] / 2 + 1;
+  g.nxh_pitch = (g.nxh + 15) / 16 * 16;
+  g.fhat_field_stride = (long long)g.nxh_pitch * g.n[1] * g.n[2];
+  g.fhat_inst_stride = 3 * g.fhat_field_stride;
+  return g;
+}
+
+static int energy_partials_fp64(bec_ws* ws, const DevTables& tb0, void* psi, void* scratch,
                                 void* fhat, void* phi) {
-  const Geom& g = ws->g;
+  const Geom g = unpadded_geom(ws->g);
+  DevTables tb = tb0;
+  tb.sktab = tb.ktab;
   const int na = n_active(ws);
   XProgram p{};
   p.itp = ws->itp;
@@ -965,7 +1014,7 @@ int bec_energy(bec_ws* ws, double* out) {
       if (hipMalloc(&ws->psi64, n * 16) != hipSuccess) return fail(BEC_ENOMEM, "psi64");
       if (hipMalloc(&ws->scratch64, n * 16) != hipSuccess) return fail(BEC_ENOMEM, "scratch64");
       if (ws->ddi) {
-        const size_t fb = 16 * (size_t)g.fhat_inst_stride * ws->B;
+        const size_t fb = 16 * (size_t)unpadded_geom(g).fhat_inst_stride * ws->B;
         if (hipMalloc(&ws->fhat64, fb) != hipSuccess) return fail(BEC_ENOMEM, "fhat64");
         HIPCHK(hipMemset(ws->fhat64, 0, fb));
         if (hipMalloc(&ws->phi64, 24 * (size_t)g.nsites * ws->B) != hipSuccess)

@@ -21,8 +21,14 @@ struct Geom {
   double inv_ntot;     // 1/nsites
   double dV;
   int nmax;            // max(n)
-  int nxh, nxh_pitch;  // rFFT half-row length and row pitch of the DDI spectra
-  long long fhat_field_stride;  // nxh_pitch*n[1]*n[2]
+  // DDI spectral grid: sn = 2n per axis with the zero-padded convolution (ddi_padded.jl),
+  // else sn = n.  Spin densities fill the first n entries of each axis, Phi is read back
+  // from the same entries.
+  int sn[3];
+  int smax;            // max(sn)
+  int spad;            // 1: zero-padded DDI convolution
+  int nxh, nxh_pitch;  // rFFT half-row length sn[0]/2+1 and row pitch of the DDI spectra
+  long long fhat_field_stride;  // nxh_pitch*sn[1]*sn[2]
   long long fhat_inst_stride;   // 3*fhat_field_stride
   int ddi_secular;
   int pad_;
@@ -30,6 +36,7 @@ struct Geom {
 
 struct DevTables {
   const double* ktab;     // [3][nmax] fftfreq k values per axis
+  const double* sktab;    // [3][smax] fftfreq k values of the DDI spectral grid
   const double* axtab;    // [B][3][nmax] omega_d^2 x_i^2 (trap), per instance
   const double* Varr;     // [B][nsites] explicit potential or null
   const InstParams* params;  // [B]

@@ -104,10 +104,13 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   // P(line*NX) + P(i).
   auto P8 = [](int e) __attribute__((always_inline)) { return e + (e >> 3); };
 
+  // DDI auxiliary lines: 2 per row, of the spectral x length (2 NX when zero-padded)
+  const int AL = g.spad ? 2 * NX : NX;
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
   cplx<T>* aux = tile + (size_t)D * TS / 8 * 9;
-  // FP64 Zeeman factors (TS/8*9 is even, so this stays 16-byte aligned)
-  cplx<double>* zf = reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * TS / 8 * 9 : 0));
+  // FP64 Zeeman factors (ROWS*AL/8*9 is even, so this stays 16-byte aligned)
+  cplx<double>* zf =
+      reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * ROWS * AL / 8 * 9 : 0));
   double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
 
   const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
@@ -145,6 +148,30 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   double norm_acc = 0.0, ke_acc = 0.0, phimax = 0.0;
 
   const int line = tid / TEAM, tt = tid - (tid / TEAM) * TEAM;
+  // FFT of the 2*ROWS auxiliary lines of length AL (NX, or 2 NX with zero padding; for
+  // the longer lines the block covers them in rounds of nthr / (2 NX / EX) lines)
+  auto aux_fft = [&](auto invc) __attribute__((always_inline)) {
+    constexpr bool INVF = decltype(invc)::value;
+    if (AL == NX) {
+      fft_team<T, NX, INVF, EX, true>(aux + P8(line * NX), 1, tt, line < 2 * ROWS, tw);
+    } else {
+      if constexpr (2 * NX <= 512) {
+        constexpr int TEAM2 = 2 * NX / EX;
+        const cplx<double>* tw2 = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<2 * NX>()]);
+        const int lpr = nthr / TEAM2, l2 = tid / TEAM2, t2 = tid - l2 * TEAM2;
+        for (int l0 = 0; l0 < 2 * ROWS; l0 += lpr) {
+          const int ln = l0 + l2;
+          const bool act = l2 < lpr && ln < 2 * ROWS;
+          fft_team<T, 2 * NX, INVF, EX, true>(aux + P8((act ? ln : 0) * 2 * NX), 1, t2, act, tw2);
+        }
+      }
+    }
+  };
+  // spectra row of tile row r: (y, z) -> y + z*sn[1]
+  auto srow_of = [&](int r) __attribute__((always_inline)) {
+    const int row = row0 + r;
+    return (row % g.n[1]) + (row / g.n[1]) * g.sn[1];
+  };
   if (st & XS_INV_FFT)
     fft_team<T, NX, true, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
 
@@ -155,23 +182,23 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     // C2R along x of the three DDI potential spectra (reference irfft semantics: bins
     // 0..NX/2 with the imaginary parts of DC and Nyquist dropped, Hermitian extension).
     const size_t fb = (size_t)inst * g.fhat_inst_stride;
-    for (int e = tid; e < TS; e += nthr) {
-      const int r = e / NX, x = e - r * NX;
-      const size_t rowoff = (size_t)(row0 + r) * g.nxh_pitch;
-      const int kk = x <= NX / 2 ? x : NX - x;
+    for (int e = tid; e < ROWS * AL; e += nthr) {
+      const int r = e / AL, x = e - r * AL;
+      const size_t rowoff = (size_t)srow_of(r) * g.nxh_pitch;
+      const int kk = x <= AL / 2 ? x : AL - x;
       cplx<T> yx = fhat[fb + rowoff + kk];
       cplx<T> yy = fhat[fb + g.fhat_field_stride + rowoff + kk];
       cplx<T> yz = fhat[fb + 2 * g.fhat_field_stride + rowoff + kk];
-      if (kk == 0 || kk == NX / 2) {
+      if (kk == 0 || kk == AL / 2) {
         yx.y = 0; yy.y = 0; yz.y = 0;
-      } else if (x > NX / 2) {
+      } else if (x > AL / 2) {
         yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
       }
-      aux[P8((2 * r) * NX + x)] = mk(yx.x - yy.y, yx.y + yy.x);
-      aux[P8((2 * r + 1) * NX + x)] = yz;
+      aux[P8((2 * r) * AL + x)] = mk(yx.x - yy.y, yx.y + yy.x);
+      aux[P8((2 * r + 1) * AL + x)] = yz;
     }
     __syncthreads();
-    fft_team<T, NX, true, EX, true>(aux + P8(line * NX), 1, tt, line < 2 * ROWS, tw);
+    aux_fft(std::true_type{});
   }
 
   auto site_coords = [&](int s, int& x, int& y, int& z) {
@@ -267,11 +294,12 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     __syncthreads();
   };
 
-  if (st & XS_LOAD_PHI) {
-    // max |Phi| for the imaginary-time overflow guard (ground_state.jl:4-29)
+  if ((st & XS_LOAD_PHI) && !g.spad) {
+    // max |Phi| for the imaginary-time overflow guard (ground_state.jl:4-29); the guard
+    // reads the unpadded Phi buffers, which the padded convolution does not write
     for (int s = tid; s < TS; s += nthr) {
       const int r = s / NX, x = s - r * NX;
-      const cplx<T> a = aux[P8((2 * r) * NX + x)], b = aux[P8((2 * r + 1) * NX + x)];
+      const cplx<T> a = aux[P8((2 * r) * AL + x)], b = aux[P8((2 * r + 1) * AL + x)];
       phimax = fmax(phimax, fmax(fabs((double)a.x), fmax(fabs((double)a.y), fabs((double)b.x))));
     }
   }
@@ -280,7 +308,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     T* ph = phi + (size_t)inst * 3 * g.nsites + (size_t)row0 * NX;
     for (int s2 = tid; s2 < TS; s2 += nthr) {
       const int r = s2 / NX, x = s2 - r * NX;
-      const cplx<T> a = aux[P8((2 * r) * NX + x)], b = aux[P8((2 * r + 1) * NX + x)];
+      const cplx<T> a = aux[P8((2 * r) * AL + x)], b = aux[P8((2 * r + 1) * AL + x)];
       ph[s2] = a.x;
       ph[g.nsites + s2] = a.y;
       ph[2 * g.nsites + s2] = b.x;
@@ -314,22 +342,28 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       T fx, fy, fz;
       spin_vec<T, D>(sp, fx, fy, fz);
       const int r = s / NX, x = s - r * NX;
-      aux[P8((2 * r) * NX + x)] = mk(fx, fy);
-      aux[P8((2 * r + 1) * NX + x)] = mk(fz, (T)0);
+      aux[P8((2 * r) * AL + x)] = mk(fx, fy);
+      aux[P8((2 * r + 1) * AL + x)] = mk(fz, (T)0);
+    }
+    if (AL > NX) {  // zero padding: entries NX..2NX-1 of every line
+      for (int e = tid; e < 2 * ROWS * NX; e += nthr) {
+        const int l = e / NX, x = NX + (e - l * NX);
+        aux[P8(l * AL + x)] = mk((T)0, (T)0);
+      }
     }
     __syncthreads();
-    fft_team<T, NX, false, EX, true>(aux + P8(line * NX), 1, tt, line < 2 * ROWS, tw);
+    aux_fft(std::false_type{});
     const size_t fb = (size_t)inst * g.fhat_inst_stride;
-    constexpr int NH = NX / 2 + 1;
+    const int NH = AL / 2 + 1;
     for (int e = tid; e < ROWS * NH; e += nthr) {
       const int r = e / NH, k = e - r * NH;
-      const cplx<T> z1 = aux[P8((2 * r) * NX + k)];
-      const cplx<T> z1n = conjc(aux[P8((2 * r) * NX + ((NX - k) & (NX - 1)))]);
+      const cplx<T> z1 = aux[P8((2 * r) * AL + k)];
+      const cplx<T> z1n = conjc(aux[P8((2 * r) * AL + ((AL - k) & (AL - 1)))]);
       const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
-      const size_t o = fb + (size_t)(row0 + r) * g.nxh_pitch + k;
+      const size_t o = fb + (size_t)srow_of(r) * g.nxh_pitch + k;
       fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
       fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
-      fhat[o + 2 * g.fhat_field_stride] = aux[P8((2 * r + 1) * NX + k)];
+      fhat[o + 2 * g.fhat_field_stride] = aux[P8((2 * r + 1) * AL + k)];
     }
   }
 
@@ -557,10 +591,16 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
 
   const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
   const int valid = ddi_data ? g.nxh : g.n[0];
+  const int n1 = ddi_data ? g.sn[1] : g.n[1];
   const long long fstride = ddi_data ? g.fhat_field_stride : g.nsites;
   const long long istride = ddi_data ? g.fhat_inst_stride : g.inst_stride;
-  const long long estride = axis == 1 ? (long long)pitch : (long long)pitch * g.n[1];
-  const long long ostride = axis == 1 ? (long long)pitch * g.n[1] : (long long)pitch;
+  const long long estride = axis == 1 ? (long long)pitch : (long long)pitch * n1;
+  const long long ostride = axis == 1 ? (long long)pitch * n1 : (long long)pitch;
+  // zero padding (DDI spectra of a padded grid): a forward transform reads only the first
+  // n[axis] entries (the rest are zero), and an inverse one keeps only those outputs
+  const bool padded = ddi_data && g.spad;
+  const int n_in = (padded && MODE != SM_INV) ? g.n[axis] : N;
+  const int n_out = (padded && MODE != SM_FWD) ? g.n[axis] : N;
   const int nxch = (valid + TB - 1) / TB;
   const int xch = blockIdx.x % nxch;
   const int other = blockIdx.x / nxch;
@@ -571,7 +611,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
     const int xi = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
     cplx<T> v = mk((T)0, (T)0);
-    if (x0 + xi < valid) v = data[base + (size_t)f * fstride + (size_t)e * estride + xi];
+    if (x0 + xi < valid && e < n_in) v = data[base + (size_t)f * fstride + (size_t)e * estride + xi];
     tile[idx] = v;
   }
   // per-axis phase tables for the kinetic multiply
@@ -617,10 +657,10 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       const int x = x0 + xi2;
       // rfftfreq along x (Nyquist +), fftfreq along y and z (ddi.jl:66-68)
       double kx = 0.0;
-      if (x < valid) kx = (x == g.n[0] / 2) ? -tb.ktab[g.n[0] / 2] : tb.ktab[x];
+      if (x < valid) kx = (x == g.sn[0] / 2) ? -tb.sktab[g.sn[0] / 2] : tb.sktab[x];
       double ky, kz;
-      if (axis == 2) { ky = tb.ktab[g.nmax + other]; kz = tb.ktab[2 * g.nmax + e]; }
-      else { ky = tb.ktab[g.nmax + e]; kz = 0.0; }
+      if (axis == 2) { ky = tb.sktab[g.smax + other]; kz = tb.sktab[2 * g.smax + e]; }
+      else { ky = tb.sktab[g.smax + e]; kz = 0.0; }
       double k2 = kx * kx;
       if (g.ndim >= 2) k2 += ky * ky;
       if (g.ndim >= 3) k2 += kz * kz;
@@ -642,7 +682,8 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       const cplx<T> fx = tile[(0 * N + e) * TB + xi2];
       const cplx<T> fy = tile[(1 * N + e) * TB + xi2];
       const cplx<T> fz = tile[(2 * N + e) * TB + xi2];
-      const double s = C * g.inv_ntot;
+      // normalisation of the inverse transform of the (possibly padded) spectral grid
+      const double s = C / ((double)g.sn[0] * (double)g.sn[1] * (double)g.sn[2]);
       auto mix = [&](double a, double b, double c) -> cplx<T> {
         const double re = a * (double)fx.x + b * (double)fy.x + c * (double)fz.x;
         const double im = a * (double)fx.y + b * (double)fy.y + c * (double)fz.y;
@@ -685,7 +726,8 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
 
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
     const int xi2 = idx % TB, rest2 = idx / TB, e = rest2 % N, f2 = rest2 / N;
-    if (x0 + xi2 < valid) data[base + (size_t)f2 * fstride + (size_t)e * estride + xi2] = tile[idx];
+    if (x0 + xi2 < valid && e < n_out)
+      data[base + (size_t)f2 * fstride + (size_t)e * estride + xi2] = tile[idx];
   }
 }
 
@@ -887,9 +929,9 @@ int xpass_rows(int D, int NX, int nrows) {
   return r;
 }
 
-static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux) {
-  // padded tiles: 9 slots per 8 elements
-  return csz * ((size_t)D * rows * NX + (has_aux ? 2 * rows * NX : 0)) / 8 * 9 +
+static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int AL) {
+  // padded tiles: 9 slots per 8 elements; AL: length of the DDI auxiliary lines
+  return csz * ((size_t)D * rows * NX + (has_aux ? 2 * (size_t)rows * AL : 0)) / 8 * 9 +
          sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double);
 }
 
@@ -898,7 +940,7 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
                           const void* src, void* dst, void* fhat, int n_active, void* phi) {
   const int rows = xpass_rows(D, NX, g.nrows);
   const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
-  const size_t lds = xpass_lds(D, rows, NX, sizeof(cplx<T>), has_aux);
+  const size_t lds = xpass_lds(D, rows, NX, sizeof(cplx<T>), has_aux, g.spad ? 2 * NX : NX);
   const int threads = D * rows * (NX / xpass_e<NX, D>());
   if (threads > xpass_max_threads<D>()) return hipErrorInvalidConfiguration;
   dim3 grid(xpass_grid(g), n_active);
@@ -984,6 +1026,13 @@ hipError_t launch_sites(hipStream_t st, const Geom& g, const DevTables& tb, cons
 
 int strided_tb(int N, int fields, int csize) { return strided_tb_c(N, fields, csize); }
 
+// Lines of the other transverse axis that a y/z pass visits.  Zero-padded DDI spectra: the
+// y passes only touch the z < n[2] planes that hold data (or are kept); the z pass all sn[1].
+static int strided_other(const Geom& g, int axis, int ddi_data) {
+  if (!ddi_data) return axis == 1 ? g.n[2] : g.n[1];
+  return axis == 1 ? g.n[2] : g.sn[1];
+}
+
 template <typename T, int N, int MODE, int FIELDS>
 static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, int axis,
                             void* data, int n_active, double kin_frac, double dt_override,
@@ -992,7 +1041,7 @@ static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, 
   constexpr int TB = strided_tb_c(N, FIELDS, (int)sizeof(cplx<T>));
   const int valid = ddi_data ? g.nxh : g.n[0];
   const int nxch = (valid + TB - 1) / TB;
-  const int n_other = axis == 1 ? g.n[2] : g.n[1];
+  const int n_other = strided_other(g, axis, ddi_data);
   const int ngroups = ddi_data ? 1 : g.D / FIELDS;
   dim3 grid(nxch * n_other, ngroups, n_active);
   const int threads = FIELDS * TB * (N / 8);
@@ -1030,13 +1079,14 @@ template <typename T>
 hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, int axis, int mode,
                           void* data, int n_active, double kin_frac, double dt_override, int itp,
                           int ddi_data, int* nblocks_out) {
+  const int len = ddi_data ? g.sn[axis] : g.n[axis];
   if (nblocks_out) {
-    const int tbw = strided_tb(g.n[axis], ddi_data ? 3 : 1, (int)sizeof(cplx<T>));
+    const int tbw = strided_tb(len, ddi_data ? 3 : 1, (int)sizeof(cplx<T>));
     const int valid = ddi_data ? g.nxh : g.n[0];
-    const int n_other = axis == 1 ? g.n[2] : g.n[1];
+    const int n_other = strided_other(g, axis, ddi_data);
     *nblocks_out = ((valid + tbw - 1) / tbw) * n_other * (ddi_data ? 1 : g.D);
   }
-  switch (g.n[axis]) {
+  switch (len) {
     case 16: return strided_mode<T, 16>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 32: return strided_mode<T, 32>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 64: return strided_mode<T, 64>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);

@@ -258,6 +258,48 @@ def test_ddi_2d_spin6_energy():
     assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
 
 
+@pytest.mark.parametrize("itp", [False, True])
+def test_ddi_padded_3d_split_step(itp):
+    """Zero-padded DDI convolution (ddi_padded.jl): spectra on the (2n)^3 grid, Phi cropped;
+    the energy's DDI term stays on the unpadded convolution (energy.jl:111-121)."""
+    n, L = (16, 32, 16), (12.0, 14.0, 12.0)
+    og, _ = grids(n, L)
+    psi = random_psi(og.shape, 3, 23) * O.gaussian(og, (2.5, 2.5, 2.5))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 1, psi, 0.002, itp, omega=(1, 1, 1), c0=6.0, c1=-0.4, ddi=True, cdd=2.0,
+                    padding=True)
+    assert ows.Qpad is not None
+    for _ in range(4):
+        O.split_step(ows)
+    A.split_step(gws, 4)
+    err = rel_l2(gws.get_psi(), ows.psi)
+    print("padded err", err)
+    assert err <= PSI_TOL
+    eo, eg = O.energy_terms(ows), A.energy_terms(gws)
+    for k in ("kin", "trap", "c0", "c1", "ddi"):
+        assert abs(eg[k] - eo[k]) <= 1e-12 * max(1.0, abs(eo[k])), k
+
+
+def test_ddi_padded_spin6_2d_and_fp32():
+    """Padded DDI for spin 6 on a 2-D grid (FP64 parity) and the same steps FP32-bulk."""
+    n, L = (32, 16), (10.0, 8.0)
+    og, _ = grids(n, L)
+    psi = random_psi(og.shape, 13, 6) * O.gaussian(og, (2.0, 2.0))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, 6, psi, 0.001, False, omega=(1, 1), p=0.5, q=0.1, c0=5.0, c1=0.3,
+                    ddi=True, cdd=1.5, padding=True)
+    _, gws32 = pair(n, L, 6, psi, 0.001, False, omega=(1, 1), p=0.5, q=0.1, c0=5.0, c1=0.3,
+                    ddi=True, cdd=1.5, padding=True, precision="fp32")
+    for _ in range(3):
+        O.split_step(ows)
+    A.split_step(gws, 3)
+    A.split_step(gws32, 3)
+    assert rel_l2(gws.get_psi(), ows.psi) <= PSI_TOL
+    e32 = rel_l2(gws32.get_psi(), ows.psi)
+    print("fp32 padded err", e32)
+    assert e32 <= 1e-5
+
+
 def test_strang_and_yoshida_core():
     n, L = (16, 16, 32), (10.0, 10.0, 12.0)
     og, _ = grids(n, L)

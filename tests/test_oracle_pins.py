@@ -505,22 +505,14 @@ def test_ddi_q_traceless_and_secular():
     assert np.allclose(Qs["xx"], -Qs["zz"] / 2) and np.allclose(Qs["xy"], 0)
 
 
-def test_ddi_potential_brute_force():
+def _brute_phi(Fs, Q, cdd, n):
     """Phi from the explicit definition: Phi_a(r) = Re (1/N) sum_k w(kx) e^{i k.r}
     C sum_b Q_ab(k) F_b(k), the sum over the half spectrum kx = 0..nx/2 with
     w = 1 at kx = 0, nx/2 and 2 otherwise, and the imaginary part of the kx = 0
     and kx = nx/2 planes dropped after the y, z inverse sums (c2r semantics)."""
-    n, dx = (4, 6, 4), (0.9, 0.7, 1.1)
-    r = rng(8)
-    Fs = [r.normal(size=(n[2], n[1], n[0])) for _ in range(3)]
-    Q = O.ddi_q(n, dx)
-    cdd = 1.7
-    phi = O.ddi_potential(*Fs, Q, cdd)
     nx, ny, nz = n
     N = nx * ny * nz
-    X = np.arange(nx)
-    Yv = np.arange(ny)
-    Zv = np.arange(nz)
+    X, Yv, Zv = np.arange(nx), np.arange(ny), np.arange(nz)
     Fk = []
     for F in Fs:  # brute-force forward DFT onto the half spectrum
         out = np.zeros((nz, ny, nx // 2 + 1), complex)
@@ -532,6 +524,7 @@ def test_ddi_potential_brute_force():
                     out[kz, ky, kx] = np.sum(F * e)
         Fk.append(out)
     names = [["xx", "xy", "xz"], ["xy", "yy", "yz"], ["xz", "yz", "zz"]]
+    res_all = []
     for a in range(3):
         Pk = cdd * sum(Q[names[a][b]] * Fk[b] for b in range(3))
         res = np.zeros((nz, ny, nx))
@@ -545,7 +538,41 @@ def test_ddi_potential_brute_force():
                 res += np.real(Zp)[:, :, None] * np.cos(2 * np.pi * kx * X / nx)[None, None, :]
             else:
                 res += 2 * np.real(Zp[:, :, None] * np.exp(2j * np.pi * kx * X / nx)[None, None, :])
-        assert np.allclose(phi[a], res / N, atol=1e-12)
+        res_all.append(res / N)
+    return res_all
+
+
+def test_ddi_potential_brute_force():
+    n, dx = (4, 6, 4), (0.9, 0.7, 1.1)
+    r = rng(8)
+    Fs = [r.normal(size=(n[2], n[1], n[0])) for _ in range(3)]
+    Q = O.ddi_q(n, dx)
+    phi = O.ddi_potential(*Fs, Q, 1.7)
+    ref = _brute_phi(Fs, Q, 1.7, n)
+    for a in range(3):
+        assert np.allclose(phi[a], ref[a], atol=1e-12)
+
+
+def test_ddi_potential_padded_brute_force():
+    """ddi_padded.jl:6-85: F in the first n entries of a (2n)^3 zero grid (same dx, so
+    dk is halved), the explicit c2r convolution there, Phi cropped to the first n."""
+    n, dx = (4, 2, 4), (0.9, 0.7, 1.1)
+    r = rng(9)
+    Fs = [r.normal(size=(n[2], n[1], n[0])) for _ in range(3)]
+    pn = tuple(2 * v for v in n)
+    Qp = O.ddi_q(pn, dx)
+    phi = O.ddi_potential_padded(*Fs, Qp, 0.8)
+    pads = []
+    for F in Fs:
+        P = np.zeros((pn[2], pn[1], pn[0]))
+        P[:n[2], :n[1], :n[0]] = F
+        pads.append(P)
+    ref = _brute_phi(pads, Qp, 0.8, pn)
+    for a in range(3):
+        assert np.allclose(phi[a], ref[a][:n[2], :n[1], :n[0]], atol=1e-12)
+    # the padded potential differs from the periodic one (no wrap-around images)
+    per = O.ddi_potential(*Fs, O.ddi_q(n, dx), 0.8)
+    assert np.max(np.abs(per[0] - phi[0])) > 1e-3
 
 
 # --- ground state / integrators ---------------------------------------------------
