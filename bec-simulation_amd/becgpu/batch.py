@@ -1,0 +1,198 @@
+"""Batched ground-state solves: many independent find_ground_state problems of mixed
+shapes, grouped into one workspace per (grid, spin, precision, dt, n_steps, tol) with one
+instance per solve, advanced together on the device.
+
+The per-solve semantics are exactly those of find_ground_state (ground_state.jl:80-163):
+ITP split steps with normalisation every step, the overflow guard on the first 10 steps, a
+convergence test every max(1, n_steps // 10) steps on |dE| and max|dpsi|/max|psi|, and the
+energy recomputed at exit.  Each instance retires when it converges; the kernels only visit
+the active instances.  Workspaces of different shapes run on their own HIP streams; the
+host issues their steps back to back and only waits on a workspace at its convergence
+checks, so the groups overlap on the device.
+
+find_ground_state_multistart (ground_state.jl:239-271) over a list of initial states is
+the special case of one group; `multistart_batched` returns the same record as the
+sequential driver.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .api import Workspace, _check_itp_overflow, _validate_itp_interactions, _validate_itp_zeeman
+from .physics import (Grid, HarmonicTrap, InteractionParams, NoPotential, SimParams, SpinSystem,
+                      ZeemanParams, init_psi)
+
+
+@dataclass
+class ITPJob:
+    """One find_ground_state problem."""
+    grid: Grid
+    F: int
+    interactions: InteractionParams
+    psi0: np.ndarray
+    zeeman: ZeemanParams = field(default_factory=ZeemanParams)
+    potential: object = field(default_factory=NoPotential)
+    dt: float = 0.001
+    n_steps: int = 10000
+    tol: float = 1e-10
+    tag: object = None
+
+
+def _group_key(job: ITPJob, precision):
+    return (tuple(job.grid.config.n_points), tuple(job.grid.config.box_size), job.F, precision,
+            float(job.dt), int(job.n_steps), float(job.tol))
+
+
+class _Group:
+    """The instances of one workspace and their ITP loop state."""
+
+    def __init__(self, jobs, idx, precision):
+        j0 = jobs[0]
+        self.idx = idx
+        self.jobs = jobs
+        self.save_every = max(1, j0.n_steps // 10)
+        self.n_steps = j0.n_steps
+        self.tol = j0.tol
+        sp = SimParams(dt=j0.dt, n_steps=j0.n_steps, imaginary_time=True, normalize_every=1,
+                       save_every=self.save_every)
+        self.ws = Workspace(grid=j0.grid, F=j0.F, n_instances=len(jobs), sim_params=sp,
+                            precision=precision)
+        for i, j in enumerate(jobs):
+            self.ws.set_instance(i, interactions=j.interactions, zeeman=j.zeeman,
+                                 potential=j.potential, dt=j.dt)
+            self.ws.set_psi(np.asarray(j.psi0, dtype=complex), i)
+        self.step = 0
+        self.running = list(range(len(jobs)))
+        et = self.ws.energy_terms()
+        self.E_prev = {i: float(et[i, L.E_TOTAL]) for i in self.running}
+        self.ws.copy_slot(1, 0)
+        self.out = {i: dict(converged=False, dE=float("nan"), dpsi=float("nan"), steps=0)
+                    for i in self.running}
+        self.t_done = {}
+
+    @property
+    def done(self):
+        return not self.running or self.step >= self.n_steps
+
+    def issue(self):
+        """Queue the steps up to the next check point (1 at a time for the first 10)."""
+        if self.step < 10:
+            k = 1
+        else:
+            k = min(self.save_every - self.step % self.save_every, self.n_steps - self.step)
+        L.check(L.lib().bec_split_step(self.ws.handle, int(k)), "bec_split_step")
+        self.step += k
+        return k
+
+    def check(self, t0):
+        """Overflow guard / convergence test after issue(); waits on this workspace only."""
+        if self.step <= 10:
+            _check_itp_overflow(self.ws, self.step, self.running)
+        if self.step % self.save_every == 0:
+            et = self.ws.energy_terms()
+            dd = self.ws.diff(0, 1, 1)
+            still = []
+            for i in self.running:
+                E = float(et[i, L.E_TOTAL])
+                dE = abs(E - self.E_prev[i])
+                pmax = dd[i, 1]
+                dpsi = float(dd[i, 0] / pmax) if pmax > 0 else 0.0
+                self.out[i].update(dE=dE, dpsi=dpsi, steps=self.step)
+                if dE < self.tol and dpsi < self.tol:
+                    self.out[i]["converged"] = True
+                    self.t_done[i] = time.perf_counter() - t0
+                else:
+                    self.E_prev[i] = E
+                    still.append(i)
+            if still != self.running:
+                self.running = still
+                if still:
+                    self.ws.set_active(still)
+            if self.running:
+                self.ws.copy_slot(1, 0)
+        else:
+            for i in self.running:
+                self.out[i]["steps"] = self.step
+        if self.done:
+            for i in self.running:
+                self.t_done.setdefault(i, time.perf_counter() - t0)
+
+    def finish(self, keep_psi):
+        self.ws.set_active(range(len(self.jobs)))
+        et = self.ws.energy_terms()
+        res = []
+        for i in range(len(self.jobs)):
+            r = dict(self.out[i])
+            r["energy"] = float(et[i, L.E_TOTAL])
+            r["time_s"] = self.t_done.get(i, float("nan"))
+            if keep_psi:
+                r["psi"] = self.ws.get_psi(i)
+            res.append(r)
+        return res
+
+
+def solve_batch(jobs, *, precision="fp64", keep_psi=False):
+    """Run every job to convergence or n_steps.  Returns (results in job order, stats)."""
+    for j in jobs:
+        _validate_itp_zeeman(j.zeeman, j.F, j.dt)
+        _validate_itp_interactions(j.interactions, j.F, j.dt, psi=j.psi0)
+    groups = {}
+    for k, j in enumerate(jobs):
+        groups.setdefault(_group_key(j, precision), []).append(k)
+    t0 = time.perf_counter()
+    gs = [_Group([jobs[k] for k in idx], idx, precision) for idx in groups.values()]
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    live = list(gs)
+    while live:
+        for g in live:          # queue work on every stream first ...
+            g.issue()
+        for g in live:          # ... then wait on each group at its check point
+            g.check(t0)
+        live = [g for g in live if not g.done]
+    for g in gs:
+        g.ws.synchronize()
+    wall = time.perf_counter() - t0
+    results = [None] * len(jobs)
+    for g in gs:
+        for i, r in enumerate(g.finish(keep_psi)):
+            results[g.idx[i]] = r
+    for g in gs:
+        g.ws.close()
+    stats = dict(wall_s=wall, setup_s=t_setup, n_groups=len(gs), solves=len(jobs),
+                 solves_per_s=len(jobs) / wall,
+                 group_sizes=[len(g.jobs) for g in gs],
+                 group_steps=[g.step for g in gs])
+    return results, stats
+
+
+def multistart_batched(*, initial_states=("polar", "ferromagnetic", "uniform",
+                                          "antiferromagnetic"),
+                       n_random=3, seed=42, grid, atom, interactions, zeeman=ZeemanParams(),
+                       potential=NoPotential(), dt=0.001, n_steps=10000, tol=1e-10,
+                       precision="fp64"):
+    """find_ground_state_multistart (ground_state.jl:239-271) with all starts in one batch;
+    best = first minimum of the energies, as the sequential driver."""
+    starts = []
+    for state in initial_states:
+        if state == "random":
+            for i in range(1, n_random + 1):
+                starts.append(("random", i, init_psi(grid, SpinSystem(atom.F), "random",
+                                                     seed=seed + i)))
+        else:
+            starts.append((state, 0, init_psi(grid, SpinSystem(atom.F), state)))
+    jobs = [ITPJob(grid=grid, F=atom.F, interactions=interactions, psi0=p, zeeman=zeeman,
+                   potential=potential, dt=dt, n_steps=n_steps, tol=tol, tag=(s, i))
+            for s, i, p in starts]
+    res, stats = solve_batch(jobs, precision=precision, keep_psi=True)
+    best = min(range(len(res)), key=lambda k: res[k]["energy"])
+    all_results = [dict(initial_state=jobs[k].tag[0], idx=jobs[k].tag[1], **res[k])
+                   for k in range(len(res))]
+    return dict(converged=res[best]["converged"], energy=res[best]["energy"],
+                initial_state=jobs[best].tag[0], psi=res[best]["psi"],
+                all_results=all_results, stats=stats)

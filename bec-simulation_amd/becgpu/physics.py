@@ -409,9 +409,21 @@ class SplitMix64:
         z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
         return z ^ (z >> 31)
 
+    def next_u64_array(self, n):
+        """The next n outputs at once (state k*gamma ahead, then the same mixing)."""
+        g = np.uint64(0x9E3779B97F4A7C15)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.s) + g * np.arange(1, n + 1, dtype=np.uint64)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.s = (self.s + n * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        return z
+
     def uniform(self, n):
-        return np.array([((self.next_u64() >> 11) + 0.5) * (1.0 / 9007199254740992.0)
-                         for _ in range(n)])
+        """U(0,1): the top 53 bits of each output, centred in its interval."""
+        return ((self.next_u64_array(n) >> np.uint64(11)).astype(np.float64) + 0.5) * \
+            (1.0 / 9007199254740992.0)
 
     def randn_complex(self, n):
         """Complex normal with E|z|^2 = 1 (Box-Muller; re, im ~ N(0, 1/2))."""

@@ -160,12 +160,12 @@ def cfg3_inputs(G, n=128):
     return grid, inter, psi, c_extra
 
 
-def run_yoshida3d(G, A, steps, warmup, precision):
+def run_yoshida3d(G, A, steps, warmup, precision, padding=False):
     grid, inter, psi, _ = cfg3_inputs(G)
     sp = G.SimParams(dt=1e-3, n_steps=500)
     ws = G.make_workspace(grid=grid, atom=G.Eu151, interactions=inter,
                           potential=G.HarmonicTrap(1.0, 1.0, 1.0), sim_params=sp, psi_init=psi,
-                          enable_ddi=True, c_dd=1.0, precision=precision)
+                          enable_ddi=True, c_dd=1.0, precision=precision, ddi_padding=padding)
     ad = G.AdaptiveDtParams(dt_init=1e-3)
     r = A.run_simulation_yoshida(ws, adaptive=ad, t_end=1e9, save_interval=0.05,
                                  keep_snapshots=False, max_accepted=warmup)
@@ -184,6 +184,55 @@ def run_yoshida3d(G, A, steps, warmup, precision):
     work = ws.profile_work()
     ws.profile(False)
     return ws, grid, (t1 - t0), prof, psi, r2, work
+
+
+def cfg4_jobs(G, n_jobs=256, seed=42, n_steps=10000):
+    """256 independent 2-D spin-1 ground states: sides uniform in {64, 128, 256} (seeded),
+    box 20^2, c0 = 10, c1 ~ U[-1, 1], starts cycling polar / ferromagnetic / uniform /
+    antiferromagnetic / seeded random, dt = 0.005, tol = 1e-8, n_steps = 10000, harmonic
+    trap omega = (1, 1) (the trap of the 1-D ground-state configuration; without one no
+    solve converges within 10000 steps)."""
+    from becgpu.batch import ITPJob
+    gen = G.physics.SplitMix64(seed)
+    u = gen.uniform(2 * n_jobs)
+    sizes = (64, 128, 256)
+    kinds = ("polar", "ferromagnetic", "uniform", "antiferromagnetic", "random")
+    grids = {n: G.make_grid(G.GridConfig((n, n), (20.0, 20.0))) for n in sizes}
+    jobs = []
+    for k in range(n_jobs):
+        n = sizes[min(2, int(u[2 * k] * 3))]
+        kind = kinds[k % len(kinds)]
+        psi0 = G.init_psi(grids[n], G.SpinSystem(1), kind, seed=seed + k)
+        jobs.append(ITPJob(grid=grids[n], F=1, interactions=G.InteractionParams(10.0, 2 * u[2 * k + 1] - 1),
+                           psi0=psi0, potential=G.HarmonicTrap(1.0, 1.0), dt=0.005,
+                           n_steps=n_steps, tol=1e-8, tag=(n, kind)))
+    return jobs
+
+
+def run_batch2d(G, A, n_jobs, n_plain):
+    """Batched FP32-bulk and FP64 solves/s of the 256-job set, against the plain per-solve
+    FP64 path (find_ground_state one job at a time) on an evenly spaced subsample."""
+    from becgpu.batch import solve_batch
+    jobs = cfg4_jobs(G, n_jobs)
+    r32, s32 = solve_batch(jobs, precision="fp32")
+    r64, s64 = solve_batch(jobs, precision="fp64")
+    sub = jobs[::max(1, n_jobs // n_plain)][:n_plain]
+    t0 = time.perf_counter()
+    for j in sub:
+        r = A.find_ground_state(grid=j.grid, atom=G.Rb87, interactions=j.interactions,
+                                potential=j.potential, dt=j.dt, n_steps=j.n_steps, tol=j.tol,
+                                psi_init=j.psi0)
+        r["workspace"].close()
+    plain = len(sub) / (time.perf_counter() - t0)
+    rel = [abs(a["energy"] - b["energy"]) / abs(b["energy"]) for a, b in zip(r32, r64)]
+    conv = [e for e, a, b in zip(rel, r32, r64) if a["converged"] and b["converged"]]
+    err = max(conv) if conv else float("nan")
+    lat = np.array([r["time_s"] for r in r32])
+    return dict(fp32=s32, fp64=s64, plain_fp64_solves_per_s=plain, plain_sample=len(sub),
+                max_rel_energy_err_fp32=err, max_rel_energy_err_all=max(rel),
+                n_both_converged=len(conv), converged_fp32=sum(r["converged"] for r in r32),
+                steps_fp32=[r["steps"] for r in r32], p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)))
 
 
 def cpu_baseline_yoshida3d(budget_s=20.0):
@@ -256,7 +305,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="yoshida3d_spin6_ddi_128",
-                    choices=["yoshida3d_spin6_ddi_128", "rt3d_spin1_128", "itp1d_spin1_64"])
+                    choices=["yoshida3d_spin6_ddi_128", "yoshida3d_spin6_ddipad_128",
+                             "rt3d_spin1_128", "itp1d_spin1_64", "batch2d_itp_256"])
+    ap.add_argument("--jobs", type=int, default=256, help="batch2d_itp_256: number of solves")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -270,14 +321,41 @@ def main():
     if args.precision is None:
         args.precision = "fp32" if args.workload.startswith("yoshida") else "fp64"
     extra = {}
-    if args.workload == "yoshida3d_spin6_ddi_128":
+    if args.workload == "batch2d_itp_256":
+        b = run_batch2d(G, A, args.jobs, n_plain=12)
+        line = {
+            "metric": "batched ITP solves/s (256 x 2-D spin-1, mixed 64^2/128^2/256^2)",
+            "value": b["fp32"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+            "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 reductions and phases)",
+            "data": "synthetic",
+            "config": {"workload": "batched find_ground_state, 2-D spin-1", "jobs": args.jobs,
+                       "sides": [64, 128, 256], "box": 20.0, "c0": 10.0, "c1": "U[-1,1] seed 42",
+                       "trap": "harmonic omega=(1,1)",
+                       "dt": 0.005, "tol": 1e-8, "n_steps": 10000},
+            "fp64_batched_solves_per_s": b["fp64"]["solves_per_s"],
+            "plain_fp64_solves_per_s": b["plain_fp64_solves_per_s"],
+            "speedup_vs_plain_fp64": b["fp32"]["solves_per_s"] / b["plain_fp64_solves_per_s"],
+            "max_rel_energy_err_fp32_vs_fp64": b["max_rel_energy_err_fp32"],
+            "max_rel_energy_err_note": "over the solves converged in both precisions",
+            "max_rel_energy_err_incl_unconverged": b["max_rel_energy_err_all"],
+            "n_converged_both": b["n_both_converged"],
+            "latency_p50_s": b["p50_s"], "latency_p99_s": b["p99_s"],
+            "converged": b["converged_fp32"], "groups": b["fp32"]["group_sizes"],
+            "plain_sample": b["plain_sample"],
+        }
+        print(json.dumps(line))
+        return
+    if args.workload.startswith("yoshida3d_spin6"):
+        pad = args.workload == "yoshida3d_spin6_ddipad_128"
         ws, grid, secs, prof, psi, r2, work = run_yoshida3d(G, A, args.steps, args.warmup,
-                                                            args.precision)
+                                                            args.precision, padding=pad)
         # Yoshida attempt = 6V + 4K + 3 psi copies + L2 reduction (B_alg per site-step)
-        b_alg = 10852.5 if args.precision == "fp64" else 5426.3
-        cfg = {"workload": "3D spin-6 dipolar 128^3, adaptive Yoshida-4 (run_simulation_yoshida!)",
+        b_alg = (10852.5 if args.precision == "fp64" else 5426.3) if not pad else None
+        cfg = {"workload": "3D spin-6 dipolar 128^3, adaptive Yoshida-4 (run_simulation_yoshida!)"
+                           + (", zero-padded 256^3 DDI" if pad else ""),
                "grid": [128, 128, 128], "F": 6, "box": 24.0, "c0": 10.0, "c_dd": 1.0,
-               "tensor": "c2..c12 ~ U[-1,1] seed 42", "dt_init": 1e-3, "ddi_padding": False}
+               "tensor": "c2..c12 ~ U[-1,1] seed 42", "dt_init": 1e-3, "ddi_padding": pad}
         extra = {"accepted": r2["n_accepted"], "rejected": r2["n_rejected"],
                  "final_dt": r2["final_dt"]}
     elif args.workload == "rt3d_spin1_128":
@@ -323,7 +401,8 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 reductions and phases)",
         "data": "synthetic", "config": cfg,
-        "step_hbm_frac": round(value * b_alg / HBM_PEAK, 4), "b_alg_per_site_step": b_alg,
+        "step_hbm_frac": round(value * b_alg / HBM_PEAK, 4) if b_alg else None,
+        "b_alg_per_site_step": b_alg,
         "roofline": roof, "hbm_kernels": hbm, "kernels": kernels, "cpu_baseline": cpu,
     }
     line.update(extra)

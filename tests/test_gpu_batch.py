@@ -1,0 +1,68 @@
+"""Batched ground states (becgpu.batch): every solve of a mixed-shape batch must equal the
+same solve run alone through find_ground_state (identical arithmetic per instance), and the
+FP64 batch must match the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import spinor_oracle as O
+import becgpu as G
+from becgpu import api as A
+from becgpu.batch import ITPJob, multistart_batched, solve_batch
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    grids = {n: G.make_grid(G.GridConfig((n, n), (20.0, 20.0))) for n in (32, 64)}
+    kinds = ("polar", "ferromagnetic", "uniform", "antiferromagnetic", "random")
+    jobs = []
+    for k in range(7):
+        n = 32 if k % 3 else 64
+        psi0 = G.init_psi(grids[n], G.SpinSystem(1), kinds[k % 5], seed=100 + k)
+        jobs.append(ITPJob(grid=grids[n], F=1, interactions=G.InteractionParams(10.0, -1 + 0.3 * k),
+                           psi0=psi0, potential=G.HarmonicTrap(1.0, 1.0), dt=0.005, n_steps=400,
+                           tol=1e-7))
+    return jobs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_single_solves(precision):
+    jobs = _jobs()
+    res, stats = solve_batch(jobs, precision=precision, keep_psi=True)
+    print(stats)
+    assert stats["n_groups"] == 2
+    for j, r in zip(jobs, res):
+        s = A.find_ground_state(grid=j.grid, atom=G.Rb87, interactions=j.interactions,
+                                potential=j.potential, dt=j.dt, n_steps=j.n_steps, tol=j.tol,
+                                psi_init=j.psi0, precision=precision)
+        assert r["steps"] == s["steps"] and r["converged"] == s["converged"]
+        assert r["energy"] == s["energy"]
+        assert np.array_equal(r["psi"], s["workspace"].get_psi())
+
+
+def test_batch_fp64_matches_oracle():
+    jobs = _jobs()[:3]
+    res, _ = solve_batch(jobs, precision="fp64", keep_psi=True)
+    for j, r in zip(jobs, res):
+        og = O.make_grid(j.grid.config.n_points, j.grid.config.box_size)
+        ows = O.make_workspace(og, 1, j.psi0, j.dt, True, omega=(1.0, 1.0), c0=10.0,
+                               c1=j.interactions.c1, n_steps=j.n_steps)
+        ro = O.find_ground_state(ows, j.n_steps, j.tol)
+        assert r["steps"] == ro["steps"]
+        assert abs(r["energy"] - ro["energy"]) <= 1e-12 * abs(ro["energy"])
+        assert rel_l2(r["psi"], ows.psi) <= 1e-10
+
+
+def test_multistart_batched_matches_sequential():
+    grid = G.make_grid(G.GridConfig((32, 32), (20.0, 20.0)))
+    kw = dict(grid=grid, atom=G.Rb87, interactions=G.InteractionParams(10.0, -0.5),
+              potential=G.HarmonicTrap(1.0, 1.0), dt=0.005, n_steps=300, tol=1e-7)
+    b = multistart_batched(initial_states=("polar", "ferromagnetic", "random"), n_random=2, **kw)
+    s = A.find_ground_state_multistart(initial_states=("polar", "ferromagnetic", "random"),
+                                       n_random=2, **kw)
+    assert b["initial_state"] == s["initial_state"]
+    assert b["energy"] == s["energy"]
+    assert [r["energy"] for r in b["all_results"]] == [r["energy"] for r in s["all_results"]]
