@@ -36,6 +36,21 @@ template <typename T>
 __device__ __forceinline__ cplx<T> conjc(cplx<T> a) { return mk(a.x, -a.y); }
 template <typename T>
 __device__ __forceinline__ T abs2(cplx<T> a) { return a.x * a.x + a.y * a.y; }
+
+// Arithmetic of operators that run in more than one kernel (the diagonal step runs in the
+// x pass and in the lane-per-site pass, depending on its neighbours in the schedule).
+// Multiply-adds are fused within these expressions only and never with the caller's
+// operations after inlining, so every kernel gives the same bits for the same site and a
+// result does not depend on how a run is split into launches.
+template <typename T>
+__device__ __forceinline__ T density_acc(T n, cplx<T> a) {
+#pragma clang fp contract(on)
+  return n + (a.x * a.x + a.y * a.y);
+}
+__device__ __forceinline__ cplx<double> cmul_fixed(cplx<double> a, cplx<double> b) {
+#pragma clang fp contract(on)
+  return cplx<double>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
 // multiply by -i (forward) or +i (inverse)
 template <typename T, bool INV>
 __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) {
@@ -117,6 +132,7 @@ __device__ __forceinline__ cplx<T> mul64(cplx<T> a, cplx<double> w) {
 // without FP64 temporaries per component).
 template <typename T>
 __device__ __forceinline__ cplx<T> mul_split(cplx<T> a, cplx<double> f) {
+#pragma clang fp contract(on)
   if constexpr (sizeof(T) == 8) {
     return a * f;
   } else {
@@ -300,6 +316,7 @@ __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool acti
 // Common factor of all components: cis(-(V + c0 n + c_lhy n^1.5) dt) or its ITP exponential.
 __device__ __forceinline__ cplx<double> diag_base(double nd, double Vtrap, const InstParams& P,
                                                   double dtf, bool itp) {
+#pragma clang fp contract(on)
   const double a = (Vtrap + P.c0 * nd + P.c_lhy * nd * sqrt(nd)) * dtf;
   if (itp) return mk(exp(-a), 0.0);
   double sn, cs;
@@ -312,10 +329,10 @@ __device__ __forceinline__ void op_diag(cplx<T>* s, double Vtrap, const InstPara
                                         bool itp, const cplx<double>* zf) {
   T n = 0;
 #pragma unroll
-  for (int c = 0; c < D; ++c) n += abs2(s[c]);
+  for (int c = 0; c < D; ++c) n = density_acc(n, s[c]);
   const cplx<double> base = diag_base((double)n, Vtrap, P, dtf, itp);
 #pragma unroll
-  for (int c = 0; c < D; ++c) s[c] = mul_split<T>(s[c], base * zf[c]);
+  for (int c = 0; c < D; ++c) s[c] = mul_split<T>(s[c], cmul_fixed(base, zf[c]));
 }
 
 // observables.jl:68-88 spin density: Fz = sum m|psi|^2,
@@ -520,7 +537,7 @@ __device__ __forceinline__ void op_loss(cplx<T>* s, const InstParams& P, double 
   if (!P.has_loss) return;
   double n = 0.0;
 #pragma unroll
-  for (int c = 0; c < D; ++c) n += (double)abs2(s[c]);
+  for (int c = 0; c < D; ++c) n += (double)density_acc<T>(0, s[c]);
 #pragma unroll
   for (int c = 0; c < D; ++c)
     if (!(P.loss[c] < 1e-30)) s[c] = scale64(s[c], loss_factor(P.loss[c], n, dtf));

@@ -243,13 +243,13 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
           // all D values live in VGPRs)
           T n = 0;
 #pragma unroll
-          for (int c = 0; c < D; ++c) n += abs2(tile[P8(c * TS + s)]);
+          for (int c = 0; c < D; ++c) n = density_acc(n, tile[P8(c * TS + s)]);
           const cplx<double> base = diag_base((double)n, trap_at(s), P, dtf, itp);
           __asm__ volatile("" ::: "memory");
 #pragma unroll
           for (int c = 0; c < D; ++c) {
             const int e = P8(c * TS + s);
-            tile[e] = mul_split<T>(tile[e], base * zfi[c]);
+            tile[e] = mul_split<T>(tile[e], cmul_fixed(base, zfi[c]));
           }
         });
       } else if (op.kind == OP_SPINMIX && D == 3) {
@@ -267,7 +267,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
             // components stay in LDS (density sweep, then the per-component factors)
             double n = 0.0;
 #pragma unroll
-            for (int c = 0; c < D; ++c) n += (double)abs2(tile[P8(c * TS + s)]);
+            for (int c = 0; c < D; ++c) n += (double)density_acc<T>(0, tile[P8(c * TS + s)]);
             __asm__ volatile("" ::: "memory");
 #pragma unroll
             for (int c = 0; c < D; ++c) {

@@ -299,6 +299,43 @@ def cpu_baseline_itp1d(budget_s=10.0):
                 sample=f"{n} ITP split_step! of the 64-point spin-1 workload, FP64 numpy oracle")
 
 
+def run_serve(G, n_req, rate, precision, n_check=10):
+    """The request stream served by becgpu.serve, then an evenly spaced sample of the cheapest
+    served requests re-run alone on the plain FP64 path: their energy error and their
+    one-at-a-time service time."""
+    from becgpu.serve import Server, poisson_mix, solve_alone
+    reqs = poisson_mix(n_req, rate=rate, seed=42, precision=precision)
+    srv = Server(keep_psi=False)
+    res, stats = srv.run(reqs, realtime=True)
+    lat = np.array([r.latency for r in res])
+    wait = np.array([r.admitted - r.arrival for r in res])
+    by = {r.rid: r for r in res}
+    cost = sorted(reqs, key=lambda q: q.sites * (2 * q.F + 1) * q.n_steps)
+    sample = cost[:max(1, len(cost) // 2)][::max(1, len(cost) // (2 * n_check))][:n_check]
+    errs, alone_s = [], []
+    for q in sample:
+        q64 = type(q)(**{**q.__dict__, "precision": "fp64"})
+        t0 = time.perf_counter()
+        a = solve_alone(q64, keep_psi=False)
+        alone_s.append(time.perf_counter() - t0)
+        errs.append(abs(by[q.rid].energy - a.energy) / abs(a.energy))
+    kinds = {}
+    for nd in (1, 2, 3):
+        sel = [r.latency for r in res if len(reqs[r.rid].n) == nd]
+        if sel:
+            kinds[f"{nd}d"] = dict(n=len(sel), p50_s=float(np.percentile(sel, 50)),
+                                   p99_s=float(np.percentile(sel, 99)))
+    return dict(stats=stats, solves_per_s=len(res) / stats["wall_s"],
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                wait_p50_s=float(np.percentile(wait, 50)), wait_p99_s=float(np.percentile(wait, 99)),
+                by_dim=kinds, arrivals_span_s=reqs[-1].arrival,
+                converged_itp=sum(1 for r in res if r.converged),
+                n_itp=sum(1 for q in reqs if q.kind == "itp"),
+                errors=sum(1 for r in res if r.error),
+                max_rel_energy_err_vs_fp64=max(errs) if errs else None,
+                err_sample=[q.rid for q in sample], sample_alone_fp64_s=alone_s)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,8 +343,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="yoshida3d_spin6_ddi_128",
                     choices=["yoshida3d_spin6_ddi_128", "yoshida3d_spin6_ddipad_128",
-                             "rt3d_spin1_128", "itp1d_spin1_64", "batch2d_itp_256"])
+                             "rt3d_spin1_128", "itp1d_spin1_64", "batch2d_itp_256", "serve_mix"])
     ap.add_argument("--jobs", type=int, default=256, help="batch2d_itp_256: number of solves")
+    ap.add_argument("--requests", type=int, default=64, help="serve_mix: requests in the stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve_mix: mean arrivals per s")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -319,8 +358,35 @@ def main():
     from becgpu import api as A
 
     if args.precision is None:
-        args.precision = "fp32" if args.workload.startswith("yoshida") else "fp64"
+        args.precision = "fp32" if args.workload.startswith(("yoshida", "serve")) else "fp64"
     extra = {}
+    if args.workload == "serve_mix":
+        b = run_serve(G, args.requests, args.rate, args.precision)
+        line = {
+            "metric": "served solves/s with p50/p99 latency (mixed ITP and real-time requests)",
+            "value": b["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": None,
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64" if args.precision == "fp64" else "f32 (FP64 reductions and phases)",
+            "data": "synthetic",
+            "config": {"workload": "Poisson request stream (becgpu.serve.poisson_mix, seed 42)",
+                       "requests": args.requests, "rate_per_s": args.rate,
+                       "dims": "1/2/3 w.p. 0.5/0.35/0.15", "F": "1/2/6 w.p. 0.6/0.25/0.15",
+                       "sides": "1-D/2-D {64,128,256}, 3-D {64,128}", "c0": "U[5,20]",
+                       "c1": "U[-1,1]", "n_steps": "U[500,5000]", "itp_tol": 1e-8,
+                       "slots_per_pool": 8, "hbm_budget_GiB": 200},
+            "latency_p50_s": b["p50_s"], "latency_p99_s": b["p99_s"],
+            "admission_wait_p50_s": b["wait_p50_s"], "admission_wait_p99_s": b["wait_p99_s"],
+            "latency_by_dim": b["by_dim"], "arrivals_span_s": b["arrivals_span_s"],
+            "wall_s": b["stats"]["wall_s"], "pools_created": b["stats"]["pools_created"],
+            "peak_pool_GiB": b["stats"]["peak_pool_bytes"] / 2 ** 30,
+            "itp_converged": f'{b["converged_itp"]}/{b["n_itp"]}', "errors": b["errors"],
+            "max_rel_energy_err_vs_fp64": b["max_rel_energy_err_vs_fp64"],
+            "err_sample_rids": b["err_sample"],
+            "sample_alone_fp64_s": [round(v, 4) for v in b["sample_alone_fp64_s"]],
+        }
+        print(json.dumps(line))
+        return
     if args.workload == "batch2d_itp_256":
         b = run_batch2d(G, A, args.jobs, n_plain=12)
         line = {
