@@ -70,6 +70,7 @@ SIGNATURES = {
     "bec_loss_step": ([C.c_void_p, C.c_double], C.c_int),
     "bec_normalize_constrained": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_synchronize": ([C.c_void_p], C.c_int),
+    "bec_query": ([C.c_void_p], C.c_int),
     "bec_last_error": ([], C.c_char_p),
     "bec_workspace_bytes": ([C.POINTER(BecConfig)], C.c_longlong),
 }

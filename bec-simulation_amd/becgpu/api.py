@@ -251,6 +251,13 @@ class Workspace:
     def synchronize(self):
         L.check(L.lib().bec_synchronize(self.handle), "bec_synchronize")
 
+    def query(self):
+        """True when all work queued on this workspace has finished (does not block)."""
+        rc = L.lib().bec_query(self.handle)
+        if rc < 0:
+            L.check(rc, "bec_query")
+        return rc == 0
+
 
 # ---------------------------------------------------------------------------
 # make_workspace

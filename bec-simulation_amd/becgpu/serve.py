@@ -109,6 +109,7 @@ class _Pool:
         self.k = 0
         self.t_issue = 0.0
         self.steps_issued = 0
+        self.in_flight = False
 
     @property
     def active(self):
@@ -130,6 +131,7 @@ class Server:
         self.peak_used = 0
         self.grids = {}
         self.pools_created = 0
+        self.host_issue_s = 0.0
 
     # -- admission -------------------------------------------------------------
     def _grid(self, r):
@@ -146,13 +148,13 @@ class Server:
         key = _pool_key(r)
         p = self.pools.get(key)
         if p is not None:
-            return p if p.free else None
+            return p if (p.free and not p.in_flight) else None
         need = self.pool_bytes(r)
         if need > self.budget:
             raise ValueError(f"request {r.rid}: a pool of {need} bytes exceeds the HBM budget")
         if self.used + need > self.budget:
             for k, q in list(self.pools.items()):   # reclaim idle pools, oldest first
-                if not q.slot:
+                if not q.slot and not q.in_flight:
                     self.used -= q.bytes
                     q.ws.close()
                     del self.pools[k]
@@ -196,16 +198,20 @@ class Server:
 
     # -- stepping ------------------------------------------------------------------
     def _issue(self, p: _Pool):
+        """Queue one chunk on the pool's stream: up to the nearest event of its slots, and
+        about chunk_ms of the pool's measured pace (a few steps while it is unmeasured)."""
         k = min(st["next"] - st["step"] for st in p.slot.values())
-        if p.ms_per_step:
-            k = min(k, max(1, int(self.chunk_ms / p.ms_per_step)))
+        k = min(k, max(1, int(self.chunk_ms / p.ms_per_step)) if p.ms_per_step else 4)
         k = max(1, min(k, self.max_chunk))
+        t = time.perf_counter()
         L.check(L.lib().bec_split_step(p.ws.handle, int(k)), "bec_split_step")
+        self.host_issue_s += time.perf_counter() - t
         for st in p.slot.values():
             st["step"] += k
         p.k = k
         p.steps_issued += k
-        p.t_issue = time.perf_counter()
+        p.t_issue = t
+        p.in_flight = True
 
     def _retire(self, p, s, et, t0, results, error=None):
         st = p.slot.pop(s)
@@ -217,7 +223,8 @@ class Server:
         p.free.append(s)
 
     def _events(self, p: _Pool, t0, results):
-        p.ws.synchronize()
+        """The pool's chunk has finished: convergence checks and retirements of its due slots."""
+        p.in_flight = False
         ms = (time.perf_counter() - p.t_issue) * 1e3 / p.k
         p.ms_per_step = ms if p.ms_per_step is None else 0.7 * p.ms_per_step + 0.3 * ms
         due = [s for s, st in p.slot.items() if st["step"] >= st["next"]]
@@ -260,12 +267,21 @@ class Server:
     def run(self, requests, realtime=True):
         """Serve `requests`.  realtime: each request becomes visible at its arrival time on
         the wall clock (latency = finish - arrival); otherwise one request is released per
-        loop iteration.  Returns (results in completion order, stats)."""
+        loop iteration.  The loop polls the pools' streams without blocking: a pool whose
+        chunk has finished is checked, takes waiting requests and is issued its next chunk
+        while the other pools' chunks still run.  Returns (results in completion order,
+        stats)."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         queue, results = [], []
+        self.host_issue_s = 0.0
         t0 = time.perf_counter()
         it = 0
         while pending or queue or any(p.slot for p in self.pools.values()):
+            progressed = False
+            for p in list(self.pools.values()):
+                if p.in_flight and p.ws.query():
+                    self._events(p, t0, results)
+                    progressed = True
             now = time.perf_counter() - t0
             if realtime:
                 while pending and pending[0].arrival <= now:
@@ -279,25 +295,26 @@ class Server:
                     waiting.append(r)
                 else:
                     self._admit(p, r, time.perf_counter() - t0, results)
+                    progressed = True
             queue = waiting
-            busy = [p for p in self.pools.values() if p.slot]
-            if not busy:
+            for p in list(self.pools.values()):
+                if p.slot and not p.in_flight:
+                    self._issue(p)
+                    progressed = True
+            if not any(p.in_flight for p in self.pools.values()):
                 if queue:
                     raise RuntimeError("requests wait for memory that no pool will release")
                 if pending and realtime:
                     time.sleep(max(0.0, min(0.005, pending[0].arrival - now)))
-                continue
-            for p in busy:            # queue a chunk on every stream first ...
-                self._issue(p)
-            for p in busy:            # ... then wait on each pool and handle its events
-                self._events(p, t0, results)
+            elif not progressed:
+                time.sleep(5e-5)
             it += 1
         wall = time.perf_counter() - t0
         for p in self.pools.values():
             p.ws.close()
         self.pools, self.used = {}, 0
         return results, dict(wall_s=wall, iterations=it, pools_created=self.pools_created,
-                             peak_pool_bytes=self.peak_used)
+                             peak_pool_bytes=self.peak_used, host_issue_s=self.host_issue_s)
 
 
 def solve_alone(r: Request, keep_psi=True):

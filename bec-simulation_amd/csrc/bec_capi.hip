@@ -924,6 +924,14 @@ int bec_synchronize(bec_ws* ws) {
   return BEC_OK;
 }
 
+int bec_query(bec_ws* ws) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  const hipError_t e = hipStreamQuery(ws->stream);
+  if (e == hipErrorNotReady) return 1;
+  HIPCHK(e);
+  return BEC_OK;
+}
+
 static int fetch_red(bec_ws* ws) {
   HIPCHK(hipMemcpyAsync(ws->h_red, ws->d_red, (size_t)ws->B * RS_COUNT * 8, hipMemcpyDeviceToHost,
                         ws->stream));

@@ -299,7 +299,7 @@ def cpu_baseline_itp1d(budget_s=10.0):
                 sample=f"{n} ITP split_step! of the 64-point spin-1 workload, FP64 numpy oracle")
 
 
-def run_serve(G, n_req, rate, precision, n_check=10):
+def run_serve(G, n_req, rate, precision, n_check=10, sequential=False):
     """The request stream served by becgpu.serve, then an evenly spaced sample of the cheapest
     served requests re-run alone on the plain FP64 path: their energy error and their
     one-at-a-time service time."""
@@ -312,13 +312,20 @@ def run_serve(G, n_req, rate, precision, n_check=10):
     by = {r.rid: r for r in res}
     cost = sorted(reqs, key=lambda q: q.sites * (2 * q.F + 1) * q.n_steps)
     sample = cost[:max(1, len(cost) // 2)][::max(1, len(cost) // (2 * n_check))][:n_check]
-    errs, alone_s = [], []
+    errs, alone_s, err_rows = [], [], []
     for q in sample:
         q64 = type(q)(**{**q.__dict__, "precision": "fp64"})
         t0 = time.perf_counter()
         a = solve_alone(q64, keep_psi=False)
         alone_s.append(time.perf_counter() - t0)
         errs.append(abs(by[q.rid].energy - a.energy) / abs(a.energy))
+        err_rows.append([q.rid, q.kind, len(q.n), q.n[0], q.F, q.n_steps, errs[-1]])
+    seq = None
+    if sequential:
+        t0 = time.perf_counter()
+        for q in reqs:
+            solve_alone(q, keep_psi=False)
+        seq = time.perf_counter() - t0
     kinds = {}
     for nd in (1, 2, 3):
         sel = [r.latency for r in res if len(reqs[r.rid].n) == nd]
@@ -333,7 +340,7 @@ def run_serve(G, n_req, rate, precision, n_check=10):
                 n_itp=sum(1 for q in reqs if q.kind == "itp"),
                 errors=sum(1 for r in res if r.error),
                 max_rel_energy_err_vs_fp64=max(errs) if errs else None,
-                err_sample=[q.rid for q in sample], sample_alone_fp64_s=alone_s)
+                err_rows=err_rows, sample_alone_fp64_s=alone_s, sequential_s=seq)
 
 
 def main():
@@ -347,6 +354,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=256, help="batch2d_itp_256: number of solves")
     ap.add_argument("--requests", type=int, default=64, help="serve_mix: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serve_mix: mean arrivals per s")
+    ap.add_argument("--sequential", action="store_true",
+                    help="serve_mix: also time every request alone, one after another")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -361,7 +370,7 @@ def main():
         args.precision = "fp32" if args.workload.startswith(("yoshida", "serve")) else "fp64"
     extra = {}
     if args.workload == "serve_mix":
-        b = run_serve(G, args.requests, args.rate, args.precision)
+        b = run_serve(G, args.requests, args.rate, args.precision, sequential=args.sequential)
         line = {
             "metric": "served solves/s with p50/p99 latency (mixed ITP and real-time requests)",
             "value": b["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": None,
@@ -382,7 +391,10 @@ def main():
             "peak_pool_GiB": b["stats"]["peak_pool_bytes"] / 2 ** 30,
             "itp_converged": f'{b["converged_itp"]}/{b["n_itp"]}', "errors": b["errors"],
             "max_rel_energy_err_vs_fp64": b["max_rel_energy_err_vs_fp64"],
-            "err_sample_rids": b["err_sample"],
+            "err_sample": {"columns": ["rid", "kind", "dim", "side", "F", "n_steps",
+                                       "rel_energy_err"], "rows": b["err_rows"]},
+            "host_issue_s": b["stats"]["host_issue_s"],
+            "sequential_alone_s": b["sequential_s"],
             "sample_alone_fp64_s": [round(v, 4) for v in b["sample_alone_fp64_s"]],
         }
         print(json.dumps(line))

@@ -162,6 +162,11 @@ int bec_profile_read(bec_ws* ws, double* ms, long long* count);
 int bec_profile_work(bec_ws* ws, double* out);
 
 int bec_synchronize(bec_ws* ws);
+/* Non-blocking completion test of the work queued on the workspace's stream: 0 when all
+ * of it has finished, 1 while some is still running, a negative code on a device error.
+ * No counterpart in the reference (its calls are synchronous); a request loop polls it
+ * to serve each workspace as soon as its own work ends. */
+int bec_query(bec_ws* ws);
 const char* bec_last_error(void);
 /* Bytes of device memory held by a workspace of this configuration (admission). */
 long long bec_workspace_bytes(const bec_config* cfg);
