@@ -260,6 +260,51 @@ class Workspace:
 
 
 # ---------------------------------------------------------------------------
+# Spectral derivatives (fft_utils.jl)
+# ---------------------------------------------------------------------------
+def _grid_config(grid: Grid):
+    cfg = L.BecConfig()
+    nd = grid.ndim
+    cfg.ndim = nd
+    for d in range(3):
+        cfg.n[d] = grid.config.n_points[d] if d < nd else 1
+        cfg.L[d] = grid.config.box_size[d] if d < nd else 1.0
+    cfg.F, cfg.n_instances = 1, 1
+    xt = np.ascontiguousarray(np.concatenate(grid.x), dtype=np.float64)
+    kt = np.ascontiguousarray(np.concatenate(grid.k), dtype=np.float64)
+    cfg.x_tab = xt.ctypes.data_as(C.POINTER(C.c_double))
+    cfg.k_tab = kt.ctypes.data_as(C.POINTER(C.c_double))
+    return cfg, (xt, kt)
+
+
+def fft_partial_derivative(f, grid: Grid, dim: int):
+    """_fft_partial_derivative (fft_utils.jl:4-19): Re IFFT(i k_dim FFT(f)) of a real field
+    of the grid's shape; dim is the reference's 1-based axis (1 = x, the fastest)."""
+    a = np.ascontiguousarray(f, dtype=np.float64)
+    if a.shape != grid.shape:
+        raise ValueError(f"field shape {a.shape} != grid shape {grid.shape}")
+    if not 1 <= dim <= grid.ndim:
+        raise ValueError(f"dim must be in 1..{grid.ndim}")
+    cfg, keep = _grid_config(grid)
+    out = np.empty(grid.shape)
+    L.check(L.lib().bec_fft_derivative(C.byref(cfg), a.ctypes.data, dim - 1, out.ctypes.data),
+            "bec_fft_derivative")
+    return out
+
+
+def fft_gradient(f, grid: Grid):
+    """_fft_gradient (fft_utils.jl:25-44): the tuple (d/dx, d/dy, ...) of a real field."""
+    a = np.ascontiguousarray(f, dtype=np.float64)
+    if a.shape != grid.shape:
+        raise ValueError(f"field shape {a.shape} != grid shape {grid.shape}")
+    cfg, keep = _grid_config(grid)
+    out = np.empty((grid.ndim,) + grid.shape)
+    L.check(L.lib().bec_fft_gradient(C.byref(cfg), a.ctypes.data, out.ctypes.data),
+            "bec_fft_gradient")
+    return tuple(out[d] for d in range(grid.ndim))
+
+
+# ---------------------------------------------------------------------------
 # make_workspace
 # ---------------------------------------------------------------------------
 def _tensor_active(F, interactions: InteractionParams):

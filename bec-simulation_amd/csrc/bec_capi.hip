@@ -924,6 +924,72 @@ int bec_synchronize(bec_ws* ws) {
   return BEC_OK;
 }
 
+// fft_utils.jl:4-19,25-44: d_d f = Re IFFT(i k_d FFT(f)) of a real field.  A private FP64
+// spin-1 workspace of the grid holds f in component d for every d < ndim, so one schedule
+// (forward transforms, i k_d / N on component d, inverse transforms) yields the gradient.
+static int fft_gradient_impl(const bec_config* grid, const double* f, int only, double* out) {
+  if (!grid || !f || !out) return fail(BEC_EINVAL, "null argument");
+  bec_config c = *grid;
+  c.F = 1;
+  c.n_instances = 1;
+  c.precision = BEC_FP64;
+  c.imaginary_time = 0;
+  c.normalize_every = 0;
+  c.enable_ddi = 0;
+  c.ddi_secular = 0;
+  c.ddi_padding = 0;
+  c.tensor = 0;
+  bec_ws* ws = nullptr;
+  CHK(bec_ws_create(&c, &ws));
+  struct Destroy {
+    bec_ws* w;
+    ~Destroy() { bec_ws_destroy(w); }
+  } destroy{ws};
+  const int nd = ws->g.ndim;
+  if (only >= nd) return fail(BEC_EINVAL, "dim must be in [0, ndim)");
+  const size_t N = (size_t)ws->g.nsites;
+  std::vector<double> h(2 * 3 * N, 0.0);
+  for (int d = 0; d < nd; ++d)
+    for (size_t i = 0; i < N; ++i) h[2 * (d * N + i)] = f[i];
+  CHK(bec_set_psi(ws, 0, h.data()));
+  CHK(sync_params(ws));
+  XProgram p{};
+  if (nd == 1) {
+    p.stages = XS_KIN_1D;
+    p.deriv = 1;
+    CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+  } else {
+    p.stages = XS_FWD_FFT;
+    CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+    if (nd == 3) {
+      CHK(do_strided(ws, 1, SM_FWD, ws->slot[0], 0.0, 0.0, 0, 0));
+      CHK(do_strided(ws, 2, SM_DERIV, ws->slot[0], 0.0, 0.0, 0, 0));
+      CHK(do_strided(ws, 1, SM_INV, ws->slot[0], 0.0, 0.0, 0, 0));
+    } else {
+      CHK(do_strided(ws, 1, SM_DERIV, ws->slot[0], 0.0, 0.0, 0, 0));
+    }
+    XProgram q{};
+    q.stages = XS_INV_FFT;
+    CHK(do_x(ws, q, ws->slot[0], ws->slot[0]));
+  }
+  CHK(bec_get_psi(ws, 0, h.data()));
+  for (int d = 0; d < nd; ++d) {
+    if (only >= 0 && d != only) continue;
+    double* o = out + (only >= 0 ? 0 : d * N);
+    for (size_t i = 0; i < N; ++i) o[i] = h[2 * (d * N + i)];
+  }
+  return BEC_OK;
+}
+
+int bec_fft_derivative(const bec_config* grid, const double* f, int dim, double* out) {
+  if (dim < 0) return fail(BEC_EINVAL, "dim must be in [0, ndim)");
+  return fft_gradient_impl(grid, f, dim, out);
+}
+
+int bec_fft_gradient(const bec_config* grid, const double* f, double* out) {
+  return fft_gradient_impl(grid, f, -1, out);
+}
+
 int bec_query(bec_ws* ws) {
   if (!ws) return fail(BEC_EINVAL, "null workspace");
   const hipError_t e = hipStreamQuery(ws->stream);

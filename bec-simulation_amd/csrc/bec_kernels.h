@@ -55,6 +55,7 @@ enum StridedMode : int {
   SM_KIN = 2,   // forward, kinetic phase/N, inverse
   SM_DDI = 3,   // 3 fields: forward, C_dd Q(k)/N contraction, inverse
   SM_KE = 4,    // forward then sum k^2 |v|^2 (no store)
+  SM_DERIV = 5, // forward, i k_c / N on component c (c < ndim), inverse (FP64 only)
 };
 
 // x-axis pass: tiles of whole x rows, all components; executes `prog`.

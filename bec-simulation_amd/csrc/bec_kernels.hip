@@ -323,9 +323,14 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     for (int e = tid; e < D * TS; e += nthr) {
       const int x = e % NX;
       const double k = tb.ktab[x];
-      const double a = 0.5 * (k * k) * kdt;
-      cplx<double> ph = itp ? mk(exp(-a), 0.0) : cis_neg<double>(a);
-      ph = ph * g.inv_ntot;
+      cplx<double> ph;
+      if (prog.deriv) {  // fft_utils.jl:4-19 along x: i k_x / N on component 0
+        ph = mk(0.0, e < TS ? k * g.inv_ntot : 0.0);
+      } else {
+        const double a = 0.5 * (k * k) * kdt;
+        ph = itp ? mk(exp(-a), 0.0) : cis_neg<double>(a);
+        ph = ph * g.inv_ntot;
+      }
       tile[P8(e)] = mul64(tile[P8(e)], ph);
     }
     __syncthreads();
@@ -697,6 +702,19 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       tile[(2 * N + e) * TB + xi2] = pz;
     }
     __syncthreads();
+  } else if (MODE == SM_DERIV) {
+    // fft_utils.jl:4-19,25-44: component c holds f and takes i k_c / N (c < ndim)
+    const int c = blockIdx.y;
+    for (int idx = tid; idx < N * TB; idx += nthr) {
+      const int xi2 = idx % TB, e = idx / TB;
+      const int x = x0 + xi2;
+      double k = 0.0;
+      if (c == 0) k = x < valid ? tb.ktab[x] : 0.0;
+      else if (c == 1) k = tb.ktab[g.nmax + (axis == 1 ? e : other)];
+      else if (c == 2 && g.ndim == 3) k = tb.ktab[2 * g.nmax + e];
+      tile[idx] = mul64(tile[idx], mk(0.0, k * g.inv_ntot));
+    }
+    __syncthreads();
   } else if (MODE == SM_KE) {
     double acc = 0.0;
     for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
@@ -721,7 +739,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
     return;
   }
 
-  if (MODE == SM_INV || MODE == SM_KIN || MODE == SM_DDI)
+  if (MODE == SM_INV || MODE == SM_KIN || MODE == SM_DDI || MODE == SM_DERIV)
     fft_team<T, N, true>(lbase, TB, tt, act, tw);
 
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
@@ -1071,6 +1089,11 @@ static hipError_t strided_mode(hipStream_t st, const Geom& g, const DevTables& t
     case SM_INV: return strided_n<T, N, SM_INV, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
     case SM_KIN: return strided_n<T, N, SM_KIN, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
     case SM_KE: return strided_n<T, N, SM_KE, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
+    case SM_DERIV:
+      if constexpr (sizeof(T) == 8)
+        return strided_n<T, N, SM_DERIV, 1>(st, g, tb, axis, data, n_active, kin_frac, dt_override, itp, 0);
+      else
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

@@ -61,6 +61,7 @@ struct XProgram {
   int itp;            // imaginary time
   double dt_override; // > 0: use instead of the per-instance dt
   double kin_frac;    // 1-D kinetic step dt fraction
+  int deriv;          // KIN_1D multiplies by i k / N (spectral derivative) instead of the phase
   Op A[kMaxOps];
   Op B[kMaxOps];
 };

@@ -161,6 +161,15 @@ int bec_profile_read(bec_ws* ws, double* ms, long long* count);
  * counters.  No counterpart in the reference. */
 int bec_profile_work(bec_ws* ws, double* out);
 
+/* Spectral derivatives of a real field f[N] (memory order as psi without the component
+ * axis) on the grid that cfg describes (ndim, n, L, x_tab, k_tab; every other field is
+ * ignored), in FP64: out = Re IFFT(i k_dim FFT(f)).
+ * bec_fft_derivative replaces _fft_partial_derivative (src/fft_utils.jl:4-19); dim is
+ * 0-based (the reference's dim - 1).  bec_fft_gradient replaces _fft_gradient
+ * (src/fft_utils.jl:25-44) and writes out[ndim][N]. */
+int bec_fft_derivative(const bec_config* cfg, const double* f, int dim, double* out);
+int bec_fft_gradient(const bec_config* cfg, const double* f, double* out);
+
 int bec_synchronize(bec_ws* ws);
 /* Non-blocking completion test of the work queued on the workspace's stream: 0 when all
  * of it has finished, 1 while some is still running, a negative code on a device error.

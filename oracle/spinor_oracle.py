@@ -623,6 +623,20 @@ def kinetic_step(psi, phase):
     return np.fft.ifftn(np.fft.fftn(psi, axes=ax) * phase[None], axes=ax)
 
 
+def fft_partial_derivative(f, grid: Grid, dim):
+    """fft_utils.jl:4-19: complex FFT of the real field, times i k_dim along the reference's
+    axis dim (1 = x, the last numpy axis), normalised inverse FFT, real part."""
+    fk = np.fft.fftn(np.asarray(f, dtype=complex))
+    return np.real(np.fft.ifftn(1j * _axis_field(grid, dim - 1, grid.k[dim - 1]) * fk))
+
+
+def fft_gradient(f, grid: Grid):
+    """fft_utils.jl:25-44: one forward FFT, then the same multiply/inverse per axis."""
+    fk = np.fft.fftn(np.asarray(f, dtype=complex))
+    return tuple(np.real(np.fft.ifftn(1j * _axis_field(grid, d, grid.k[d]) * fk))
+                 for d in range(grid.ndim))
+
+
 def ddi_step(ws: OracleWS, dtf, itp):
     """ddi.jl:168-201 / ddi_padded.jl:90-124."""
     fx, fy, fz = spin_density(ws.psi, ws.F)
