@@ -100,6 +100,8 @@ static void prof_end(bec_ws* ws, int idx) {
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
+static constexpr size_t kWorkBytes = sizeof(unsigned long long) * kWorkSlots * kWorkStride;
+
 static bool is_pow2_supported(int n) {
   return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
 }
@@ -632,8 +634,9 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMemset(ws->d_red, 0, redb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
   const size_t pb = (size_t)RS_COUNT * ws->B * ws->max_blocks * 8;
   if (hipMalloc(&ws->d_partials, pb) != hipSuccess) return cleanup(BEC_ENOMEM, "partials");
-  if (hipMalloc(&ws->d_work, 2 * sizeof(unsigned long long)) != hipSuccess)
+  if (hipMalloc(&ws->d_work, kWorkBytes) != hipSuccess)
     return cleanup(BEC_ENOMEM, "work counters");
+  if (hipMemset(ws->d_work, 0, kWorkBytes) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
   if (hipMemset(ws->d_partials, 0, pb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
   if (hipHostMalloc(&ws->h_red, redb) != hipSuccess) return cleanup(BEC_ENOMEM, "pinned red");
   // twiddles exp(-2 pi i m / N), computed in long double and stored in FP64 for both
@@ -1191,7 +1194,7 @@ int bec_profile_enable(bec_ws* ws, int on) {
   ws->prof_on = on != 0;
   ws->ev_used = 0;
   ws->prof_cls.clear();
-  HIPCHK(hipMemset(ws->d_work, 0, 2 * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(ws->d_work, 0, kWorkBytes));
   for (double& b : ws->prof_bytes) b = 0.0;
   refresh_tables(ws);
   return BEC_OK;
@@ -1200,17 +1203,22 @@ int bec_profile_enable(bec_ws* ws, int on) {
 int bec_profile_work(bec_ws* ws, double* out) {
   if (!ws || !out) return fail(BEC_EINVAL, "null argument");
   HIPCHK(hipStreamSynchronize(ws->stream));
-  unsigned long long h[2] = {0, 0};
-  HIPCHK(hipMemcpy(h, ws->d_work, sizeof(h), hipMemcpyDeviceToHost));
-  out[0] = (double)h[0];
-  out[1] = (double)h[1];
+  std::vector<unsigned long long> h((size_t)kWorkSlots * kWorkStride);
+  HIPCHK(hipMemcpy(h.data(), ws->d_work, kWorkBytes, hipMemcpyDeviceToHost));
+  unsigned long long nmv = 0, nrot = 0;
+  for (int sl = 0; sl < kWorkSlots; ++sl) {
+    nmv += h[(size_t)sl * kWorkStride];
+    nrot += h[(size_t)sl * kWorkStride + 1];
+  }
+  out[0] = (double)nmv;
+  out[1] = (double)nrot;
   out[2] = ws->tensor && ws->F == 6 ? tensor_matvec_flops<6>() : 0.0;
   out[3] = ws->F == 6 ? euler_flops<6>() : ws->F == 2 ? euler_flops<2>() : euler_flops<1>();
   for (int k = 0; k < BEC_K_COUNT; ++k) {
     out[4 + k] = ws->prof_bytes[k];
     ws->prof_bytes[k] = 0.0;
   }
-  HIPCHK(hipMemset(ws->d_work, 0, sizeof(h)));
+  HIPCHK(hipMemset(ws->d_work, 0, kWorkBytes));
   return BEC_OK;
 }
 

@@ -31,8 +31,11 @@ struct Geom {
   long long fhat_field_stride;  // nxh_pitch*sn[1]*sn[2]
   long long fhat_inst_stride;   // 3*fhat_field_stride
   int ddi_secular;
-  int pad_;
+  int xrows;           // x-pass rows per block chosen by the tile autotuner (0: size heuristic)
 };
+
+constexpr int kWorkSlots = 256;
+constexpr int kWorkStride = 8;
 
 struct DevTables {
   const double* ktab;     // [3][nmax] fftfreq k values per axis
@@ -44,7 +47,10 @@ struct DevTables {
   const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), FP64 for every T
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
-  unsigned long long* work;  // profiling only (else null): [0] tensor matvecs, [1] spin rotations
+  // profiling only (else null): kWorkSlots counter pairs, one per 64-byte line, so that the
+  // blocks of a launch spread their atomics over many addresses; slot s holds
+  // [s * kWorkStride] tensor matvecs and [s * kWorkStride + 1] spin rotations
+  unsigned long long* work;
   int max_blocks;
   int maxB;
 };
@@ -88,6 +94,10 @@ constexpr int strided_tb_c(int N, int fields, int csize) {
 }
 int strided_tb(int N, int fields, int csize);
 int xpass_rows(int D, int NX, int nrows);
+// rows per x-pass block for this geometry (the tuned value when set)
+int xpass_rows_g(const Geom& g);
+// whether `rows` rows per block is a valid x-pass tile for this geometry
+bool xpass_rows_ok(const Geom& g, int rows);
 // Blocks per instance of an x-pass launch (= number of per-block partial sums it writes).
 int xpass_grid(const Geom& g);
 

@@ -564,15 +564,18 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
   int nmv = 0, nrot = 0;
   if (site < g.nsites) site_ops<T, D>(g, tb, prog, psi, phi, s_ops, zf, inst, site, dt, itp, nmv, nrot);
   if (tb.work) {  // profiling: executed tensor matvecs and spin rotations
-    __shared__ unsigned int s_cnt[2];
-    if (tid == 0) s_cnt[0] = s_cnt[1] = 0u;
-    __syncthreads();
-    if (nmv) atomicAdd(&s_cnt[0], (unsigned int)nmv);
-    if (nrot) atomicAdd(&s_cnt[1], (unsigned int)nrot);
-    __syncthreads();
-    if (tid == 0) {
-      atomicAdd(&tb.work[0], (unsigned long long)s_cnt[0]);
-      atomicAdd(&tb.work[1], (unsigned long long)s_cnt[1]);
+    // wave sums (no LDS, no barrier), then one atomic per wave into a spread slot
+    int a = nmv, b = nrot;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o);
+      b += __shfl_xor(b, o);
+    }
+    if ((tid & 63) == 0) {
+      const unsigned slot = (blockIdx.x * (blockDim.x >> 6) + (tid >> 6)) % kWorkSlots;
+      unsigned long long* w = tb.work + (size_t)slot * kWorkStride;
+      if (a) atomicAdd(&w[0], (unsigned long long)a);
+      if (b) atomicAdd(&w[1], (unsigned long long)b);
     }
   }
 }
@@ -953,10 +956,19 @@ static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int A
          sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double);
 }
 
+int xpass_rows_g(const Geom& g) {
+  return g.xrows > 0 ? g.xrows : xpass_rows(g.D, g.n[0], g.nrows);
+}
+
+bool xpass_rows_ok(const Geom& g, int rows) {
+  return rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
+         g.D * rows * (g.n[0] / 8) <= 512;
+}
+
 template <typename T, int NX, int D>
 static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
                           const void* src, void* dst, void* fhat, int n_active, void* phi) {
-  const int rows = xpass_rows(D, NX, g.nrows);
+  const int rows = xpass_rows_g(g);
   const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
   const size_t lds = xpass_lds(D, rows, NX, sizeof(cplx<T>), has_aux, g.spad ? 2 * NX : NX);
   const int threads = D * rows * (NX / xpass_e<NX, D>());
@@ -970,7 +982,7 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
   return hipGetLastError();
 }
 
-int xpass_grid(const Geom& g) { return g.nrows / xpass_rows(g.D, g.n[0], g.nrows); }
+int xpass_grid(const Geom& g) { return g.nrows / xpass_rows_g(g); }
 
 template <typename T, int D>
 static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,

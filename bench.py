@@ -63,6 +63,65 @@ def _traffic(workload, kernel):
         return None
 
 
+def _kernel_class(name):
+    """Bench kernel class of a HIP kernel symbol (as the profiled pass groups launches)."""
+    if "site_kernel" in name:
+        return "sites"
+    if "xpass_kernel" in name:
+        return "xpass"
+    if "energy_kernel" in name:
+        return "energy"
+    if "strided_kernel" in name:
+        args = [a.strip() for a in name.split("<", 1)[1].split(">", 1)[0].split(",")]
+        mode, fields = int(args[2]), int(args[4])
+        if fields == 3:
+            return "ddi_conv" if mode == 3 else "ddi_fft"
+        return {0: "fwd", 1: "inv", 2: "kin", 4: "ke", 5: "deriv"}.get(mode)
+    return None
+
+
+def pmc_summary(fetch_csv, write_csv, copy_bytes, workload, out_path, precision):
+    """HBM bytes per launch by kernel class from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch), each scaled so that the largest __amd_rocclr_copyBuffer
+    dispatch, a copy of `copy_bytes`, reads and writes exactly that many bytes."""
+    import csv
+    import collections
+
+    def load(path, counter):
+        d = collections.defaultdict(list)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return d
+
+    fe, wr = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    cf = max(max(v) for k, v in fe.items() if "copyBuffer" in k)
+    cw = max(max(v) for k, v in wr.items() if "copyBuffer" in k)
+    sf, sw = copy_bytes / 1024 / cf, copy_bytes / 1024 / cw
+    cls = collections.defaultdict(lambda: [0, 0.0, 0.0])
+    for k, v in fe.items():
+        c = _kernel_class(k)
+        # an FP32-bulk workload runs FP64 kernels only inside the energy evaluation
+        if c is not None and precision == "fp32" and "<double" in k:
+            c = "energy"
+        if "widen_kernel" in k:
+            c = "energy"
+        if c is None or k not in wr:
+            continue
+        cls[c][0] += len(v)
+        cls[c][1] += sum(v) * 1024 * sf
+        cls[c][2] += sum(wr[k]) * 1024 * sw
+    out = {"workload": workload, "precision": precision, "copy_bytes": copy_bytes,
+           "scale_fetch": sf, "scale_write": sw,
+           "kernels": {c: {"launches": n, "fetch_bytes_per_launch": f / n,
+                           "write_bytes_per_launch": w / n, "hbm_bytes_per_launch": (f + w) / n}
+                       for c, (n, f, w) in sorted(cls.items())}}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def _rooflines(prof, work, precision):
     """Per-class achieved figures from the profiled pass: HBM classes in GB/s of
     algorithmic bytes; the lane-per-site pass in TFLOP/s of executed flops."""
@@ -358,7 +417,17 @@ def main():
                     help="serve_mix: also time every request alone, one after another")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/<workload>_traffic.json from two rocprofv3 --pmc runs")
+    ap.add_argument("--copy-bytes", type=float, default=None,
+                    help="--pmc-summary: bytes of the largest copyBuffer dispatch")
     args = ap.parse_args()
+    if args.pmc_summary:
+        prec = args.precision or ("fp32" if args.workload.startswith(("yoshida", "serve"))
+                                  else "fp64")
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_bytes, args.workload,
+                    os.path.join(ROOT, "profiles", f"{args.workload}_traffic.json"), prec)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; sharding across GPUs is a "
                                    "design note only, not built", "n_gpus": args.gpus}))

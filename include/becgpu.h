@@ -57,7 +57,7 @@ typedef struct bec_config {
   int normalize_every; /* SimParams.normalize_every (ITP renormalisation cadence) */
   int enable_ddi;      /* make_workspace(enable_ddi=...) */
   int ddi_secular;     /* make_ddi_params(secular=...) */
-  int ddi_padding;     /* make_workspace(ddi_padding=...) (not built: BEC_EUNSUP) */
+  int ddi_padding;     /* make_workspace(ddi_padding=...): 2x zero-padded DDI, sides <= 256 */
   int tensor;          /* general-F tensor step replaces spin mixing (TensorInteractionCache
                         * active, initialization.jl:164-185); couplings via bec_set_tensor */
   /* Cell-centred coordinates and fftfreq wavenumbers of each axis, concatenated

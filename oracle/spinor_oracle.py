@@ -27,7 +27,12 @@ a retyped copy of its own formula):
                             m-ordering of the decay; density dependence; L3 uniform)
   CG / 6j ................. sympy exact + the reference test's known values
   DDI ..................... direct O(N^2) k-sum convolution on small grids;
-                            uniform polarised E_ddi = 0
+                            uniform polarised E_ddi = 0; padded variant: brute-force
+                            linear (non-periodic) convolution on the doubled grid
+  constrained Mz .......... closed form of the two-level Newton problem
+  adaptive ITP ............ fixed-dt ITP limit; harmonic E = 1/2
+  FFT derivative .......... exact gradients of band-limited trigonometric fields on
+                            anisotropic grids; Gaussian derivative closed form
   ITP / energy ............ uniform-state energies; c0=0 harmonic E = D/2;
                             3D F=6 golden density cuts (examples/eu151/density_64.json)
   Yoshida ................. coefficient closed form; 4th-order convergence ratio
