@@ -71,6 +71,8 @@ SIGNATURES = {
     "bec_normalize_constrained": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_synchronize": ([C.c_void_p], C.c_int),
     "bec_query": ([C.c_void_p], C.c_int),
+    "bec_autotune": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+    "bec_split_step_graph": ([C.c_void_p, C.c_int], C.c_int),
     "bec_fft_derivative": ([C.POINTER(BecConfig), C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_fft_gradient": ([C.POINTER(BecConfig), C.c_void_p, C.c_void_p], C.c_int),
     "bec_last_error": ([], C.c_char_p),

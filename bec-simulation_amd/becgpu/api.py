@@ -52,7 +52,7 @@ class Workspace:
 
     def __init__(self, *, grid: Grid, F: int, n_instances: int, sim_params: SimParams,
                  precision="fp64", enable_ddi=False, secular_ddi=False, ddi_padding=False,
-                 tensor=False):
+                 tensor=False, autotune=True):
         self.grid = grid
         self.F = F
         self.sys = SpinSystem(F)
@@ -93,6 +93,7 @@ class Workspace:
         self.zeeman = [ZeemanParams()] * n_instances
         self._loss = [False] * n_instances
         self._active = list(range(n_instances))
+        self.xrows = self.autotune() if autotune else None
 
     # -- handle ------------------------------------------------------------
     @property
@@ -251,6 +252,13 @@ class Workspace:
     def synchronize(self):
         L.check(L.lib().bec_synchronize(self.handle), "bec_synchronize")
 
+    def autotune(self):
+        """Pick the x-pass tile for this shape (timed once per shape and process; psi is not
+        modified).  Returns the x rows per block in use."""
+        r = C.c_int(0)
+        L.check(L.lib().bec_autotune(self.handle, C.byref(r)), "bec_autotune")
+        return int(r.value)
+
     def query(self):
         """True when all work queued on this workspace has finished (does not block)."""
         rc = L.lib().bec_query(self.handle)
@@ -394,9 +402,13 @@ def _record_snapshot(ws, res: SimulationResult, keep_psi=True, inst=0):
 # ---------------------------------------------------------------------------
 # Integrators
 # ---------------------------------------------------------------------------
-def split_step(ws: Workspace, n: int = 1):
-    """split_step! (split_step.jl:10-35), n times."""
-    L.check(L.lib().bec_split_step(ws.handle, int(n)), "bec_split_step")
+def split_step(ws: Workspace, n: int = 1, graph: bool = False):
+    """split_step! (split_step.jl:10-35), n times.  graph: issue the n steps as one captured
+    HIP graph (cached per step count and active set; same results)."""
+    if graph:
+        L.check(L.lib().bec_split_step_graph(ws.handle, int(n)), "bec_split_step_graph")
+    else:
+        L.check(L.lib().bec_split_step(ws.handle, int(n)), "bec_split_step")
     if not ws.sim_params.imaginary_time:
         for i in ws.active:
             for _ in range(n):

@@ -51,8 +51,9 @@ def _group_key(job: ITPJob, precision):
 class _Group:
     """The instances of one workspace and their ITP loop state."""
 
-    def __init__(self, jobs, idx, precision):
+    def __init__(self, jobs, idx, precision, graphs=False):
         j0 = jobs[0]
+        self.graphs = graphs
         self.idx = idx
         self.jobs = jobs
         self.save_every = max(1, j0.n_steps // 10)
@@ -85,7 +86,10 @@ class _Group:
             k = 1
         else:
             k = min(self.save_every - self.step % self.save_every, self.n_steps - self.step)
-        L.check(L.lib().bec_split_step(self.ws.handle, int(k)), "bec_split_step")
+        if self.graphs:
+            L.check(L.lib().bec_split_step_graph(self.ws.handle, int(k)), "bec_split_step_graph")
+        else:
+            L.check(L.lib().bec_split_step(self.ws.handle, int(k)), "bec_split_step")
         self.step += k
         return k
 
@@ -136,8 +140,9 @@ class _Group:
         return res
 
 
-def solve_batch(jobs, *, precision="fp64", keep_psi=False):
-    """Run every job to convergence or n_steps.  Returns (results in job order, stats)."""
+def solve_batch(jobs, *, precision="fp64", keep_psi=False, graphs=False):
+    """Run every job to convergence or n_steps.  Returns (results in job order, stats).
+    graphs: issue each group's steps between check points as captured HIP graphs."""
     for j in jobs:
         _validate_itp_zeeman(j.zeeman, j.F, j.dt)
         _validate_itp_interactions(j.interactions, j.F, j.dt, psi=j.psi0)
@@ -145,7 +150,7 @@ def solve_batch(jobs, *, precision="fp64", keep_psi=False):
     for k, j in enumerate(jobs):
         groups.setdefault(_group_key(j, precision), []).append(k)
     t0 = time.perf_counter()
-    gs = [_Group([jobs[k] for k in idx], idx, precision) for idx in groups.values()]
+    gs = [_Group([jobs[k] for k in idx], idx, precision, graphs) for idx in groups.values()]
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     live = list(gs)

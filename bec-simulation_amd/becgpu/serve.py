@@ -117,15 +117,18 @@ class _Pool:
 
 
 class Server:
-    """Single host thread; device-resident pools; continuous batching."""
+    """Single host thread; device-resident pools; continuous batching.  graphs: issue each
+    chunk as a captured HIP graph (bec_split_step_graph), chunk lengths rounded down to
+    powers of two so that each pool replays a few cached graphs."""
 
     def __init__(self, *, hbm_budget_bytes=200 << 30, slots_per_pool=8, chunk_ms=20.0,
-                 max_chunk=500, keep_psi=False):
+                 max_chunk=512, keep_psi=False, graphs=False):
         self.budget = int(hbm_budget_bytes)
         self.slots_per_pool = int(slots_per_pool)
         self.chunk_ms = float(chunk_ms)
         self.max_chunk = int(max_chunk)
         self.keep_psi = keep_psi
+        self.graphs = graphs
         self.pools = {}
         self.used = 0
         self.peak_used = 0
@@ -204,7 +207,12 @@ class Server:
         k = min(k, max(1, int(self.chunk_ms / p.ms_per_step)) if p.ms_per_step else 4)
         k = max(1, min(k, self.max_chunk))
         t = time.perf_counter()
-        L.check(L.lib().bec_split_step(p.ws.handle, int(k)), "bec_split_step")
+        if self.graphs:
+            # chunks of 2^j steps, so that a pool replays a few captured graphs
+            k = 1 << (k.bit_length() - 1)
+            L.check(L.lib().bec_split_step_graph(p.ws.handle, int(k)), "bec_split_step_graph")
+        else:
+            L.check(L.lib().bec_split_step(p.ws.handle, int(k)), "bec_split_step")
         self.host_issue_s += time.perf_counter() - t
         for st in p.slot.values():
             st["step"] += k

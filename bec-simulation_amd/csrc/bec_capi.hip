@@ -5,6 +5,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -72,7 +75,21 @@ struct bec_ws {
   size_t ev_used = 0;
   std::vector<int> prof_cls;
   double prof_bytes[BEC_K_COUNT] = {};  // algorithmic HBM bytes issued per kernel class
+  // captured split-step runs (bec_split_step_graph), keyed by step count, active list and
+  // the schedule flags; dropped whenever a captured kernel argument could change
+  struct Graph {
+    int n;
+    std::vector<int> active;
+    int flags;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
 };
+
+static void drop_graphs(bec_ws* ws) {
+  for (auto& g : ws->graphs) (void)hipGraphExecDestroy(g.exec);
+  ws->graphs.clear();
+}
 
 // Kernel classes reported by bec_profile_read (order = BEC_K_* in becgpu.h).
 // `bytes`: algorithmic HBM bytes of the launch (each array the pass must touch read
@@ -107,6 +124,7 @@ static bool is_pow2_supported(int n) {
 }
 
 static void refresh_tables(bec_ws* ws) {
+  drop_graphs(ws);  // captured launches hold the previous tables by value
   DevTables& t = ws->tb;
   t.ktab = ws->d_ktab;
   t.sktab = ws->d_sktab ? ws->d_sktab : ws->d_ktab;
@@ -478,7 +496,8 @@ static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out
   const int nrows = ny * nz;
   const int rows = xpass_rows(D, nx, nrows);
   int nxb = nrows / rows;
-  int maxb = std::max(nxb, 1024);
+  // the tile autotuner may pick down to a quarter of the default rows per block
+  int maxb = std::max(std::min(nrows, 4 * nxb), 1024);
   // strided SM_KE partial count
   for (int axis = 1; axis < c->ndim; ++axis) {
     const int N = axis == 1 ? ny : nz;
@@ -679,6 +698,7 @@ void bec_ws_destroy(bec_ws* ws) {
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
   if (ws->h_red) (void)hipHostFree(ws->h_red);
   for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
+  drop_graphs(ws);
   if (ws->stream) (void)hipStreamDestroy(ws->stream);
   delete ws;
 }
@@ -823,6 +843,55 @@ int bec_split_step(bec_ws* ws, int n_steps) {
       items.push_back({IT_NORM, 0.0});
   }
   return run_items(ws, items, ws->itp, 0.0);
+}
+
+// Which operators the schedule compiler includes for the active instances (the captured
+// launch sequence depends on these and on nothing else in the parameters: every value is
+// read from device memory by the kernels).
+static int sched_flags(const bec_ws* ws) {
+  int f = 0;
+  for (int inst : ws->active) {
+    if (std::fabs(ws->hp[inst].c1) >= 1e-30) f |= 1;
+    if (std::fabs(ws->hp[inst].c2) >= 1e-30) f |= 2;
+    if (ws->hp[inst].has_loss) f |= 4;
+  }
+  return f;
+}
+
+int bec_split_step_graph(bec_ws* ws, int n_steps) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (n_steps < 0) return fail(BEC_EINVAL, "n_steps < 0");
+  if (n_steps == 0 || n_active(ws) == 0) return BEC_OK;
+  if (ws->prof_on) return bec_split_step(ws, n_steps);  // per-launch events are not captured
+  CHK(sync_params(ws));  // parameters reach the device before (never inside) the capture
+  const int flags = sched_flags(ws);
+  bec_ws::Graph* hit = nullptr;
+  for (auto& g : ws->graphs)
+    if (g.n == n_steps && g.flags == flags && g.active == ws->active) hit = &g;
+  if (!hit) {
+    constexpr size_t kMaxGraphs = 32;
+    if (ws->graphs.size() >= kMaxGraphs) {
+      (void)hipGraphExecDestroy(ws->graphs.front().exec);
+      ws->graphs.erase(ws->graphs.begin());
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = bec_split_step(ws, n_steps);
+    const hipError_t ec = hipStreamEndCapture(ws->stream, &graph);
+    if (rc != BEC_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(ec);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(ei);
+    ws->graphs.push_back({n_steps, ws->active, flags, exec});
+    hit = &ws->graphs.back();
+  }
+  HIPCHK(hipGraphLaunch(hit->exec, ws->stream));
+  return BEC_OK;
 }
 
 int bec_leapfrog_segment(bec_ws* ws, int n_steps) {
@@ -991,6 +1060,89 @@ int bec_fft_derivative(const bec_config* grid, const double* f, int dim, double*
 
 int bec_fft_gradient(const bec_config* grid, const double* f, double* out) {
   return fft_gradient_impl(grid, f, -1, out);
+}
+
+// Tile autotuner.  The x pass's rows per block set its LDS tile, its thread count and the
+// number of blocks in flight; the best value depends on (precision, n_x, D, number of rows).
+// Candidates (powers of two that divide the rows, within 512 threads and the partials
+// buffer) are timed on the workspace's own data: an inverse FFT + diagonal step + forward
+// FFT pass from psi into the scratch buffer, one warm-up and three timed launches each.
+// A candidate replaces the size heuristic only when it is 3 % faster.  Decisions are kept
+// per process and shape, so every workspace of a shape uses the same tiles (the block
+// partition is also the partition of the block-partial reductions).
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_tune;
+
+int bec_autotune(bec_ws* ws, int* rows_out) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  const auto key = std::make_tuple((int)ws->prec, ws->g.n[0], ws->g.D, ws->g.nrows);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) {
+    if (ws->g.xrows != it->second) drop_graphs(ws);
+    ws->g.xrows = it->second;
+    if (rows_out) *rows_out = xpass_rows_g(ws->g);
+    return BEC_OK;
+  }
+  Geom g = ws->g;
+  g.xrows = 0;
+  const int r0 = xpass_rows_g(g);
+  std::vector<int> cand;
+  for (int r = 1; r <= g.nrows; r *= 2) {
+    g.xrows = r;
+    if (xpass_rows_ok(g, r) && g.nrows / r <= ws->max_blocks && r * 4 >= r0) cand.push_back(r);
+  }
+  int best = 0;
+  if (cand.size() > 1 && n_active(ws) > 0) {
+    CHK(sync_params(ws));
+    XProgram p{};
+    p.stages = XS_INV_FFT | XS_FWD_FFT | XS_OPS_A;
+    p.itp = ws->itp;
+    p.nA = 1;
+    p.A[0].kind = OP_DIAG;
+    p.A[0].frac = 0.5;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double t_best = 0.0, t_def = 0.0;
+    int rc = BEC_OK;
+    for (int r : cand) {
+      ws->g.xrows = r;
+      hipError_t e = hipSuccess;
+      for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+        if (k == 1) e = hipEventRecord(e0, ws->stream);
+        if (e == hipSuccess)
+          e = ws->prec == BEC_FP32
+                  ? launch_xpass<float>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->scratch,
+                                        ws->fhat, n_active(ws), ws->phi)
+                  : launch_xpass<double>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->scratch,
+                                         ws->fhat, n_active(ws), ws->phi);
+      }
+      if (e == hipSuccess) e = hipEventRecord(e1, ws->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(BEC_EDEVICE, std::string("autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      if (r == r0) t_def = ms;
+      if (best == 0 || ms < t_best) {
+        best = r;
+        t_best = ms;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ws->g.xrows = 0;
+    if (rc != BEC_OK) return rc;
+    if (t_def > 0.0 && t_best > 0.97 * t_def) best = 0;
+  }
+  g_tune[key] = best;
+  if (ws->g.xrows != best) drop_graphs(ws);
+  ws->g.xrows = best;
+  if (rows_out) *rows_out = xpass_rows_g(ws->g);
+  return BEC_OK;
 }
 
 int bec_query(bec_ws* ws) {

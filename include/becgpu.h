@@ -104,6 +104,12 @@ int bec_set_active(bec_ws* ws, const int* insts, int n);
 /* split_step! x n_steps (split_step.jl:10-35): V(dt/2) K(dt) V(dt/2), ITP
  * renormalisation every normalize_every steps.  t/step bookkeeping is host-side. */
 int bec_split_step(bec_ws* ws, int n_steps);
+/* The same n_steps as one captured HIP graph, replayed on later calls with the same step
+ * count, active list and set of active operators (no counterpart in the reference: it
+ * removes the per-kernel launch cost of small grids).  Parameters (dt, couplings, trap,
+ * loss rates) may change between replays: the kernels read them from device memory.
+ * Results are bitwise those of bec_split_step.  With profiling on it runs uncaptured. */
+int bec_split_step_graph(bec_ws* ws, int n_steps);
 /* One leapfrog segment of _run_simulation_leapfrog! (simulation.jl:49-94):
  * V(dt/2) [K(dt) V(dt)]^(n-1) K(dt) V(dt/2). */
 int bec_leapfrog_segment(bec_ws* ws, int n_steps);
@@ -171,6 +177,12 @@ int bec_fft_derivative(const bec_config* cfg, const double* f, int dim, double* 
 int bec_fft_gradient(const bec_config* cfg, const double* f, double* out);
 
 int bec_synchronize(bec_ws* ws);
+/* Tile autotuner (no counterpart in the reference, whose FFTW plans are tuned by
+ * FFTW_MEASURE, src/grid.jl:37-53): times the x-pass tile candidates on this workspace's
+ * data and keeps the fastest for the shape (precision, n_x, D, grid rows) for the rest of
+ * the process; later workspaces of the shape reuse the decision without timing.  psi is
+ * not modified.  rows_out (optional): the x rows per block now in use. */
+int bec_autotune(bec_ws* ws, int* rows_out);
 /* Non-blocking completion test of the work queued on the workspace's stream: 0 when all
  * of it has finished, 1 while some is still running, a negative code on a device error.
  * No counterpart in the reference (its calls are synchronous); a request loop polls it
