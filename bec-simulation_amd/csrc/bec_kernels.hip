@@ -112,10 +112,44 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   cplx<double>* zf =
       reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * ROWS * AL / 8 * 9 : 0));
   double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
+  cplx<double>* tws = reinterpret_cast<cplx<double>*>(rsh + 16);  // NX twiddles
 
-  const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
+  const cplx<double>* twg = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
+  const cplx<double>* tw = tws;
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
+
+  // Every global load of the prologue is issued before the first wait: each thread loads
+  // exactly EX tile elements (the launch has D*ROWS*NX/EX threads) and up to three
+  // twiddles (nthr >= 3 NX / 8), so the block waits for one memory latency, not for one
+  // per element, and the FFT passes read their twiddles from LDS.
+  // (indices are clamped rather than guarded, so the arrays stay in registers)
+  constexpr int TWK = 3;
+  cplx<double> twv[TWK];
+#pragma unroll
+  for (int k = 0; k < TWK; ++k) twv[k] = twg[min(tid + k * nthr, NX - 1)];
+  // The tile (D contiguous chunks of ROWS*NX elements) and the twiddles into LDS, before
+  // any code with a call (the Zeeman factors' sin/cos) so that nothing is live across it.
+  // No branch separates a load from its LDS write (a guarded write lets the compiler sink
+  // the load into the guard and wait for it there): the launch has exactly D*TS/EX threads,
+  // so every tile index is in range, and a twiddle index past NX - 1 rewrites entry NX - 1
+  // with the value it already holds.
+  if (!(st & XS_NO_LOAD)) {
+    cplx<T> ld[EX];
+#pragma unroll
+    for (int k = 0; k < EX; ++k) {
+      const int e = tid + k * nthr;
+      const int c = e / TS, r = e - c * TS;
+      ld[k] = src[ibase + (size_t)c * g.nsites + r];
+    }
+#pragma unroll
+    for (int k = 0; k < TWK; ++k) tws[min(tid + k * nthr, NX - 1)] = twv[k];
+#pragma unroll
+    for (int k = 0; k < EX; ++k) tile[P8(tid + k * nthr)] = ld[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < TWK; ++k) tws[min(tid + k * nthr, NX - 1)] = twv[k];
+  }
 
   // Op lists in LDS (A at [0, kMaxOps), B at [kMaxOps, 2 kMaxOps)), copied with
   // compile-time indices so the by-value kernel argument is never indexed at run time.
@@ -135,13 +169,6 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       const double dtf = op.frac * dt;
       if (itp) zf[idx] = mk(exp(-(P.zee[c] - P.zee_shift) * dtf), 0.0);
       else zf[idx] = cis_neg<double>(P.zee[c] * dtf);
-    }
-  }
-  // Load the tile: D contiguous chunks of ROWS*NX elements.
-  if (!(st & XS_NO_LOAD)) {
-    for (int e = tid; e < D * TS; e += nthr) {
-      const int c = e / TS, r = e - c * TS;
-      tile[P8(e)] = src[ibase + (size_t)c * g.nsites + r];
     }
   }
   __syncthreads();
@@ -595,7 +622,9 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
   double* tabs = reinterpret_cast<double*>(tile + FIELDS * N * TB);  // [2*(N+TB+1)] (re,im) pairs
   double* rsh = tabs + 2 * (N + TB + 1);
-  const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<N>()]);
+  cplx<double>* tws = reinterpret_cast<cplx<double>*>(rsh + 16);  // N twiddles
+  const cplx<double>* twg = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<N>()]);
+  const cplx<double>* tw = tws;
 
   const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
   const int valid = ddi_data ? g.nxh : g.n[0];
@@ -616,12 +645,27 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   const size_t base = (size_t)inst * istride + (size_t)blockIdx.y * FIELDS * fstride +
                       (size_t)other * ostride + x0;
 
-  for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
+  // All global loads first (each thread owns exactly 8 tile elements: the launch has
+  // FIELDS*TB*N/8 threads), then the LDS writes: one memory latency per block.
+  cplx<T> ld[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int idx = tid + k * nthr;
     const int xi = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
-    cplx<T> v = mk((T)0, (T)0);
-    if (x0 + xi < valid && e < n_in) v = data[base + (size_t)f * fstride + (size_t)e * estride + xi];
-    tile[idx] = v;
+    const bool in = idx < FIELDS * N * TB && x0 + xi < valid && e < n_in;
+    // out-of-range lanes read element 0 of their line and store zero
+    const cplx<T> v = data[base + (in ? (size_t)f * fstride + (size_t)e * estride + xi : 0)];
+    ld[k] = in ? v : mk((T)0, (T)0);
   }
+  cplx<double> twv[2];  // N <= 2 * nthr (nthr = FIELDS*TB*N/8 >= N/2); clamped, not guarded
+#pragma unroll
+  for (int k = 0; k < 2; ++k) twv[k] = twg[min(tid + k * nthr, N - 1)];
+  // unguarded writes (the launch has exactly FIELDS*N*TB/8 threads; a twiddle index past
+  // N - 1 rewrites entry N - 1 with the value it holds), so no load is sunk into a branch
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tile[tid + k * nthr] = ld[k];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) tws[min(tid + k * nthr, N - 1)] = twv[k];
   // per-axis phase tables for the kinetic multiply
   if (MODE == SM_KIN) {
     const InstParams& P = tb.params[inst];
@@ -953,7 +997,8 @@ int xpass_rows(int D, int NX, int nrows) {
 static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int AL) {
   // padded tiles: 9 slots per 8 elements; AL: length of the DDI auxiliary lines
   return csz * ((size_t)D * rows * NX + (has_aux ? 2 * (size_t)rows * AL : 0)) / 8 * 9 +
-         sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double);
+         sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double) +
+         sizeof(cplx<double>) * NX;  // twiddles
 }
 
 int xpass_rows_g(const Geom& g) {
@@ -1075,7 +1120,8 @@ static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, 
   const int ngroups = ddi_data ? 1 : g.D / FIELDS;
   dim3 grid(nxch * n_other, ngroups, n_active);
   const int threads = FIELDS * TB * (N / 8);
-  const size_t lds = sizeof(cplx<T>) * FIELDS * N * TB + sizeof(double) * (2 * (N + TB + 1) + 16);
+  const size_t lds = sizeof(cplx<T>) * FIELDS * N * TB + sizeof(double) * (2 * (N + TB + 1) + 16) +
+                     sizeof(cplx<double>) * N;  // twiddles
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)strided_kernel<T, N, MODE, TB, FIELDS>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
