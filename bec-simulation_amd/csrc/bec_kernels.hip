@@ -98,7 +98,9 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   const bool itp = prog.itp != 0;
   const int st = prog.stages;
   const bool has_aux = (st & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
-  const int TS = ROWS * NX;  // sites in this tile
+  const int TS = ROWS * NX;  // sites in this tile (a power of two, as are the grid sides)
+  const int ts_shift = __ffs(TS) - 1;
+  const int n1_shift = __ffs(g.n[1]) - 1, n1_mask = g.n[1] - 1;
   // LDS tiles are padded (one slot per 8 elements, lds_at<true>): element e of the tile
   // lives at P(e) = e + e/8; lines start at multiples of NX, so P(line*NX + i) =
   // P(line*NX) + P(i).
@@ -139,7 +141,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
     for (int k = 0; k < EX; ++k) {
       const int e = tid + k * nthr;
-      const int c = e / TS, r = e - c * TS;
+      const int c = e >> ts_shift, r = e & (TS - 1);
       ld[k] = src[ibase + (size_t)c * g.nsites + r];
     }
 #pragma unroll
@@ -197,7 +199,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   // spectra row of tile row r: (y, z) -> y + z*sn[1]
   auto srow_of = [&](int r) __attribute__((always_inline)) {
     const int row = row0 + r;
-    return (row % g.n[1]) + (row / g.n[1]) * g.sn[1];
+    return (row & n1_mask) + (row >> n1_shift) * g.sn[1];
   };
   if (st & XS_INV_FFT)
     fft_team<T, NX, true, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
@@ -232,8 +234,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     const int r = s / NX;
     x = s - r * NX;
     const int row = row0 + r;
-    y = row % g.n[1];
-    z = row / g.n[1];
+    y = row & n1_mask;
+    z = row >> n1_shift;
   };
   auto trap_at = [&](int s) -> double {
     int x, y, z;
@@ -416,9 +418,11 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     }
   }
 
-  if (!(st & XS_NO_STORE)) {
-    for (int e = tid; e < D * TS; e += nthr) {
-      const int c = e / TS, r = e - c * TS;
+  if (!(st & XS_NO_STORE)) {  // EX elements per thread (exact launch shape)
+#pragma unroll
+    for (int k = 0; k < EX; ++k) {
+      const int e = tid + k * nthr;
+      const int c = e >> ts_shift, r = e & (TS - 1);
       dst[ibase + (size_t)c * g.nsites + r] = tile[P8(e)];
     }
   }
@@ -443,9 +447,11 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 __device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, int inst,
                                             long long site) {
   if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + site];
-  const int x = (int)(site % g.n[0]);
-  const long long row = site / g.n[0];
-  const int y = (int)(row % g.n[1]), z = (int)(row / g.n[1]);
+  // grid sides are powers of two and an instance has fewer than 2^31 sites
+  const int sidx = (int)site;
+  const int x = sidx & (g.n[0] - 1);
+  const int row = sidx >> (__ffs(g.n[0]) - 1);
+  const int y = row & (g.n[1] - 1), z = row >> (__ffs(g.n[1]) - 1);
   const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
   double v = ax[x];
   if (g.ndim >= 2) v += ax[g.nmax + y];
