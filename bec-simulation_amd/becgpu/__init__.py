@@ -8,6 +8,7 @@ from .physics import (AdaptiveDtParams, AtomSpecies, Eu151, GridConfig, Harmonic
                       ZeemanParams, add_noise, cell_volume, classify_phase, init_psi,
                       init_psi_thomas_fermi, make_grid, spin_matrices)
 from .api import (SimulationResult, Workspace, energy_terms, fft_gradient,  # noqa: F401
-                  fft_partial_derivative, find_ground_state, find_ground_state_multistart, magnetization, make_workspace, run_simulation,
+                  fft_partial_derivative, find_ground_state, find_ground_state_multistart,
+                  magnetization, make_workspace, run_simulation, run_simulation_adaptive,
                   run_simulation_yoshida, scan_continuation, split_step, strang_core,
                   total_energy, total_norm, yoshida_core)

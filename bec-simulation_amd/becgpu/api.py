@@ -508,6 +508,104 @@ def run_simulation_yoshida(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end:
     return dict(result=res, n_accepted=n_acc, n_rejected=n_rej, final_dt=dt)
 
 
+def run_simulation_adaptive(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end: float,
+                            save_interval: float, callback=None, keep_snapshots=True):
+    """run_simulation_adaptive! (adaptive.jl:57-67), real time.  error_mode "step_change"
+    (adaptive.jl:69-194): V(dt/2) K(dt) with the closing half potential deferred and merged
+    into the next step (FSAL; flushed at save points and at the end), error = L2 change of
+    the step, factors 0.9 sqrt(tol/err) in [0.5, 2].  error_mode "richardson"
+    (adaptive.jl:203-296): one Strang step of dt against two of dt/2, factors
+    0.9 cbrt(tol/err).  The saved states stay on the device (slots 2 and 3)."""
+    lib = L.lib()
+
+    def hp(dth):
+        L.check(lib.bec_half_potential(ws.handle, float(dth), 0), "bec_half_potential")
+
+    def l2(a, b):  # sum |a - b|^2 / sum |b|^2 (_wavefunction_l2_change, adaptive.jl:42-50)
+        d = ws.diff(a, b, 0)[0]
+        return float(d[0] / max(d[1], 1e-300))
+
+    tol = adaptive.tol
+    rich = adaptive.error_mode == "richardson"
+    dt = min(max(adaptive.dt_init, adaptive.dt_min), adaptive.dt_max)
+    res = SimulationResult()
+    _record_snapshot(ws, res, keep_snapshots)
+    next_save = ws.t[0] + save_interval
+    n_acc = n_rej = 0
+    fsal, fsal_dt = False, 0.0
+    while ws.t[0] < t_end - 1e-14:
+        t = ws.t[0]
+        dt_step = min(dt, t_end - t)
+        rem = next_save - t
+        if 1e-14 < rem < dt_step:
+            dt_step = rem
+        dt_step = max(dt_step, adaptive.dt_min)
+        clamped = dt_step < dt * 0.99
+        may_reject = (not clamped) and dt_step > adaptive.dt_min * 1.01
+        if rich:
+            ws.copy_slot(2, 0)                       # psi_saved
+            strang_core(ws, dt_step)
+            ws.copy_slot(3, 0)                       # psi_full
+            ws.copy_slot(0, 2)
+            strang_core(ws, dt_step / 2)
+            strang_core(ws, dt_step / 2)
+            err = l2(3, 0)
+            if may_reject and err > tol:
+                ws.copy_slot(0, 2)
+                dt = max(dt_step * max(0.5, 0.9 * (tol / err) ** (1 / 3)), adaptive.dt_min)
+                n_rej += 1
+                continue
+        else:
+            if not clamped:
+                ws.copy_slot(2, 0)                   # psi_old
+            prev = (fsal, fsal_dt)
+            if fsal and abs(fsal_dt - dt_step) < 1e-14:
+                hp(dt_step)
+            elif fsal:
+                hp(fsal_dt / 2)
+                hp(dt_step / 2)
+            else:
+                hp(dt_step / 2)
+            fsal = False
+            L.check(lib.bec_kinetic(ws.handle, float(dt_step), 0), "bec_kinetic")
+            err = 0.0
+            if may_reject:
+                err = l2(0, 2)
+                if err > tol:
+                    ws.copy_slot(0, 2)
+                    fsal, fsal_dt = prev
+                    dt = max(dt_step * max(0.5, 0.9 * math.sqrt(tol / err)), adaptive.dt_min)
+                    n_rej += 1
+                    continue
+            fsal, fsal_dt = True, dt_step
+        if ws.has_loss:
+            L.check(lib.bec_loss_step(ws.handle, dt_step), "bec_loss_step")
+        ws.t[0] += dt_step
+        ws.step[0] += 1
+        n_acc += 1
+        if not clamped:
+            if rich:
+                fac = min(2.0, 0.9 * (tol / err) ** (1 / 3)) if err > 1e-300 else 2.0
+            else:
+                if not may_reject:
+                    err = l2(0, 2)
+                fac = min(2.0, 0.9 * math.sqrt(tol / err)) if err > 1e-300 else 2.0
+            dt = min(max(dt * fac, adaptive.dt_min), adaptive.dt_max)
+        if ws.t[0] >= next_save - 1e-14:
+            if fsal:
+                hp(fsal_dt / 2)
+                fsal = False
+            _record_snapshot(ws, res, keep_snapshots)
+            if callback is not None:
+                callback(ws, n_acc)
+            next_save += save_interval
+    if fsal:
+        hp(fsal_dt / 2)
+    if not res.times or abs(res.times[-1] - ws.t[0]) > 1e-12:
+        _record_snapshot(ws, res, keep_snapshots)
+    return dict(result=res, n_accepted=n_acc, n_rejected=n_rej, final_dt=dt)
+
+
 # ---------------------------------------------------------------------------
 # Ground state drivers (ground_state.jl)
 # ---------------------------------------------------------------------------

@@ -389,6 +389,8 @@ class AdaptiveDtParams:
             raise ValueError("dt_max must be >= dt_min")
         if not self.tol > 0:
             raise ValueError("tol must be positive")
+        if self.error_mode not in ("step_change", "richardson"):
+            raise ValueError(f"error_mode must be step_change or richardson, got {self.error_mode}")
 
 
 # ---------------------------------------------------------------------------

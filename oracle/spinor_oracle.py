@@ -31,6 +31,9 @@ a retyped copy of its own formula):
                             linear (non-periodic) convolution on the doubled grid
   constrained Mz .......... closed form of the two-level Newton problem
   adaptive ITP ............ fixed-dt ITP limit; harmonic E = 1/2
+  adaptive Strang ......... fixed-dt limits (step-change == split steps, Richardson == two
+                            half steps) with commuting potentials; the reference's
+                            run_simulation_adaptive! tests (norm, energy vs fixed dt, growth)
   FFT derivative .......... exact gradients of band-limited trigonometric fields on
                             anisotropic grids; Gaussian derivative closed form
   ITP / energy ............ uniform-state energies; c0=0 harmonic E = D/2;
@@ -961,6 +964,91 @@ def run_simulation_yoshida(ws: OracleWS, *, dt_init=1e-3, dt_min=1e-5, dt_max=1e
         if ws.t >= next_save - 1e-14:
             _snapshot(ws, rec)
             next_save += save_interval
+    if not rec["times"] or abs(rec["times"][-1] - ws.t) > 1e-12:
+        _snapshot(ws, rec)
+    return dict(result=rec, n_accepted=n_acc, n_rejected=n_rej, final_dt=dt)
+
+
+def run_simulation_adaptive(ws: OracleWS, *, dt_init=1e-3, dt_min=1e-5, dt_max=1e-2, tol=1e-3,
+                            error_mode="step_change", t_end, save_interval):
+    """adaptive.jl:57-67 (run_simulation_adaptive!, real time): the step-change loop
+    (adaptive.jl:69-194: V(dt/2) K(dt) with the closing half potential deferred and merged
+    into the next step (FSAL), error = L2 change of the step, factors 0.9 sqrt(tol/err)
+    clamped to [0.5, 2]) or the Richardson loop (adaptive.jl:203-296: one Strang step of dt
+    against two of dt/2, factors 0.9 cbrt(tol/err))."""
+    dt = min(max(dt_init, dt_min), dt_max)
+    rec = dict(times=[], energies=[], norms=[], mags=[], psi=[])
+    _snapshot(ws, rec)
+    next_save = ws.t + save_interval
+    n_acc = n_rej = 0
+    fsal, fsal_dt = False, 0.0
+
+    def hp(dth):
+        half_potential_step(ws, dth, False)
+
+    while ws.t < t_end - 1e-14:
+        dt_step = min(dt, t_end - ws.t)
+        rem = next_save - ws.t
+        if 1e-14 < rem < dt_step:
+            dt_step = rem
+        dt_step = max(dt_step, dt_min)
+        clamped = dt_step < dt * 0.99
+        may_reject = (not clamped) and dt_step > dt_min * 1.01
+        if error_mode == "richardson":
+            saved = ws.psi.copy()
+            strang_core(ws, dt_step)
+            full = ws.psi.copy()
+            ws.psi = saved.copy()
+            strang_core(ws, dt_step / 2)
+            strang_core(ws, dt_step / 2)
+            err = l2_change(full, ws.psi)
+            if may_reject and err > tol:
+                ws.psi = saved
+                dt = max(dt_step * max(0.5, 0.9 * (tol / err) ** (1 / 3)), dt_min)
+                n_rej += 1
+                continue
+        else:
+            old = ws.psi.copy() if not clamped else None
+            prev = (fsal, fsal_dt)
+            if fsal and abs(fsal_dt - dt_step) < 1e-14:
+                hp(dt_step)
+            elif fsal:
+                hp(fsal_dt / 2)
+                hp(dt_step / 2)
+            else:
+                hp(dt_step / 2)
+            fsal = False
+            ws.psi = kinetic_step(ws.psi, kinetic_phase(ws.grid, dt_step, False))
+            err = 0.0
+            if may_reject:
+                err = l2_change(ws.psi, old)
+                if err > tol:
+                    ws.psi = old
+                    fsal, fsal_dt = prev
+                    dt = max(dt_step * max(0.5, 0.9 * math.sqrt(tol / err)), dt_min)
+                    n_rej += 1
+                    continue
+            fsal, fsal_dt = True, dt_step
+        apply_loss(ws, dt_step)
+        ws.t += dt_step
+        ws.step += 1
+        n_acc += 1
+        if not clamped:
+            if error_mode == "richardson":
+                fac = min(2.0, 0.9 * (tol / err) ** (1 / 3)) if err > 1e-300 else 2.0
+            else:
+                if not may_reject:
+                    err = l2_change(ws.psi, old)
+                fac = min(2.0, 0.9 * math.sqrt(tol / err)) if err > 1e-300 else 2.0
+            dt = min(max(dt * fac, dt_min), dt_max)
+        if ws.t >= next_save - 1e-14:
+            if fsal:
+                hp(fsal_dt / 2)
+                fsal = False
+            _snapshot(ws, rec)
+            next_save += save_interval
+    if fsal:
+        hp(fsal_dt / 2)
     if not rec["times"] or abs(rec["times"][-1] - ws.t) > 1e-12:
         _snapshot(ws, rec)
     return dict(result=rec, n_accepted=n_acc, n_rejected=n_rej, final_dt=dt)

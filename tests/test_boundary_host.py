@@ -156,3 +156,13 @@ def test_host_loss_rates_match_oracle(F):
     from becgpu import physics as P
     assert np.allclose(P.dipolar_relaxation_rates(F, 0.7), O.dipolar_relaxation_rates(F, 0.7),
                        rtol=1e-12, atol=1e-15)
+
+
+def test_adaptive_dt_params_validation():
+    """types.jl:343-361 / test_adaptive_dt.jl:28-47."""
+    p = G.AdaptiveDtParams()
+    assert (p.dt_init, p.dt_min, p.dt_max, p.tol, p.error_mode) == (0.001, 1e-5, 0.01, 1e-3, "step_change")
+    assert G.AdaptiveDtParams(error_mode="richardson").error_mode == "richardson"
+    for bad in (dict(dt_init=-1.0), dict(dt_min=0.1, dt_max=0.01), dict(tol=-1.0), dict(error_mode="foo")):
+        with pytest.raises(ValueError):
+            G.AdaptiveDtParams(**bad)

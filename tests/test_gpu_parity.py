@@ -469,3 +469,32 @@ def test_fft_gradient_and_derivative(n, L):
     got = G.fft_gradient(np.cos(ph) * np.ones(og.shape), gg)
     for a in range(og.ndim):
         assert np.max(np.abs(got[a] + kv[a] * np.sin(ph))) < 1e-12 * max(kv)
+
+
+@pytest.mark.parametrize("mode", ["step_change", "richardson"])
+@pytest.mark.parametrize("n,L,F", [((64,), (20.0,), 1), ((32, 32), (12.0, 12.0), 2)])
+def test_run_simulation_adaptive(mode, n, L, F):
+    """run_simulation_adaptive! (adaptive.jl:57-296) vs the oracle: same accept/reject
+    sequence, final dt, snapshots and psi."""
+    og, gg = grids(n, L)
+    D = 2 * F + 1
+    psi = O.init_psi(og, F, "uniform") * (1 + 0.05 * np.random.default_rng(3).normal(size=og.shape))[None]
+    psi = psi + 0.02 * random_psi(og.shape, D, 4)
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    omega = (1.0,) * len(n)
+    ows, gws = pair(n, L, F, psi, 0.001, False, omega=omega, c0=10.0, c1=-0.5, q=0.1)
+    ro = O.run_simulation_adaptive(ows, dt_init=0.002, dt_min=1e-5, dt_max=0.01, tol=1e-4,
+                                   error_mode=mode, t_end=0.15, save_interval=0.05)
+    rg = G.run_simulation_adaptive(gws, adaptive=G.AdaptiveDtParams(dt_init=0.002, dt_min=1e-5,
+                                                                    dt_max=0.01, tol=1e-4,
+                                                                    error_mode=mode),
+                                   t_end=0.15, save_interval=0.05)
+    print(mode, n, ro["n_accepted"], ro["n_rejected"], rg["n_accepted"], rg["n_rejected"])
+    assert (rg["n_accepted"], rg["n_rejected"]) == (ro["n_accepted"], ro["n_rejected"])
+    assert rg["final_dt"] == pytest.approx(ro["final_dt"], rel=1e-9)
+    assert np.allclose(rg["result"].times, ro["result"]["times"], rtol=0, atol=1e-12)
+    for a, b in zip(rg["result"].energies, ro["result"]["energies"]):
+        assert abs(a - b) <= E_TOL * abs(b)
+    got = gws.get_psi(0)
+    for c in range(D):
+        assert rel_l2(got[c], ows.psi[c]) <= PSI_TOL

@@ -666,3 +666,61 @@ def test_fft_derivative_gaussian_spectral_accuracy():
     s = 1.0
     f = np.exp(-x ** 2 / (2 * s * s))
     assert np.max(np.abs(O.fft_partial_derivative(f, g, 1) + x / s ** 2 * f)) < 1e-12
+
+
+def _adaptive_ws(c1=-0.5, n=64, L=20.0):
+    g = O.make_grid((n,), (L,))
+    psi = O.init_psi(g, 1, "uniform") * (1 + 0.05 * rng(7).normal(size=g.shape))[None]
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * g.dV)
+    return g, psi
+
+
+def test_adaptive_strang_fixed_dt_limits():
+    """With dt_min = dt_max = dt_init nothing adapts: the step-change (FSAL) loop must equal
+    split_step! (V(dt/2) K V(dt/2)) with the half potentials merged, and the Richardson loop
+    two split steps of dt/2 each, to roundoff when the potential operators commute (c1 = 0:
+    merging V(a) V(b) = V(a + b) is exact up to rounding)."""
+    g, psi = _adaptive_ws()
+    dt = 0.004
+    for mode, sub in (("step_change", 1), ("richardson", 2)):
+        a = O.make_workspace(g, 1, psi, dt, False, omega=(1.0,), c0=10.0, c1=0.0, q=0.2)
+        out = O.run_simulation_adaptive(a, dt_init=dt, dt_min=dt, dt_max=dt, tol=1e-3,
+                                        error_mode=mode, t_end=0.2, save_interval=0.1)
+        assert out["n_rejected"] == 0 and out["n_accepted"] == 50
+        b = O.make_workspace(g, 1, psi, dt / sub, False, omega=(1.0,), c0=10.0, c1=0.0, q=0.2)
+        for _ in range(50 * sub):
+            O.split_step(b)
+        assert np.max(np.abs(a.psi - b.psi)) < 1e-11, mode
+        assert out["result"]["times"] == pytest.approx([0.0, 0.1, 0.2])
+
+
+@pytest.mark.parametrize("mode", ["step_change", "richardson"])
+def test_adaptive_strang_reference_behaviour(mode):
+    """test_adaptive_dt.jl:55-131: norm conserved to 1e-4 at every save, the final energy
+    within 1 % of a fixed-dt leapfrog run, dt grows from a small initial value."""
+    g, psi = _adaptive_ws()
+    a = O.make_workspace(g, 1, psi, 0.001, False, omega=(1.0,), c0=10.0, c1=-0.5)
+    out = O.run_simulation_adaptive(a, dt_init=0.001, dt_min=1e-5, dt_max=0.01, tol=1e-3,
+                                    error_mode=mode, t_end=0.5, save_interval=0.1)
+    rec = out["result"]
+    assert out["n_accepted"] > 0 and len(rec["times"]) >= 3
+    assert rec["times"][0] == 0.0 and rec["times"][-1] >= 0.5 - 0.01
+    assert all(abs(nn - rec["norms"][0]) <= 1e-4 * rec["norms"][0] for nn in rec["norms"])
+    f = O.make_workspace(g, 1, psi, 0.001, False, omega=(1.0,), c0=10.0, c1=-0.5, n_steps=500,
+                         save_every=500)
+    rf = O.run_simulation(f)
+    assert abs(rf["energies"][-1] - rec["energies"][-1]) / abs(rf["energies"][-1]) < 0.01
+    b = O.make_workspace(g, 1, psi, 0.001, False, omega=(1.0,), c0=10.0, c1=-0.5)
+    out2 = O.run_simulation_adaptive(b, dt_init=1e-4, dt_min=1e-5, dt_max=0.01, tol=1e-3,
+                                     error_mode=mode, t_end=0.5, save_interval=0.5)
+    assert out2["final_dt"] > 1e-4
+
+
+def test_adaptive_strang_rejects_and_shrinks():
+    """A tolerance far below the first step's change forces rejections; each rejection
+    shrinks dt by at least the 0.5 floor factor's bound and the accepted steps obey tol."""
+    g, psi = _adaptive_ws()
+    a = O.make_workspace(g, 1, psi, 0.01, False, omega=(1.0,), c0=10.0, c1=-0.5)
+    out = O.run_simulation_adaptive(a, dt_init=0.01, dt_min=1e-5, dt_max=0.01, tol=1e-9,
+                                    t_end=0.02, save_interval=0.02)
+    assert out["n_rejected"] > 0 and out["final_dt"] < 0.01
