@@ -267,15 +267,15 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
       int j = t + b * TEAM;
       int k = j % Ns;
       if (Ns > 1) {
-        // one table load per butterfly; the other twiddles are its powers (FP64), which
-        // keeps the per-element cache traffic of the pass at the data itself
-        cplx<double> w1 = tw[k * (N / (Ns * R))];
-        if (INV) w1.y = -w1.y;
-        cplx<double> wr = w1;
+        // W_N^{st r} for r = 1..R-1 straight from the table (st r < N): the callers keep
+        // the table in LDS, so this costs LDS reads instead of FP64 products, and every
+        // twiddle is the correctly rounded value rather than an accumulated power
+        const int st = k * (N / (Ns * R));
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          v[b * R + r] = mul64(v[b * R + r], wr);
-          if (r + 1 < R) wr = wr * w1;
+          cplx<double> w = tw[st * r];
+          if (INV) w.y = -w.y;
+          v[b * R + r] = mul64(v[b * R + r], w);
         }
       }
       dftR<T, R, INV>(&v[b * R]);
