@@ -155,11 +155,13 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 
   // Op lists in LDS (A at [0, kMaxOps), B at [kMaxOps, 2 kMaxOps)), copied with
   // compile-time indices so the by-value kernel argument is never indexed at run time.
+  // Slot i is copied by thread i, or by thread 0 when the block is smaller than that (1-D
+  // lines of 16 or 32 points run blocks of 6 or 12 threads).
   __shared__ Op s_ops[2 * kMaxOps];
 #pragma unroll
   for (int i = 0; i < kMaxOps; ++i) {
-    if (tid == i) s_ops[i] = prog.A[i];
-    if (tid == kMaxOps + i) s_ops[kMaxOps + i] = prog.B[i];
+    if (tid == (i < nthr ? i : 0)) s_ops[i] = prog.A[i];
+    if (tid == (kMaxOps + i < nthr ? kMaxOps + i : 0)) s_ops[kMaxOps + i] = prog.B[i];
   }
   __syncthreads();
   // Zeeman factors of every diagonal op (uniform over the block).
@@ -581,7 +583,7 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
   const bool itp = prog.itp != 0;
 #pragma unroll
   for (int i = 0; i < kMaxOps; ++i)
-    if (tid == i) s_ops[i] = prog.ops[i];
+    if (tid == (i < (int)blockDim.x ? i : 0)) s_ops[i] = prog.ops[i];
   __syncthreads();
   for (int idx = tid; idx < kMaxOps * D; idx += blockDim.x) {
     const int i = idx / D, c = idx % D;

@@ -498,3 +498,24 @@ def test_run_simulation_adaptive(mode, n, L, F):
     got = gws.get_psi(0)
     for c in range(D):
         assert rel_l2(got[c], ows.psi[c]) <= PSI_TOL
+
+
+@pytest.mark.parametrize("nx", [16, 32])
+@pytest.mark.parametrize("F,itp", [(1, False), (1, True), (2, False)])
+def test_short_1d_lines(nx, F, itp):
+    """1-D lines of 16 and 32 points run x-pass blocks with fewer threads than op slots
+    (6 or 12 for spin 1): every operator before and after the kinetic step must still run."""
+    n, L = (nx,), (10.0,)
+    og, gg = grids(n, L)
+    D = 2 * F + 1
+    psi = O.init_psi(og, F, "uniform") + 0.05 * random_psi(og.shape, D, 11)
+    psi /= math.sqrt(np.sum(np.abs(psi) ** 2) * og.dV)
+    ows, gws = pair(n, L, F, psi, 0.01, itp, omega=(1.0,), c0=1.0, c1=-0.3, q=0.2,
+                    n_steps=40, save_every=20)
+    ro = O.run_simulation(ows)
+    rg = A.run_simulation(gws, keep_snapshots=False)
+    got = gws.get_psi(0)
+    for c in range(D):
+        assert rel_l2(got[c], ows.psi[c]) <= PSI_TOL
+    for a, b in zip(rg.energies, ro["energies"]):
+        assert abs(a - b) <= E_TOL * abs(b)
