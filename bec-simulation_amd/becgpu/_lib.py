@@ -16,7 +16,8 @@ BEC_FP64, BEC_FP32 = 0, 1
 (E_KIN, E_TRAP, E_ZEEMAN, E_C0, E_C1, E_DDI, E_LHY, E_TENSOR, E_NORM, E_MZ,
  E_TOTAL, E_COUNT) = range(12)
 E_NAMES = ["kin", "trap", "zeeman", "c0", "c1", "ddi", "lhy", "tensor", "norm", "mz", "total"]
-K_NAMES = ["xpass", "fwd", "kin", "inv", "ke", "ddi_fft", "ddi_conv", "reduce", "sites", "energy"]
+K_NAMES = ["xpass", "fwd", "kin", "inv", "ke", "ddi_fft", "ddi_conv", "reduce", "sites", "energy",
+           "xsite"]
 
 
 class BecConfig(C.Structure):
@@ -70,6 +71,7 @@ SIGNATURES = {
     "bec_loss_step": ([C.c_void_p, C.c_double], C.c_int),
     "bec_normalize_constrained": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_synchronize": ([C.c_void_p], C.c_int),
+    "bec_set_xsite": ([C.c_void_p, C.c_int], C.c_int),
     "bec_query": ([C.c_void_p], C.c_int),
     "bec_autotune": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "bec_split_step_graph": ([C.c_void_p, C.c_int], C.c_int),

@@ -240,14 +240,24 @@ class Workspace:
         return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(L.K_NAMES)}
 
     def profile_work(self):
-        """Work since profile(True) or the last call: lane-per-site pass counts
-        {"tensor_matvecs", "rotations", "flops_per_matvec", "flops_per_rotation", "flops"}
-        and "bytes": algorithmic HBM bytes issued per kernel class."""
-        out = np.zeros(4 + len(L.K_NAMES))
+        """Work since profile(True) or the last call: per-site operator counts
+        {"tensor_matvecs", "rotations", "flops_per_matvec", "flops_per_rotation", "flops"},
+        "fft_flops" of the fused x/site launches, and "bytes": algorithmic HBM bytes issued
+        per kernel class."""
+        out = np.zeros(5 + len(L.K_NAMES))
         L.check(L.lib().bec_profile_work(self.handle, out.ctypes.data), "bec_profile_work")
         return {"tensor_matvecs": out[0], "rotations": out[1], "flops_per_matvec": out[2],
                 "flops_per_rotation": out[3], "flops": out[0] * out[2] + out[1] * out[3],
+                "fft_flops": float(out[4 + len(L.K_NAMES)]),
                 "bytes": {n: float(out[4 + i]) for i, n in enumerate(L.K_NAMES)}}
+
+    def set_xsite(self, on: bool) -> bool:
+        """Run the heavy per-site operators of F = 2 and 6 in the fused x/site kernel (on,
+        the default) or in separate x and lane-per-site passes; returns whether the fused
+        kernel is in use for this workspace's shape."""
+        r = L.lib().bec_set_xsite(self.handle, 1 if on else 0)
+        L.check(min(r, 0), "bec_set_xsite")
+        return r == 1
 
     def synchronize(self):
         L.check(L.lib().bec_synchronize(self.handle), "bec_synchronize")

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <tuple>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -75,6 +76,9 @@ struct bec_ws {
   size_t ev_used = 0;
   std::vector<int> prof_cls;
   double prof_bytes[BEC_K_COUNT] = {};  // algorithmic HBM bytes issued per kernel class
+  double prof_fft_flops = 0.0;          // FFT flops issued by fused x/site launches
+  // heavy per-site operators of F = 2 and 6 run in the fused x/site kernel (bec_set_xsite)
+  bool xsite = true;
   // captured split-step runs (bec_split_step_graph), keyed by step count, active list and
   // the schedule flags; dropped whenever a captured kernel argument could change
   struct Graph {
@@ -213,6 +217,32 @@ static int do_sites(bec_ws* ws, const SProgram& p) {
   HIPCHK(e);
   return BEC_OK;
 }
+// Fused x pass + lane-per-site pass on psi in place (slot 0).
+static int do_xs(bec_ws* ws, const XProgram& p) {
+  double b = 0.0;
+  if (ws->prof_on) {
+    b = 2.0 * psi_bytes(ws);
+    if (p.stages & XS_WRITE_FHAT) b += half_field_bytes(ws);
+    if (p.stages & XS_LOAD_PHI) b += half_field_bytes(ws);
+    // complex FFTs of n_x points: D lines per grid row for each tile FFT, two
+    // auxiliary lines per row (of the spectral x length) for each DDI stage
+    const Geom& g = ws->g;
+    auto fl = [](double n) { return 5.0 * n * std::log2(n); };
+    const double rows = (double)g.nrows * n_active(ws);
+    const int ntile = ((p.stages & XS_INV_FFT) ? 1 : 0) + ((p.stages & XS_FWD_FFT) ? 1 : 0);
+    const int naux = ((p.stages & XS_LOAD_PHI) ? 1 : 0) + ((p.stages & XS_WRITE_FHAT) ? 1 : 0);
+    ws->prof_fft_flops += rows * (ntile * g.D * fl(g.n[0]) + naux * 2.0 * fl(g.sn[0]));
+  }
+  const int pe = prof_begin(ws, BEC_K_XSITE, b);
+  hipError_t e = ws->prec == BEC_FP32
+                     ? launch_xsite<float>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->fhat,
+                                           n_active(ws))
+                     : launch_xsite<double>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->fhat,
+                                            n_active(ws));
+  prof_end(ws, pe);
+  HIPCHK(e);
+  return BEC_OK;
+}
 static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_frac, double dt_ovr,
                       int itp, int ddi_data, int* nb = nullptr) {
   hipError_t e;
@@ -261,11 +291,16 @@ struct Sched {
   XProgram p{};
   SProgram sp{};  // open lane-per-site program (13-component operators)
   bool scale_pending = false;
+  // fuse: heavy operators join the x program, which then runs in the fused x/site kernel
+  // (heavy: the open x program holds one)
+  bool fuse = false;
+  bool heavy = false;
 
   void reset() {
     std::memset(&p, 0, sizeof(p));
     p.itp = itp;
     p.dt_override = dt_ovr;
+    heavy = false;
   }
   void reset_s() {
     std::memset(&sp, 0, sizeof(sp));
@@ -290,10 +325,15 @@ struct Sched {
     if (empty()) return BEC_OK;
     if (p.nA) p.stages |= XS_OPS_A;
     if (p.nB) p.stages |= XS_OPS_B;
-    // passes that only read psi (norm, spin density) do not write it back
-    const int ro = XS_NORM | XS_WRITE_FHAT;
-    if (p.nA == 0 && p.nB == 0 && (p.stages & ~ro) == 0) p.stages |= XS_NO_STORE;
-    int r = do_x(ws, p, ws->slot[0], ws->slot[0]);
+    int r;
+    if (heavy) {
+      r = do_xs(ws, p);
+    } else {
+      // passes that only read psi (norm, spin density) do not write it back
+      const int ro = XS_NORM | XS_WRITE_FHAT;
+      if (p.nA == 0 && p.nB == 0 && (p.stages & ~ro) == 0) p.stages |= XS_NO_STORE;
+      r = do_x(ws, p, ws->slot[0], ws->slot[0]);
+    }
     reset();
     return r;
   }
@@ -301,6 +341,15 @@ struct Sched {
     if (scale_pending) {
       scale_pending = false;
       CHK(add_op(OP_SCALE, 0.0));
+    }
+    if (fuse) {  // one op list; the x stages after it close the program
+      if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
+      if (p.nA >= kMaxOps) CHK(flush());
+      p.A[p.nA].kind = kind;
+      p.A[p.nA].frac = frac;
+      ++p.nA;
+      if (site_op(kind)) heavy = true;
+      return BEC_OK;
     }
     if (sp.n > 0 || site_op(kind)) {
       if (!empty()) {  // the x pass before the site kernel must store psi first
@@ -367,9 +416,15 @@ struct Sched {
       p.stages |= XS_WRITE_FHAT;
       CHK(flush());
       CHK(ddi_conv());
-      // C2R of Phi into its real-space buffer; the rotation runs per site
-      p.stages |= XS_LOAD_PHI | XS_STORE_PHI | XS_NO_LOAD | XS_NO_STORE;
-      CHK(flush());
+      if (fuse) {
+        // C2R of Phi in LDS of the launch that rotates (the op list is empty here, so the
+        // rotation cannot be split from it)
+        p.stages |= XS_LOAD_PHI;
+      } else {
+        // C2R of Phi into its real-space buffer; the rotation runs per site
+        p.stages |= XS_LOAD_PHI | XS_STORE_PHI | XS_NO_LOAD | XS_NO_STORE;
+        CHK(flush());
+      }
       CHK(add_op(OP_DDI_ROT, f));
     }
     if (nematic) CHK(add_op(OP_NEMATIC, f / 2));
@@ -435,6 +490,7 @@ static int run_items(bec_ws* ws, const std::vector<Item>& items, int itp, double
   s.ws = ws;
   s.itp = itp;
   s.dt_ovr = dt_ovr;
+  s.fuse = ws->xsite && xsite_ok(ws->g);
   s.reset();
   s.reset_s();
   for (const Item& it : items) {
@@ -587,6 +643,10 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   g.fhat_inst_stride = 3 * g.fhat_field_stride;
   g.ddi_secular = c->ddi_secular;
   ws_bytes(c, &ws->max_blocks, &ws->n_xblocks);
+  {
+    const char* e = getenv("BEC_XSITE");
+    ws->xsite = !(e && atoi(e) == 0);
+  }
 
   ws->hp.assign(ws->B, InstParams{});
   ws->user.assign(ws->B, bec_inst_params{});
@@ -996,6 +1056,13 @@ int bec_synchronize(bec_ws* ws) {
   return BEC_OK;
 }
 
+int bec_set_xsite(bec_ws* ws, int on) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (ws->xsite != (on != 0)) drop_graphs(ws);  // captured launches follow the old schedule
+  ws->xsite = on != 0;
+  return ws->xsite && xsite_ok(ws->g) ? 1 : 0;
+}
+
 // fft_utils.jl:4-19,25-44: d_d f = Re IFFT(i k_d FFT(f)) of a real field.  A private FP64
 // spin-1 workspace of the grid holds f in component d for every d < ndim, so one schedule
 // (forward transforms, i k_d / N on component d, inverse transforms) yields the gradient.
@@ -1370,6 +1437,8 @@ int bec_profile_work(bec_ws* ws, double* out) {
     out[4 + k] = ws->prof_bytes[k];
     ws->prof_bytes[k] = 0.0;
   }
+  out[4 + BEC_K_COUNT] = ws->prof_fft_flops;
+  ws->prof_fft_flops = 0.0;
   HIPCHK(hipMemset(ws->d_work, 0, kWorkBytes));
   return BEC_OK;
 }

@@ -18,58 +18,9 @@
 
 #include "bec_device.h"
 #include "bec_kernels.h"
+#include "bec_passes.h"
 
 namespace bec {
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-  return v;
-}
-
-// Deterministic block sum (result on thread 0).  sh: >= 16 doubles of LDS.
-__device__ double block_sum(double v, double* sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int i = 0; i < nw; ++i) r += sh[i];
-  }
-  return r;
-}
-__device__ double block_max(double v, double* sh) {
-  v = wave_max(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int i = 0; i < nw; ++i) r = fmax(r, sh[i]);
-  }
-  return r;
-}
-
-__device__ __forceinline__ double* partial_ptr(const DevTables& tb, int slot, int inst, int blk) {
-  return tb.partials + ((size_t)slot * tb.maxB + inst) * tb.max_blocks + blk;
-}
-
-template <int N>
-__host__ __device__ constexpr int ilog2c() {
-  int l = 0, v = N;
-  while (v > 1) { v >>= 1; ++l; }
-  return l;
-}
 
 // ---------------------------------------------------------------------------
 // x pass
@@ -179,30 +130,6 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   double norm_acc = 0.0, ke_acc = 0.0, phimax = 0.0;
 
   const int line = tid / TEAM, tt = tid - (tid / TEAM) * TEAM;
-  // FFT of the 2*ROWS auxiliary lines of length AL (NX, or 2 NX with zero padding; for
-  // the longer lines the block covers them in rounds of nthr / (2 NX / EX) lines)
-  auto aux_fft = [&](auto invc) __attribute__((always_inline)) {
-    constexpr bool INVF = decltype(invc)::value;
-    if (AL == NX) {
-      fft_team<T, NX, INVF, EX, true>(aux + P8(line * NX), 1, tt, line < 2 * ROWS, tw);
-    } else {
-      if constexpr (2 * NX <= 512) {
-        constexpr int TEAM2 = 2 * NX / EX;
-        const cplx<double>* tw2 = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<2 * NX>()]);
-        const int lpr = nthr / TEAM2, l2 = tid / TEAM2, t2 = tid - l2 * TEAM2;
-        for (int l0 = 0; l0 < 2 * ROWS; l0 += lpr) {
-          const int ln = l0 + l2;
-          const bool act = l2 < lpr && ln < 2 * ROWS;
-          fft_team<T, 2 * NX, INVF, EX, true>(aux + P8((act ? ln : 0) * 2 * NX), 1, t2, act, tw2);
-        }
-      }
-    }
-  };
-  // spectra row of tile row r: (y, z) -> y + z*sn[1]
-  auto srow_of = [&](int r) __attribute__((always_inline)) {
-    const int row = row0 + r;
-    return (row & n1_mask) + (row >> n1_shift) * g.sn[1];
-  };
   if (st & XS_INV_FFT)
     fft_team<T, NX, true, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
 
@@ -210,26 +137,9 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 
   if (st & XS_LOAD_PHI) {
-    // C2R along x of the three DDI potential spectra (reference irfft semantics: bins
-    // 0..NX/2 with the imaginary parts of DC and Nyquist dropped, Hermitian extension).
-    const size_t fb = (size_t)inst * g.fhat_inst_stride;
-    for (int e = tid; e < ROWS * AL; e += nthr) {
-      const int r = e / AL, x = e - r * AL;
-      const size_t rowoff = (size_t)srow_of(r) * g.nxh_pitch;
-      const int kk = x <= AL / 2 ? x : AL - x;
-      cplx<T> yx = fhat[fb + rowoff + kk];
-      cplx<T> yy = fhat[fb + g.fhat_field_stride + rowoff + kk];
-      cplx<T> yz = fhat[fb + 2 * g.fhat_field_stride + rowoff + kk];
-      if (kk == 0 || kk == AL / 2) {
-        yx.y = 0; yy.y = 0; yz.y = 0;
-      } else if (x > AL / 2) {
-        yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
-      }
-      aux[P8((2 * r) * AL + x)] = mk(yx.x - yy.y, yx.y + yy.x);
-      aux[P8((2 * r + 1) * AL + x)] = yz;
-    }
+    ddi_lines_in<T>(g, fhat, (size_t)inst * g.fhat_inst_stride, row0, ROWS, AL, aux, tid, nthr);
     __syncthreads();
-    aux_fft(std::true_type{});
+    ddi_lines_fft<T, NX, true>(tb, aux, ROWS, AL, tid, nthr, tw);
   }
 
   auto site_coords = [&](int s, int& x, int& y, int& z) {
@@ -381,26 +291,10 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
       aux[P8((2 * r) * AL + x)] = mk(fx, fy);
       aux[P8((2 * r + 1) * AL + x)] = mk(fz, (T)0);
     }
-    if (AL > NX) {  // zero padding: entries NX..2NX-1 of every line
-      for (int e = tid; e < 2 * ROWS * NX; e += nthr) {
-        const int l = e / NX, x = NX + (e - l * NX);
-        aux[P8(l * AL + x)] = mk((T)0, (T)0);
-      }
-    }
+    if (AL > NX) ddi_lines_zero_upper<T, NX>(aux, ROWS, AL, tid, nthr);
     __syncthreads();
-    aux_fft(std::false_type{});
-    const size_t fb = (size_t)inst * g.fhat_inst_stride;
-    const int NH = AL / 2 + 1;
-    for (int e = tid; e < ROWS * NH; e += nthr) {
-      const int r = e / NH, k = e - r * NH;
-      const cplx<T> z1 = aux[P8((2 * r) * AL + k)];
-      const cplx<T> z1n = conjc(aux[P8((2 * r) * AL + ((AL - k) & (AL - 1)))]);
-      const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
-      const size_t o = fb + (size_t)srow_of(r) * g.nxh_pitch + k;
-      fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
-      fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
-      fhat[o + 2 * g.fhat_field_stride] = aux[P8((2 * r + 1) * AL + k)];
-    }
+    ddi_lines_fft<T, NX, false>(tb, aux, ROWS, AL, tid, nthr, tw);
+    ddi_lines_out<T>(g, fhat, (size_t)inst * g.fhat_inst_stride, row0, ROWS, AL, aux, tid, nthr);
   }
 
   if (st & XS_NORM) {
@@ -442,23 +336,6 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     const double v = block_sum(ke_acc, rsh);
     if (tid == 0) *partial_ptr(tb, RS_KIN, inst, blockIdx.x) = v;
   }
-}
-
-// Trap potential at a flat site index (potentials.jl:5-15 via per-axis omega^2 x^2 tables,
-// or an explicit potential array).
-__device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, int inst,
-                                            long long site) {
-  if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + site];
-  // grid sides are powers of two and an instance has fewer than 2^31 sites
-  const int sidx = (int)site;
-  const int x = sidx & (g.n[0] - 1);
-  const int row = sidx >> (__ffs(g.n[0]) - 1);
-  const int y = row & (g.n[1] - 1), z = row >> (__ffs(g.n[1]) - 1);
-  const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
-  double v = ax[x];
-  if (g.ndim >= 2) v += ax[g.nmax + y];
-  if (g.ndim >= 3) v += ax[2 * g.nmax + z];
-  return 0.5 * v;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,172 @@
+// Device building blocks shared by the x pass and the fused x/site pass: block
+// reductions into per-block partials, the trap potential at a site, and the auxiliary
+// DDI lines (C2R input of the potential spectra, R2C output of the spin density) that
+// both kernels keep beside their LDS tile of x rows.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bec_device.h"
+#include "bec_kernels.h"
+
+namespace bec {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Deterministic block sum (result on thread 0).  sh: >= 16 doubles of LDS.
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += sh[i];
+  }
+  return r;
+}
+__device__ inline double block_max(double v, double* sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r = fmax(r, sh[i]);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double* partial_ptr(const DevTables& tb, int slot, int inst, int blk) {
+  return tb.partials + ((size_t)slot * tb.maxB + inst) * tb.max_blocks + blk;
+}
+
+template <int N>
+__host__ __device__ constexpr int ilog2c() {
+  int l = 0, v = N;
+  while (v > 1) { v >>= 1; ++l; }
+  return l;
+}
+
+// Trap potential at a flat site index (potentials.jl:5-15 via per-axis omega^2 x^2 tables,
+// or an explicit potential array).
+__device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, int inst,
+                                            long long site) {
+  if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + site];
+  // grid sides are powers of two and an instance has fewer than 2^31 sites
+  const int sidx = (int)site;
+  const int x = sidx & (g.n[0] - 1);
+  const int row = sidx >> (__ffs(g.n[0]) - 1);
+  const int y = row & (g.n[1] - 1), z = row >> (__ffs(g.n[1]) - 1);
+  const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
+  double v = ax[x];
+  if (g.ndim >= 2) v += ax[g.nmax + y];
+  if (g.ndim >= 3) v += ax[2 * g.nmax + z];
+  return 0.5 * v;
+}
+
+// LDS tiles are padded with one slot per 8 elements: element e lives at p8(e) = e + e/8,
+// so the stride-8 writes of the first Stockham pass do not pile onto one bank.  Lines
+// start at multiples of their length (a multiple of 8), so p8(l*N + i) = p8(l*N) + p8(i).
+__device__ __forceinline__ int p8(int e) { return e + (e >> 3); }
+
+// Row of the DDI spectra (y, z) -> y + z*sn[1] for grid row `row` = y + z*n[1].
+__device__ __forceinline__ int spec_row(const Geom& g, int row) {
+  return (row & (g.n[1] - 1)) + (row >> (__ffs(g.n[1]) - 1)) * g.sn[1];
+}
+
+// C2R input along x of the three DDI potential spectra for the ROWS tile rows from row0
+// (reference irfft semantics, ddi.jl:108-136: bins 0..AL/2 with the imaginary parts of DC
+// and Nyquist dropped, Hermitian extension).  Two-for-one: auxiliary line 2r holds
+// Phi_x + i Phi_y, line 2r+1 holds Phi_z (its inverse FFT is real).
+template <typename T>
+__device__ __forceinline__ void ddi_lines_in(const Geom& g, const cplx<T>* __restrict__ fhat,
+                                             size_t fb, int row0, int ROWS, int AL, cplx<T>* aux,
+                                             int tid, int nthr) {
+  for (int e = tid; e < ROWS * AL; e += nthr) {
+    const int r = e / AL, x = e - r * AL;
+    const size_t rowoff = (size_t)spec_row(g, row0 + r) * g.nxh_pitch;
+    const int kk = x <= AL / 2 ? x : AL - x;
+    cplx<T> yx = fhat[fb + rowoff + kk];
+    cplx<T> yy = fhat[fb + g.fhat_field_stride + rowoff + kk];
+    cplx<T> yz = fhat[fb + 2 * g.fhat_field_stride + rowoff + kk];
+    if (kk == 0 || kk == AL / 2) {
+      yx.y = 0; yy.y = 0; yz.y = 0;
+    } else if (x > AL / 2) {
+      yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
+    }
+    aux[p8((2 * r) * AL + x)] = mk(yx.x - yy.y, yx.y + yy.x);
+    aux[p8((2 * r + 1) * AL + x)] = yz;
+  }
+}
+
+// FFT of the 2*ROWS auxiliary lines of length AL (NX, or 2 NX with zero padding), in
+// rounds of nthr / (AL/8) lines.  Every thread of the block calls it.
+template <typename T, int NX, bool INV>
+__device__ __forceinline__ void ddi_lines_fft(const DevTables& tb, cplx<T>* aux, int ROWS, int AL,
+                                              int tid, int nthr, const cplx<double>* tw) {
+  if (AL == NX) {
+    constexpr int TEAM = NX / 8;
+    const int lpr = nthr / TEAM, l = tid / TEAM, t = tid - l * TEAM;
+    for (int l0 = 0; l0 < 2 * ROWS; l0 += lpr) {
+      const int ln = l0 + l;
+      const bool act = l < lpr && ln < 2 * ROWS;
+      fft_team<T, NX, INV, 8, true>(aux + p8((act ? ln : 0) * NX), 1, t, act, tw);
+    }
+  } else {
+    if constexpr (2 * NX <= 512) {
+      constexpr int TEAM2 = 2 * NX / 8;
+      const cplx<double>* tw2 = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<2 * NX>()]);
+      const int lpr = nthr / TEAM2, l2 = tid / TEAM2, t2 = tid - l2 * TEAM2;
+      for (int l0 = 0; l0 < 2 * ROWS; l0 += lpr) {
+        const int ln = l0 + l2;
+        const bool act = l2 < lpr && ln < 2 * ROWS;
+        fft_team<T, 2 * NX, INV, 8, true>(aux + p8((act ? ln : 0) * 2 * NX), 1, t2, act, tw2);
+      }
+    }
+  }
+}
+
+// Zero padding of the spin-density lines: entries NX..2NX-1 of every auxiliary line.
+template <typename T, int NX>
+__device__ __forceinline__ void ddi_lines_zero_upper(cplx<T>* aux, int ROWS, int AL, int tid,
+                                                     int nthr) {
+  for (int e = tid; e < 2 * ROWS * NX; e += nthr) {
+    const int l = e / NX, x = NX + (e - l * NX);
+    aux[p8(l * AL + x)] = mk((T)0, (T)0);
+  }
+}
+
+// R2C output along x of the spin density (after the forward FFT of the auxiliary lines):
+// the two-for-one split of line 2r into the spectra of F_x and F_y, line 2r+1 is F_z's;
+// half rows of AL/2 + 1 bins into the DDI spectra buffer.
+template <typename T>
+__device__ __forceinline__ void ddi_lines_out(const Geom& g, cplx<T>* __restrict__ fhat, size_t fb,
+                                              int row0, int ROWS, int AL, const cplx<T>* aux,
+                                              int tid, int nthr) {
+  const int NH = AL / 2 + 1;
+  for (int e = tid; e < ROWS * NH; e += nthr) {
+    const int r = e / NH, k = e - r * NH;
+    const cplx<T> z1 = aux[p8((2 * r) * AL + k)];
+    const cplx<T> z1n = conjc(aux[p8((2 * r) * AL + ((AL - k) & (AL - 1)))]);
+    const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
+    const size_t o = fb + (size_t)spec_row(g, row0 + r) * g.nxh_pitch + k;
+    fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
+    fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
+    fhat[o + 2 * g.fhat_field_stride] = aux[p8((2 * r + 1) * AL + k)];
+  }
+}
+
+}  // namespace bec

@@ -65,6 +65,8 @@ def _traffic(workload, kernel):
 
 def _kernel_class(name):
     """Bench kernel class of a HIP kernel symbol (as the profiled pass groups launches)."""
+    if "xsite_kernel" in name:
+        return "xsite"
     if "site_kernel" in name:
         return "sites"
     if "xpass_kernel" in name:
@@ -126,6 +128,7 @@ def _rooflines(prof, work, precision):
     """Per-class achieved figures from the profiled pass: HBM classes in GB/s of
     algorithmic bytes; the lane-per-site pass in TFLOP/s of executed flops."""
     out = {}
+    op_class = "xsite" if prof.get("xsite", (0, 0))[1] > 0 else "sites"
     for k, (ms, cnt) in prof.items():
         if cnt == 0 or ms <= 0:
             continue
@@ -135,16 +138,21 @@ def _rooflines(prof, work, precision):
             out[k] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                       "frac": round(gbs / 8000.0, 4), "avg_launch_us": round(ms / cnt * 1e3, 2),
                       "launches": cnt, "alg_bytes_per_launch": work["bytes"][k] / cnt}
-        elif k == "sites" and work["flops"] > 0:
+        elif k == op_class and work["flops"] > 0:
+            # per-site operator flops (tensor matvecs, rotations) land in whichever of the
+            # two kernels ran them; the fused one adds its x-line FFTs
+            fft = work.get("fft_flops", 0.0) if k == "xsite" else 0.0
+            flops = work["flops"] + fft
             peak = FP32_PEAK if precision == "fp32" else FP64_PEAK
-            tf = work["flops"] / secs / 1e12
+            tf = flops / secs / 1e12
             out[k] = {"bound": "mfma", "achieved": round(tf, 2), "peak": peak / 1e12,
                       "unit": "TFLOP/s", "frac": round(tf * 1e12 / peak, 4),
                       "engine": "VALU (no MFMA; non-packed FMA issue caps it at half the peak)",
                       "avg_launch_us": round(ms / cnt * 1e3, 2), "launches": cnt,
-                      "alg_flops_per_launch": work["flops"] / cnt,
+                      "alg_flops_per_launch": flops / cnt, "fft_flops_per_launch": fft / cnt,
                       "tensor_matvecs": work["tensor_matvecs"], "rotations": work["rotations"],
-                      "alg_bytes_per_launch": work["bytes"]["sites"] / cnt}
+                      "alg_bytes_per_launch": work["bytes"][k] / cnt,
+                      "alg_GB/s": round(work["bytes"][k] / secs / 1e9, 1)}
     return out
 
 

@@ -150,10 +150,12 @@ int bec_itp_guard(bec_ws* ws, double* out);
  *  BEC_K_REDUCE     partial-sum reduction
  *  BEC_K_SITES      lane-per-site pass (13-component tensor step and spin rotations)
  *  BEC_K_ENERGY     a whole bec_energy evaluation (all of its launches)
+ *  BEC_K_XSITE      fused x pass + lane-per-site pass (F = 2 and 6: x FFTs, DDI x lines and
+ *                   the heavy per-site operators in one launch)
  * No counterpart in the reference (which uses TimerOutputs, SpinorBEC.jl:12). */
 enum {
   BEC_K_XPASS = 0, BEC_K_FWD, BEC_K_KIN, BEC_K_INV, BEC_K_KE, BEC_K_DDI_FFT, BEC_K_DDI_CONV,
-  BEC_K_REDUCE, BEC_K_SITES, BEC_K_ENERGY, BEC_K_COUNT
+  BEC_K_REDUCE, BEC_K_SITES, BEC_K_ENERGY, BEC_K_XSITE, BEC_K_COUNT
 };
 int bec_profile_enable(bec_ws* ws, int on);
 /* ms[BEC_K_COUNT] total milliseconds and count[BEC_K_COUNT] launches since enable;
@@ -163,8 +165,10 @@ int bec_profile_read(bec_ws* ws, double* ms, long long* count);
  * out[0] tensor matvecs (Taylor terms of the tensor step), out[1] Euler spin rotations,
  * out[2] flops per tensor matvec, out[3] flops per rotation, out[4 + k] algorithmic HBM
  * bytes issued by kernel class k (BEC_K_*; 0 for BEC_K_REDUCE and BEC_K_ENERGY), so
- * out[4 + k] / time of class k is its achieved algorithmic bandwidth.  Resets the
- * counters.  No counterpart in the reference. */
+ * out[4 + k] / time of class k is its achieved algorithmic bandwidth;
+ * out[4 + BEC_K_COUNT] FFT flops of the fused x/site launches (5 N log2 N per complex
+ * line of N points).  out holds 5 + BEC_K_COUNT doubles.  Resets the counters.  No
+ * counterpart in the reference. */
 int bec_profile_work(bec_ws* ws, double* out);
 
 /* Spectral derivatives of a real field f[N] (memory order as psi without the component
@@ -177,6 +181,13 @@ int bec_fft_derivative(const bec_config* cfg, const double* f, int dim, double* 
 int bec_fft_gradient(const bec_config* cfg, const double* f, double* out);
 
 int bec_synchronize(bec_ws* ws);
+/* Kernel schedule switch (no counterpart in the reference): on (the default, or the
+ * environment variable BEC_XSITE=0 for off), spins F = 2 and 6 run every half potential
+ * step with its neighbouring x FFTs and DDI x lines in the fused x/site kernel; off, in
+ * separate x passes and lane-per-site passes.  Both give the same psi.  Returns the
+ * setting in force (the fused kernel needs 2-D/3-D grids with one x-pass tile of 64..256
+ * sites; other shapes always run separately). */
+int bec_set_xsite(bec_ws* ws, int on);
 /* Tile autotuner (no counterpart in the reference, whose FFTW plans are tuned by
  * FFTW_MEASURE, src/grid.jl:37-53): times the x-pass tile candidates on this workspace's
  * data and keeps the fastest for the shape (precision, n_x, D, grid rows) for the rest of
