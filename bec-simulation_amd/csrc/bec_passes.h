@@ -91,24 +91,39 @@ __device__ __forceinline__ int spec_row(const Geom& g, int row) {
 // (reference irfft semantics, ddi.jl:108-136: bins 0..AL/2 with the imaginary parts of DC
 // and Nyquist dropped, Hermitian extension).  Two-for-one: auxiliary line 2r holds
 // Phi_x + i Phi_y, line 2r+1 holds Phi_z (its inverse FFT is real).
+// Element e = r*AL + x of the lines in two halves: ddi_phi_fetch reads its three spectral
+// bins (so a caller can issue the loads early), ddi_phi_place writes the line entries.
+template <typename T>
+__device__ __forceinline__ void ddi_phi_fetch(const Geom& g, const cplx<T>* __restrict__ fhat,
+                                              size_t fb, int row0, int AL, int e, cplx<T>* v) {
+  const int r = e / AL, x = e - r * AL;
+  const size_t rowoff = (size_t)spec_row(g, row0 + r) * g.nxh_pitch;
+  const int kk = x <= AL / 2 ? x : AL - x;
+  v[0] = fhat[fb + rowoff + kk];
+  v[1] = fhat[fb + g.fhat_field_stride + rowoff + kk];
+  v[2] = fhat[fb + 2 * g.fhat_field_stride + rowoff + kk];
+}
+template <typename T>
+__device__ __forceinline__ void ddi_phi_place(int AL, int e, const cplx<T>* v, cplx<T>* aux) {
+  const int r = e / AL, x = e - r * AL;
+  const int kk = x <= AL / 2 ? x : AL - x;
+  cplx<T> yx = v[0], yy = v[1], yz = v[2];
+  if (kk == 0 || kk == AL / 2) {
+    yx.y = 0; yy.y = 0; yz.y = 0;
+  } else if (x > AL / 2) {
+    yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
+  }
+  aux[p8((2 * r) * AL + x)] = mk(yx.x - yy.y, yx.y + yy.x);
+  aux[p8((2 * r + 1) * AL + x)] = yz;
+}
 template <typename T>
 __device__ __forceinline__ void ddi_lines_in(const Geom& g, const cplx<T>* __restrict__ fhat,
                                              size_t fb, int row0, int ROWS, int AL, cplx<T>* aux,
                                              int tid, int nthr) {
   for (int e = tid; e < ROWS * AL; e += nthr) {
-    const int r = e / AL, x = e - r * AL;
-    const size_t rowoff = (size_t)spec_row(g, row0 + r) * g.nxh_pitch;
-    const int kk = x <= AL / 2 ? x : AL - x;
-    cplx<T> yx = fhat[fb + rowoff + kk];
-    cplx<T> yy = fhat[fb + g.fhat_field_stride + rowoff + kk];
-    cplx<T> yz = fhat[fb + 2 * g.fhat_field_stride + rowoff + kk];
-    if (kk == 0 || kk == AL / 2) {
-      yx.y = 0; yy.y = 0; yz.y = 0;
-    } else if (x > AL / 2) {
-      yx = conjc(yx); yy = conjc(yy); yz = conjc(yz);
-    }
-    aux[p8((2 * r) * AL + x)] = mk(yx.x - yy.y, yx.y + yy.x);
-    aux[p8((2 * r + 1) * AL + x)] = yz;
+    cplx<T> v[3];
+    ddi_phi_fetch<T>(g, fhat, fb, row0, AL, e, v);
+    ddi_phi_place<T>(AL, e, v, aux);
   }
 }
 

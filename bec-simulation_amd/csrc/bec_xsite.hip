@@ -27,7 +27,10 @@ __global__ void __launch_bounds__(kXsiteMaxThreads)
 xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
              cplx<T>* __restrict__ fhat, int ROWS) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int TEAM = NX / 8;
+  // values per thread in the tile FFTs: 16 for 13 components, so one round of TS/TEAM =
+  // 16*ROWS lines covers all 13*ROWS (a butterfly's arithmetic does not depend on E)
+  constexpr int EF = D > 8 ? 16 : 8;
+  constexpr int TEAM = NX / EF;
   const int tid = threadIdx.x;
   const int TS = blockDim.x;  // sites of the tile = threads (the launch has ROWS*NX threads)
   const int inst = tb.active[blockIdx.y];
@@ -47,16 +50,26 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
 
-  // Prologue: the D tile loads of this thread's site and one twiddle are all issued
-  // before the first wait; TS >= NX threads cover the twiddle table.
+  // Prologue: the D tile loads of this thread's site, one twiddle and (with the DDI
+  // potential stage) the spectral bins of this thread's AL/NX <= 2 line entries are all
+  // issued before the first wait; TS >= NX threads cover the twiddle table.
+  const size_t fb = (size_t)inst * g.fhat_inst_stride;
+  const int nph = (st & XS_LOAD_PHI) ? AL / NX : 0;
   {
     const cplx<double> twv = twg[min(tid, NX - 1)];
     cplx<T> ld[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) ld[c] = psi[ibase + (size_t)c * g.nsites + tid];
+    cplx<T> ph[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (k < nph) ddi_phi_fetch<T>(g, fhat, fb, row0, AL, tid + k * TS, ph[k]);
     tws[min(tid, NX - 1)] = twv;
 #pragma unroll
     for (int c = 0; c < D; ++c) tile[p8(c * TS + tid)] = ld[c];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (k < nph) ddi_phi_place<T>(AL, tid + k * TS, ph[k], aux);
   }
   // op list in LDS, copied with compile-time indices (the by-value kernel argument is
   // never indexed at run time), then the FP64 Zeeman factors of its diagonal steps
@@ -76,20 +89,18 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
   }
   __syncthreads();
 
-  // FFT of the D*ROWS tile lines in rounds of TS/TEAM = 8*ROWS lines
+  // FFT of the D*ROWS tile lines in rounds of TS/TEAM = (EF)*ROWS lines
   auto tile_fft = [&](auto invc) __attribute__((always_inline)) {
     constexpr bool INV = decltype(invc)::value;
     const int lpr = TS / TEAM, l = tid / TEAM, t = tid - l * TEAM;
     for (int l0 = 0; l0 < D * ROWS; l0 += lpr) {
       const int ln = l0 + l;
       const bool act = ln < D * ROWS;
-      fft_team<T, NX, INV, 8, true>(tile + p8((act ? ln : 0) * NX), 1, t, act, tws);
+      fft_team<T, NX, INV, EF, true>(tile + p8((act ? ln : 0) * NX), 1, t, act, tws);
     }
   };
   if (st & XS_INV_FFT) tile_fft(std::true_type{});
-  if (st & XS_LOAD_PHI) {
-    ddi_lines_in<T>(g, fhat, (size_t)inst * g.fhat_inst_stride, row0, ROWS, AL, aux, tid, TS);
-    __syncthreads();
+  if (st & XS_LOAD_PHI) {  // lines placed in the prologue (the op-list barrier follows)
     ddi_lines_fft<T, NX, true>(tb, aux, ROWS, AL, tid, TS, tws);
     // max |Phi| for the imaginary-time overflow guard (ground_state.jl:4-29), taken before
     // the site program (a following spin-density stage reuses the lines); the guard reads
@@ -157,7 +168,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
     if (AL > NX) ddi_lines_zero_upper<T, NX>(aux, ROWS, AL, tid, TS);
     __syncthreads();
     ddi_lines_fft<T, NX, false>(tb, aux, ROWS, AL, tid, TS, tws);
-    ddi_lines_out<T>(g, fhat, (size_t)inst * g.fhat_inst_stride, row0, ROWS, AL, aux, tid, TS);
+    ddi_lines_out<T>(g, fhat, fb, row0, ROWS, AL, aux, tid, TS);
   }
   if (st & XS_FWD_FFT) {
     __syncthreads();
