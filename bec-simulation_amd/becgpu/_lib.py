@@ -13,9 +13,10 @@ LIB_PATH = os.path.join(_PKG, "libbecgpu.so")
 
 BEC_OK, BEC_EINVAL, BEC_EDEVICE, BEC_ENOMEM, BEC_EUNSUP = 0, -1, -2, -3, -4
 BEC_FP64, BEC_FP32 = 0, 1
-(E_KIN, E_TRAP, E_ZEEMAN, E_C0, E_C1, E_DDI, E_LHY, E_TENSOR, E_NORM, E_MZ,
- E_TOTAL, E_COUNT) = range(12)
-E_NAMES = ["kin", "trap", "zeeman", "c0", "c1", "ddi", "lhy", "tensor", "norm", "mz", "total"]
+(E_KIN, E_TRAP, E_ZEEMAN, E_C0, E_C1, E_DDI, E_LHY, E_TENSOR, E_NORM, E_MZ, E_RAMAN,
+ E_TOTAL, E_COUNT) = range(13)
+E_NAMES = ["kin", "trap", "zeeman", "c0", "c1", "ddi", "lhy", "tensor", "norm", "mz", "raman",
+           "total"]
 K_NAMES = ["xpass", "fwd", "kin", "inv", "ke", "ddi_fft", "ddi_conv", "reduce", "sites", "energy",
            "xsite"]
 
@@ -68,6 +69,7 @@ SIGNATURES = {
     "bec_profile_read": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "bec_profile_work": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_set_loss": ([C.c_void_p, C.c_int, C.c_void_p, C.c_int], C.c_int),
+    "bec_set_raman": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_loss_step": ([C.c_void_p, C.c_double], C.c_int),
     "bec_normalize_constrained": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_synchronize": ([C.c_void_p], C.c_int),

@@ -161,6 +161,18 @@ class Workspace:
         L.check(L.lib().bec_set_loss(self.handle, inst, rates.ctypes.data, self.D), "bec_set_loss")
         self._loss[inst] = True
 
+    def set_raman(self, inst, raman):
+        """RamanCoupling(Omega_R, delta, k_eff) of one instance, or None (types.jl:179-183):
+        apply_raman_step! (raman.jl:8-31) runs on the device in every half potential step."""
+        if raman is None:
+            L.check(L.lib().bec_set_raman(self.handle, inst, None), "bec_set_raman")
+            return
+        k = tuple(float(v) for v in raman.k_eff)
+        if len(k) != self.grid.ndim:
+            raise ValueError(f"RamanCoupling.k_eff has {len(k)} entries for a {self.grid.ndim}-D grid")
+        r = np.ascontiguousarray([float(raman.Omega_R), float(raman.delta), *k], dtype=np.float64)
+        L.check(L.lib().bec_set_raman(self.handle, inst, r.ctypes.data), "bec_set_raman")
+
     @property
     def has_loss(self):
         return any(self._loss[i] for i in self.active)
@@ -346,8 +358,6 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
                    psi_init=None, enable_ddi=False, c_dd=float("nan"), secular_ddi=False,
                    raman=None, loss=None, ddi_padding=False, precision="fp64", fft_flags=None):
     F = atom.F
-    if raman is not None:
-        raise NotImplementedError("Raman coupling is outside the GPU hot path")
     tensor = _tensor_active(F, interactions)
     channels = None
     if tensor:
@@ -372,6 +382,8 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
         ws.set_tensor(0, channels)
     if loss is not None:
         ws.set_loss(0, loss)
+    if raman is not None:
+        ws.set_raman(0, raman)
     psi = init_psi(grid, SpinSystem(F)) if psi_init is None else np.array(psi_init, dtype=complex)
     ws.set_psi(psi, 0)
     ws.atom = atom

@@ -242,6 +242,15 @@ def precompute_cg_table(F: int):
 
 
 @dataclass(frozen=True)
+class RamanCoupling:
+    """types.jl:179-183: two-photon Raman coupling H_R = delta Fz + Omega_R/2 (e^{i k.r} F+ +
+    e^{-i k.r} F-) (raman.jl:1-6); k_eff has one entry per grid axis."""
+    Omega_R: float
+    delta: float
+    k_eff: tuple
+
+
+@dataclass(frozen=True)
 class LossParams:
     """types.jl:291-296: dipolar relaxation rate and three-body loss."""
     gamma_dr: float

@@ -52,6 +52,7 @@ struct bec_ws {
   void* fhat64 = nullptr;
   void* phi64 = nullptr;
   double* d_ktab = nullptr;
+  double* d_xtab = nullptr;   // [3][nmax] cell-centred coordinates (Raman phase k.r)
   double* d_sktab = nullptr;  // k of the zero-padded DDI grid (null: the grid's own k)
   double* d_axtab = nullptr;
   double* d_varr = nullptr;
@@ -131,6 +132,7 @@ static void refresh_tables(bec_ws* ws) {
   drop_graphs(ws);  // captured launches hold the previous tables by value
   DevTables& t = ws->tb;
   t.ktab = ws->d_ktab;
+  t.xtab = ws->d_xtab;
   t.sktab = ws->d_sktab ? ws->d_sktab : ws->d_ktab;
   t.axtab = ws->d_axtab;
   t.Varr = ws->d_varr && std::any_of(ws->has_varr.begin(), ws->has_varr.end(), [](int v) { return v; })
@@ -311,7 +313,8 @@ struct Sched {
   // Euler spin rotations (DDI, spin mixing for F >= 2) and the tensor step run in the
   // lane-per-site kernel; the x pass keeps FFTs and light pointwise work
   bool site_op(int kind) const {
-    return kind == OP_TENSOR || kind == OP_DDI_ROT || (kind == OP_SPINMIX && ws->D >= 5) ||
+    return kind == OP_TENSOR || kind == OP_DDI_ROT || kind == OP_RAMAN ||
+           (kind == OP_SPINMIX && ws->D >= 5) ||
            (kind == OP_NEMATIC && ws->D >= 13);
   }
   int flush_s() {
@@ -396,6 +399,12 @@ struct Sched {
       if (std::fabs(ws->hp[inst].c1) >= 1e-30) return true;
     return false;
   }
+  // the Raman step runs when a RamanCoupling is present (split_step.jl:77-83,101-107)
+  bool raman_active() const {
+    for (int inst : ws->active)
+      if (ws->hp[inst].has_raman) return true;
+    return false;
+  }
   // likewise the nematic step with |c2| < 1e-30 (nematic.jl:43); never with the tensor step
   bool nematic_active() const {
     if (ws->tensor) return false;
@@ -410,6 +419,8 @@ struct Sched {
     CHK(add_op(OP_DIAG, f / 2));
     if (mixing) CHK(add_op(mix, f / 2));
     if (nematic) CHK(add_op(OP_NEMATIC, f / 2));
+    const bool raman = raman_active();
+    if (raman) CHK(add_op(OP_RAMAN, f / 2));
     if (ws->ddi) {
       CHK(flush_s());
       if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
@@ -427,6 +438,7 @@ struct Sched {
       }
       CHK(add_op(OP_DDI_ROT, f));
     }
+    if (raman) CHK(add_op(OP_RAMAN, f / 2));
     if (nematic) CHK(add_op(OP_NEMATIC, f / 2));
     if (mixing) CHK(add_op(mix, f / 2));
     CHK(add_op(OP_DIAG, f / 2));
@@ -687,6 +699,17 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMalloc(&ws->d_ktab, kt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "ktab");
   if (hipMemcpy(ws->d_ktab, kt.data(), kt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(BEC_EDEVICE, "ktab copy");
+  {  // x table [3][nmax]
+    std::vector<double> xt(3 * g.nmax, 0.0);
+    int xo = 0;
+    for (int d = 0; d < c->ndim; ++d) {
+      for (int i = 0; i < c->n[d]; ++i) xt[d * g.nmax + i] = c->x_tab[xo + i];
+      xo += c->n[d];
+    }
+    if (hipMalloc(&ws->d_xtab, xt.size() * 8) != hipSuccess) return cleanup(BEC_ENOMEM, "xtab");
+    if (hipMemcpy(ws->d_xtab, xt.data(), xt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "xtab copy");
+  }
   if (g.spad) {
     // k of the padded grid (same dx, twice the points): fftfreq(2n) * 2 pi / (2n dx)
     std::vector<double> sk(3 * g.smax, 0.0);
@@ -747,7 +770,8 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->phi) (void)hipFree(ws->phi);
   for (void* q : {ws->psi64, ws->scratch64, ws->fhat64, ws->phi64})
     if (q) (void)hipFree(q);
-  for (double* p : {ws->d_ktab, ws->d_sktab, ws->d_axtab, ws->d_varr, ws->d_red, ws->d_partials})
+  for (double* p : {ws->d_ktab, ws->d_xtab, ws->d_sktab, ws->d_axtab, ws->d_varr, ws->d_red,
+                    ws->d_partials})
     if (p) (void)hipFree(p);
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
@@ -769,11 +793,13 @@ int bec_set_params(bec_ws* ws, int inst, const bec_inst_params* p) {
   if (!(p->dt > 0)) return fail(BEC_EINVAL, "dt must be positive");
   ws->user[inst] = *p;
   InstParams& q = ws->hp[inst];
-  const InstParams keep = q;  // tensor couplings and loss rates have their own setters
+  const InstParams keep = q;  // tensor couplings, loss rates, Raman have their own setters
   q = InstParams{};
   std::memcpy(q.gS, keep.gS, sizeof(q.gS));
   std::memcpy(q.loss, keep.loss, sizeof(q.loss));
   q.has_loss = keep.has_loss;
+  std::memcpy(q.raman, keep.raman, sizeof(q.raman));
+  q.has_raman = keep.has_raman;
   q.dt = p->dt;
   q.c0 = p->c0;
   q.c1 = p->c1;
@@ -914,6 +940,7 @@ static int sched_flags(const bec_ws* ws) {
     if (std::fabs(ws->hp[inst].c1) >= 1e-30) f |= 1;
     if (std::fabs(ws->hp[inst].c2) >= 1e-30) f |= 2;
     if (ws->hp[inst].has_loss) f |= 4;
+    if (ws->hp[inst].has_raman) f |= 8;
   }
   return f;
 }
@@ -1014,6 +1041,23 @@ int bec_set_loss(bec_ws* ws, int inst, const double* rates, int n) {
       q.loss[c] = rates[c];
     }
     q.has_loss = 1;
+  }
+  ws->params_dirty = true;
+  return BEC_OK;
+}
+
+int bec_set_raman(bec_ws* ws, int inst, const double* raman) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  InstParams& q = ws->hp[inst];
+  if (!raman) {
+    q.has_raman = 0;
+    for (double& r : q.raman) r = 0.0;
+  } else {
+    for (int i = 0; i < 2 + ws->g.ndim; ++i)
+      if (!std::isfinite(raman[i])) return fail(BEC_EINVAL, "Raman parameters must be finite");
+    for (int i = 0; i < 5; ++i) q.raman[i] = i < 2 + ws->g.ndim ? raman[i] : 0.0;
+    q.has_raman = 1;
   }
   ws->params_dirty = true;
   return BEC_OK;
@@ -1276,7 +1320,7 @@ static int energy_partials_fp64(bec_ws* ws, const DevTables& tb0, void* psi, voi
                                &nbe));
   const uint32_t emask = (1u << RS_NORM) | (1u << RS_TRAP) | (1u << RS_ZEE) | (1u << RS_C0) |
                          (1u << RS_C1) | (1u << RS_LHY) | (1u << RS_DDI) | (1u << RS_MZ) |
-                         (1u << RS_TENSOR);
+                         (1u << RS_TENSOR) | (1u << RS_RAMAN);
   HIPCHK(launch_reduce(ws->stream, tb, ws->d_red, na, nbe, emask, 0));
   // kinetic energy: forward FFT of every component, sum k^2 |psi_hat|^2 (energy.jl:56-65)
   if (g.ndim == 1) {
@@ -1338,8 +1382,9 @@ int bec_energy(bec_ws* ws, double* out) {
     o[BEC_E_TENSOR] = r[RS_TENSOR] * dV;  // tensor channels, or the nematic term without them
     o[BEC_E_NORM] = r[RS_NORM] * dV;
     o[BEC_E_MZ] = r[RS_MZ] * dV;
+    o[BEC_E_RAMAN] = q.has_raman ? r[RS_RAMAN] * dV : 0.0;
     o[BEC_E_TOTAL] = o[BEC_E_KIN] + o[BEC_E_TRAP] + o[BEC_E_ZEEMAN] + o[BEC_E_C0] + o[BEC_E_C1] +
-                     o[BEC_E_DDI] + o[BEC_E_LHY] + o[BEC_E_TENSOR];
+                     o[BEC_E_DDI] + o[BEC_E_LHY] + o[BEC_E_TENSOR] + o[BEC_E_RAMAN];
   }
   return BEC_OK;
 }

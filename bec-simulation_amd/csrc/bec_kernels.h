@@ -39,6 +39,7 @@ constexpr int kWorkStride = 8;
 
 struct DevTables {
   const double* ktab;     // [3][nmax] fftfreq k values per axis
+  const double* xtab;     // [3][nmax] cell-centred coordinates per axis
   const double* sktab;    // [3][smax] fftfreq k values of the DDI spectral grid
   const double* axtab;    // [B][3][nmax] omega_d^2 x_i^2 (trap), per instance
   const double* Varr;     // [B][nsites] explicit potential or null

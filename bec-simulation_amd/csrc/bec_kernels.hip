@@ -351,7 +351,7 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
   const int inst = tb.active[blockIdx.y];
   const InstParams& P = tb.params[inst];
   double e_norm = 0, e_trap = 0, e_zee = 0, e_c0 = 0, e_c1 = 0, e_lhy = 0, e_ddi = 0, e_mz = 0;
-  double e_ten = 0;
+  double e_ten = 0, e_ram = 0;
   const cplx<T>* p = psi + (size_t)inst * g.inst_stride;
   for (long long site = blockIdx.x * (long long)blockDim.x + threadIdx.x; site < g.nsites;
        site += (long long)gridDim.x * blockDim.x) {
@@ -375,6 +375,11 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
     e_c0 += n * n;
     e_c1 += (double)fx * fx + (double)fy * fy + (double)fz * fz;
     e_lhy += n * n * sqrt(n);
+    if (P.has_raman) {  // _raman_energy (energy.jl:123-135): psi^H H_R psi = phi . <F>
+      double phx, phy, phz;
+      raman_phi(g, tb, P, site, phx, phy, phz);
+      e_ram += phx * (double)fx + phy * (double)fy + phz * (double)fz;
+    }
     if (with_tensor) {
       if constexpr (D == 13) e_ten += tensor_energy_site<T, D>(sp, P);
     } else if (P.c2 != 0.0) {  // energy.jl:43-46,106-109: 1/2 c2 |A00|^2, tensor slot
@@ -404,6 +409,7 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
   v = block_sum(e_ddi, rsh); if (tid == 0) *partial_ptr(tb, RS_DDI, inst, blk) = v;
   v = block_sum(e_mz, rsh); if (tid == 0) *partial_ptr(tb, RS_MZ, inst, blk) = v;
   v = block_sum(e_ten, rsh); if (tid == 0) *partial_ptr(tb, RS_TENSOR, inst, blk) = v;
+  v = block_sum(e_ram, rsh); if (tid == 0) *partial_ptr(tb, RS_RAMAN, inst, blk) = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -435,6 +441,12 @@ __device__ __forceinline__ void site_ops(const Geom& g, const DevTables& tb, con
       op_nematic<T, D>(sp, P.c2, dtf, itp);
     } else if (op.kind == OP_LOSS) {
       op_loss<T, D>(sp, P, dtf);
+    } else if (op.kind == OP_RAMAN) {
+      if (P.has_raman) {
+        double phx, phy, phz;
+        raman_phi(g, tb, P, site, phx, phy, phz);
+        nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
+      }
     } else if (op.kind == OP_TENSOR) {
       if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
     } else if (op.kind == OP_SCALE) {

@@ -77,6 +77,26 @@ __device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, 
   return 0.5 * v;
 }
 
+// raman.jl:19-23: H_R = delta Fz + Omega_R/2 (e^{i k.r} F+ + e^{-i k.r} F-) at a flat site
+// index is phi . F with phi = (Omega_R cos k.r, -Omega_R sin k.r, delta)
+// (F+- = Fx +- i Fy), so exp(-i H_R dt) is the Euler rotation by phi; k.r is summed over the
+// axes in order in FP64.
+__device__ __forceinline__ void raman_phi(const Geom& g, const DevTables& tb, const InstParams& P,
+                                          long long site, double& phx, double& phy, double& phz) {
+  const int sidx = (int)site;
+  const int x = sidx & (g.n[0] - 1);
+  const int row = sidx >> (__ffs(g.n[0]) - 1);
+  const int y = row & (g.n[1] - 1), z = row >> (__ffs(g.n[1]) - 1);
+  double kr = P.raman[2] * tb.xtab[x];
+  if (g.ndim >= 2) kr += P.raman[3] * tb.xtab[g.nmax + y];
+  if (g.ndim >= 3) kr += P.raman[4] * tb.xtab[2 * g.nmax + z];
+  double s, c;
+  sincos_d(kr, &s, &c);
+  phx = P.raman[0] * c;
+  phy = -P.raman[0] * s;
+  phz = P.raman[1];
+}
+
 // LDS tiles are padded with one slot per 8 elements: element e lives at p8(e) = e + e/8,
 // so the stride-8 writes of the first Stockham pass do not pile onto one bank.  Lines
 // start at multiples of their length (a multiple of 8), so p8(l*N + i) = p8(l*N) + p8(i).

@@ -18,8 +18,10 @@ struct InstParams {
   double zee_shift;  // min_c E_c (subtracted in imaginary time)
   double gS[7];      // channel couplings g_S, S = 0, 2, ..., 2F (general-F tensor step)
   double loss[kMaxD]; // loss rate gamma_c + L3 per component (real time only)
+  double raman[5];   // RamanCoupling: Omega_R, delta, k_eff per axis (raman.jl, types.jl:179-183)
   int active;
   int has_loss;      // 0: no loss step (LossParams absent, or gamma_dr and L3 < 1e-30)
+  int has_raman;     // 1: the Raman step runs (RamanCoupling present)
 };
 
 enum OpKind : int {
@@ -30,6 +32,7 @@ enum OpKind : int {
   OP_TENSOR = 5,    // apply_tensor_interaction_step! (dt*frac)
   OP_NEMATIC = 6,   // apply_nematic_step! (dt*frac)
   OP_LOSS = 7,      // apply_loss_step! (dt*frac), real time
+  OP_RAMAN = 8,     // apply_raman_step! (dt*frac)
 };
 
 struct Op {
@@ -87,7 +90,8 @@ enum RedSlot : int {
   RS_MZ = 8,
   RS_PHIMAX = 9,
   RS_TENSOR = 10,
-  RS_COUNT = 11,
+  RS_RAMAN = 11,
+  RS_COUNT = 12,
 };
 
 }  // namespace bec

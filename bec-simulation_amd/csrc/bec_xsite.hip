@@ -4,8 +4,8 @@
 // One block holds ROWS whole x rows of every component in LDS, ROWS*NX sites with one
 // thread per site, and runs the stages the schedule asks for:
 //   inverse x FFT -> C2R of the three DDI potential lines (Phi stays in LDS) -> the site
-//   program on the spinor in VGPRs (diagonal, spin mixing, DDI rotation by Phi, tensor,
-//   nematic, loss, ITP scale) -> spin density and its R2C into the DDI spectra -> norm
+//   program on the spinor in VGPRs (diagonal, spin mixing, Raman, DDI rotation by Phi,
+//   tensor, nematic, loss, ITP scale) -> spin density and its R2C into the DDI spectra -> norm
 //   partials -> forward x FFT -> store.
 // Without it a half potential step of the dipolar spin-6 gas is six launches (x pass,
 // site pass, spin-density x pass, Phi x pass, site pass, x pass: four extra round trips
@@ -140,6 +140,12 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
         op_nematic<T, D>(sp, P.c2, dtf, itp);
       } else if (op.kind == OP_LOSS) {
         op_loss<T, D>(sp, P, dtf);
+      } else if (op.kind == OP_RAMAN) {
+        if (P.has_raman) {
+          double phx, phy, phz;
+          raman_phi(g, tb, P, site, phx, phy, phz);
+          nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
+        }
       } else if (op.kind == OP_TENSOR) {
         if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
       } else if (op.kind == OP_SCALE) {

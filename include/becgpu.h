@@ -39,7 +39,7 @@ extern "C" {
 /* Energy terms returned per instance by bec_energy (energy.jl:6-54). */
 enum {
   BEC_E_KIN = 0, BEC_E_TRAP, BEC_E_ZEEMAN, BEC_E_C0, BEC_E_C1, BEC_E_DDI, BEC_E_LHY,
-  BEC_E_TENSOR, BEC_E_NORM, BEC_E_MZ, BEC_E_TOTAL, BEC_E_COUNT
+  BEC_E_TENSOR, BEC_E_NORM, BEC_E_MZ, BEC_E_RAMAN, BEC_E_TOTAL, BEC_E_COUNT
 };
 
 typedef struct bec_ws bec_ws;
@@ -92,6 +92,11 @@ int bec_set_tensor(bec_ws* ws, int inst, const double* gS, int n);
  * the caller forms gamma_c = _dipolar_relaxation_rates(F, gamma_dr)), or rates = NULL for
  * no loss (LossParams absent).  Real-time split_step and leapfrog segments then apply it. */
 int bec_set_loss(bec_ws* ws, int inst, const double* rates, int n);
+/* RamanCoupling of one instance (types.jl:179-183): raman = {Omega_R, delta, k_eff[ndim]},
+ * or NULL for none.  With it, every half potential step applies apply_raman_step!
+ * (raman.jl:8-31) with dt_half/2 before and after the DDI step (split_step.jl:77-107), and
+ * bec_energy reports _raman_energy (energy.jl:123-135) in BEC_E_RAMAN. */
+int bec_set_raman(bec_ws* ws, int inst, const double* raman);
 /* apply_loss_step! with step dt (losses.jl:25-49), as run_simulation_yoshida! applies it
  * after each accepted step (yoshida.jl:64-66). */
 int bec_loss_step(bec_ws* ws, double dt);

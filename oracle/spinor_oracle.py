@@ -23,6 +23,9 @@ a retyped copy of its own formula):
                             sum_S g_S |A_SM|^2 identity
   nematic step ............ ITP: scipy.linalg.expm of the real-linear pair ODE with A00
                             frozen; the reference's nematic tests (invariants, small dt)
+  Raman step .............. spin-1 Rabi closed forms (real and imaginary time, with the
+                            e^{-i k.x} phase of the transferred components); detuning-only
+                            phases; scipy.linalg.expm of the per-site H_R; norm (test_raman.jl)
   loss step ............... the reference's loss tests (rates: m=-F stable, mean gamma_dr;
                             m-ordering of the decay; density dependence; L3 uniform)
   CG / 6j ................. sympy exact + the reference test's known values
@@ -571,6 +574,7 @@ class OracleWS:
     c_lhy: float = 0.0
     c2: float = 0.0                # nematic coupling (c_extra[1]) when the tensor path is off
     loss: tuple = None             # (gamma_dr, L3) or None (LossParams)
+    raman: tuple = None            # (Omega_R, delta, k_eff tuple) or None (RamanCoupling)
     channels: dict = None          # tensor path (g_S), or None
     T: np.ndarray = None
     cdd: float = 0.0
@@ -592,7 +596,7 @@ class OracleWS:
 def make_workspace(grid, F, psi, dt, itp, *, omega=None, V=None, p=0.0, q=0.0,
                    c0=0.0, c1=0.0, c_lhy=0.0, c_extra=(), ddi=False, cdd=0.0,
                    secular=False, padding=False, n_steps=1, normalize_every=None,
-                   save_every=None, loss=None):
+                   save_every=None, loss=None, raman=None):
     """initialization.jl:86-191 (the subset on the hot path)."""
     if V is None:
         V = harmonic_potential(grid, omega) if omega is not None else np.zeros(grid.shape)
@@ -610,7 +614,9 @@ def make_workspace(grid, F, psi, dt, itp, *, omega=None, V=None, p=0.0, q=0.0,
     ws = OracleWS(grid=grid, F=F, psi=np.array(psi, dtype=complex), dt=dt, itp=itp, V=V,
                   p=p, q=q, c0=c0, c1=c1, c_lhy=c_lhy, c2=c2, channels=ch, T=T, cdd=cdd, ddi=ddi,
                   normalize_every=normalize_every, save_every=save_every, n_steps=n_steps,
-                  loss=None if loss is None else (float(loss[0]), float(loss[1])))
+                  loss=None if loss is None else (float(loss[0]), float(loss[1])),
+                  raman=None if raman is None else (float(raman[0]), float(raman[1]),
+                                                    tuple(float(k) for k in raman[2])))
     if ddi:
         ws.Q = ddi_q(grid.n, grid.dx, secular)
         if padding:
@@ -656,8 +662,48 @@ def ddi_step(ws: OracleWS, dtf, itp):
     ws.psi = euler_rotation(ws.psi, *phi, dtf, ws.F, itp)
 
 
+def raman_kr(grid: Grid, k_eff):
+    """raman.jl:19: k_eff . r on the grid (numpy C-order shape)."""
+    kr = np.zeros(grid.shape)
+    for d in range(grid.ndim):
+        kr = kr + _axis_field(grid, d, k_eff[d] * np.asarray(grid.x[d]))
+    return kr
+
+
+def raman_hamiltonians(grid: Grid, F, raman):
+    """raman.jl:20-23: H_R = delta Fz + Omega_R/2 (e^{i k.r} F+ + e^{-i k.r} F-) per site,
+    shape (N sites, D, D) in flat C order."""
+    Omega, delta, keff = raman
+    sm = spin_matrices(F)
+    ph = np.exp(1j * raman_kr(grid, keff)).reshape(-1)
+    return (delta * sm["Fz"][None]
+            + (Omega / 2) * (ph[:, None, None] * sm["Fp"][None]
+                             + np.conj(ph)[:, None, None] * sm["Fm"][None]))
+
+
+def raman_step(psi, grid: Grid, F, raman, dtf, itp):
+    """apply_raman_step! (raman.jl:8-31): psi <- U psi per site with
+    U = _exp_i_hermitian(H_R, dt) (spinor_utils.jl:26-38): V diag(cis(-lam dt)) V^H,
+    or exp(-lam dt) in imaginary time, from LAPACK eigh of the per-site H_R."""
+    H = raman_hamiltonians(grid, F, raman)
+    lam, V = np.linalg.eigh(H)
+    e = np.exp(-lam * dtf) if itp else np.exp(-1j * lam * dtf)
+    U = np.einsum("nij,nj,nkj->nik", V, e, V.conj())
+    D = psi.shape[0]
+    out = np.einsum("nij,nj->ni", U, psi.reshape(D, -1).T)
+    return np.ascontiguousarray(out.T).reshape(psi.shape)
+
+
+def raman_energy(psi, grid: Grid, F, raman, dV):
+    """_raman_energy (energy.jl:123-135): sum_r Re(psi^H H_R psi) dV."""
+    H = raman_hamiltonians(grid, F, raman)
+    D = psi.shape[0]
+    s = psi.reshape(D, -1).T
+    return float(np.sum(np.real(np.einsum("ni,nij,nj->n", s.conj(), H, s)))) * dV
+
+
 def half_potential_step(ws: OracleWS, dth, itp):
-    """split_step.jl:48-133 (raman absent)."""
+    """split_step.jl:48-133."""
     zee = ws.zee
     ws.psi = diagonal_step(ws.psi, ws.V, zee, ws.c0, ws.c_lhy, dth / 2, itp)
     if ws.T is not None:
@@ -665,8 +711,12 @@ def half_potential_step(ws: OracleWS, dth, itp):
     else:
         ws.psi = spin_mixing_step(ws.psi, ws.F, ws.c1, dth / 2, itp)
         ws.psi = nematic_step(ws.psi, ws.F, ws.c2, dth / 2, itp)
+    if ws.raman is not None:
+        ws.psi = raman_step(ws.psi, ws.grid, ws.F, ws.raman, dth / 2, itp)
     if ws.ddi:
         ddi_step(ws, dth, itp)
+    if ws.raman is not None:
+        ws.psi = raman_step(ws.psi, ws.grid, ws.F, ws.raman, dth / 2, itp)
     if ws.T is not None:
         ws.psi = tensor_step(ws.psi, ws.T, dth / 2, itp)
     else:
@@ -769,13 +819,14 @@ def energy_terms(ws: OracleWS):
     else:
         E["ddi"] = 0.0
     E["lhy"] = (2.0 / 5.0) * ws.c_lhy * float(np.sum(n * n * np.sqrt(n))) * dV if ws.c_lhy != 0 else 0.0
+    E["raman"] = raman_energy(psi, g, ws.F, ws.raman, dV) if ws.raman is not None else 0.0
     return E
 
 
 def total_energy(ws: OracleWS):
     E = energy_terms(ws)
     return (E["kin"] + E["trap"] + E["zeeman"] + E["c0"] + E["c1"] + E["ddi"]
-            + E["lhy"] + E["tensor"])
+            + E["lhy"] + E["tensor"] + E["raman"])
 
 
 def total_norm(psi, dV):

@@ -724,3 +724,70 @@ def test_adaptive_strang_rejects_and_shrinks():
     out = O.run_simulation_adaptive(a, dt_init=0.01, dt_min=1e-5, dt_max=0.01, tol=1e-9,
                                     t_end=0.02, save_interval=0.02)
     assert out["n_rejected"] > 0 and out["final_dt"] < 0.01
+
+
+# --- Raman coupling (test_raman.jl) ----------------------------------------------
+def _m_plus_one(grid, D):
+    psi = np.zeros((D,) + grid.shape, complex)
+    psi[0] = 1.0 / math.sqrt(np.prod(grid.n) * grid.dV)
+    return psi
+
+
+@pytest.mark.parametrize("itp", [False, True])
+def test_raman_spin1_rabi_closed_form(itp):
+    """Spin 1 from m=+1 with delta = 0: U = e^{i a Fz} e^{-i beta Fx} e^{-i a Fz}, beta = Omega t,
+    a = k.x, so psi = (cos^2(b/2), -i sin(b)/sqrt2 e^{-ia}, -sin^2(b/2) e^{-2ia}) psi_0 in real
+    time and (cosh^2(b/2), -sinh(b)/sqrt2 e^{-ia}, sinh^2(b/2) e^{-2ia}) psi_0 in imaginary time.
+    A factor 2 in Omega/2, the sign of k or a swapped F+/F- fails it."""
+    g = O.make_grid((16,), (8.0,))
+    k, Om, dt, n = 2 * math.pi / 4.0, 1.3, 0.01, 37
+    psi = _m_plus_one(g, 3)
+    a0 = psi[0].copy()
+    for _ in range(n):
+        psi = O.raman_step(psi, g, 1, (Om, 0.0, (k,)), dt, itp)
+    b, ph = Om * dt * n, np.exp(-1j * k * np.asarray(g.x[0]))
+    if itp:
+        want = [np.cosh(b / 2) ** 2 * a0, -np.sinh(b) / math.sqrt(2) * ph * a0, np.sinh(b / 2) ** 2 * ph ** 2 * a0]
+    else:
+        want = [np.cos(b / 2) ** 2 * a0, -1j * np.sin(b) / math.sqrt(2) * ph * a0, -np.sin(b / 2) ** 2 * ph ** 2 * a0]
+    for c in range(3):
+        assert np.allclose(psi[c], want[c], atol=1e-13, rtol=0), c
+
+
+def test_raman_detuning_only_phases_and_zero_coupling():
+    """Omega = 0: psi_c *= e^{-i delta m dt} (m = F - c); Omega = delta = 0 leaves psi unchanged."""
+    g = O.make_grid((8, 4), (4.0, 2.0))
+    psi = rspinor(5, 32, 3).reshape(5, 4, 8)
+    out = O.raman_step(psi, g, 2, (0.0, 0.7, (1.0, -2.0)), 0.05, False)
+    for c in range(5):
+        assert np.allclose(out[c], np.exp(-1j * 0.7 * (2 - c) * 0.05) * psi[c], atol=1e-14)
+    assert np.allclose(O.raman_step(psi, g, 2, (0.0, 0.0, (0.0, 0.0)), 0.01, False), psi, atol=1e-14)
+
+
+@pytest.mark.parametrize("F", [1, 2, 6])
+def test_raman_step_matches_expm_and_keeps_norm(F):
+    """Per-site scipy expm(-i H_R dt) with H_R assembled from the textbook spin matrices
+    (F+ from sqrt(F(F+1) - m(m+1))); real-time norm conserved (test_raman.jl 'preserves norm')."""
+    D = 2 * F + 1
+    g = O.make_grid((4, 4), (3.0, 2.0))
+    psi = rspinor(D, 16, 7).reshape(D, 4, 4)
+    rc = (1.7, -0.4, (1.1, 0.6))
+    out = O.raman_step(psi, g, F, rc, 0.03, False)
+    m = np.array([F - c for c in range(D)], float)
+    Fp = np.zeros((D, D))
+    for c in range(1, D):   # <m+1| F+ |m>: row c-1 (m+1), column c (m)
+        Fp[c - 1, c] = math.sqrt(F * (F + 1) - m[c] * (m[c] + 1))
+    for iy in range(4):
+        for ix in range(4):
+            kr = 1.1 * g.x[0][ix] + 0.6 * g.x[1][iy]
+            H = -0.4 * np.diag(m) + 0.85 * (np.exp(1j * kr) * Fp + np.exp(-1j * kr) * Fp.T)
+            assert np.allclose(out[:, iy, ix], sla.expm(-1j * 0.03 * H) @ psi[:, iy, ix], atol=1e-13)
+    assert np.sum(np.abs(out) ** 2) == pytest.approx(np.sum(np.abs(psi) ** 2), rel=1e-13)
+
+
+def test_raman_energy_of_stretched_state_is_detuning():
+    """<+F| H_R |+F> = delta F per site (the F+- terms have no diagonal), so E_raman = delta F * norm."""
+    g = O.make_grid((16, 8), (8.0, 4.0))
+    psi = _m_plus_one(g, 5)
+    ws = O.make_workspace(g, 2, psi, 0.01, False, raman=(2.0, 0.3, (1.0, 0.5)))
+    assert O.energy_terms(ws)["raman"] == pytest.approx(0.3 * 2 * 1.0, rel=1e-13)
