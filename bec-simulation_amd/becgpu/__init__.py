@@ -3,7 +3,9 @@ for the hot loop of SpinorBEC.jl.  The compute runs in libbecgpu.so (hand-writte
 HIP kernels behind the C ABI in include/becgpu.h); this package is the host
 driver that keeps the reference's public entry points."""
 from .physics import (AdaptiveDtParams, AtomSpecies, Eu151, GridConfig, HarmonicTrap,  # noqa: F401
-                      InteractionParams, LossParams, RamanCoupling, Na23, NoPotential, Rb87, SimParams,
+                      InteractionParams, LossParams, RamanCoupling, GravityPotential,
+                      CompositePotential, TimeDependentZeeman, zeeman_at,
+                      interaction_params_from_constraint, Na23, NoPotential, Rb87, SimParams,
                       SpinSystem,
                       ZeemanParams, add_noise, cell_volume, classify_phase, init_psi,
                       init_psi_thomas_fermi, make_grid, spin_matrices)

@@ -23,7 +23,8 @@ from . import _lib as L
 from .physics import (AdaptiveDtParams, Grid, HarmonicTrap, InteractionParams, NoPotential,
                       SimParams, SpinSystem, ZeemanParams, c0c1_to_gS, c_extra_to_delta_gS,
                       cell_volume, classify_phase, dipolar_relaxation_rates, evaluate_potential,
-                      init_psi, spin_matrices, zeeman_energies)
+                      init_psi, spin_matrices, zeeman_energies, TimeDependentZeeman,
+                      zeeman_at)
 
 ITP_EXPONENT_LIMIT = 50.0          # ground_state.jl:1
 
@@ -91,6 +92,10 @@ class Workspace:
         self.params = [None] * n_instances
         self.interactions = [None] * n_instances
         self.zeeman = [ZeemanParams()] * n_instances
+        # TimeDependentZeeman.B_func (t -> ZeemanParams) shared by the instances, or None;
+        # the drivers apply it at the instance's time before every half potential step and
+        # energy, as zeeman_at(ws.zeeman, ws.state.t) does (split_step.jl:49-50, energy.jl:15)
+        self.zeeman_fn = None
         self._loss = [False] * n_instances
         self._active = list(range(n_instances))
         self.xrows = self.autotune() if autotune else None
@@ -148,6 +153,19 @@ class Workspace:
         self.params[inst] = p
         self.interactions[inst] = interactions
         self.zeeman[inst] = zeeman
+
+    def sync_zeeman(self):
+        """Set each active instance's Zeeman field to zeeman_fn(t) (no-op without one)."""
+        if self.zeeman_fn is None:
+            return
+        for inst in self._active:
+            z = self.zeeman_fn(float(self.t[inst]))
+            if z == self.zeeman[inst] or self.params[inst] is None:
+                continue
+            p = self.params[inst]
+            p.p, p.q = float(z.p), float(z.q)
+            L.check(L.lib().bec_set_params(self.handle, inst, C.byref(p)), "bec_set_params")
+            self.zeeman[inst] = z
 
     def set_loss(self, inst, loss):
         """LossParams(gamma_dr, L3) of one instance, or None (types.jl:291-296). The
@@ -358,6 +376,10 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
                    psi_init=None, enable_ddi=False, c_dd=float("nan"), secular_ddi=False,
                    raman=None, loss=None, ddi_padding=False, precision="fp64", fft_flags=None):
     F = atom.F
+    zeeman_fn = None
+    if isinstance(zeeman, TimeDependentZeeman):    # types.jl:173-175
+        zeeman_fn = zeeman.B_func
+        zeeman = zeeman_at(zeeman, 0.0)
     tensor = _tensor_active(F, interactions)
     channels = None
     if tensor:
@@ -384,6 +406,7 @@ def make_workspace(*, grid: Grid, atom, interactions: InteractionParams,
         ws.set_loss(0, loss)
     if raman is not None:
         ws.set_raman(0, raman)
+    ws.zeeman_fn = zeeman_fn
     psi = init_psi(grid, SpinSystem(F)) if psi_init is None else np.array(psi_init, dtype=complex)
     ws.set_psi(psi, 0)
     ws.atom = atom
@@ -412,6 +435,7 @@ def magnetization(ws: Workspace, inst=0):
 
 
 def _record_snapshot(ws, res: SimulationResult, keep_psi=True, inst=0):
+    ws.sync_zeeman()
     e = ws.energy_terms()[inst]
     res.times.append(float(ws.t[inst]))
     res.energies.append(float(e[L.E_TOTAL]))
@@ -427,6 +451,11 @@ def _record_snapshot(ws, res: SimulationResult, keep_psi=True, inst=0):
 def split_step(ws: Workspace, n: int = 1, graph: bool = False):
     """split_step! (split_step.jl:10-35), n times.  graph: issue the n steps as one captured
     HIP graph (cached per step count and active set; same results)."""
+    if ws.zeeman_fn is not None and n > 1 and not ws.sim_params.imaginary_time:
+        for _ in range(n):            # the field follows t from step to step
+            split_step(ws, 1, graph=False)
+        return
+    ws.sync_zeeman()
     if graph:
         L.check(L.lib().bec_split_step_graph(ws.handle, int(n)), "bec_split_step_graph")
     else:
@@ -456,6 +485,8 @@ def run_simulation(ws: Workspace, callback=None, keep_snapshots=True):
                     callback(ws, done)
         return res
     dt = ws.params[0].dt
+    if ws.zeeman_fn is not None:
+        return _run_leapfrog_time_dependent(ws, res, callback, keep_snapshots)
     done = 0
     while done < sp.n_steps:
         k = min(sp.save_every - done % sp.save_every, sp.n_steps - done)
@@ -471,11 +502,45 @@ def run_simulation(ws: Workspace, callback=None, keep_snapshots=True):
     return res
 
 
+def _run_leapfrog_time_dependent(ws, res, callback, keep_snapshots):
+    """_run_simulation_leapfrog! (simulation.jl:49-94) with a time-dependent Zeeman field:
+    every half potential step reads the field at the current t (the merged V(dt) of a step
+    at its start time), so the steps run as separate half potential and kinetic launches."""
+    lib = L.lib()
+    sp = ws.sim_params
+    dt = ws.params[0].dt
+
+    def hp(dth):
+        ws.sync_zeeman()
+        L.check(lib.bec_half_potential(ws.handle, float(dth), 0), "bec_half_potential")
+
+    hp(dt / 2)
+    for step in range(1, sp.n_steps + 1):
+        L.check(lib.bec_kinetic(ws.handle, float(dt), 0), "bec_kinetic")
+        is_save = step % sp.save_every == 0
+        is_last = step == sp.n_steps
+        split = is_save or is_last
+        hp(dt / 2 if split else dt)
+        if ws.has_loss:
+            L.check(lib.bec_loss_step(ws.handle, float(dt)), "bec_loss_step")
+        ws.t[0] += dt
+        ws.step[0] += 1
+        if is_save:
+            _record_snapshot(ws, res, keep_snapshots)
+            if callback is not None:
+                callback(ws, step)
+        if split and not is_last:
+            hp(dt / 2)
+    return res
+
+
 def strang_core(ws: Workspace, dt: float):
+    ws.sync_zeeman()
     L.check(L.lib().bec_strang_core(ws.handle, float(dt)), "bec_strang_core")
 
 
 def yoshida_core(ws: Workspace, dt: float):
+    ws.sync_zeeman()
     L.check(L.lib().bec_yoshida_core(ws.handle, float(dt)), "bec_yoshida_core")
 
 
@@ -541,6 +606,7 @@ def run_simulation_adaptive(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end
     lib = L.lib()
 
     def hp(dth):
+        ws.sync_zeeman()
         L.check(lib.bec_half_potential(ws.handle, float(dth), 0), "bec_half_potential")
 
     def l2(a, b):  # sum |a - b|^2 / sum |b|^2 (_wavefunction_l2_change, adaptive.jl:42-50)

@@ -346,18 +346,64 @@ class NoPotential:
     pass
 
 
+@dataclass(frozen=True)
+class GravityPotential:
+    """types.jl:199-209: V = g x_axis, axis 1-based (potentials.jl:17-24)."""
+    g: float
+    axis: int
+
+
+@dataclass(frozen=True)
+class CompositePotential:
+    """types.jl:211-213: the sum of its components (potentials.jl:26-32)."""
+    components: tuple
+
+
 def evaluate_potential(pot, grid: Grid):
-    """potentials.jl:1-15."""
+    """potentials.jl:1-32 (harmonic, gravity, composite; numpy C order (z, y, x))."""
     V = np.zeros(grid.shape)
     if isinstance(pot, NoPotential) or pot is None:
         return V
     nd = grid.ndim
-    s = np.zeros(grid.shape)
-    for d in range(nd):
+
+    def axis_field(d, vals):
         sh = [1] * nd
         sh[nd - 1 - d] = grid.config.n_points[d]
-        s = s + (pot.omega[d] ** 2 * grid.x[d] ** 2).reshape(sh)
+        return np.asarray(vals).reshape(sh)
+
+    if isinstance(pot, GravityPotential):
+        if not 1 <= pot.axis <= nd:
+            raise ValueError(f"axis must be between 1 and {nd}")
+        return V + axis_field(pot.axis - 1, pot.g * np.asarray(grid.x[pot.axis - 1]))
+    if isinstance(pot, CompositePotential):
+        for c in pot.components:
+            V = V + evaluate_potential(c, grid)
+        return V
+    if not isinstance(pot, HarmonicTrap):
+        raise NotImplementedError(f"potential {type(pot).__name__} is not built")
+    s = np.zeros(grid.shape)
+    for d in range(nd):
+        s = s + axis_field(d, pot.omega[d] ** 2 * grid.x[d] ** 2)
     return 0.5 * s
+
+
+@dataclass(frozen=True)
+class TimeDependentZeeman:
+    """types.jl:173-175: B_func(t) -> ZeemanParams, evaluated at the state's time by every
+    half potential step and energy (zeeman.jl:20-21)."""
+    B_func: object
+
+
+def zeeman_at(z, t: float):
+    """zeeman.jl:20-21."""
+    return z.B_func(t) if isinstance(z, TimeDependentZeeman) else z
+
+
+def interaction_params_from_constraint(*, c_total: float, c1_ratio: float = 0.0, F: int,
+                                       c_extra=()):
+    """interactions.jl:182-187: c0 = c_total / (1 + F^2 c1_ratio), c1 = c1_ratio c0."""
+    c0 = c_total / (1.0 + F ** 2 * c1_ratio)
+    return InteractionParams(c0, c1_ratio * c0, 0.0, tuple(float(v) for v in c_extra))
 
 
 @dataclass(frozen=True)

@@ -575,6 +575,7 @@ class OracleWS:
     c2: float = 0.0                # nematic coupling (c_extra[1]) when the tensor path is off
     loss: tuple = None             # (gamma_dr, L3) or None (LossParams)
     raman: tuple = None            # (Omega_R, delta, k_eff tuple) or None (RamanCoupling)
+    zeeman_fn: object = None       # TimeDependentZeeman: t -> (p, q), or None
     channels: dict = None          # tensor path (g_S), or None
     T: np.ndarray = None
     cdd: float = 0.0
@@ -590,6 +591,10 @@ class OracleWS:
 
     @property
     def zee(self):
+        """zeeman_at(ws.zeeman, ws.state.t) (zeeman.jl:20-21, split_step.jl:49-50)."""
+        if self.zeeman_fn is not None:
+            p, q = self.zeeman_fn(self.t)
+            return zeeman_diag(self.F, p, q)
         return zeeman_diag(self.F, self.p, self.q)
 
 
