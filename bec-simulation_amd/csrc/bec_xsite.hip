@@ -216,8 +216,9 @@ bool xsite_ok(const Geom& g) {
   const int nx = g.n[0];
   if (nx < 16 || nx > 256) return false;
   const int ts = xpass_rows_g(g) * nx;
-  // a tile of at least one wave, one thread per site
-  return ts >= 64 && ts <= kXsiteMaxThreads;
+  // one thread per site (tiles below one wave, from short lines or the tile autotuner's
+  // choice, leave lanes idle but keep the x pass's block partition)
+  return ts >= 16 && ts <= kXsiteMaxThreads;
 }
 
 template <typename T, int NX, int D>
@@ -225,7 +226,7 @@ static hipError_t xsite_n(hipStream_t st, const Geom& g, const DevTables& tb, co
                           void* psi, void* fhat, int n_active) {
   const int rows = xpass_rows_g(g);
   const int threads = rows * NX;
-  if (threads < 64 || threads > kXsiteMaxThreads) return hipErrorInvalidConfiguration;
+  if (threads < 16 || threads > kXsiteMaxThreads) return hipErrorInvalidConfiguration;
   const bool has_aux = (prog.stages & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
   const size_t lds = xsite_lds(D, rows, NX, sizeof(cplx<T>), has_aux, g.spad ? 2 * NX : NX);
   if (lds > 48 * 1024)

@@ -190,7 +190,7 @@ int bec_synchronize(bec_ws* ws);
  * environment variable BEC_XSITE=0 for off), spins F = 2 and 6 run every half potential
  * step with its neighbouring x FFTs and DDI x lines in the fused x/site kernel; off, in
  * separate x passes and lane-per-site passes.  Both give the same psi.  Returns the
- * setting in force (the fused kernel needs 2-D/3-D grids with one x-pass tile of 64..256
+ * setting in force (the fused kernel needs 2-D/3-D grids with one x-pass tile of 16..256
  * sites; other shapes always run separately). */
 int bec_set_xsite(bec_ws* ws, int on);
 /* Tile autotuner (no counterpart in the reference, whose FFTW plans are tuned by
