@@ -563,19 +563,31 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   for (int k = 0; k < 8; ++k) tile[tid + k * nthr] = ld[k];
 #pragma unroll
   for (int k = 0; k < 2; ++k) tws[min(tid + k * nthr, N - 1)] = twv[k];
-  // per-axis phase tables for the kinetic multiply
+  // per-axis phase tables for the kinetic multiply: entries [0, N) along the line, [N, N+TB)
+  // for the tile's x columns; the factor of the other transverse axis (y for the z pass) and
+  // the 1/N of the inverse transform are folded into the line entries, so each element
+  // takes one FP64 complex product of two table entries
   if (MODE == SM_KIN) {
     const InstParams& P = tb.params[inst];
     const double dt = (dt_override > 0.0 ? dt_override : P.dt) * kin_frac;
-    for (int i = tid; i < N + TB + 1; i += nthr) {
-      double k;
-      if (i < N) k = tb.ktab[axis * g.nmax + i];
-      else if (i < N + TB) k = (x0 + (i - N) < g.n[0]) ? tb.ktab[x0 + (i - N)] : 0.0;
-      else k = (g.ndim == 3 && axis == 2) ? tb.ktab[g.nmax + other] : 0.0;  // y for the z pass
+    auto phase = [&](double k, double& re, double& im) __attribute__((always_inline)) {
       const double a = 0.5 * (k * k) * dt;
-      double re, im;
       if (itp) { re = exp(-a); im = 0.0; }
       else { double s, c; sincos_d(a, &s, &c); re = c; im = -s; }
+    };
+    double ore, oim;  // y factor of the z pass
+    phase((g.ndim == 3 && axis == 2) ? tb.ktab[g.nmax + other] : 0.0, ore, oim);
+    for (int i = tid; i < N + TB; i += nthr) {
+      double k;
+      if (i < N) k = tb.ktab[axis * g.nmax + i];
+      else k = (x0 + (i - N) < g.n[0]) ? tb.ktab[x0 + (i - N)] : 0.0;
+      double re, im;
+      phase(k, re, im);
+      if (i < N) {
+        const cplx<double> v = (mk(re, im) * mk(ore, oim)) * g.inv_ntot;
+        re = v.x;
+        im = v.y;
+      }
       tabs[2 * i] = re;
       tabs[2 * i + 1] = im;
     }
@@ -592,10 +604,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       const int xi2 = idx % TB, e = (idx / TB) % N;
       const cplx<double> pa = mk(tabs[2 * e], tabs[2 * e + 1]);
       const cplx<double> px = mk(tabs[2 * (N + xi2)], tabs[2 * (N + xi2) + 1]);
-      const cplx<double> po = mk(tabs[2 * (N + TB)], tabs[2 * (N + TB) + 1]);
-      cplx<double> ph = (px * pa) * po;
-      ph = ph * g.inv_ntot;
-      tile[idx] = mul64(tile[idx], ph);
+      tile[idx] = mul64(tile[idx], px * pa);
     }
     __syncthreads();
   } else if (MODE == SM_DDI) {
