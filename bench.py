@@ -293,7 +293,7 @@ def run_batch2d(G, A, n_jobs, n_plain):
     plain = len(sub) / (time.perf_counter() - t0)
     rel = [abs(a["energy"] - b["energy"]) / abs(b["energy"]) for a, b in zip(r32, r64)]
     conv = [e for e, a, b in zip(rel, r32, r64) if a["converged"] and b["converged"]]
-    err = max(conv) if conv else float("nan")
+    err = max(conv) if conv else None   # null in the JSON line when no pair converged
     lat = np.array([r["time_s"] for r in r32])
     return dict(fp32=s32, fp64=s64, plain_fp64_solves_per_s=plain, plain_sample=len(sub),
                 max_rel_energy_err_fp32=err, max_rel_energy_err_all=max(rel),
