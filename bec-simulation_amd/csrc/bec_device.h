@@ -317,7 +317,10 @@ __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool acti
 __device__ __forceinline__ cplx<double> diag_base(double nd, double Vtrap, const InstParams& P,
                                                   double dtf, bool itp) {
 #pragma clang fp contract(on)
-  const double a = (Vtrap + P.c0 * nd + P.c_lhy * nd * sqrt(nd)) * dtf;
+  // without LHY (c_lhy = 0, uniform over the block) the term is fma(0, sqrt(n), x) = x, so
+  // skipping the FP64 square root gives the same bits
+  const double a = P.c_lhy != 0.0 ? (Vtrap + P.c0 * nd + P.c_lhy * nd * sqrt(nd)) * dtf
+                                  : (Vtrap + P.c0 * nd) * dtf;
   if (itp) return mk(exp(-a), 0.0);
   double sn, cs;
   sincos_d(a, &sn, &cs);
