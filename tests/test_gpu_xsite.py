@@ -67,6 +67,8 @@ def test_spin6_tensor_ddi_yoshida_fused_equals_split(precision, padding):
     assert ns["xsite"] == 0 and ns["sites"] > 0, ns
     a, b = gf.get_psi(), gs.get_psi()
     assert np.array_equal(a, b), float(np.max(np.abs(a - b)))
+    if not padding:   # max |Phi| partials (the padded convolution leaves them to the energy)
+        assert np.array_equal(gf.itp_guard(), gs.itp_guard())
     if precision == "fp64":
         O.yoshida_core(ows, 1e-3)
         O.yoshida_core(ows, 1e-3)
@@ -107,9 +109,16 @@ def test_spin2_mixing_ddi_fused_equals_split(itp):
     A.split_step(gs, 3)
     a, b = gf.get_psi(), gs.get_psi()
     assert rel_l2(a, b) <= 1e-13
+    # ITP overflow guard (ground_state.jl:4-29): norm and max |Phi| of the last convolution,
+    # the latter from the fused kernel's own partials
+    qf, qs = gf.itp_guard(), gs.itp_guard()
+    assert qf[0, 1] > 0.0
+    assert np.allclose(qf, qs, rtol=1e-12, atol=0)
     for _ in range(3):
         O.split_step(ows)
     assert rel_l2(a, ows.psi) <= 1e-10
+    phi = ows._last_phi   # the oracle's last convolution
+    assert qf[0, 1] == pytest.approx(max(float(np.max(np.abs(f))) for f in phi), rel=1e-9)
 
 
 def test_fused_graph_replays_match_eager():
