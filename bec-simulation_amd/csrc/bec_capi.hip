@@ -64,7 +64,7 @@ struct bec_ws {
   double* d_cpart = nullptr;   // constrained normalisation: per-component partials, targets,
   double* d_ctarget = nullptr; // factors (allocated on first use)
   double* d_cfac = nullptr;
-  void* d_tw[10] = {};
+  void* d_tw[kNumFftSizes] = {};
   double* h_red = nullptr;  // pinned
   int max_blocks = 0;
   int n_xblocks = 0;
@@ -124,8 +124,11 @@ static void prof_end(bec_ws* ws, int idx) {
 // ---------------------------------------------------------------------------
 static constexpr size_t kWorkBytes = sizeof(unsigned long long) * kWorkSlots * kWorkStride;
 
-static bool is_pow2_supported(int n) {
-  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+// Grid sides with built x-line FFTs: 2^k (8..512), 3 * 2^k (24..192), 5 * 2^k (40..160).
+// (Zero-padded DDI lines are twice the side: up to 384 and 320 for the latter two.)
+static bool side_supported(int n) {
+  if (n == 320 || n == 384) return false;
+  return fft_size_id(n) >= 0;
 }
 
 static void refresh_tables(bec_ws* ws) {
@@ -140,7 +143,7 @@ static void refresh_tables(bec_ws* ws) {
                : nullptr;
   t.params = ws->d_params;
   t.active = ws->d_active;
-  for (int i = 0; i < 10; ++i) t.tw[i] = ws->d_tw[i];
+  for (int i = 0; i < kNumFftSizes; ++i) t.tw[i] = ws->d_tw[i];
   t.red = ws->d_red;
   t.partials = ws->d_partials;
   t.max_blocks = ws->max_blocks;
@@ -260,6 +263,17 @@ static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_fra
   else
     e = launch_strided<double>(ws->stream, ws->g, ws->tb, axis, mode, data, n_active(ws), kin_frac,
                                dt_ovr, itp, ddi_data, nb);
+  prof_end(ws, pe);
+  HIPCHK(e);
+  return BEC_OK;
+}
+// 1-D DDI: the Q_ab product of the three half spectra (read and written once)
+static int do_ddi_line(bec_ws* ws) {
+  const int pe = prof_begin(ws, 6, ws->prof_on ? 2.0 * half_field_bytes(ws) : 0.0);
+  const hipError_t e =
+      ws->prec == BEC_FP32
+          ? launch_ddi_line<float>(ws->stream, ws->g, ws->tb, ws->fhat, n_active(ws))
+          : launch_ddi_line<double>(ws->stream, ws->g, ws->tb, ws->fhat, n_active(ws));
   prof_end(ws, pe);
   HIPCHK(e);
   return BEC_OK;
@@ -384,6 +398,8 @@ struct Sched {
       CHK(do_strided(ws, 1, SM_FWD, ws->fhat, 0, 0, itp, 1));
       CHK(do_strided(ws, 2, SM_DDI, ws->fhat, 0, 0, itp, 1));
       CHK(do_strided(ws, 1, SM_INV, ws->fhat, 0, 0, itp, 1));
+    } else if (nd == 1) {
+      CHK(do_ddi_line(ws));
     } else {
       CHK(do_strided(ws, 1, SM_DDI, ws->fhat, 0, 0, itp, 1));
     }
@@ -604,15 +620,14 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     if (c->n[d] <= 0 || (c->n[d] % 2) != 0)
       return fail(BEC_EINVAL, "n_points must be positive even integers");
     if (!(c->L[d] > 0)) return fail(BEC_EINVAL, "box_size must be positive");
-    if (!is_pow2_supported(c->n[d]))
-      return fail(BEC_EUNSUP, "grid sides must be powers of two in [16, 512] on this build");
+    if (!side_supported(c->n[d]))
+      return fail(BEC_EUNSUP, "grid sides must be 2^k in [8, 512], 3*2^k in [24, 192] or 5*2^k in [40, 160] on this build");
   }
   if (!(c->F == 1 || c->F == 2 || c->F == 6)) return fail(BEC_EUNSUP, "F must be 1, 2 or 6");
   if (c->F == 6 && c->n[0] > 256) return fail(BEC_EUNSUP, "F=6 supports x sides up to 256");
   if (c->n_instances < 1) return fail(BEC_EINVAL, "n_instances must be >= 1");
   if (c->precision != BEC_FP64 && c->precision != BEC_FP32)
     return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
-  if (c->enable_ddi && c->ndim < 2) return fail(BEC_EUNSUP, "DDI needs a 2-D or 3-D grid here");
   if (c->ddi_padding && !c->enable_ddi) return fail(BEC_EINVAL, "ddi_padding needs enable_ddi");
   if (c->ddi_padding)
     for (int d = 0; d < c->ndim; ++d)
@@ -743,8 +758,8 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipHostMalloc(&ws->h_red, redb) != hipSuccess) return cleanup(BEC_ENOMEM, "pinned red");
   // twiddles exp(-2 pi i m / N), computed in long double and stored in FP64 for both
   // precisions (FP32-bulk butterflies multiply by them in FP64 with one rounding)
-  for (int l = 3; l <= 9; ++l) {
-    const int N = 1 << l;
+  for (int l = 0; l < kNumFftSizes; ++l) {
+    const int N = kFftSizes[l];
     std::vector<double> buf(2 * N);
     for (int m = 0; m < N; ++m) {
       const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
@@ -778,7 +793,7 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->d_work) (void)hipFree(ws->d_work);
   for (double* q : {ws->d_cpart, ws->d_ctarget, ws->d_cfac})
     if (q) (void)hipFree(q);
-  for (int i = 0; i < 10; ++i)
+  for (int i = 0; i < kNumFftSizes; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
   if (ws->h_red) (void)hipHostFree(ws->h_red);
   for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
@@ -1309,6 +1324,8 @@ static int energy_partials_fp64(bec_ws* ws, const DevTables& tb0, void* psi, voi
       CHK(S(1, SM_FWD, fhat, 1, nullptr));
       CHK(S(2, SM_DDI, fhat, 1, nullptr));
       CHK(S(1, SM_INV, fhat, 1, nullptr));
+    } else if (g.ndim == 1) {
+      HIPCHK(launch_ddi_line<double>(ws->stream, g, tb, fhat, na));
     } else {
       CHK(S(1, SM_DDI, fhat, 1, nullptr));
     }

@@ -217,24 +217,86 @@ __device__ __forceinline__ void dft8(cplx<T>* v) {
   v[7] = e3 - o3;
 }
 
+// Radix 3 (w = exp(-+2 pi i/3)): y0 = v0 + (v1 + v2), y1,2 = v0 - (v1 + v2)/2 -+ i sin(pi/3) (v1 - v2);
+// every product by a constant and each output formed in FP64 with one rounding.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft3(cplx<T>* v) {
+  constexpr double S3 = 0.86602540378443864676;  // sin(pi/3)
+  const double ax = (double)v[1].x + (double)v[2].x, ay = (double)v[1].y + (double)v[2].y;
+  const double bx = ((double)v[1].x - (double)v[2].x) * S3, by = ((double)v[1].y - (double)v[2].y) * S3;
+  const double mx = (double)v[0].x - 0.5 * ax, my = (double)v[0].y - 0.5 * ay;
+  // -i b (forward) or +i b (inverse)
+  const double tx = INV ? -by : by, ty = INV ? bx : -bx;
+  v[0] = mk((T)((double)v[0].x + ax), (T)((double)v[0].y + ay));
+  v[1] = mk((T)(mx + tx), (T)(my + ty));
+  v[2] = mk((T)(mx - tx), (T)(my - ty));
+}
+
+// Radix 5 (w = exp(-+2 pi i/5)), with a_k = v_k + v_{5-k}, b_k = v_k - v_{5-k}:
+// y_{1,4} = v0 + c1 a1 + c2 a2 -+ i (s1 b1 + s2 b2), y_{2,3} = v0 + c2 a1 + c1 a2 -+ i (s2 b1 - s1 b2)
+// (c_k = cos(2 pi k/5), s_k = sin(2 pi k/5)); FP64 inside, one rounding per output.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft5(cplx<T>* v) {
+  constexpr double C1 = 0.30901699437494742410, C2 = -0.80901699437494742410;
+  constexpr double S1 = 0.95105651629515357212, S2 = 0.58778525229247312917;
+  const double x0 = v[0].x, y0 = v[0].y;
+  const double a1x = (double)v[1].x + (double)v[4].x, a1y = (double)v[1].y + (double)v[4].y;
+  const double a2x = (double)v[2].x + (double)v[3].x, a2y = (double)v[2].y + (double)v[3].y;
+  const double b1x = (double)v[1].x - (double)v[4].x, b1y = (double)v[1].y - (double)v[4].y;
+  const double b2x = (double)v[2].x - (double)v[3].x, b2y = (double)v[2].y - (double)v[3].y;
+  const double m1x = x0 + C1 * a1x + C2 * a2x, m1y = y0 + C1 * a1y + C2 * a2y;
+  const double m2x = x0 + C2 * a1x + C1 * a2x, m2y = y0 + C2 * a1y + C1 * a2y;
+  const double n1x = S1 * b1x + S2 * b2x, n1y = S1 * b1y + S2 * b2y;
+  const double n2x = S2 * b1x - S1 * b2x, n2y = S2 * b1y - S1 * b2y;
+  // -i n (forward) or +i n (inverse)
+  const double t1x = INV ? -n1y : n1y, t1y = INV ? n1x : -n1x;
+  const double t2x = INV ? -n2y : n2y, t2y = INV ? n2x : -n2x;
+  v[0] = mk((T)(x0 + a1x + a2x), (T)(y0 + a1y + a2y));
+  v[1] = mk((T)(m1x + t1x), (T)(m1y + t1y));
+  v[4] = mk((T)(m1x - t1x), (T)(m1y - t1y));
+  v[2] = mk((T)(m2x + t2x), (T)(m2y + t2y));
+  v[3] = mk((T)(m2x - t2x), (T)(m2y - t2y));
+}
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dftR(cplx<T>* v) {
   if constexpr (R == 2) dft2<T, INV>(v[0], v[1]);
+  else if constexpr (R == 3) dft3<T, INV>(v);
   else if constexpr (R == 4) dft4<T, INV>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 5) dft5<T, INV>(v);
   else dft8<T, INV>(v);
 }
 
-// Radix plan for a length-N line: a list of radices (each 2, 4 or 8) whose
-// product is N, every pass processed by N/E threads holding E (8 or 16) values each.
+// Radix plan for a length-N line: radices (2, 3, 4, 5 or 8) whose product is N, and E,
+// the values each thread holds (N/E threads per line); every radix divides E.  Lines of
+// 2^k points use radix 8 first and E = 8 (or 16 where a caller asks for it); lines of
+// 3 * 2^k and 5 * 2^k points (24 ... 384) use radix 4 passes with one radix-3 or radix-5
+// pass and E = 12 or 20.
 template <int N>
 struct RadixPlan;
-template <> struct RadixPlan<8>   { static constexpr int np = 1; static constexpr int r[3] = {8, 0, 0}; };
-template <> struct RadixPlan<16>  { static constexpr int np = 2; static constexpr int r[3] = {8, 2, 0}; };
-template <> struct RadixPlan<32>  { static constexpr int np = 2; static constexpr int r[3] = {8, 4, 0}; };
-template <> struct RadixPlan<64>  { static constexpr int np = 2; static constexpr int r[3] = {8, 8, 0}; };
-template <> struct RadixPlan<128> { static constexpr int np = 3; static constexpr int r[3] = {8, 4, 4}; };
-template <> struct RadixPlan<256> { static constexpr int np = 3; static constexpr int r[3] = {8, 8, 4}; };
-template <> struct RadixPlan<512> { static constexpr int np = 3; static constexpr int r[3] = {8, 8, 8}; };
+template <> struct RadixPlan<8>   { static constexpr int np = 1, e = 8;  static constexpr int r[5] = {8}; };
+template <> struct RadixPlan<16>  { static constexpr int np = 2, e = 8;  static constexpr int r[5] = {8, 2}; };
+template <> struct RadixPlan<32>  { static constexpr int np = 2, e = 8;  static constexpr int r[5] = {8, 4}; };
+template <> struct RadixPlan<64>  { static constexpr int np = 2, e = 8;  static constexpr int r[5] = {8, 8}; };
+template <> struct RadixPlan<128> { static constexpr int np = 3, e = 8;  static constexpr int r[5] = {8, 4, 4}; };
+template <> struct RadixPlan<256> { static constexpr int np = 3, e = 8;  static constexpr int r[5] = {8, 8, 4}; };
+template <> struct RadixPlan<512> { static constexpr int np = 3, e = 8;  static constexpr int r[5] = {8, 8, 8}; };
+template <> struct RadixPlan<24>  { static constexpr int np = 3, e = 12; static constexpr int r[5] = {4, 2, 3}; };
+template <> struct RadixPlan<48>  { static constexpr int np = 3, e = 12; static constexpr int r[5] = {4, 4, 3}; };
+template <> struct RadixPlan<96>  { static constexpr int np = 4, e = 12; static constexpr int r[5] = {4, 4, 2, 3}; };
+template <> struct RadixPlan<192> { static constexpr int np = 4, e = 12; static constexpr int r[5] = {4, 4, 4, 3}; };
+template <> struct RadixPlan<384> { static constexpr int np = 5, e = 12; static constexpr int r[5] = {4, 4, 4, 2, 3}; };
+template <> struct RadixPlan<40>  { static constexpr int np = 3, e = 20; static constexpr int r[5] = {4, 2, 5}; };
+template <> struct RadixPlan<80>  { static constexpr int np = 3, e = 20; static constexpr int r[5] = {4, 4, 5}; };
+template <> struct RadixPlan<160> { static constexpr int np = 4, e = 20; static constexpr int r[5] = {4, 4, 2, 5}; };
+template <> struct RadixPlan<320> { static constexpr int np = 4, e = 20; static constexpr int r[5] = {4, 4, 4, 5}; };
+
+// Values per thread of a length-N line FFT: the plan's E, or E2 (a multiple of 8) for a
+// power-of-two line whose caller holds more values per thread.
+template <int N, int E2 = 8>
+constexpr int fft_e() {
+  return RadixPlan<N>::e == 8 ? E2 : RadixPlan<N>::e;
+}
 
 // One Stockham pass: butterflies j = t + b*(N/E), b < E/R.
 // in[j + r*N/R] -> twiddle W_N^{(j mod Ns) r N/(Ns R)} -> DFT_R -> out[(j/Ns)Ns R + j mod Ns + r Ns]
@@ -290,21 +352,22 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* base, int ES, int t, bool
 // Full length-N FFT (unnormalised) of the line at base[e*ES], in place in LDS.
 // Every thread of the block must call it (it synchronises); `active` marks the
 // threads that own a (line, team index) pair.
+template <typename T, int N, bool INV, int E, bool PAD, int I>
+__device__ __forceinline__ void fft_passes(cplx<T>* base, int ES, int t, bool active, int Ns,
+                                           const cplx<double>* __restrict__ tw) {
+  using P = RadixPlan<N>;
+  if constexpr (I < P::np) {
+    static_assert(E % P::r[I] == 0, "every radix of the plan divides the values per thread");
+    stockham_pass<T, N, P::r[I], INV, E, PAD>(base, ES, t, active, Ns, tw);
+    fft_passes<T, N, INV, E, PAD, I + 1>(base, ES, t, active, Ns * P::r[I], tw);
+  }
+}
+
 template <typename T, int N, bool INV, int E = 8, bool PAD = false>
 __device__ __forceinline__ void fft_team(cplx<T>* base, int ES, int t, bool active,
                                          const cplx<double>* __restrict__ tw) {
   static_assert(N % E == 0, "line shorter than the per-thread register block");
-  using P = RadixPlan<N>;
-  int Ns = 1;
-  stockham_pass<T, N, P::r[0], INV, E, PAD>(base, ES, t, active, Ns, tw);
-  Ns *= P::r[0];
-  if constexpr (P::np > 1) {
-    stockham_pass<T, N, P::r[1], INV, E, PAD>(base, ES, t, active, Ns, tw);
-    Ns *= P::r[1];
-  }
-  if constexpr (P::np > 2) {
-    stockham_pass<T, N, P::r[2], INV, E, PAD>(base, ES, t, active, Ns, tw);
-  }
+  fft_passes<T, N, INV, E, PAD, 0>(base, ES, t, active, 1, tw);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,6 +37,19 @@ struct Geom {
 constexpr int kWorkSlots = 256;
 constexpr int kWorkStride = 8;
 
+// Line lengths the FFTs are built for: 2^k, 3 * 2^k and 5 * 2^k from 8 to 512 (every one a
+// multiple of 8).  Twiddle table i belongs to kFftSizes[i].
+constexpr int kFftSizes[] = {8, 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512};
+constexpr int kNumFftSizes = 16;
+constexpr int fft_size_id(int n) {
+  for (int i = 0; i < kNumFftSizes; ++i)
+    if (kFftSizes[i] == n) return i;
+  return -1;
+}
+// Values per thread of a line FFT of a built length (the radix plan's E): 12 for 3 * 2^k,
+// 20 for 5 * 2^k, else 8.
+constexpr int line_e(int n) { return n % 3 == 0 ? 12 : n % 5 == 0 ? 20 : 8; }
+
 struct DevTables {
   const double* ktab;     // [3][nmax] fftfreq k values per axis
   const double* xtab;     // [3][nmax] cell-centred coordinates per axis
@@ -45,7 +58,7 @@ struct DevTables {
   const double* Varr;     // [B][nsites] explicit potential or null
   const InstParams* params;  // [B]
   const int* active;      // active instance list
-  const void* tw[10];     // twiddle tables per log2(N): exp(-2 pi i m/N), FP64 for every T
+  const void* tw[kNumFftSizes];  // per line length: exp(-2 pi i m/N), FP64 for every T
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
   // profiling only (else null): kWorkSlots counter pairs, one per 64-byte line, so that the
@@ -96,6 +109,12 @@ template <typename T>
 hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, int axis, int mode,
                           void* data, int n_active, double kin_frac, double dt_override, int itp,
                           int ddi_data, int* nblocks_out);
+
+// 1-D grids: Q_ab product of the DDI half spectra (the step the y/z spectra passes carry
+// on 2-D and 3-D grids).
+template <typename T>
+hipError_t launch_ddi_line(hipStream_t st, const Geom& g, const DevTables& tb, void* fhat,
+                           int n_active);
 
 // Tile shapes (shared by the launchers and the workspace sizing).
 constexpr int strided_tb_c(int N, int fields, int csize) {

@@ -29,9 +29,10 @@ namespace bec {
 // (13-component Euler rotations and the tensor step) keep their spinors in VGPRs.
 template <int D>
 constexpr int xpass_max_threads() { return 512; }
-// Values per FFT thread along x.  With E <= D every tile has at most one site per thread.
+// Values per FFT thread along x (the line's radix plan: 8, 12 or 20).  With E <= D every
+// tile has at most one site per thread.
 template <int NX, int D>
-constexpr int xpass_e() { return 8; }
+constexpr int xpass_e() { return fft_e<NX>(); }
 
 template <typename T, int NX, int D>
 __global__ void __launch_bounds__(xpass_max_threads<D>())
@@ -50,8 +51,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   const int st = prog.stages;
   const bool has_aux = (st & (XS_LOAD_PHI | XS_WRITE_FHAT)) != 0;
   const int TS = ROWS * NX;  // sites in this tile (a power of two, as are the grid sides)
-  const int ts_shift = __ffs(TS) - 1;
-  const int n1_shift = __ffs(g.n[1]) - 1, n1_mask = g.n[1] - 1;
+  const int rows_shift = __ffs(ROWS) - 1;  // rows per block: a power of two
   // LDS tiles are padded (one slot per 8 elements, lds_at<true>): element e of the tile
   // lives at P(e) = e + e/8; lines start at multiples of NX, so P(line*NX + i) =
   // P(line*NX) + P(i).
@@ -60,24 +60,26 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
   // DDI auxiliary lines: 2 per row, of the spectral x length (2 NX when zero-padded)
   const int AL = g.spad ? 2 * NX : NX;
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
-  cplx<T>* aux = tile + (size_t)D * TS / 8 * 9;
+  // the padded tile rounded up to an even element count (odd only for 8-point 1-D lines),
+  // so that what follows starts 16-byte aligned
+  cplx<T>* aux = tile + ((size_t)D * TS / 8 * 9 + 1) / 2 * 2;
   // FP64 Zeeman factors (ROWS*AL/8*9 is even, so this stays 16-byte aligned)
   cplx<double>* zf =
       reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * ROWS * AL / 8 * 9 : 0));
   double* rsh = reinterpret_cast<double*>(zf + 2 * kMaxOps * D);
   cplx<double>* tws = reinterpret_cast<cplx<double>*>(rsh + 16);  // NX twiddles
 
-  const cplx<double>* twg = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
+  const cplx<double>* twg = tw_table<NX>(tb);
   const cplx<double>* tw = tws;
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
 
   // Every global load of the prologue is issued before the first wait: each thread loads
-  // exactly EX tile elements (the launch has D*ROWS*NX/EX threads) and up to three
-  // twiddles (nthr >= 3 NX / 8), so the block waits for one memory latency, not for one
+  // exactly EX tile elements (the launch has D*ROWS*NX/EX threads) and up to EX/3
+  // twiddles (nthr >= 3 NX / EX), so the block waits for one memory latency, not for one
   // per element, and the FFT passes read their twiddles from LDS.
   // (indices are clamped rather than guarded, so the arrays stay in registers)
-  constexpr int TWK = 3;
+  constexpr int TWK = (EX + 2) / 3;
   cplx<double> twv[TWK];
 #pragma unroll
   for (int k = 0; k < TWK; ++k) twv[k] = twg[min(tid + k * nthr, NX - 1)];
@@ -92,7 +94,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
     for (int k = 0; k < EX; ++k) {
       const int e = tid + k * nthr;
-      const int c = e >> ts_shift, r = e & (TS - 1);
+      const int c = (e / NX) >> rows_shift, r = e - c * TS;
       ld[k] = src[ibase + (size_t)c * g.nsites + r];
     }
 #pragma unroll
@@ -146,8 +148,8 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     const int r = s / NX;
     x = s - r * NX;
     const int row = row0 + r;
-    y = row & n1_mask;
-    z = row >> n1_shift;
+    z = div_side(row, g.n[1]);
+    y = row - z * g.n[1];
   };
   auto trap_at = [&](int s) -> double {
     int x, y, z;
@@ -318,7 +320,7 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 #pragma unroll
     for (int k = 0; k < EX; ++k) {
       const int e = tid + k * nthr;
-      const int c = e >> ts_shift, r = e & (TS - 1);
+      const int c = (e / NX) >> rows_shift, r = e - c * TS;
       dst[ibase + (size_t)c * g.nsites + r] = tile[P8(e)];
     }
   }
@@ -513,14 +515,15 @@ __global__ void __launch_bounds__(1024)
 strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, double kin_frac,
                double dt_override, int itp, int ddi_data) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int TEAM = N / 8;
+  constexpr int E = fft_e<N>();  // values per thread (8, 12 or 20)
+  constexpr int TEAM = N / E;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int inst = tb.active[blockIdx.z];
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem);
   double* tabs = reinterpret_cast<double*>(tile + FIELDS * N * TB);  // [2*(N+TB+1)] (re,im) pairs
   double* rsh = tabs + 2 * (N + TB + 1);
   cplx<double>* tws = reinterpret_cast<cplx<double>*>(rsh + 16);  // N twiddles
-  const cplx<double>* twg = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<N>()]);
+  const cplx<double>* twg = tw_table<N>(tb);
   const cplx<double>* tw = tws;
 
   const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
@@ -542,11 +545,11 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   const size_t base = (size_t)inst * istride + (size_t)blockIdx.y * FIELDS * fstride +
                       (size_t)other * ostride + x0;
 
-  // All global loads first (each thread owns exactly 8 tile elements: the launch has
-  // FIELDS*TB*N/8 threads), then the LDS writes: one memory latency per block.
-  cplx<T> ld[8];
+  // All global loads first (each thread owns exactly E tile elements: the launch has
+  // FIELDS*TB*N/E threads), then the LDS writes: one memory latency per block.
+  cplx<T> ld[E];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < E; ++k) {
     const int idx = tid + k * nthr;
     const int xi = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
     const bool in = idx < FIELDS * N * TB && x0 + xi < valid && e < n_in;
@@ -554,15 +557,17 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
     const cplx<T> v = data[base + (in ? (size_t)f * fstride + (size_t)e * estride + xi : 0)];
     ld[k] = in ? v : mk((T)0, (T)0);
   }
-  cplx<double> twv[2];  // N <= 2 * nthr (nthr = FIELDS*TB*N/8 >= N/2); clamped, not guarded
+  // N <= TWK * nthr (nthr = FIELDS*TB*N/E); clamped, not guarded
+  constexpr int TWK = (E + FIELDS * TB - 1) / (FIELDS * TB);
+  cplx<double> twv[TWK];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) twv[k] = twg[min(tid + k * nthr, N - 1)];
-  // unguarded writes (the launch has exactly FIELDS*N*TB/8 threads; a twiddle index past
+  for (int k = 0; k < TWK; ++k) twv[k] = twg[min(tid + k * nthr, N - 1)];
+  // unguarded writes (the launch has exactly FIELDS*N*TB/E threads; a twiddle index past
   // N - 1 rewrites entry N - 1 with the value it holds), so no load is sunk into a branch
 #pragma unroll
-  for (int k = 0; k < 8; ++k) tile[tid + k * nthr] = ld[k];
+  for (int k = 0; k < E; ++k) tile[tid + k * nthr] = ld[k];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) tws[min(tid + k * nthr, N - 1)] = twv[k];
+  for (int k = 0; k < TWK; ++k) tws[min(tid + k * nthr, N - 1)] = twv[k];
   // per-axis phase tables for the kinetic multiply: entries [0, N) along the line, [N, N+TB)
   // for the tile's x columns; the factor of the other transverse axis (y for the z pass) and
   // the 1/N of the inverse transform are folded into the line entries, so each element
@@ -597,7 +602,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   const int xi = tid % TB, rest = tid / TB, tt = rest % TEAM, f = rest / TEAM;
   const bool act = f < FIELDS;
   cplx<T>* lbase = tile + (act ? f : 0) * N * TB + xi;
-  if (MODE != SM_INV) fft_team<T, N, false>(lbase, TB, tt, act, tw);
+  if (MODE != SM_INV) fft_team<T, N, false, E>(lbase, TB, tt, act, tw);
 
   if (MODE == SM_KIN) {
     for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
@@ -619,40 +624,15 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
       double ky, kz;
       if (axis == 2) { ky = tb.sktab[g.smax + other]; kz = tb.sktab[2 * g.smax + e]; }
       else { ky = tb.sktab[g.smax + e]; kz = 0.0; }
-      double k2 = kx * kx;
-      if (g.ndim >= 2) k2 += ky * ky;
-      if (g.ndim >= 3) k2 += kz * kz;
-      double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
-      if (k2 != 0.0) {
-        const double ik = 1.0 / k2;
-        if (g.ddi_secular) {
-          const double q = kz * kz * ik - 1.0 / 3.0;
-          qzz = q; qxx = -q / 2.0; qyy = -q / 2.0;
-        } else {
-          qxx = kx * kx * ik - 1.0 / 3.0;
-          qyy = ky * ky * ik - 1.0 / 3.0;
-          qzz = kz * kz * ik - 1.0 / 3.0;
-          qxy = kx * ky * ik;
-          qxz = kx * kz * ik;
-          qyz = ky * kz * ik;
-        }
-      }
-      const cplx<T> fx = tile[(0 * N + e) * TB + xi2];
-      const cplx<T> fy = tile[(1 * N + e) * TB + xi2];
-      const cplx<T> fz = tile[(2 * N + e) * TB + xi2];
+      cplx<T> fx = tile[(0 * N + e) * TB + xi2];
+      cplx<T> fy = tile[(1 * N + e) * TB + xi2];
+      cplx<T> fz = tile[(2 * N + e) * TB + xi2];
       // normalisation of the inverse transform of the (possibly padded) spectral grid
       const double s = C / ((double)g.sn[0] * (double)g.sn[1] * (double)g.sn[2]);
-      auto mix = [&](double a, double b, double c) -> cplx<T> {
-        const double re = a * (double)fx.x + b * (double)fy.x + c * (double)fz.x;
-        const double im = a * (double)fx.y + b * (double)fy.y + c * (double)fz.y;
-        return mk((T)(re * s), (T)(im * s));
-      };
-      const cplx<T> px = mix(qxx, qxy, qxz);
-      const cplx<T> py = mix(qxy, qyy, qyz);
-      const cplx<T> pz = mix(qxz, qyz, qzz);
-      tile[(0 * N + e) * TB + xi2] = px;
-      tile[(1 * N + e) * TB + xi2] = py;
-      tile[(2 * N + e) * TB + xi2] = pz;
+      ddi_qmix<T>(kx, ky, kz, g.ndim, g.ddi_secular != 0, s, fx, fy, fz);
+      tile[(0 * N + e) * TB + xi2] = fx;
+      tile[(1 * N + e) * TB + xi2] = fy;
+      tile[(2 * N + e) * TB + xi2] = fz;
     }
     __syncthreads();
   } else if (MODE == SM_DERIV) {
@@ -693,7 +673,7 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   }
 
   if (MODE == SM_INV || MODE == SM_KIN || MODE == SM_DDI || MODE == SM_DERIV)
-    fft_team<T, N, true>(lbase, TB, tt, act, tw);
+    fft_team<T, N, true, E>(lbase, TB, tt, act, tw);
 
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
     const int xi2 = idx % TB, rest2 = idx / TB, e = rest2 % N, f2 = rest2 / N;
@@ -889,7 +869,7 @@ static int xpass_tile_elems() {
 
 int xpass_rows(int D, int NX, int nrows) {
   int target = xpass_tile_elems() / (D * NX);
-  const int e = 8;
+  const int e = line_e(NX);
   const int maxthr = 512;
   while (target > 1 && D * target * (NX / e) > maxthr) --target;
   if (target < 1) target = 1;
@@ -902,7 +882,9 @@ int xpass_rows(int D, int NX, int nrows) {
 
 static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int AL) {
   // padded tiles: 9 slots per 8 elements; AL: length of the DDI auxiliary lines
-  return csz * ((size_t)D * rows * NX + (has_aux ? 2 * (size_t)rows * AL : 0)) / 8 * 9 +
+  // (the tile part rounded up to an even element count, as the kernel lays it out)
+  return csz * (((size_t)D * rows * NX / 8 * 9 + 1) / 2 * 2 +
+                (has_aux ? 2 * (size_t)rows * AL / 8 * 9 : 0)) +
          sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double) +
          sizeof(cplx<double>) * NX;  // twiddles
 }
@@ -913,7 +895,7 @@ int xpass_rows_g(const Geom& g) {
 
 bool xpass_rows_ok(const Geom& g, int rows) {
   return rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
-         g.D * rows * (g.n[0] / 8) <= 512;
+         g.D * rows * (g.n[0] / line_e(g.n[0])) <= 512;
 }
 
 template <typename T, int NX, int D>
@@ -939,6 +921,14 @@ template <typename T, int D>
 static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
                           const void* src, void* dst, void* fhat, int n_active, void* phi) {
   switch (g.n[0]) {
+    case 8: return xpass_n<T, 8, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 24: return xpass_n<T, 24, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 40: return xpass_n<T, 40, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 48: return xpass_n<T, 48, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 80: return xpass_n<T, 80, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 96: return xpass_n<T, 96, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 160: return xpass_n<T, 160, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 192: return xpass_n<T, 192, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 16: return xpass_n<T, 16, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 32: return xpass_n<T, 32, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 64: return xpass_n<T, 64, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
@@ -1025,7 +1015,7 @@ static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, 
   const int n_other = strided_other(g, axis, ddi_data);
   const int ngroups = ddi_data ? 1 : g.D / FIELDS;
   dim3 grid(nxch * n_other, ngroups, n_active);
-  const int threads = FIELDS * TB * (N / 8);
+  const int threads = FIELDS * TB * (N / fft_e<N>());
   const size_t lds = sizeof(cplx<T>) * FIELDS * N * TB + sizeof(double) * (2 * (N + TB + 1) + 16) +
                      sizeof(cplx<double>) * N;  // twiddles
   if (lds > 65536)
@@ -1074,6 +1064,16 @@ hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, in
     *nblocks_out = ((valid + tbw - 1) / tbw) * n_other * (ddi_data ? 1 : g.D);
   }
   switch (len) {
+    case 8: return strided_mode<T, 8>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 24: return strided_mode<T, 24>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 40: return strided_mode<T, 40>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 48: return strided_mode<T, 48>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 80: return strided_mode<T, 80>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 96: return strided_mode<T, 96>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 160: return strided_mode<T, 160>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 192: return strided_mode<T, 192>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 320: return strided_mode<T, 320>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 384: return strided_mode<T, 384>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 16: return strided_mode<T, 16>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 32: return strided_mode<T, 32>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 64: return strided_mode<T, 64>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
@@ -1083,6 +1083,35 @@ hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, in
     default: return hipErrorInvalidValue;
   }
 }
+
+// 1-D grids: the x pass leaves the three half spectra (nxh bins each) and no transverse
+// pass follows to carry the k-space product, so it runs on its own, one bin per thread
+// (ddi.jl:108-136 with N = 1: k = k_x, rfftfreq with the Nyquist bin positive).
+template <typename T>
+__global__ void __launch_bounds__(64) ddi_line_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ fhat) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= g.nxh) return;
+  const int inst = tb.active[blockIdx.y];
+  const double kx = (x == g.sn[0] / 2) ? -tb.sktab[g.sn[0] / 2] : tb.sktab[x];
+  const double s = tb.params[inst].cdd / (double)g.sn[0];
+  cplx<T>* f = fhat + (size_t)inst * g.fhat_inst_stride + x;
+  cplx<T> fx = f[0], fy = f[g.fhat_field_stride], fz = f[2 * g.fhat_field_stride];
+  ddi_qmix<T>(kx, 0.0, 0.0, 1, g.ddi_secular != 0, s, fx, fy, fz);
+  f[0] = fx;
+  f[g.fhat_field_stride] = fy;
+  f[2 * g.fhat_field_stride] = fz;
+}
+
+template <typename T>
+hipError_t launch_ddi_line(hipStream_t st, const Geom& g, const DevTables& tb, void* fhat,
+                           int n_active) {
+  if (g.ndim != 1) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((g.nxh + 63) / 64), n_active);
+  hipLaunchKernelGGL((ddi_line_kernel<T>), grid, dim3(64), 0, st, g, tb, (cplx<T>*)fhat);
+  return hipGetLastError();
+}
+template hipError_t launch_ddi_line<float>(hipStream_t, const Geom&, const DevTables&, void*, int);
+template hipError_t launch_ddi_line<double>(hipStream_t, const Geom&, const DevTables&, void*, int);
 
 hipError_t launch_reduce(hipStream_t st, const DevTables& tb, double* red, int n_active,
                          int nblocks, uint32_t slot_mask, uint32_t max_mask) {

@@ -53,11 +53,17 @@ __device__ __forceinline__ double* partial_ptr(const DevTables& tb, int slot, in
   return tb.partials + ((size_t)slot * tb.maxB + inst) * tb.max_blocks + blk;
 }
 
+// Twiddle table exp(-2 pi i m / N), m < N, of a built line length.
 template <int N>
-__host__ __device__ constexpr int ilog2c() {
-  int l = 0, v = N;
-  while (v > 1) { v >>= 1; ++l; }
-  return l;
+__device__ __forceinline__ const cplx<double>* tw_table(const DevTables& tb) {
+  static_assert(fft_size_id(N) >= 0, "no FFT of this length is built");
+  return reinterpret_cast<const cplx<double>*>(tb.tw[fft_size_id(N)]);
+}
+
+// a / n for a grid side n (a shift when n is a power of two; n is uniform over the
+// launch, so the branch is too)
+__device__ __forceinline__ int div_side(int a, int n) {
+  return (n & (n - 1)) == 0 ? a >> (__ffs(n) - 1) : a / n;
 }
 
 // Trap potential at a flat site index (potentials.jl:5-15 via per-axis omega^2 x^2 tables,
@@ -65,11 +71,11 @@ __host__ __device__ constexpr int ilog2c() {
 __device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, int inst,
                                             long long site) {
   if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + site];
-  // grid sides are powers of two and an instance has fewer than 2^31 sites
+  // an instance has fewer than 2^31 sites
   const int sidx = (int)site;
-  const int x = sidx & (g.n[0] - 1);
-  const int row = sidx >> (__ffs(g.n[0]) - 1);
-  const int y = row & (g.n[1] - 1), z = row >> (__ffs(g.n[1]) - 1);
+  const int row = div_side(sidx, g.n[0]);
+  const int x = sidx - row * g.n[0];
+  const int z = div_side(row, g.n[1]), y = row - z * g.n[1];
   const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
   double v = ax[x];
   if (g.ndim >= 2) v += ax[g.nmax + y];
@@ -84,9 +90,9 @@ __device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, 
 __device__ __forceinline__ void raman_phi(const Geom& g, const DevTables& tb, const InstParams& P,
                                           long long site, double& phx, double& phy, double& phz) {
   const int sidx = (int)site;
-  const int x = sidx & (g.n[0] - 1);
-  const int row = sidx >> (__ffs(g.n[0]) - 1);
-  const int y = row & (g.n[1] - 1), z = row >> (__ffs(g.n[1]) - 1);
+  const int row = div_side(sidx, g.n[0]);
+  const int x = sidx - row * g.n[0];
+  const int z = div_side(row, g.n[1]), y = row - z * g.n[1];
   double kr = P.raman[2] * tb.xtab[x];
   if (g.ndim >= 2) kr += P.raman[3] * tb.xtab[g.nmax + y];
   if (g.ndim >= 3) kr += P.raman[4] * tb.xtab[2 * g.nmax + z];
@@ -104,7 +110,8 @@ __device__ __forceinline__ int p8(int e) { return e + (e >> 3); }
 
 // Row of the DDI spectra (y, z) -> y + z*sn[1] for grid row `row` = y + z*n[1].
 __device__ __forceinline__ int spec_row(const Geom& g, int row) {
-  return (row & (g.n[1] - 1)) + (row >> (__ffs(g.n[1]) - 1)) * g.sn[1];
+  const int z = div_side(row, g.n[1]);
+  return (row - z * g.n[1]) + z * g.sn[1];
 }
 
 // C2R input along x of the three DDI potential spectra for the ROWS tile rows from row0
@@ -148,27 +155,29 @@ __device__ __forceinline__ void ddi_lines_in(const Geom& g, const cplx<T>* __res
 }
 
 // FFT of the 2*ROWS auxiliary lines of length AL (NX, or 2 NX with zero padding), in
-// rounds of nthr / (AL/8) lines.  Every thread of the block calls it.
+// rounds of nthr / (AL/E) lines (E values per thread).  Every thread of the block calls it.
 template <typename T, int NX, bool INV>
 __device__ __forceinline__ void ddi_lines_fft(const DevTables& tb, cplx<T>* aux, int ROWS, int AL,
                                               int tid, int nthr, const cplx<double>* tw) {
   if (AL == NX) {
-    constexpr int TEAM = NX / 8;
+    constexpr int E = fft_e<NX>();
+    constexpr int TEAM = NX / E;
     const int lpr = nthr / TEAM, l = tid / TEAM, t = tid - l * TEAM;
     for (int l0 = 0; l0 < 2 * ROWS; l0 += lpr) {
       const int ln = l0 + l;
       const bool act = l < lpr && ln < 2 * ROWS;
-      fft_team<T, NX, INV, 8, true>(aux + p8((act ? ln : 0) * NX), 1, t, act, tw);
+      fft_team<T, NX, INV, E, true>(aux + p8((act ? ln : 0) * NX), 1, t, act, tw);
     }
   } else {
     if constexpr (2 * NX <= 512) {
-      constexpr int TEAM2 = 2 * NX / 8;
-      const cplx<double>* tw2 = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<2 * NX>()]);
+      constexpr int E2 = fft_e<2 * NX>();
+      constexpr int TEAM2 = 2 * NX / E2;
+      const cplx<double>* tw2 = tw_table<2 * NX>(tb);
       const int lpr = nthr / TEAM2, l2 = tid / TEAM2, t2 = tid - l2 * TEAM2;
       for (int l0 = 0; l0 < 2 * ROWS; l0 += lpr) {
         const int ln = l0 + l2;
         const bool act = l2 < lpr && ln < 2 * ROWS;
-        fft_team<T, 2 * NX, INV, 8, true>(aux + p8((act ? ln : 0) * 2 * NX), 1, t2, act, tw2);
+        fft_team<T, 2 * NX, INV, E2, true>(aux + p8((act ? ln : 0) * 2 * NX), 1, t2, act, tw2);
       }
     }
   }
@@ -195,13 +204,50 @@ __device__ __forceinline__ void ddi_lines_out(const Geom& g, cplx<T>* __restrict
   for (int e = tid; e < ROWS * NH; e += nthr) {
     const int r = e / NH, k = e - r * NH;
     const cplx<T> z1 = aux[p8((2 * r) * AL + k)];
-    const cplx<T> z1n = conjc(aux[p8((2 * r) * AL + ((AL - k) & (AL - 1)))]);
+    const cplx<T> z1n = conjc(aux[p8((2 * r) * AL + (k == 0 ? 0 : AL - k))]);
     const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
     const size_t o = fb + (size_t)spec_row(g, row0 + r) * g.nxh_pitch + k;
     fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
     fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
     fhat[o + 2 * g.fhat_field_stride] = aux[p8((2 * r + 1) * AL + k)];
   }
+}
+
+// ddi.jl:7-82 and :108-136 at one spectral point: Q_ab = k_a k_b / k^2 - delta_ab / 3 with
+// Q(0) = 0 (secular: Q_zz = k_z^2 / k^2 - 1/3, Q_xx = Q_yy = -Q_zz / 2, no off-diagonal
+// terms), then (F_x, F_y, F_z) <- s Q (F_x, F_y, F_z) in FP64 with one rounding per output
+// (s = c_dd / number of spectral points, the inverse transform's normalisation).  Axes
+// above the grid's dimension contribute k = 0.
+template <typename T>
+__device__ __forceinline__ void ddi_qmix(double kx, double ky, double kz, int ndim, bool secular,
+                                         double s, cplx<T>& fx, cplx<T>& fy, cplx<T>& fz) {
+  double k2 = kx * kx;
+  if (ndim >= 2) k2 += ky * ky;
+  if (ndim >= 3) k2 += kz * kz;
+  double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
+  if (k2 != 0.0) {
+    const double ik = 1.0 / k2;
+    if (secular) {
+      const double q = kz * kz * ik - 1.0 / 3.0;
+      qzz = q; qxx = -q / 2.0; qyy = -q / 2.0;
+    } else {
+      qxx = kx * kx * ik - 1.0 / 3.0;
+      qyy = ky * ky * ik - 1.0 / 3.0;
+      qzz = kz * kz * ik - 1.0 / 3.0;
+      qxy = kx * ky * ik;
+      qxz = kx * kz * ik;
+      qyz = ky * kz * ik;
+    }
+  }
+  const cplx<T> ax = fx, ay = fy, az = fz;
+  auto mix = [&](double a, double b, double c) -> cplx<T> {
+    const double re = a * (double)ax.x + b * (double)ay.x + c * (double)az.x;
+    const double im = a * (double)ax.y + b * (double)ay.y + c * (double)az.y;
+    return mk((T)(re * s), (T)(im * s));
+  };
+  fx = mix(qxx, qxy, qxz);
+  fy = mix(qxy, qyy, qyz);
+  fz = mix(qxz, qyz, qzz);
 }
 
 }  // namespace bec

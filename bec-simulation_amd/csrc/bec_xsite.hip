@@ -46,7 +46,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
       reinterpret_cast<cplx<double>*>(aux + (has_aux ? 2 * ROWS * AL / 8 * 9 : 0));
   double* rsh = reinterpret_cast<double*>(zf + kMaxOps * D);
   cplx<double>* tws = reinterpret_cast<cplx<double>*>(rsh + 16);  // NX twiddles
-  const cplx<double>* twg = reinterpret_cast<const cplx<double>*>(tb.tw[ilog2c<NX>()]);
+  const cplx<double>* twg = tw_table<NX>(tb);
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
 
@@ -214,7 +214,7 @@ bool xsite_ok(const Geom& g) {
   if (g.D != 5 && g.D != 13) return false;
   if (g.ndim < 2) return false;
   const int nx = g.n[0];
-  if (nx < 16 || nx > 256) return false;
+  if (nx < 16 || nx > 256 || (nx & (nx - 1)) != 0) return false;  // power-of-two lines
   const int ts = xpass_rows_g(g) * nx;
   // one thread per site (tiles below one wave, from short lines or the tile autotuner's
   // choice, leave lanes idle but keep the x pass's block partition)

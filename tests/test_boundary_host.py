@@ -56,9 +56,10 @@ def _cfg(**kw):
     (dict(n=(63, 1, 1)), L.BEC_EINVAL),
     (dict(ndim=4), L.BEC_EINVAL),
     (dict(F=3), L.BEC_EUNSUP),
-    (dict(n=(96, 1, 1)), L.BEC_EUNSUP),
+    (dict(n=(72, 1, 1)), L.BEC_EUNSUP),
+    (dict(n=(320, 1, 1)), L.BEC_EUNSUP),
     (dict(B=0), L.BEC_EINVAL),
-    (dict(ddi=1), L.BEC_EUNSUP),
+    (dict(n=(4, 1, 1)), L.BEC_EUNSUP),
 ])
 def test_create_rejects_invalid_configs(kw, code):
     c, buf = _cfg(**kw)

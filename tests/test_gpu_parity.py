@@ -500,7 +500,7 @@ def test_run_simulation_adaptive(mode, n, L, F):
         assert rel_l2(got[c], ows.psi[c]) <= PSI_TOL
 
 
-@pytest.mark.parametrize("nx", [16, 32])
+@pytest.mark.parametrize("nx", [8, 16, 32])
 @pytest.mark.parametrize("F,itp", [(1, False), (1, True), (2, False)])
 def test_short_1d_lines(nx, F, itp):
     """1-D lines of 16 and 32 points run x-pass blocks with fewer threads than op slots

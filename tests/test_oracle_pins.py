@@ -553,6 +553,26 @@ def test_ddi_potential_brute_force():
         assert np.allclose(phi[a], ref[a], atol=1e-12)
 
 
+@pytest.mark.parametrize("secular", [False, True])
+def test_ddi_potential_1d_closed_form(secular):
+    """ddi.jl:51-80 with N = 1 (k_y = k_z = 0): off k = 0 the full tensor is
+    diag(2/3, -1/3, -1/3), the secular one diag(1/6, 1/6, -1/3), and the k = 0 mode is
+    dropped, so single Fourier modes come back scaled (Nyquist mode included)."""
+    n, L = 16, 7.0
+    x = np.arange(n) * (L / n)
+    k1 = 2 * math.pi / L
+    fx = np.cos(k1 * x)
+    fy = 0.5 * np.sin(2 * k1 * x) + 0.25
+    fz = 0.3 + np.cos(8 * k1 * x)            # the Nyquist mode
+    Q = O.ddi_q((n,), (L / n,), secular=secular)
+    px, py, pz = O.ddi_potential(fx, fy, fz, Q, 1.7)
+    qa, qz = (1 / 6, -1 / 3) if secular else (2 / 3, -1 / 3)
+    qb = qa if secular else -1 / 3
+    assert np.allclose(px, 1.7 * qa * np.cos(k1 * x), atol=1e-13)
+    assert np.allclose(py, 1.7 * qb * 0.5 * np.sin(2 * k1 * x), atol=1e-13)
+    assert np.allclose(pz, 1.7 * qz * np.cos(8 * k1 * x), atol=1e-13)
+
+
 def test_ddi_potential_padded_brute_force():
     """ddi_padded.jl:6-85: F in the first n entries of a (2n)^3 zero grid (same dx, so
     dk is halved), the explicit c2r convolution there, Phi cropped to the first n."""
