@@ -43,7 +43,7 @@ from .physics import (GridConfig, HarmonicTrap, InteractionParams, SimParams, Sp
 class Request:
     rid: int
     kind: str                 # "itp" or "rt"
-    n: tuple                  # grid points per axis (powers of two, 16..512)
+    n: tuple                  # grid points per axis (2^k 8..512, 3*2^k 24..192, 5*2^k 40..160)
     L: tuple                  # box length per axis
     F: int
     c0: float
@@ -333,8 +333,14 @@ def solve_alone(r: Request, keep_psi=True):
     return res[0]
 
 
-def poisson_mix(n_req, *, rate=200.0, seed=42, steps=(500, 5000), sides12=(64, 128, 256),
-                sides3=(64, 128), precision="fp64"):
+# Built grid sides inside the stream's ranges ({64 ... 256} for 1-D and 2-D requests,
+# {48 ... 128} for 3-D): 2^k, 3 * 2^k and 5 * 2^k.
+SIDES12 = (64, 80, 96, 128, 160, 192, 256)
+SIDES3 = (48, 64, 80, 96, 128)
+
+
+def poisson_mix(n_req, *, rate=200.0, seed=42, steps=(500, 5000), sides12=SIDES12,
+                sides3=SIDES3, precision="fp64"):
     """A seeded request stream: exponential inter-arrival times of mean 1/rate; dimension
     1/2/3 with probability 0.5/0.35/0.15; F = 1/2/6 with probability 0.6/0.25/0.15; side
     uniform over `sides12` (1-D, 2-D) or `sides3` (3-D); c0 ~ U[5, 20], c1 ~ U[-1, 1];

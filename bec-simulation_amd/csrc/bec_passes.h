@@ -60,22 +60,26 @@ __device__ __forceinline__ const cplx<double>* tw_table(const DevTables& tb) {
   return reinterpret_cast<const cplx<double>*>(tb.tw[fft_size_id(N)]);
 }
 
-// a / n for a grid side n (a shift when n is a power of two; n is uniform over the
-// launch, so the branch is too)
+// a / n for a grid side n: a shift when n is a power of two (POW2: known to be, as in the
+// fused x/site kernel, which only runs such grids), else a division (n is uniform over
+// the launch, so the branch is too)
+template <bool POW2 = false>
 __device__ __forceinline__ int div_side(int a, int n) {
-  return (n & (n - 1)) == 0 ? a >> (__ffs(n) - 1) : a / n;
+  if constexpr (POW2) return a >> (__ffs(n) - 1);
+  else return (n & (n - 1)) == 0 ? a >> (__ffs(n) - 1) : a / n;
 }
 
 // Trap potential at a flat site index (potentials.jl:5-15 via per-axis omega^2 x^2 tables,
 // or an explicit potential array).
+template <bool POW2 = false>
 __device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, int inst,
                                             long long site) {
   if (tb.Varr) return tb.Varr[(size_t)inst * g.nsites + site];
   // an instance has fewer than 2^31 sites
   const int sidx = (int)site;
-  const int row = div_side(sidx, g.n[0]);
+  const int row = div_side<POW2>(sidx, g.n[0]);
   const int x = sidx - row * g.n[0];
-  const int z = div_side(row, g.n[1]), y = row - z * g.n[1];
+  const int z = div_side<POW2>(row, g.n[1]), y = row - z * g.n[1];
   const double* ax = tb.axtab + (size_t)inst * 3 * g.nmax;
   double v = ax[x];
   if (g.ndim >= 2) v += ax[g.nmax + y];
@@ -87,12 +91,13 @@ __device__ __forceinline__ double trap_site(const Geom& g, const DevTables& tb, 
 // index is phi . F with phi = (Omega_R cos k.r, -Omega_R sin k.r, delta)
 // (F+- = Fx +- i Fy), so exp(-i H_R dt) is the Euler rotation by phi; k.r is summed over the
 // axes in order in FP64.
+template <bool POW2 = false>
 __device__ __forceinline__ void raman_phi(const Geom& g, const DevTables& tb, const InstParams& P,
                                           long long site, double& phx, double& phy, double& phz) {
   const int sidx = (int)site;
-  const int row = div_side(sidx, g.n[0]);
+  const int row = div_side<POW2>(sidx, g.n[0]);
   const int x = sidx - row * g.n[0];
-  const int z = div_side(row, g.n[1]), y = row - z * g.n[1];
+  const int z = div_side<POW2>(row, g.n[1]), y = row - z * g.n[1];
   double kr = P.raman[2] * tb.xtab[x];
   if (g.ndim >= 2) kr += P.raman[3] * tb.xtab[g.nmax + y];
   if (g.ndim >= 3) kr += P.raman[4] * tb.xtab[2 * g.nmax + z];
@@ -109,8 +114,9 @@ __device__ __forceinline__ void raman_phi(const Geom& g, const DevTables& tb, co
 __device__ __forceinline__ int p8(int e) { return e + (e >> 3); }
 
 // Row of the DDI spectra (y, z) -> y + z*sn[1] for grid row `row` = y + z*n[1].
+template <bool POW2 = false>
 __device__ __forceinline__ int spec_row(const Geom& g, int row) {
-  const int z = div_side(row, g.n[1]);
+  const int z = div_side<POW2>(row, g.n[1]);
   return (row - z * g.n[1]) + z * g.sn[1];
 }
 
@@ -120,11 +126,11 @@ __device__ __forceinline__ int spec_row(const Geom& g, int row) {
 // Phi_x + i Phi_y, line 2r+1 holds Phi_z (its inverse FFT is real).
 // Element e = r*AL + x of the lines in two halves: ddi_phi_fetch reads its three spectral
 // bins (so a caller can issue the loads early), ddi_phi_place writes the line entries.
-template <typename T>
+template <typename T, bool POW2 = false>
 __device__ __forceinline__ void ddi_phi_fetch(const Geom& g, const cplx<T>* __restrict__ fhat,
                                               size_t fb, int row0, int AL, int e, cplx<T>* v) {
   const int r = e / AL, x = e - r * AL;
-  const size_t rowoff = (size_t)spec_row(g, row0 + r) * g.nxh_pitch;
+  const size_t rowoff = (size_t)spec_row<POW2>(g, row0 + r) * g.nxh_pitch;
   const int kk = x <= AL / 2 ? x : AL - x;
   v[0] = fhat[fb + rowoff + kk];
   v[1] = fhat[fb + g.fhat_field_stride + rowoff + kk];
@@ -196,7 +202,7 @@ __device__ __forceinline__ void ddi_lines_zero_upper(cplx<T>* aux, int ROWS, int
 // R2C output along x of the spin density (after the forward FFT of the auxiliary lines):
 // the two-for-one split of line 2r into the spectra of F_x and F_y, line 2r+1 is F_z's;
 // half rows of AL/2 + 1 bins into the DDI spectra buffer.
-template <typename T>
+template <typename T, bool POW2 = false>
 __device__ __forceinline__ void ddi_lines_out(const Geom& g, cplx<T>* __restrict__ fhat, size_t fb,
                                               int row0, int ROWS, int AL, const cplx<T>* aux,
                                               int tid, int nthr) {
@@ -206,7 +212,7 @@ __device__ __forceinline__ void ddi_lines_out(const Geom& g, cplx<T>* __restrict
     const cplx<T> z1 = aux[p8((2 * r) * AL + k)];
     const cplx<T> z1n = conjc(aux[p8((2 * r) * AL + (k == 0 ? 0 : AL - k))]);
     const cplx<T> sum = z1 + z1n, dif = z1 - z1n;
-    const size_t o = fb + (size_t)spec_row(g, row0 + r) * g.nxh_pitch + k;
+    const size_t o = fb + (size_t)spec_row<POW2>(g, row0 + r) * g.nxh_pitch + k;
     fhat[o] = mk(sum.x * (T)0.5, sum.y * (T)0.5);
     fhat[o + g.fhat_field_stride] = mk(dif.y * (T)0.5, -dif.x * (T)0.5);
     fhat[o + 2 * g.fhat_field_stride] = aux[p8((2 * r + 1) * AL + k)];

@@ -63,7 +63,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
     cplx<T> ph[2][3];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
-      if (k < nph) ddi_phi_fetch<T>(g, fhat, fb, row0, AL, tid + k * TS, ph[k]);
+      if (k < nph) ddi_phi_fetch<T, true>(g, fhat, fb, row0, AL, tid + k * TS, ph[k]);
     tws[min(tid, NX - 1)] = twv;
 #pragma unroll
     for (int c = 0; c < D; ++c) tile[p8(c * TS + tid)] = ld[c];
@@ -129,7 +129,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
       const Op op = s_ops[i];
       const double dtf = op.frac * dt;
       if (op.kind == OP_DIAG) {
-        op_diag<T, D>(sp, trap_site(g, tb, inst, site), P, dtf, itp, &zf[i * D]);
+        op_diag<T, D>(sp, trap_site<true>(g, tb, inst, site), P, dtf, itp, &zf[i * D]);
       } else if (op.kind == OP_SPINMIX) {
         nrot += op_spin_mixing<T, D>(sp, P, dtf, itp);
       } else if (op.kind == OP_DDI_ROT) {
@@ -143,7 +143,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
       } else if (op.kind == OP_RAMAN) {
         if (P.has_raman) {
           double phx, phy, phz;
-          raman_phi(g, tb, P, site, phx, phy, phz);
+          raman_phi<true>(g, tb, P, site, phx, phy, phz);
           nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
         }
       } else if (op.kind == OP_TENSOR) {
@@ -174,7 +174,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
     if (AL > NX) ddi_lines_zero_upper<T, NX>(aux, ROWS, AL, tid, TS);
     __syncthreads();
     ddi_lines_fft<T, NX, false>(tb, aux, ROWS, AL, tid, TS, tws);
-    ddi_lines_out<T>(g, fhat, fb, row0, ROWS, AL, aux, tid, TS);
+    ddi_lines_out<T, true>(g, fhat, fb, row0, ROWS, AL, aux, tid, TS);
   }
   if (st & XS_FWD_FFT) {
     __syncthreads();
@@ -214,7 +214,10 @@ bool xsite_ok(const Geom& g) {
   if (g.D != 5 && g.D != 13) return false;
   if (g.ndim < 2) return false;
   const int nx = g.n[0];
-  if (nx < 16 || nx > 256 || (nx & (nx - 1)) != 0) return false;  // power-of-two lines
+  if (nx < 16 || nx > 256) return false;
+  // power-of-two sides on every axis: site coordinates and spectra rows by shifts
+  for (int d = 0; d < 3; ++d)
+    if ((g.n[d] & (g.n[d] - 1)) != 0) return false;
   const int ts = xpass_rows_g(g) * nx;
   // one thread per site (tiles below one wave, from short lines or the tile autotuner's
   // choice, leave lanes idle but keep the x pass's block partition)

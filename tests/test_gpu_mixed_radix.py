@@ -82,9 +82,10 @@ def test_3d(n, L, F, itp):
 
 
 @pytest.mark.parametrize("padding", [False, True])
-def test_spin6_fused_kernel_with_mixed_transverse_sides(padding):
-    """x side a power of two (the fused x/site kernel runs) with y, z sides of 48 and 24:
-    site coordinates and DDI spectra rows by division; fused == split bit for bit."""
+def test_spin6_mixed_transverse_sides(padding):
+    """x side a power of two with y, z sides of 48 and 24: the fused x/site kernel (built for
+    power-of-two grids) declines the shape, and the split passes take site coordinates and
+    DDI spectra rows by division."""
     n, L = (32, 48, 24), (12.0, 14.0, 10.0)
     og, _ = grids(n, L)
     psi = _psi(n, L, 6, 9)
@@ -92,12 +93,15 @@ def test_spin6_fused_kernel_with_mixed_transverse_sides(padding):
                    ddi=True, cdd=1.0, padding=padding)
     _, gs = pair(n, L, 6, psi, 1e-3, False, omega=(1.0, 1.0, 1.0), c0=10.0, c_extra=_cextra(4),
                  ddi=True, cdd=1.0, padding=padding)
-    assert gf.set_xsite(True)
+    assert not gf.set_xsite(True)
     gs.set_xsite(False)
+    gf.profile(True)
     for _ in range(2):
         O.split_step(ows)
     A.split_step(gf, 2)
     A.split_step(gs, 2)
+    launches = {k: v[1] for k, v in gf.profile_read().items()}
+    assert launches["xsite"] == 0 and launches["sites"] > 0, launches
     a, b = gf.get_psi(), gs.get_psi()
     assert np.array_equal(a, b), float(np.max(np.abs(a - b)))
     assert rel_l2(a, ows.psi) <= PSI_TOL

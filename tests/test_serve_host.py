@@ -1,7 +1,7 @@
 """Host side of the request server: the seeded request stream (no device calls)."""
 import numpy as np
 
-from becgpu.serve import Request, _pool_key, poisson_mix
+from becgpu.serve import SIDES12, SIDES3, Request, _pool_key, poisson_mix
 
 
 def test_poisson_mix_is_seeded_and_shaped():
@@ -25,7 +25,7 @@ def test_poisson_mix_is_seeded_and_shaped():
         assert 500 <= r.n_steps <= 5000
         assert 5.0 <= r.c0 <= 20.0 and -1.0 <= r.c1 <= 1.0
         assert len(set(r.n)) == 1
-        assert r.n[0] in ((64, 128) if len(r.n) == 3 else (64, 128, 256))
+        assert r.n[0] in (SIDES3 if len(r.n) == 3 else SIDES12)
         assert r.dt == (0.005 if r.kind == "itp" else 0.001)
     kinds = np.array([r.kind == "itp" for r in a])
     assert abs(kinds.mean() - 0.5) < 5 * np.sqrt(0.25 / len(a))
