@@ -691,8 +691,11 @@ static int xpass_tile_elems() {
 }
 
 int xpass_rows(int D, int NX, int nrows) {
-  int target = xpass_tile_elems() / (D * NX);
+  // the element budget is for 8 values per thread; lines with E = 12 or 20 take up to twice
+  // it so that their blocks keep about the same thread count (at most twice: an FP64 spin-1
+  // tile with zero-padded DDI lines then still fits in LDS, about 130 KB)
   const int e = line_e(NX);
+  int target = xpass_tile_elems() * std::min(e, 16) / 8 / (D * NX);
   const int maxthr = 512;
   while (target > 1 && D * target * (NX / e) > maxthr) --target;
   if (target < 1) target = 1;
@@ -717,8 +720,11 @@ int xpass_rows_g(const Geom& g) {
 }
 
 bool xpass_rows_ok(const Geom& g, int rows) {
+  // the LDS bound is taken for FP64 with the DDI lines, so one choice fits every program
+  const int AL = g.spad ? 2 * g.n[0] : g.n[0];
   return rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
-         g.D * rows * (g.n[0] / line_e(g.n[0])) <= 512;
+         g.D * rows * (g.n[0] / line_e(g.n[0])) <= 512 &&
+         xpass_lds(g.D, rows, g.n[0], sizeof(cplx<double>), true, AL) <= 160 * 1024;
 }
 
 template <typename T, int NX, int D>
