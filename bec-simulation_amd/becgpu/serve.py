@@ -43,7 +43,8 @@ from .physics import (GridConfig, HarmonicTrap, InteractionParams, SimParams, Sp
 class Request:
     rid: int
     kind: str                 # "itp" or "rt"
-    n: tuple                  # grid points per axis (2^k 8..512, 3*2^k 24..192, 5*2^k 40..160)
+    n: tuple                  # grid points per axis (2^k 8..512, 3*2^k 24..192, 5*2^k 40..160,
+                              # 7*2^k 56..224)
     L: tuple                  # box length per axis
     F: int
     c0: float
@@ -334,9 +335,10 @@ def solve_alone(r: Request, keep_psi=True):
 
 
 # Built grid sides inside the stream's ranges ({64 ... 256} for 1-D and 2-D requests,
-# {48 ... 128} for 3-D): 2^k, 3 * 2^k and 5 * 2^k.
-SIDES12 = (64, 80, 96, 128, 160, 192, 256)
-SIDES3 = (48, 64, 80, 96, 128)
+# {48 ... 128} for 3-D): 2^k, 3 * 2^k, 5 * 2^k and 7 * 2^k (every multiple of 16 in the
+# 3-D range).
+SIDES12 = (64, 80, 96, 112, 128, 160, 192, 224, 256)
+SIDES3 = (48, 64, 80, 96, 112, 128)
 
 
 def poisson_mix(n_req, *, rate=200.0, seed=42, steps=(500, 5000), sides12=SIDES12,

@@ -124,8 +124,8 @@ static void prof_end(bec_ws* ws, int idx) {
 // ---------------------------------------------------------------------------
 static constexpr size_t kWorkBytes = sizeof(unsigned long long) * kWorkSlots * kWorkStride;
 
-// Grid sides with built x-line FFTs: 2^k (8..512), 3 * 2^k (24..192), 5 * 2^k (40..160).
-// (Zero-padded DDI lines are twice the side: up to 384 and 320 for the latter two.)
+// Grid sides with built x-line FFTs: 2^k (8..512), 3 * 2^k (24..192), 5 * 2^k (40..160),
+// 7 * 2^k (56..224).  (Zero-padded DDI lines are twice the side and must be built too.)
 static bool side_supported(int n) {
   if (n == 320 || n == 384) return false;
   return fft_size_id(n) >= 0;
@@ -621,7 +621,7 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
       return fail(BEC_EINVAL, "n_points must be positive even integers");
     if (!(c->L[d] > 0)) return fail(BEC_EINVAL, "box_size must be positive");
     if (!side_supported(c->n[d]))
-      return fail(BEC_EUNSUP, "grid sides must be 2^k in [8, 512], 3*2^k in [24, 192] or 5*2^k in [40, 160] on this build");
+      return fail(BEC_EUNSUP, "grid sides must be 2^k in [8, 512], 3*2^k in [24, 192], 5*2^k in [40, 160] or 7*2^k in [56, 224] on this build");
   }
   if (!(c->F == 1 || c->F == 2 || c->F == 6)) return fail(BEC_EUNSUP, "F must be 1, 2 or 6");
   if (c->F == 6 && c->n[0] > 256) return fail(BEC_EUNSUP, "F=6 supports x sides up to 256");
@@ -631,8 +631,8 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (c->ddi_padding && !c->enable_ddi) return fail(BEC_EINVAL, "ddi_padding needs enable_ddi");
   if (c->ddi_padding)
     for (int d = 0; d < c->ndim; ++d)
-      if (c->n[d] > 256)
-        return fail(BEC_EUNSUP, "zero-padded DDI supports sides up to 256 (padded FFTs <= 512)");
+      if (c->n[d] > 256 || fft_size_id(2 * c->n[d]) < 0)
+        return fail(BEC_EUNSUP, "zero-padded DDI needs built lines of twice the side (sides up to 256, not 224)");
   if (!c->x_tab || !c->k_tab) return fail(BEC_EINVAL, "missing tables");
 
   bec_ws* ws = new bec_ws();

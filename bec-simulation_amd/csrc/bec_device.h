@@ -258,20 +258,58 @@ __device__ __forceinline__ void dft5(cplx<T>* v) {
   v[3] = mk((T)(m2x - t2x), (T)(m2y - t2y));
 }
 
+// Radix 7, the same symmetric form: a_k = v_k + v_{7-k}, b_k = v_k - v_{7-k} (k = 1..3),
+// y_j = v0 + sum_k cos(2 pi jk/7) a_k -+ i sum_k sin(2 pi jk/7) b_k, y_{7-j} with the other
+// sign (jk reduced mod 7: the cosines are C1..C3 permuted, the sines +-S1..S3).
+template <typename T, bool INV>
+__device__ __forceinline__ void dft7(cplx<T>* v) {
+  constexpr double C1 = 0.62348980185873353053, C2 = -0.22252093395631440429,
+                   C3 = -0.90096886790241912624;
+  constexpr double S1 = 0.78183148246802980871, S2 = 0.97492791218182360702,
+                   S3 = 0.43388373911755812048;
+  const double x0 = v[0].x, y0 = v[0].y;
+  double ax[4], ay[4], bx[4], by[4];
+#pragma unroll
+  for (int k = 1; k <= 3; ++k) {
+    ax[k] = (double)v[k].x + (double)v[7 - k].x;
+    ay[k] = (double)v[k].y + (double)v[7 - k].y;
+    bx[k] = (double)v[k].x - (double)v[7 - k].x;
+    by[k] = (double)v[k].y - (double)v[7 - k].y;
+  }
+  constexpr double cc[3][3] = {{C1, C2, C3}, {C2, C3, C1}, {C3, C1, C2}};
+  constexpr double ss[3][3] = {{S1, S2, S3}, {S2, -S3, -S1}, {S3, -S1, S2}};
+  cplx<T> out[7];
+  out[0] = mk((T)(x0 + ax[1] + ax[2] + ax[3]), (T)(y0 + ay[1] + ay[2] + ay[3]));
+#pragma unroll
+  for (int j = 1; j <= 3; ++j) {
+    const double mx = x0 + cc[j - 1][0] * ax[1] + cc[j - 1][1] * ax[2] + cc[j - 1][2] * ax[3];
+    const double my = y0 + cc[j - 1][0] * ay[1] + cc[j - 1][1] * ay[2] + cc[j - 1][2] * ay[3];
+    const double nx = ss[j - 1][0] * bx[1] + ss[j - 1][1] * bx[2] + ss[j - 1][2] * bx[3];
+    const double ny = ss[j - 1][0] * by[1] + ss[j - 1][1] * by[2] + ss[j - 1][2] * by[3];
+    // -i n (forward) or +i n (inverse)
+    const double tx = INV ? -ny : ny, ty = INV ? nx : -nx;
+    out[j] = mk((T)(mx + tx), (T)(my + ty));
+    out[7 - j] = mk((T)(mx - tx), (T)(my - ty));
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) v[k] = out[k];
+}
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dftR(cplx<T>* v) {
   if constexpr (R == 2) dft2<T, INV>(v[0], v[1]);
   else if constexpr (R == 3) dft3<T, INV>(v);
   else if constexpr (R == 4) dft4<T, INV>(v[0], v[1], v[2], v[3]);
   else if constexpr (R == 5) dft5<T, INV>(v);
+  else if constexpr (R == 7) dft7<T, INV>(v);
   else dft8<T, INV>(v);
 }
 
-// Radix plan for a length-N line: radices (2, 3, 4, 5 or 8) whose product is N, and E,
+// Radix plan for a length-N line: radices (2, 3, 4, 5, 7 or 8) whose product is N, and E,
 // the values each thread holds (N/E threads per line); every radix divides E.  Lines of
 // 2^k points use radix 8 first and E = 8 (or 16 where a caller asks for it); lines of
-// 3 * 2^k and 5 * 2^k points (24 ... 384) use radix 4 passes with one radix-3 or radix-5
-// pass and E = 12 or 20.
+// 3 * 2^k, 5 * 2^k and 7 * 2^k points (24 ... 384) use radix 4 passes with one radix-3, -5
+// or -7 pass and E = 12, 20 or 28.
 template <int N>
 struct RadixPlan;
 template <> struct RadixPlan<8>   { static constexpr int np = 1, e = 8;  static constexpr int r[5] = {8}; };
@@ -290,6 +328,9 @@ template <> struct RadixPlan<40>  { static constexpr int np = 3, e = 20; static 
 template <> struct RadixPlan<80>  { static constexpr int np = 3, e = 20; static constexpr int r[5] = {4, 4, 5}; };
 template <> struct RadixPlan<160> { static constexpr int np = 4, e = 20; static constexpr int r[5] = {4, 4, 2, 5}; };
 template <> struct RadixPlan<320> { static constexpr int np = 4, e = 20; static constexpr int r[5] = {4, 4, 4, 5}; };
+template <> struct RadixPlan<56>  { static constexpr int np = 3, e = 28; static constexpr int r[5] = {4, 2, 7}; };
+template <> struct RadixPlan<112> { static constexpr int np = 3, e = 28; static constexpr int r[5] = {4, 4, 7}; };
+template <> struct RadixPlan<224> { static constexpr int np = 4, e = 28; static constexpr int r[5] = {4, 4, 2, 7}; };
 
 // Values per thread of a length-N line FFT: the plan's E, or E2 (a multiple of 8) for a
 // power-of-two line whose caller holds more values per thread.

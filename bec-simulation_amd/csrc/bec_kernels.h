@@ -37,18 +37,19 @@ struct Geom {
 constexpr int kWorkSlots = 256;
 constexpr int kWorkStride = 8;
 
-// Line lengths the FFTs are built for: 2^k, 3 * 2^k and 5 * 2^k from 8 to 512 (every one a
-// multiple of 8).  Twiddle table i belongs to kFftSizes[i].
-constexpr int kFftSizes[] = {8, 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512};
-constexpr int kNumFftSizes = 16;
+// Line lengths the FFTs are built for: 2^k, 3 * 2^k, 5 * 2^k and 7 * 2^k from 8 to 512
+// (every one a multiple of 8).  Twiddle table i belongs to kFftSizes[i].
+constexpr int kFftSizes[] = {8,  16,  24,  32,  40,  48,  56,  64,  80,  96,
+                             112, 128, 160, 192, 224, 256, 320, 384, 512};
+constexpr int kNumFftSizes = 19;
 constexpr int fft_size_id(int n) {
   for (int i = 0; i < kNumFftSizes; ++i)
     if (kFftSizes[i] == n) return i;
   return -1;
 }
 // Values per thread of a line FFT of a built length (the radix plan's E): 12 for 3 * 2^k,
-// 20 for 5 * 2^k, else 8.
-constexpr int line_e(int n) { return n % 3 == 0 ? 12 : n % 5 == 0 ? 20 : 8; }
+// 20 for 5 * 2^k, 28 for 7 * 2^k, else 8.
+constexpr int line_e(int n) { return n % 3 == 0 ? 12 : n % 5 == 0 ? 20 : n % 7 == 0 ? 28 : 8; }
 
 struct DevTables {
   const double* ktab;     // [3][nmax] fftfreq k values per axis

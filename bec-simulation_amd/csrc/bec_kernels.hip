@@ -690,10 +690,12 @@ static int xpass_tile_elems() {
   return v;
 }
 
+static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int AL);
+
 int xpass_rows(int D, int NX, int nrows) {
-  // the element budget is for 8 values per thread; lines with E = 12 or 20 take up to twice
-  // it so that their blocks keep about the same thread count (at most twice: an FP64 spin-1
-  // tile with zero-padded DDI lines then still fits in LDS, about 130 KB)
+  // the element budget is for 8 values per thread; lines with E = 12, 20 or 28 take up to
+  // twice it so that their blocks keep about the same thread count (at most twice: an FP64
+  // spin-1 tile with zero-padded DDI lines then still fits in LDS, about 130 KB)
   const int e = line_e(NX);
   int target = xpass_tile_elems() * std::min(e, 16) / 8 / (D * NX);
   const int maxthr = 512;
@@ -703,6 +705,8 @@ int xpass_rows(int D, int NX, int nrows) {
   while (r * 2 <= target) r *= 2;
   while (r > 1 && (nrows % r) != 0) r /= 2;
   if (r > nrows) r = nrows;
+  // LDS bound for the largest program (FP64, zero-padded DDI lines)
+  while (r > 1 && xpass_lds(D, r, NX, 16, true, 2 * NX) > 160 * 1024) r /= 2;
   return r;
 }
 
@@ -758,6 +762,9 @@ static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, co
     case 96: return xpass_n<T, 96, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 160: return xpass_n<T, 160, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 192: return xpass_n<T, 192, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 56: return xpass_n<T, 56, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 112: return xpass_n<T, 112, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
+    case 224: return xpass_n<T, 224, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 16: return xpass_n<T, 16, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 32: return xpass_n<T, 32, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
     case 64: return xpass_n<T, 64, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);

@@ -175,7 +175,7 @@ __device__ __forceinline__ void ddi_lines_fft(const DevTables& tb, cplx<T>* aux,
       fft_team<T, NX, INV, E, true>(aux + p8((act ? ln : 0) * NX), 1, t, act, tw);
     }
   } else {
-    if constexpr (2 * NX <= 512) {
+    if constexpr (fft_size_id(2 * NX) >= 0) {  // padded lines of a built length only
       constexpr int E2 = fft_e<2 * NX>();
       constexpr int TEAM2 = 2 * NX / E2;
       const cplx<double>* tw2 = tw_table<2 * NX>(tb);

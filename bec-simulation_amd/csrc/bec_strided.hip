@@ -266,6 +266,9 @@ hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, in
     case 96: return strided_mode<T, 96>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 160: return strided_mode<T, 160>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 192: return strided_mode<T, 192>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 56: return strided_mode<T, 56>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 112: return strided_mode<T, 112>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
+    case 224: return strided_mode<T, 224>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 320: return strided_mode<T, 320>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 384: return strided_mode<T, 384>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 16: return strided_mode<T, 16>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);

@@ -49,8 +49,9 @@ typedef struct bec_ws bec_ws;
 typedef struct bec_config {
   int ndim;            /* 1, 2 or 3 */
   int n[3];            /* grid points per axis (x, y, z); unused axes 1.  Built: 2^k in
-                          [8, 512], 3*2^k in [24, 192], 5*2^k in [40, 160]; other even
-                          sides (valid in the reference) return BEC_EUNSUP */
+                          [8, 512], 3*2^k in [24, 192], 5*2^k in [40, 160], 7*2^k in
+                          [56, 224]; other even sides (valid in the reference) return
+                          BEC_EUNSUP */
   double L[3];         /* box size per axis */
   int F;               /* spin (2F+1 components); F in {1, 2, 6} */
   int n_instances;     /* batch size B */

@@ -1,9 +1,9 @@
-"""Grid sides of 3 * 2^k and 5 * 2^k points (radix-3 and radix-5 Stockham passes) against the
-FP64 oracle.
+"""Grid sides of 3 * 2^k, 5 * 2^k and 7 * 2^k points (radix-3, -5 and -7 Stockham passes)
+against the FP64 oracle.
 
 The reference takes any positive even side (grid.jl: GridConfig validation, FFTW plans of
-any length).  Built here: 2^k (8..512), 3 * 2^k (24..192) and 5 * 2^k (40..160) sides, with
-zero-padded DDI lines of twice the side (up to 384 and 320).  Sides are mixed per axis so
+any length).  Built here: 2^k (8..512), 3 * 2^k (24..192), 5 * 2^k (40..160) and 7 * 2^k
+(56..224) sides, with zero-padded DDI lines of twice the side (up to 384, 320 and 224).  Sides are mixed per axis so
 that a swapped axis or a wrong radix order fails; the FFT itself is checked through the
 spectral gradient (every mode, Nyquist included) to 1e-12.
 """
@@ -52,7 +52,7 @@ def _check(ows, gws, nsteps, psi_tol=PSI_TOL, e_tol=E_TOL):
     assert de <= e_tol
 
 
-@pytest.mark.parametrize("n", [24, 40, 48, 80, 96, 160, 192])
+@pytest.mark.parametrize("n", [24, 40, 48, 56, 80, 96, 112, 160, 192, 224])
 def test_1d_lines(n):
     L = (16.0,)
     psi = _psi((n,), L, 1, n)
@@ -73,7 +73,9 @@ def test_2d(n, L, F, itp):
 
 @pytest.mark.parametrize("n,L,F,itp", [((48, 40, 24), (12.0, 10.0, 8.0), 1, False),
                                        ((24, 48, 80), (8.0, 12.0, 16.0), 2, True),
-                                       ((40, 24, 16), (10.0, 8.0, 6.0), 6, False)])
+                                       ((40, 24, 16), (10.0, 8.0, 6.0), 6, False),
+                                       ((112, 56, 48), (16.0, 10.0, 9.0), 1, True),
+                                       ((56, 16, 112), (10.0, 6.0, 16.0), 2, False)])
 def test_3d(n, L, F, itp):
     psi = _psi(n, L, F, 7)
     kw = dict(c_extra=_cextra(3)) if F == 6 else dict(c1=-0.4)
@@ -112,7 +114,7 @@ def test_spin6_mixed_transverse_sides(padding):
 @pytest.mark.parametrize("padding", [False, True])
 @pytest.mark.parametrize("secular", [False, True])
 @pytest.mark.parametrize("n,L", [((24, 40, 48), (10.0, 12.0, 14.0)), ((80, 48), (16.0, 12.0)),
-                                 ((96,), (20.0,))])
+                                 ((96,), (20.0,)), ((56, 112), (10.0, 16.0))])
 def test_ddi(n, L, secular, padding):
     """DDI on mixed sides: half spectra of 13, 21, 25 ... bins, padded lines of 48 ... 192."""
     psi = _psi(n, L, 1, 11)
@@ -121,9 +123,10 @@ def test_ddi(n, L, secular, padding):
     _check(ows, gws, 8)
 
 
-@pytest.mark.parametrize("n,L", [((192,), (24.0,)), ((160, 32), (20.0, 8.0)), ((64, 192), (12.0, 20.0))])
+@pytest.mark.parametrize("n,L", [((192,), (24.0,)), ((160, 32), (20.0, 8.0)), ((64, 192), (12.0, 20.0)),
+                                 ((112, 56), (16.0, 10.0))])
 def test_ddi_padded_long_lines(n, L):
-    """Zero-padded DDI with 384- and 320-point spectral lines (x pass and y pass)."""
+    """Zero-padded DDI with 384-, 320- and 224-point spectral lines (x pass and y pass)."""
     psi = _psi(n, L, 1, 13)
     ows, gws = pair(n, L, 1, psi, 0.002, True, omega=(1.0,) * len(n), c0=10.0, c1=-0.5,
                     ddi=True, cdd=2.0, padding=True)
@@ -131,7 +134,7 @@ def test_ddi_padded_long_lines(n, L):
 
 
 @pytest.mark.parametrize("n,L", [((48,), (10.0,)), ((80, 24), (12.0, 8.0)),
-                                 ((40, 96, 48), (9.0, 14.0, 11.0))])
+                                 ((40, 96, 48), (9.0, 14.0, 11.0)), ((224, 56), (20.0, 9.0))])
 def test_fft_gradient(n, L):
     """Spectral gradient (fft_utils.jl:4-44): forward and inverse line FFTs of every built
     radix against numpy on a seeded random field."""
