@@ -31,7 +31,9 @@ a retyped copy of its own formula):
   CG / 6j ................. sympy exact + the reference test's known values
   DDI ..................... direct O(N^2) k-sum convolution on small grids;
                             uniform polarised E_ddi = 0; padded variant: brute-force
-                            linear (non-periodic) convolution on the doubled grid
+                            linear (non-periodic) convolution on the doubled grid;
+                            1-D: single Fourier modes scaled by the closed-form
+                            diag(2/3, -1/3, -1/3) (secular diag(1/6, 1/6, -1/3))
   constrained Mz .......... closed form of the two-level Newton problem
   adaptive ITP ............ fixed-dt ITP limit; harmonic E = 1/2
   adaptive Strang ......... fixed-dt limits (step-change == split steps, Richardson == two
