@@ -156,8 +156,8 @@ def _rooflines(prof, work, precision):
     return out
 
 
-def run_rt3d(G, A, steps, warmup, precision):
-    grid, trap, psi = _cfg2_inputs(G, precision=precision)
+def run_rt3d(G, A, steps, warmup, precision, n=128):
+    grid, trap, psi = _cfg2_inputs(G, n=n, precision=precision)
     save_every = 10
     sp = G.SimParams(dt=1e-3, n_steps=max(steps, 1), imaginary_time=False, save_every=save_every)
     ws = G.make_workspace(grid=grid, atom=G.Rb87, interactions=G.InteractionParams(10.0, 0.5),
@@ -426,6 +426,9 @@ def main():
     ap.add_argument("--sequential", action="store_true",
                     help="serve_mix: also time every request alone, one after another")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
+    ap.add_argument("--side", type=int, default=128,
+                    help="rt3d_spin1_128: cube side (the named workload is 128; other built "
+                         "sides, e.g. 96 or 160, measure the mixed-radix FFTs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/<workload>_traffic.json from two rocprofv3 --pmc runs")
@@ -516,10 +519,12 @@ def main():
         extra = {"accepted": r2["n_accepted"], "rejected": r2["n_rejected"],
                  "final_dt": r2["final_dt"]}
     elif args.workload == "rt3d_spin1_128":
-        ws, grid, secs, prof, psi, work = run_rt3d(G, A, args.steps, args.warmup, args.precision)
+        n = args.side
+        ws, grid, secs, prof, psi, work = run_rt3d(G, A, args.steps, args.warmup, args.precision,
+                                                   n=n)
         b_alg = 288.0 if args.precision == "fp64" else 144.0
-        cfg = {"workload": "3D spin-1 real-time leapfrog 128^3 (run_simulation!, save_every=10)",
-               "grid": [128, 128, 128], "F": 1, "dt": 1e-3, "c0": 10.0, "c1": 0.5, "q": 0.1}
+        cfg = {"workload": f"3D spin-1 real-time leapfrog {n}^3 (run_simulation!, save_every=10)",
+               "grid": [n, n, n], "F": 1, "dt": 1e-3, "c0": 10.0, "c1": 0.5, "q": 0.1}
     else:
         ws, grid, secs, prof, psi, work = run_itp1d(G, A, args.steps, args.warmup, args.precision)
         b_alg = 384.0 if args.precision == "fp64" else 192.0
@@ -533,7 +538,8 @@ def main():
     roof = dict(roofs.get(dom, {"bound": "hbm", "achieved": None, "peak": 8000.0, "unit": "GB/s",
                                 "frac": None}))
     roof["kernel"] = dom
-    tr = _traffic(args.workload, dom)
+    # measured traffic is per workload at its named size
+    tr = _traffic(args.workload, dom) if args.side == 128 else None
     roof["traffic"] = tr
     if tr is not None:
         roof["traffic_source"] = f"profiles/{args.workload}_traffic.json"
@@ -546,7 +552,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         if args.workload == "rt3d_spin1_128":
-            cpu = cpu_baseline_rt3d(psi)
+            cpu = cpu_baseline_rt3d(psi, grid_n=args.side)
         elif args.workload == "itp1d_spin1_64":
             cpu = cpu_baseline_itp1d()
         else:
