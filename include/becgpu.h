@@ -193,8 +193,8 @@ int bec_synchronize(bec_ws* ws);
  * environment variable BEC_XSITE=0 for off), spins F = 2 and 6 run every half potential
  * step with its neighbouring x FFTs and DDI x lines in the fused x/site kernel; off, in
  * separate x passes and lane-per-site passes.  Both give the same psi.  Returns the
- * setting in force (the fused kernel needs 2-D/3-D grids with one x-pass tile of 16..256
- * sites; other shapes always run separately). */
+ * setting in force (the fused kernel needs 2-D/3-D grids whose sides are all powers of two,
+ * with one x-pass tile of 16..256 sites; other shapes always run separately). */
 int bec_set_xsite(bec_ws* ws, int on);
 /* Tile autotuner (no counterpart in the reference, whose FFTW plans are tuned by
  * FFTW_MEASURE, src/grid.jl:37-53): times the x-pass tile candidates on this workspace's
