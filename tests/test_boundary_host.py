@@ -58,6 +58,8 @@ def _cfg(**kw):
     (dict(F=3), L.BEC_EUNSUP),
     (dict(n=(72, 1, 1)), L.BEC_EUNSUP),
     (dict(n=(320, 1, 1)), L.BEC_EUNSUP),
+    (dict(n=(120, 1, 1)), L.BEC_EUNSUP),
+    (dict(n=(448, 1, 1)), L.BEC_EUNSUP),
     (dict(B=0), L.BEC_EINVAL),
     (dict(n=(4, 1, 1)), L.BEC_EUNSUP),
 ])
