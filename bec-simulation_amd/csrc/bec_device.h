@@ -764,13 +764,32 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
 template <typename T>
 __device__ __forceinline__ T cabs1(cplx<T> a) { return fabs(a.x) + fabs(a.y); }
 
+// Storage of the Taylor sum of op_tensor.  RegAcc keeps it in VGPRs; LdsAcc keeps it in the
+// calling thread's own D slots of an LDS tile (slot of component c: at(c)), which takes
+// 2D VGPRs off the step's live set (the FP64 spin-6 step then fits 256 VGPRs).  Both give
+// the same bits: the update below is written with one rounding per operation.
+template <typename T, int D>
+struct RegAcc {
+  cplx<T> a[D];
+  __device__ __forceinline__ cplx<T> get(int c) const { return a[c]; }
+  __device__ __forceinline__ void set(int c, cplx<T> v) { a[c] = v; }
+};
+template <typename T, typename At>
+struct LdsAcc {
+  cplx<T>* base;
+  At at;
+  __device__ __forceinline__ cplx<T> get(int c) const { return base[at(c)]; }
+  __device__ __forceinline__ void set(int c, cplx<T> v) { base[at(c)] = v; }
+};
+
 // psi <- exp(-i h dt) psi (real time) or exp(-h dt) psi (imaginary time), h built from the
 // incoming spinor.  The reference diagonalises h (LAPACK eigen); here the action of the
 // exponential is summed as a Taylor series to working precision, with the step split into
 // nsub pieces when the norm bound theta = 2 |dt| n sum_S |g_S| exceeds 1/2.  Sites with
 // n < 1e-20 are left unchanged (tensor_interaction.jl:212).
-template <typename T, int D>
-__device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp) {
+template <typename T, int D, typename Acc>
+__device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp,
+                                         Acc& acc) {
   constexpr int F = (D - 1) / 2;
   double n = 0.0;
 #pragma unroll
@@ -789,16 +808,16 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
   const double hstep = dtf / nsub;
   const T eps = sizeof(T) == 8 ? (T)1e-17 : (T)3e-9;
   const double n_in = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
-  cplx<T> psi0[D], v[D], w[D], acc[D];
+  cplx<T> psi0[D], v[D], w[D];
   int nmv = 0;
 #pragma unroll
-  for (int c = 0; c < D; ++c) psi0[c] = s[c];
+  for (int c = 0; c < D; ++c) {
+    psi0[c] = s[c];
+    acc.set(c, s[c]);
+  }
   for (int sub = 0; sub < nsub; ++sub) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      v[c] = s[c];
-      acc[c] = s[c];
-    }
+    for (int c = 0; c < D; ++c) v[c] = acc.get(c);
     for (int k = 1; k <= 60; ++k) {
       tensor_matvec<T, D>(psi0, v, w, g);
       ++nmv;
@@ -806,18 +825,26 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
       T nv = 0, na = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
+#pragma clang fp contract(off)
         v[c] = itp ? mk(-w[c].x * f, -w[c].y * f) : mk(w[c].y * f, -w[c].x * f);
-        acc[c] = acc[c] + v[c];
+        const cplx<T> a = acc.get(c) + v[c];
+        acc.set(c, a);
         nv += cabs1(v[c]);
-        na += cabs1(acc[c]);
+        na += cabs1(a);
       }
       if (nv <= eps * na) break;
     }
-#pragma unroll
-    for (int c = 0; c < D; ++c) s[c] = acc[c];
   }
+#pragma unroll
+  for (int c = 0; c < D; ++c) s[c] = acc.get(c);
   if (!itp) restore_norm<T, D>(s, n_in);
   return nmv;
+}
+
+template <typename T, int D>
+__device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp) {
+  RegAcc<T, D> acc;
+  return op_tensor<T, D>(s, P, dtf, itp, acc);
 }
 
 // Tensor interaction energy density 1/2 sum_S g_S sum_M |A_SM|^2 with the pair amplitude

@@ -23,7 +23,7 @@ namespace bec {
 constexpr int kXsiteMaxThreads = 256;
 
 template <typename T, int NX, int D>
-__global__ void __launch_bounds__(kXsiteMaxThreads)
+__global__ void __launch_bounds__(kXsiteMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
 xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
              cplx<T>* __restrict__ fhat, int ROWS) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -147,7 +147,15 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
           nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
         }
       } else if (op.kind == OP_TENSOR) {
-        if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
+        if constexpr (D == 13 && sizeof(T) == 8) {
+          // FP64: the Taylor sum lives in this thread's own tile slots (written back
+          // below), which keeps the step within 256 VGPRs (2 waves/SIMD)
+          auto at = [TS, tid](int c) __attribute__((always_inline)) { return p8(c * TS + tid); };
+          LdsAcc<T, decltype(at)> acc{tile, at};
+          nmv += op_tensor<T, D>(sp, P, dtf, itp, acc);
+        } else if constexpr (D == 13) {
+          nmv += op_tensor<T, D>(sp, P, dtf, itp);
+        }
       } else if (op.kind == OP_SCALE) {
         const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 #pragma unroll

@@ -1,7 +1,8 @@
 """Benchmark of the MI355X split-step path.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
-(default workload: yoshida3d_spin6_ddi_128, FP32-bulk)
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--precision P] [--no-cpu]
+(default workload: yoshida3d_spin6_ddi_128 in FP64, with the FP32-bulk run of the same
+workload and its error against FP64 reported beside it under "fp32_bulk")
 
 Prints ONE JSON line.  A step is one accepted time step of the whole grid
 (all 2F+1 components); value = site-steps/s with psi resident in HBM.
@@ -50,14 +51,19 @@ FP64_PEAK = 78.6e12   # FLOP/s, MI355X FP64 vector / matrix
 HBM_CLASSES = ("xpass", "fwd", "kin", "inv", "ddi_fft", "ddi_conv")
 
 
-def _traffic(workload, kernel):
+def _traffic_path(workload, precision):
+    return os.path.join(ROOT, "profiles", f"{workload}_{precision}_traffic.json")
+
+
+def _traffic(workload, kernel, precision):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE passes, scaled by a copy kernel of known size), if a summary for this
-    workload is committed under profiles/ as JSON; otherwise None."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    workload at this precision is committed under profiles/ as JSON; otherwise None."""
     try:
-        with open(path) as f:
+        with open(_traffic_path(workload, precision)) as f:
             d = json.load(f)
+        if d.get("precision") != precision:
+            return None
         return d["kernels"][kernel]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
@@ -145,9 +151,11 @@ def _rooflines(prof, work, precision):
             flops = work["flops"] + fft
             peak = FP32_PEAK if precision == "fp32" else FP64_PEAK
             tf = flops / secs / 1e12
-            out[k] = {"bound": "mfma", "achieved": round(tf, 2), "peak": peak / 1e12,
-                      "unit": "TFLOP/s", "frac": round(tf * 1e12 / peak, 4),
-                      "engine": "VALU (no MFMA; non-packed FMA issue caps it at half the peak)",
+            engine = ("VALU FP64 FMA (the FP64 vector and matrix peaks are equal)"
+                      if precision == "fp64" else
+                      "VALU FP32 FMA without packed FP32 (issue caps it at half the peak)")
+            out[k] = {"bound": "valu", "achieved": round(tf, 2), "peak": peak / 1e12,
+                      "unit": "TFLOP/s", "frac": round(tf * 1e12 / peak, 4), "engine": engine,
                       "avg_launch_us": round(ms / cnt * 1e3, 2), "launches": cnt,
                       "alg_flops_per_launch": flops / cnt, "fft_flops_per_launch": fft / cnt,
                       "tensor_matvecs": work["tensor_matvecs"], "rotations": work["rotations"],
@@ -227,7 +235,7 @@ def cfg3_inputs(G, n=128):
     return grid, inter, psi, c_extra
 
 
-def run_yoshida3d(G, A, steps, warmup, precision, padding=False):
+def run_yoshida3d(G, A, steps, warmup, precision, padding=False, profiled=True):
     grid, inter, psi, _ = cfg3_inputs(G)
     sp = G.SimParams(dt=1e-3, n_steps=500)
     ws = G.make_workspace(grid=grid, atom=G.Eu151, interactions=inter,
@@ -243,6 +251,8 @@ def run_yoshida3d(G, A, steps, warmup, precision, padding=False):
                                   keep_snapshots=False, max_accepted=steps)
     ws.synchronize()
     t1 = time.perf_counter()
+    if not profiled:
+        return ws, grid, (t1 - t0), None, psi, r2, None
     ws.profile(True)
     A.run_simulation_yoshida(ws, adaptive=G.AdaptiveDtParams(dt_init=r2["final_dt"]), t_end=1e9,
                              save_interval=0.05, keep_snapshots=False,
@@ -251,6 +261,35 @@ def run_yoshida3d(G, A, steps, warmup, precision, padding=False):
     work = ws.profile_work()
     ws.profile(False)
     return ws, grid, (t1 - t0), prof, psi, r2, work
+
+
+def fp32_companion(G, A, steps, warmup, padding, nsites, n_err=10):
+    """The same spin-6 workload in FP32-bulk: its rate over the same number of accepted
+    steps, and its error against the FP64 path on a fixed-dt run (n_err Yoshida cores at
+    dt = 1e-3 from the same initial state in both precisions; relative L2 of psi over all
+    components and relative energy difference)."""
+    ws32, _, secs32, _, _, r32, _ = run_yoshida3d(G, A, steps, warmup, "fp32", padding=padding,
+                                                   profiled=False)
+    ws32.close()
+    grid, inter, psi, _ = cfg3_inputs(G)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        ws = G.make_workspace(grid=grid, atom=G.Eu151, interactions=inter,
+                              potential=G.HarmonicTrap(1.0, 1.0, 1.0),
+                              sim_params=G.SimParams(dt=1e-3, n_steps=n_err), psi_init=psi,
+                              enable_ddi=True, c_dd=1.0, precision=prec, ddi_padding=padding)
+        for _ in range(n_err):
+            A.yoshida_core(ws, 1e-3)
+        out[prec] = (ws.get_psi(), A.total_energy(ws))
+        ws.close()
+    (p64, e64), (p32, e32) = out["fp64"], out["fp32"]
+    rel_l2 = float(np.sqrt(np.sum(np.abs(p32 - p64) ** 2) / np.sum(np.abs(p64) ** 2)))
+    return {"value": steps * nsites / secs32, "unit": "site-steps/s",
+            "ms_per_step": secs32 / steps * 1e3, "accepted": r32["n_accepted"],
+            "rejected": r32["n_rejected"],
+            "dtype": "f32 (FP64 reductions, phase arguments, DDI kernel, renormalisation)",
+            "err_vs_fp64": {"yoshida_cores": n_err, "dt": 1e-3, "rel_l2_psi": rel_l2,
+                            "rel_energy": abs(e32 - e64) / abs(e64)}}
 
 
 def cfg4_jobs(G, n_jobs=256, seed=42, n_steps=10000):
@@ -302,10 +341,33 @@ def run_batch2d(G, A, n_jobs, n_plain, graphs=False):
                 p99_s=float(np.percentile(lat, 99)))
 
 
-def cpu_baseline_yoshida3d(budget_s=20.0):
-    """FP64 oracle on a bounded sample of the spin-6 dipolar workload: Yoshida
-    attempts (Strang core + Yoshida core + L2 error, yoshida.jl:50-56) on the same
-    configuration at 32^3 (same box and couplings), scaled to site-steps/s."""
+def _cpu_worker(job):
+    """One single-threaded oracle worker of the CPU baseline: (units of work, seconds)."""
+    kind, budget_s, kw = job
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        return {"yoshida3d": _cpu_yoshida3d, "rt3d": _cpu_rt3d,
+                "itp1d": _cpu_itp1d}[kind](budget_s, **kw)
+
+
+def cpu_pool(kind, budget_s, **kw):
+    """Run the oracle sample `kind` on every host core this job may use (at most 16, the
+    host share of one GPU), one single-threaded process per core, all at once; the
+    throughput is the sum over the processes.  Called before anything touches the GPU, so
+    the worker processes are plain forks of a process without GPU state."""
+    import multiprocessing as mp
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(kind, budget_s, kw)] * cores)
+    units = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
+    return cores, units, wall, res[0][2]
+
+
+def _cpu_yoshida3d(budget_s):
+    """FP64 oracle on a bounded sample of the spin-6 dipolar workload: Yoshida attempts
+    (Strang core + Yoshida core + L2 error, yoshida.jl:50-56) on the same configuration at
+    32^3 (same box and couplings); one attempt is one accepted step when none is rejected."""
     import becgpu as G
     from oracle import spinor_oracle as O
     n = 32
@@ -324,16 +386,23 @@ def cpu_baseline_yoshida3d(budget_s=20.0):
         O.yoshida_core(ows, dt)
         O.l2_change(ows.psi, ps)
         att += 1
-    el = time.perf_counter() - t0
-    return dict(value=att * n ** 3 / el, unit="site-steps/s", cores=1, kind="port",
-                sample=f"{att} Yoshida attempts (dt=1e-3) of the spin-6 dipolar workload at 32^3, "
-                       f"FP64 numpy oracle (LAPACK eigh per site), {el:.1f} s")
+    return att * n ** 3, time.perf_counter() - t0, "Yoshida attempts (dt=1e-3) at 32^3"
 
 
-def cpu_baseline_rt3d(psi, grid_n=128, budget_s=15.0):
-    """FP64 oracle (numpy/pocketfft, single thread) on a bounded sample of the same
-    workload: leapfrog steps on the same 128^3 input until ~budget_s of CPU time."""
+def cpu_baseline_yoshida3d(budget_s=20.0):
+    cores, units, wall, what = cpu_pool("yoshida3d", budget_s)
+    return dict(value=units / wall, unit="site-steps/s", cores=cores, kind="port",
+                sample=f"{units // 32 ** 3} {what} of the spin-6 dipolar workload (same box, "
+                       f"couplings and DDI), FP64 numpy oracle (pocketfft, LAPACK eigh per "
+                       f"site), {cores} single-threaded processes for {wall:.1f} s")
+
+
+def _cpu_rt3d(budget_s, grid_n=128):
+    """FP64 oracle (numpy/pocketfft, single thread) on a bounded sample of the spin-1
+    real-time workload: leapfrog steps on the same input until ~budget_s."""
+    import becgpu as G
     from oracle import spinor_oracle as O
+    _, _, psi = _cfg2_inputs(G, n=grid_n)
     og = O.make_grid((grid_n,) * 3, (20.0,) * 3)
     ows = O.make_workspace(og, 1, psi, 1e-3, False, omega=(1, 1, 1), q=0.1, c0=10.0, c1=0.5,
                            n_steps=1000, save_every=10)
@@ -344,14 +413,17 @@ def cpu_baseline_rt3d(psi, grid_n=128, budget_s=15.0):
         ows.psi = O.kinetic_step(ows.psi, ows.kphase)
         O.half_potential_step(ows, 1e-3, False)
         n += 1
-    dt = time.perf_counter() - t0
-    sites = grid_n ** 3
-    return dict(value=n * sites / dt, unit="site-steps/s", cores=1, kind="port",
-                sample=f"{n} leapfrog steps (K + merged V(dt)) of the 128^3 spin-1 workload, "
-                       f"FP64 numpy oracle, {dt:.1f} s")
+    return n * grid_n ** 3, time.perf_counter() - t0, f"leapfrog steps (K + merged V(dt)) at {grid_n}^3"
 
 
-def cpu_baseline_itp1d(budget_s=10.0):
+def cpu_baseline_rt3d(grid_n=128, budget_s=15.0):
+    cores, units, wall, what = cpu_pool("rt3d", budget_s, grid_n=grid_n)
+    return dict(value=units / wall, unit="site-steps/s", cores=cores, kind="port",
+                sample=f"{units // grid_n ** 3} {what} of the spin-1 real-time workload, FP64 "
+                       f"numpy oracle, {cores} single-threaded processes for {wall:.1f} s")
+
+
+def _cpu_itp1d(budget_s):
     from oracle import spinor_oracle as O
     og = O.make_grid((64,), (20.0,))
     psi = O.init_psi(og, 1, "polar")
@@ -361,9 +433,14 @@ def cpu_baseline_itp1d(budget_s=10.0):
     while time.perf_counter() - t0 < budget_s:
         O.split_step(ows)
         n += 1
-    dt = time.perf_counter() - t0
-    return dict(value=n * 64 / dt, unit="site-steps/s", cores=1, kind="port",
-                sample=f"{n} ITP split_step! of the 64-point spin-1 workload, FP64 numpy oracle")
+    return n * 64, time.perf_counter() - t0, "ITP split_step! at 64 points"
+
+
+def cpu_baseline_itp1d(budget_s=10.0):
+    cores, units, wall, what = cpu_pool("itp1d", budget_s)
+    return dict(value=units / wall, unit="site-steps/s", cores=cores, kind="port",
+                sample=f"{units // 64} {what} of the 1-D spin-1 workload, FP64 numpy oracle, "
+                       f"{cores} single-threaded processes for {wall:.1f} s")
 
 
 def run_serve(G, n_req, rate, precision, n_check=10, sequential=False):
@@ -430,26 +507,37 @@ def main():
                     help="rt3d_spin1_128: cube side (the named workload is 128; other built "
                          "sides, e.g. 96 or 160, measure the mixed-radix FFTs)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="yoshida3d_*: skip the FP32-bulk companion run of the FP64 line")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/<workload>_traffic.json from two rocprofv3 --pmc runs")
     ap.add_argument("--copy-bytes", type=float, default=None,
                     help="--pmc-summary: bytes of the largest copyBuffer dispatch")
     args = ap.parse_args()
     if args.pmc_summary:
-        prec = args.precision or ("fp32" if args.workload.startswith(("yoshida", "serve"))
-                                  else "fp64")
+        prec = args.precision or ("fp32" if args.workload.startswith("serve") else "fp64")
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_bytes, args.workload,
-                    os.path.join(ROOT, "profiles", f"{args.workload}_traffic.json"), prec)
+                    _traffic_path(args.workload, prec), prec)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; sharding across GPUs is a "
                                    "design note only, not built", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.precision is None:
+        args.precision = "fp32" if args.workload.startswith("serve") else "fp64"
+    # the CPU baseline runs first, in forked single-threaded processes, before this process
+    # touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        if args.workload == "rt3d_spin1_128":
+            cpu = cpu_baseline_rt3d(grid_n=args.side)
+        elif args.workload == "itp1d_spin1_64":
+            cpu = cpu_baseline_itp1d()
+        elif args.workload.startswith("yoshida3d_spin6"):
+            cpu = cpu_baseline_yoshida3d()
     import becgpu as G
     from becgpu import api as A
 
-    if args.precision is None:
-        args.precision = "fp32" if args.workload.startswith(("yoshida", "serve")) else "fp64"
     extra = {}
     if args.workload == "serve_mix":
         b = run_serve(G, args.requests, args.rate, args.precision, sequential=args.sequential)
@@ -518,6 +606,8 @@ def main():
                "tensor": "c2..c12 ~ U[-1,1] seed 42", "dt_init": 1e-3, "ddi_padding": pad}
         extra = {"accepted": r2["n_accepted"], "rejected": r2["n_rejected"],
                  "final_dt": r2["final_dt"]}
+        if args.precision == "fp64" and not args.no_fp32:
+            extra["fp32_bulk"] = fp32_companion(G, A, args.steps, args.warmup, pad, nsites=128 ** 3)
     elif args.workload == "rt3d_spin1_128":
         n = args.side
         ws, grid, secs, prof, psi, work = run_rt3d(G, A, args.steps, args.warmup, args.precision,
@@ -539,24 +629,17 @@ def main():
                                 "frac": None}))
     roof["kernel"] = dom
     # measured traffic is per workload at its named size
-    tr = _traffic(args.workload, dom) if args.side == 128 else None
+    tr = _traffic(args.workload, dom, args.precision) if args.side == 128 else None
     roof["traffic"] = tr
     if tr is not None:
-        roof["traffic_source"] = f"profiles/{args.workload}_traffic.json"
+        roof["traffic_source"] = os.path.relpath(_traffic_path(args.workload, args.precision),
+                                                 ROOT)
     total_ms = sum(v[0] for v in prof.values())
     kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
                    "share": round(v[0] / max(1e-9, total_ms), 3)}
                for k, v in prof.items() if v[1] > 0}
     hbm = {k: {"GB/s": r["achieved"], "frac": r["frac"]} for k, r in roofs.items()
            if r["bound"] == "hbm"}
-    cpu = None
-    if not args.no_cpu:
-        if args.workload == "rt3d_spin1_128":
-            cpu = cpu_baseline_rt3d(psi, grid_n=args.side)
-        elif args.workload == "itp1d_spin1_64":
-            cpu = cpu_baseline_itp1d()
-        else:
-            cpu = cpu_baseline_yoshida3d()
     line = {
         "metric": "spinor site-steps/s & %HBM peak",
         "value": value, "unit": "site-steps/s", "n_gpus": 1, "steps": args.steps,

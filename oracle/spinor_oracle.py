@@ -474,7 +474,10 @@ def tensor_step(psi, T, dtf, itp):
     skipped where n < 1e-20."""
     D = psi.shape[0]
     sp = psi.reshape(D, -1)
-    h = np.einsum("abcd,cp,dp->pab", T, np.conj(sp), sp)
+    # h[p, a, b] = sum_{c,d} T[a, b, c, d] conj(psi_c) psi_d, as one (D^2 x D^2) @ (D^2 x N)
+    # product over the sites
+    pair = (np.conj(sp)[:, None, :] * sp[None, :, :]).reshape(D * D, -1)
+    h = (T.reshape(D * D, D * D) @ pair).T.reshape(-1, D, D)
     lam, vec = np.linalg.eigh(h, UPLO="U")
     s = np.einsum("pjk,jp->pk", np.conj(vec), sp)
     ph = np.exp(-lam * dtf) if itp else np.exp(-1j * lam * dtf)
