@@ -65,6 +65,7 @@ SIGNATURES = {
     "bec_copy_slot": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "bec_diff": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "bec_itp_guard": ([C.c_void_p, C.c_void_p], C.c_int),
+    "bec_spin_density": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "bec_profile_read": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "bec_profile_work": ([C.c_void_p, C.c_void_p], C.c_int),

@@ -253,6 +253,14 @@ class Workspace:
     def copy_slot(self, dst, src):
         L.check(L.lib().bec_copy_slot(self.handle, dst, src), "bec_copy_slot")
 
+    def spin_density(self, inst=0):
+        """_compute_spin_density! (observables.jl:61-88): (Fx, Fy, Fz), each of the grid's
+        numpy shape, in FP64."""
+        out = np.zeros((3,) + self.grid.shape)
+        L.check(L.lib().bec_spin_density(self.handle, int(inst), out.ctypes.data),
+                "bec_spin_density")
+        return out[0], out[1], out[2]
+
     def itp_guard(self):
         out = np.zeros((self.B, 2))
         L.check(L.lib().bec_itp_guard(self.handle, out.ctypes.data), "bec_itp_guard")
@@ -885,7 +893,9 @@ def find_ground_state_multistart(*, initial_states=("polar", "ferromagnetic", "u
 def scan_continuation(*, param_values, make_interactions, grid, atom, initial_state="polar",
                       energy_jump_threshold=0.1, n_steps_continuation=500, n_steps_fresh=5000,
                       **kw):
-    """ground_state.jl:328-378."""
+    """ground_state.jl:328-378.  Each record is the reference's (param, energy, converged,
+    phase, psi) plus `fallback`: whether the multistart search replaced the continued
+    solve after an energy jump."""
     results = []
     prev_psi = None
     prev_E = float("nan")
@@ -897,14 +907,17 @@ def scan_continuation(*, param_values, make_interactions, grid, atom, initial_st
         else:
             r = find_ground_state(grid=grid, atom=atom, interactions=inter,
                                   initial_state=initial_state, n_steps=n_steps_fresh, **kw)
+        fallback = False
         if (not math.isnan(prev_E)
                 and abs(r["energy"] - prev_E) / max(abs(prev_E), 1e-30) > energy_jump_threshold):
+            r["workspace"].close()
             r = find_ground_state_multistart(grid=grid, atom=atom, interactions=inter,
                                              n_steps=n_steps_fresh, **kw)
+            fallback = True
         psi = r["workspace"].get_psi(0)
         ph = classify_phase(psi, atom.F, grid)
         results.append(dict(param=val, energy=r["energy"], converged=r["converged"],
-                            phase=ph["phase"], psi=psi.copy()))
+                            phase=ph["phase"], psi=psi.copy(), fallback=fallback))
         prev_psi = psi
         prev_E = r["energy"]
     return results

@@ -16,7 +16,7 @@ REPO = os.path.dirname(ROOT)
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG_DIR, "libbecgpu.so")
 
-SOURCES = ["bec_kernels.hip", "bec_strided.hip", "bec_xsite.hip", "bec_capi.hip"]
+SOURCES = ["bec_kernels.hip", "bec_strided.hip", "bec_xsite.hip", "bec_aux.hip", "bec_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

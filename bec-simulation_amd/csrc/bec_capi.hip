@@ -61,6 +61,7 @@ struct bec_ws {
   InstParams* d_params = nullptr;
   int* d_active = nullptr;
   unsigned long long* d_work = nullptr;  // profiling counters (tensor matvecs, spin rotations)
+  double* d_sdens = nullptr;  // bec_spin_density output staging, [3][nsites] (first use)
   double* d_cpart = nullptr;   // constrained normalisation: per-component partials, targets,
   double* d_ctarget = nullptr; // factors (allocated on first use)
   double* d_cfac = nullptr;
@@ -607,6 +608,30 @@ static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out
   return bytes;
 }
 
+// Geometry of a workspace with D components on the grid c describes (validated by the caller).
+static void fill_geom(const bec_config* c, int D, Geom& g) {
+  g = Geom{};
+  g.ndim = c->ndim;
+  for (int d = 0; d < 3; ++d) g.n[d] = d < c->ndim ? c->n[d] : 1;
+  g.D = D;
+  g.nrows = g.n[1] * g.n[2];
+  g.nsites = (long long)g.n[0] * g.n[1] * g.n[2];
+  g.inst_stride = g.nsites * D;
+  g.inv_ntot = 1.0 / (double)g.nsites;
+  double dV = 1.0;
+  for (int d = 0; d < c->ndim; ++d) dV *= c->L[d] / c->n[d];
+  g.dV = dV;
+  g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
+  g.spad = c->enable_ddi && c->ddi_padding ? 1 : 0;
+  for (int d = 0; d < 3; ++d) g.sn[d] = d < c->ndim ? g.n[d] * (g.spad ? 2 : 1) : 1;
+  g.smax = std::max(g.sn[0], std::max(g.sn[1], g.sn[2]));
+  g.nxh = g.sn[0] / 2 + 1;
+  g.nxh_pitch = (g.nxh + 15) / 16 * 16;
+  g.fhat_field_stride = (long long)g.nxh_pitch * g.sn[1] * g.sn[2];
+  g.fhat_inst_stride = 3 * g.fhat_field_stride;
+  g.ddi_secular = c->ddi_secular;
+}
+
 long long bec_workspace_bytes(const bec_config* cfg) {
   if (!cfg) return -1;
   return ws_bytes(cfg, nullptr, nullptr);
@@ -650,25 +675,7 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   }
   ws->csz = ws->prec == BEC_FP32 ? 8 : 16;
   Geom& g = ws->g;
-  g.ndim = c->ndim;
-  for (int d = 0; d < 3; ++d) g.n[d] = d < c->ndim ? c->n[d] : 1;
-  g.D = ws->D;
-  g.nrows = g.n[1] * g.n[2];
-  g.nsites = (long long)g.n[0] * g.n[1] * g.n[2];
-  g.inst_stride = g.nsites * ws->D;
-  g.inv_ntot = 1.0 / (double)g.nsites;
-  double dV = 1.0;
-  for (int d = 0; d < c->ndim; ++d) dV *= c->L[d] / c->n[d];
-  g.dV = dV;
-  g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
-  g.spad = c->enable_ddi && c->ddi_padding ? 1 : 0;
-  for (int d = 0; d < 3; ++d) g.sn[d] = d < c->ndim ? g.n[d] * (g.spad ? 2 : 1) : 1;
-  g.smax = std::max(g.sn[0], std::max(g.sn[1], g.sn[2]));
-  g.nxh = g.sn[0] / 2 + 1;
-  g.nxh_pitch = (g.nxh + 15) / 16 * 16;
-  g.fhat_field_stride = (long long)g.nxh_pitch * g.sn[1] * g.sn[2];
-  g.fhat_inst_stride = 3 * g.fhat_field_stride;
-  g.ddi_secular = c->ddi_secular;
+  fill_geom(c, ws->D, g);
   ws_bytes(c, &ws->max_blocks, &ws->n_xblocks);
   {
     const char* e = getenv("BEC_XSITE");
@@ -791,7 +798,7 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
   if (ws->d_work) (void)hipFree(ws->d_work);
-  for (double* q : {ws->d_cpart, ws->d_ctarget, ws->d_cfac})
+  for (double* q : {ws->d_cpart, ws->d_ctarget, ws->d_cfac, ws->d_sdens})
     if (q) (void)hipFree(q);
   for (int i = 0; i < kNumFftSizes; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
@@ -1446,6 +1453,20 @@ int bec_diff(bec_ws* ws, int a, int b, int mode, double* out) {
     out[2 * inst] = ws->h_red[(size_t)inst * RS_COUNT + RS_NORM];
     out[2 * inst + 1] = ws->h_red[(size_t)inst * RS_COUNT + RS_KIN];
   }
+  return BEC_OK;
+}
+
+int bec_spin_density(bec_ws* ws, int inst, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  if (inst < 0 || inst >= ws->B) return fail(BEC_EINVAL, "instance out of range");
+  const long long N = ws->g.nsites;
+  if (!ws->d_sdens && hipMalloc(&ws->d_sdens, 3 * (size_t)N * 8) != hipSuccess)
+    return fail(BEC_ENOMEM, "spin density buffer");
+  const void* psi = (const char*)ws->slot[0] + ws->csz * (size_t)ws->g.inst_stride * inst;
+  HIPCHK(ws->prec == BEC_FP32 ? launch_spin_density<float>(ws->stream, psi, ws->d_sdens, N, ws->D)
+                              : launch_spin_density<double>(ws->stream, psi, ws->d_sdens, N, ws->D));
+  HIPCHK(hipMemcpyAsync(out, ws->d_sdens, 3 * (size_t)N * 8, hipMemcpyDeviceToHost, ws->stream));
+  HIPCHK(hipStreamSynchronize(ws->stream));
   return BEC_OK;
 }
 

@@ -711,6 +711,21 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
 // Clebsch-Gordan table of spin F in tens:: order, evaluated by the compiler.
 template <int F>
 inline constexpr tens::CGTab<F> kCG{};
+// The same values in constant memory.  VALU literals are 32 bits, so an FP64 CG value used
+// as an instruction operand costs two s_mov_b32 per use; read from here, the compiler
+// fetches eight values per s_load_dwordx16 instead.
+template <int F>
+__constant__ tens::CGTab<F> kCGConst = tens::CGTab<F>{};
+using const_f64_ptr = const __attribute__((address_space(4))) double*;
+
+// CG value at table index I as an operand of type T (FP32: literal; FP64: loaded through
+// cgp, a pointer to kCGConst<F> that the caller makes opaque on every matvec, so the loads
+// stay next to their use instead of being hoisted out of the Taylor loop and spilled).
+template <typename T, int F, int I>
+__device__ __forceinline__ T cg_operand(const_f64_ptr cgp) {
+  if constexpr (sizeof(T) == 8) return cgp[I];
+  else return (T)kCG<F>.v[I];
+}
 
 // w = h v with the mean-field matrix of the spinor psi,
 //   h_{m m'} = sum_S g_S sum_M CG(m, M-m | S M) CG(m', M-m' | S M) conj(psi_{M-m}) psi_{M-m'},
@@ -720,7 +735,7 @@ inline constexpr tens::CGTab<F> kCG{};
 // cgt: table in tens:: order; every index below is a compile-time constant.
 template <typename T, int D>
 __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>* v, cplx<T>* w,
-                                              const T* g) {
+                                              const T* g, const_f64_ptr cgp) {
   constexpr int F = (D - 1) / 2;
 #pragma unroll
   for (int c = 0; c < D; ++c) w[c] = mk((T)0, (T)0);
@@ -740,8 +755,8 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       if constexpr (m1 != m2) p = p + psi[F - m1] * v[F - m2];
       static_for<0, NS>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
-        constexpr T c = (T)kCG<F>.v[tens::off(F, M, m1, S0) + k];
-        if constexpr (c != (T)0) Bs[k] = Bs[k] + p * c;
+        constexpr int I = tens::off(F, M, m1, S0) + k;
+        if constexpr ((T)kCG<F>.v[I] != (T)0) Bs[k] = Bs[k] + p * cg_operand<T, F, I>(cgp);
       });
     });
 #pragma unroll
@@ -752,8 +767,8 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       cplx<T> C = mk((T)0, (T)0);
       static_for<0, NS>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
-        constexpr T c = (T)kCG<F>.v[tens::off(F, M, m1, S0) + k];
-        if constexpr (c != (T)0) C = C + Bs[k] * c;
+        constexpr int I = tens::off(F, M, m1, S0) + k;
+        if constexpr ((T)kCG<F>.v[I] != (T)0) C = C + Bs[k] * cg_operand<T, F, I>(cgp);
       });
       w[F - m1] = w[F - m1] + conjc(psi[F - m2]) * C;
       if constexpr (m1 != m2) w[F - m2] = w[F - m2] + conjc(psi[F - m1]) * C;
@@ -819,7 +834,9 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
 #pragma unroll
     for (int c = 0; c < D; ++c) v[c] = acc.get(c);
     for (int k = 1; k <= 60; ++k) {
-      tensor_matvec<T, D>(psi0, v, w, g);
+      const_f64_ptr cgp = (const_f64_ptr)kCGConst<F>.v;
+      if constexpr (sizeof(T) == 8) asm volatile("" : "+s"(cgp));
+      tensor_matvec<T, D>(psi0, v, w, g, cgp);
       ++nmv;
       const T f = (T)(hstep / k);
       T nv = 0, na = 0;

@@ -130,6 +130,11 @@ bool xpass_rows_ok(const Geom& g, int rows);
 // Blocks per instance of an x-pass launch (= number of per-block partial sums it writes).
 int xpass_grid(const Geom& g);
 
+// Spin density (Fx, Fy, Fz) of one instance's psi into out[3][nsites], FP64 (bec_aux.hip).
+template <typename T>
+hipError_t launch_spin_density(hipStream_t st, const void* psi, double* out, long long nsites,
+                               int D);
+
 // dst[i] = (double)src[i], i < n (FP32 psi widened for FP64 energy evaluation)
 hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n);
 

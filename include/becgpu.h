@@ -143,6 +143,10 @@ int bec_copy_slot(bec_ws* ws, int dst_slot, int src_slot);
 /* mode 0: out[B][2] = {sum |a-b|^2, sum |b|^2}  (_wavefunction_l2_change, adaptive.jl:42-50)
  * mode 1: out[B][2] = {max |a-b|, max |a|}      (dpsi test, ground_state.jl:148-149) */
 int bec_diff(bec_ws* ws, int slot_a, int slot_b, int mode, double* out);
+/* _compute_spin_density! (observables.jl:61-88) of instance inst, in FP64:
+ * out[3][N] = Fx, Fy, Fz per site in psi's spatial memory order, with
+ * Fz = sum_m m |psi_m|^2 and Fx + i Fy = sum_m f+(m) conj(psi_{m+1}) psi_m. */
+int bec_spin_density(bec_ws* ws, int inst, double* out);
 /* ITP guard (ground_state.jl:4-29): out[B][2] = {sum |psi|^2 (NaN if any NaN),
  * max |Phi| of the last DDI convolution (0 without DDI)}. */
 int bec_itp_guard(bec_ws* ws, double* out);

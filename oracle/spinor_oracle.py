@@ -44,6 +44,16 @@ a retyped copy of its own formula):
   ITP / energy ............ uniform-state energies; c0=0 harmonic E = D/2;
                             3D F=6 golden density cuts (examples/eu151/density_64.json)
   Yoshida ................. coefficient closed form; 4th-order convergence ratio
+  adaptive Yoshida driver . fixed-dt limit (dt_min = dt_max: repeated S4 steps, overshoot
+                            of t_end, save points); commuting operators (err ~ 0: dt x3 per
+                            step to dt_max, clamped steps landing on save points, exact
+                            final state); the reference's adaptive test (norm, snapshots)
+  Thomas-Fermi ............ 1-D mu_TF = (3 c0 / (4 sqrt 2))^(2/3), 3-D mu_TF =
+                            (15 c0 / (4 pi))^(2/5) / 2 (harmonic trap, omega = 1)
+  classify_phase .......... closed-form order parameters of the ferromagnetic, polar,
+                            nematic and cyclic states
+  multistart / continuation uniform ground states of the box: E = (c0 + min(c1, 0)) / (2L)
+                            in 1-D; the best start and the continuation fallback follow
 """
 from __future__ import annotations
 
@@ -945,6 +955,103 @@ def find_ground_state_adaptive(ws: OracleWS, n_steps, tol, dt_max):
                 ws.kphase = kinetic_phase(ws.grid, ws.dt, True)
     return dict(converged=converged, energy=total_energy(ws), dE=dE, dpsi=dpsi, steps=total,
                 final_dt=ws.dt)
+
+
+def find_ground_state_multistart(make_ws, starts, n_steps, tol):
+    """ground_state.jl:239-271: one fresh find_ground_state per start, in order; best =
+    the first minimum of the energies (Julia argmin).  starts: [(label, idx, psi0)] in the
+    reference's order (each named state with idx 0; a :random entry expands to idx
+    1..n_random at its place); make_ws(psi0) builds the ITP workspace of one solve."""
+    results = []
+    for label, idx, psi0 in starts:
+        ws = make_ws(psi0)
+        r = find_ground_state(ws, n_steps, tol)
+        results.append(dict(initial_state=label, idx=idx, ws=ws, **r))
+    best = results[0]
+    for r in results[1:]:
+        if r["energy"] < best["energy"]:
+            best = r
+    return dict(ws=best["ws"], converged=best["converged"], energy=best["energy"],
+                initial_state=best["initial_state"], all_results=results)
+
+
+def scan_continuation(param_values, make_ws, initial_psi, multistart_starts, *, tol,
+                      energy_jump_threshold=0.1, n_steps_continuation=500, n_steps_fresh=5000):
+    """ground_state.jl:328-378: the first point from initial_psi with n_steps_fresh steps,
+    every later point from a copy of the previous ground state with n_steps_continuation
+    steps; if the energy moved by more than energy_jump_threshold relative to the previous
+    point, the point is solved again by find_ground_state_multistart (n_steps_fresh) over
+    multistart_starts.  make_ws(val, psi0) builds the ITP workspace for parameter val.
+    Each record also says whether the multistart fallback ran."""
+    out = []
+    prev_psi, prev_E = None, float("nan")
+    for val in param_values:
+        if prev_psi is not None:
+            ws = make_ws(val, prev_psi.copy())
+            r = find_ground_state(ws, n_steps_continuation, tol)
+        else:
+            ws = make_ws(val, np.array(initial_psi, dtype=complex))
+            r = find_ground_state(ws, n_steps_fresh, tol)
+        fallback = False
+        if (not math.isnan(prev_E)
+                and abs(r["energy"] - prev_E) / max(abs(prev_E), 1e-30) > energy_jump_threshold):
+            m = find_ground_state_multistart(lambda p0, v=val: make_ws(v, p0), multistart_starts,
+                                             n_steps_fresh, tol)
+            ws, r, fallback = m["ws"], m, True
+        ph = classify_phase(ws.psi, ws.F, ws.grid)
+        out.append(dict(param=val, energy=r["energy"], converged=r["converged"],
+                        phase=ph["phase"], psi=ws.psi.copy(), fallback=fallback))
+        prev_psi, prev_E = ws.psi.copy(), r["energy"]
+    return out
+
+
+def pair_amplitude(psi, F, S, M):
+    """observables.jl:118-143: A_SM = sum_{m1} CG(F m1; F M-m1 | S M) psi_m1 psi_{M-m1}
+    (entries with |CG| < 1e-15 skipped, clebsch_gordan.jl:166-180)."""
+    A = np.zeros(psi.shape[1:], complex)
+    for m1 in range(-F, F + 1):
+        m2 = M - m1
+        if abs(m2) > F:
+            continue
+        a = cg(F, m1, m2, S, M)
+        if abs(a) < 1e-15:
+            continue
+        A += a * psi[F - m1] * psi[F - m2]
+    return A
+
+
+def classify_phase(psi, F, grid: Grid):
+    """diagnostics.jl:166-212 with pair_amplitude_spectrum (observables.jl:227-246):
+    spin order sum|f|^2 dV / (F^2 sum n^2 dV); channel weights w_S = sum_M sum|A_SM|^2 dV;
+    nematic order w_0 / (sum n^2 dV / D); phase ferromagnetic (spin order > 0.9), polar
+    (F = 1) or nematic (nematic order > 0.9), cyclic (normalised w_2F > 0.5), else mixed;
+    vacuum below a total norm of 1e-30."""
+    D = 2 * F + 1
+    dV = grid.dV
+    n = density(psi)
+    n_sum = float(np.sum(n)) * dV
+    n_sq = float(np.sum(n ** 2)) * dV
+    if n_sum < 1e-30:
+        return dict(spin_order=0.0, nematic_order=0.0, channel_weights={}, phase="vacuum",
+                    magnetization_density=0.0)
+    fx, fy, fz = spin_density(psi, F)
+    spin_order = float(np.sum(fx ** 2 + fy ** 2 + fz ** 2)) * dV / (float(F) ** 2 * n_sq)
+    w = {S: sum(float(np.sum(np.abs(pair_amplitude(psi, F, S, M)) ** 2)) * dV
+                for M in range(-S, S + 1)) for S in range(0, 2 * F + 1, 2)}
+    tot = sum(w.values())
+    cw = {S: (v / tot if tot > 0 else 0.0) for S, v in w.items()}
+    nematic_order = w.get(0, 0.0) / (n_sq / D)
+    mz = magnetization(psi, F, dV) / n_sum
+    if spin_order > 0.9:
+        phase = "ferromagnetic"
+    elif nematic_order > 0.9:
+        phase = "polar" if F == 1 else "nematic"
+    elif cw.get(2 * F, 0.0) > 0.5:
+        phase = "cyclic"
+    else:
+        phase = "mixed"
+    return dict(spin_order=spin_order, nematic_order=nematic_order, channel_weights=cw,
+                phase=phase, magnetization_density=mz)
 
 
 def _snapshot(ws, rec):

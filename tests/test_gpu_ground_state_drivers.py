@@ -1,0 +1,153 @@
+"""find_ground_state_multistart, scan_continuation and the ITP overflow guard on the GPU,
+against the oracle's drivers (ground_state.jl:4-29, 239-271, 328-378).
+
+The same initial states go to both sides (the named states, and the seeded :random ones from
+this package's generator, since Julia's RNG cannot be reproduced).  Bars: every solve's
+energy to 1e-12 relative, the same convergence flags and step counts, psi to relative L2
+1e-10, the same phase labels and the same points where the continuation falls back to
+multistart.  The best start is compared when its energy is not tied with another start
+(both sides take the first minimum, but a tie at round-off may break differently).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import spinor_oracle as O
+import becgpu as G
+from becgpu import api as A
+from helpers import grids, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+STATES = ("polar", "ferromagnetic", "uniform", "antiferromagnetic")
+
+
+def _best_label_comparable(results):
+    es = sorted(r["energy"] for r in results)
+    return len(es) < 2 or (es[1] - es[0]) > 1e-9 * abs(es[0])
+
+
+def test_multistart_matches_oracle():
+    """1-D spin-1 in a harmonic trap, c1 < 0, the four named starts and two seeded random
+    spinors (initial_states with :random, n_random = 2)."""
+    n, L = (32,), (10.0,)
+    og, gg = grids(n, L)
+    c0, c1, dt, n_steps, tol = 10.0, -0.5, 0.01, 600, 1e-9
+    states = STATES + ("random",)
+    m = A.find_ground_state_multistart(initial_states=states, n_random=2, seed=42, grid=gg,
+                                       atom=G.Rb87, interactions=G.InteractionParams(c0, c1),
+                                       potential=G.HarmonicTrap(1.0), dt=dt, n_steps=n_steps,
+                                       tol=tol)
+    starts = [(s, 0, O.init_psi(og, 1, s)) for s in STATES]
+    starts += [("random", i, G.init_psi(gg, G.SpinSystem(1), "random", seed=42 + i))
+               for i in (1, 2)]
+    om = O.find_ground_state_multistart(
+        lambda p0: O.make_workspace(og, 1, p0, dt, True, omega=(1.0,), c0=c0, c1=c1,
+                                    n_steps=n_steps), starts, n_steps, tol)
+    assert len(m["all_results"]) == len(om["all_results"]) == 6
+    for g, o in zip(m["all_results"], om["all_results"]):
+        assert (g["initial_state"], g["idx"]) == (o["initial_state"], o["idx"])
+        assert g["converged"] == o["converged"] and g["steps"] == o["steps"]
+        assert abs(g["energy"] - o["energy"]) <= 1e-12 * abs(o["energy"])
+        assert rel_l2(g["workspace"].get_psi(), o["ws"].psi) <= 1e-10
+    assert abs(m["energy"] - om["energy"]) <= 1e-12 * abs(om["energy"])
+    if _best_label_comparable(om["all_results"]):
+        assert m["initial_state"] == om["initial_state"]
+
+
+def test_continuation_with_fallback_matches_oracle():
+    """Box of side 10 without a trap, c1 = -0.5, c0 = 10, 10.5, 20 from the polar state: the
+    last point jumps by 90 % and falls back to multistart on both sides."""
+    n, L = (16,), (10.0,)
+    og, gg = grids(n, L)
+    vals, dt, tol, ns = [10.0, 10.5, 20.0], 0.05, 1e-10, 1500
+    res = A.scan_continuation(param_values=vals,
+                              make_interactions=lambda v: G.InteractionParams(v, -0.5), grid=gg,
+                              atom=G.Rb87, initial_state="polar", n_steps_continuation=ns,
+                              n_steps_fresh=ns, dt=dt, tol=tol)
+    starts = [(s, 0, O.init_psi(og, 1, s)) for s in STATES]
+    ores = O.scan_continuation(vals, lambda v, p0: O.make_workspace(og, 1, p0, dt, True, c0=v,
+                                                                    c1=-0.5, n_steps=ns),
+                               O.init_psi(og, 1, "polar"), starts, tol=tol,
+                               n_steps_continuation=ns, n_steps_fresh=ns)
+    assert [r["fallback"] for r in res] == [r["fallback"] for r in ores] == [False, False, True]
+    for g, o in zip(res, ores):
+        assert g["param"] == o["param"] and g["phase"] == o["phase"]
+        assert g["converged"] == o["converged"]
+        assert abs(g["energy"] - o["energy"]) <= 1e-12 * abs(o["energy"])
+        assert rel_l2(g["psi"], o["psi"]) <= 1e-10
+
+
+def test_continuation_c1_sweep_in_trap_matches_oracle():
+    """test_continuation.jl:6-64 parameters (16 points, box 10, c0 = 10, dt = 0.01, tol 1e-6,
+    100 / 500 steps), sweeping c1 from ferromagnetic to antiferromagnetic coupling, plus a
+    harmonic trap so that the states differ from the uniform box states."""
+    n, L = (16,), (10.0,)
+    og, gg = grids(n, L)
+    vals, dt, tol = [-2.0, -1.0, 0.0, 1.0, 2.0], 0.01, 1e-6
+    res = A.scan_continuation(param_values=vals,
+                              make_interactions=lambda v: G.InteractionParams(10.0, v), grid=gg,
+                              atom=G.Rb87, initial_state="polar", n_steps_continuation=100,
+                              n_steps_fresh=500, dt=dt, tol=tol, potential=G.HarmonicTrap(1.0))
+    starts = [(s, 0, O.init_psi(og, 1, s)) for s in STATES]
+
+    def mk(v, p0):
+        return O.make_workspace(og, 1, p0, dt, True, omega=(1.0,), c0=10.0, c1=v, n_steps=500)
+
+    ores = O.scan_continuation(vals, mk, O.init_psi(og, 1, "polar"), starts, tol=tol,
+                               n_steps_continuation=100, n_steps_fresh=500)
+    assert [r["param"] for r in res] == vals
+    for g, o in zip(res, ores):
+        assert (g["phase"], g["fallback"], g["converged"]) == (o["phase"], o["fallback"],
+                                                                o["converged"])
+        assert abs(g["energy"] - o["energy"]) <= 1e-12 * abs(o["energy"])
+        assert rel_l2(g["psi"], o["psi"]) <= 1e-10
+
+
+def test_itp_guard_detects_nan():
+    """test_analytical_validation.jl:287-298: a NaN in psi makes the ITP overflow check throw
+    (ArgumentError there, ValueError here), before any step and after one step."""
+    n, L = (16,), (10.0,)
+    _, gg = grids(n, L)
+    psi = np.zeros((3, 16), complex)
+    psi[0, 0] = np.nan
+    ws = G.make_workspace(grid=gg, atom=G.Rb87, interactions=G.InteractionParams(100.0, 0.0),
+                          sim_params=G.SimParams(dt=0.001, n_steps=1, imaginary_time=True),
+                          psi_init=psi)
+    with pytest.raises(ValueError, match="NaN detected in ITP at step 1"):
+        A._check_itp_overflow(ws, 1)
+    A.split_step(ws, 1)
+    with pytest.raises(ValueError, match="NaN"):
+        A._check_itp_overflow(ws, 1)
+    ok = G.make_workspace(grid=gg, atom=G.Rb87, interactions=G.InteractionParams(100.0, 0.0),
+                          sim_params=G.SimParams(dt=0.001, n_steps=1, imaginary_time=True),
+                          psi_init=G.init_psi(gg, G.SpinSystem(1), "polar"))
+    A.split_step(ok, 1)
+    A._check_itp_overflow(ok, 1)
+
+
+def test_itp_guard_detects_ddi_overflow():
+    """ground_state.jl:12-27: max |Phi| dt / 2 > 50 after a DDI step throws; the same state
+    with a small c_dd passes.  The workspace is built directly (find_ground_state's pre-flight
+    check would refuse this c_dd)."""
+    n, L = (32, 32), (8.0, 8.0)
+    og, gg = grids(n, L)
+    psi = G.init_psi(gg, G.SpinSystem(1), "ferromagnetic")
+    for cdd, bad in ((2e5, True), (1.0, False)):
+        ws = G.make_workspace(grid=gg, atom=G.Rb87, interactions=G.InteractionParams(1.0, 0.0),
+                              sim_params=G.SimParams(dt=0.01, n_steps=1, imaginary_time=True),
+                              psi_init=psi, enable_ddi=True, c_dd=cdd)
+        A.split_step(ws, 1)
+        phimax = float(ws.itp_guard()[0, 1])
+        fx, fy, fz = O.spin_density(psi, 1)
+        Q = O.ddi_q(og.n, og.dx)
+        ref = max(float(np.max(np.abs(f))) for f in O.ddi_potential(fx, fy, fz, Q, cdd))
+        assert phimax > 0.0
+        if bad:
+            assert phimax * 0.01 / 2 > 50 or not math.isfinite(phimax)
+            with pytest.raises(ValueError):
+                A._check_itp_overflow(ws, 1)
+        else:
+            assert phimax == pytest.approx(ref, rel=0.2)   # Phi of the evolved state
+            A._check_itp_overflow(ws, 1)
