@@ -80,6 +80,7 @@ SIGNATURES = {
     "bec_split_step_graph": ([C.c_void_p, C.c_int], C.c_int),
     "bec_fft_derivative": ([C.POINTER(BecConfig), C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_fft_gradient": ([C.POINTER(BecConfig), C.c_void_p, C.c_void_p], C.c_int),
+    "bec_fft_gradient_noalloc": ([C.POINTER(BecConfig), C.c_void_p, C.c_void_p], C.c_int),
     "bec_last_error": ([], C.c_char_p),
     "bec_workspace_bytes": ([C.POINTER(BecConfig)], C.c_longlong),
 }

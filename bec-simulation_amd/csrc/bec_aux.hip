@@ -1,6 +1,8 @@
-// Small pointwise kernels outside the split step: the spin-density observable.
+// Small pointwise kernels outside the split step: the spin-density observable and the
+// packing of real fields for the spectral-derivative entry points.
 #include <hip/hip_runtime.h>
 
+#include "bec_aux.h"
 #include "bec_device.h"
 #include "bec_kernels.h"
 
@@ -41,6 +43,40 @@ hipError_t launch_spin_density(hipStream_t st, const void* psi, double* out, lon
   if (nb > 8192) nb = 8192;
   hipLaunchKernelGGL((spin_density_kernel<T>), dim3((unsigned)nb), dim3(256), 0, st,
                      (const cplx<T>*)psi, out, nsites, D);
+  return hipGetLastError();
+}
+
+__global__ void grad_pack_kernel(const double* __restrict__ f, cplx<double>* __restrict__ psi,
+                                 long long n, int nd) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = f[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) psi[d * n + i] = mk(d < nd ? v : 0.0, 0.0);
+  }
+}
+
+__global__ void grad_unpack_kernel(const cplx<double>* __restrict__ psi, double* __restrict__ out,
+                                   long long n, int nd) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    for (int d = 0; d < nd; ++d) out[d * n + i] = psi[d * n + i].x;
+}
+
+static unsigned grid_of(long long n) {
+  long long nb = (n + 255) / 256;
+  return (unsigned)(nb > 8192 ? 8192 : (nb < 1 ? 1 : nb));
+}
+
+hipError_t launch_grad_pack(hipStream_t st, const double* f, void* psi, long long n, int nd) {
+  hipLaunchKernelGGL(grad_pack_kernel, dim3(grid_of(n)), dim3(256), 0, st, f,
+                     (cplx<double>*)psi, n, nd);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_unpack(hipStream_t st, const void* psi, double* out, long long n, int nd) {
+  hipLaunchKernelGGL(grad_unpack_kernel, dim3(grid_of(n)), dim3(256), 0, st,
+                     (const cplx<double>*)psi, out, n, nd);
   return hipGetLastError();
 }
 

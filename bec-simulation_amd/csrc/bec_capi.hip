@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/becgpu.h"
+#include "bec_aux.h"
 #include "bec_kernels.h"
 
 using namespace bec;
@@ -1193,6 +1194,159 @@ int bec_fft_derivative(const bec_config* grid, const double* f, int dim, double*
 
 int bec_fft_gradient(const bec_config* grid, const double* f, double* out) {
   return fft_gradient_impl(grid, f, -1, out);
+}
+
+// ---------------------------------------------------------------------------
+// bec_fft_gradient_noalloc: bec_fft_gradient for callers that cannot allocate device memory.
+// Every device buffer of the private spin-1 workspace (psi, the staging of f and of the
+// result, axis tables, twiddles, parameter block, active list, reduction partials) lives in
+// one file-scope array per GPU.  Host code never passes the array's name to the device: the
+// address comes from hipGetSymbolAddress, once per device, on that device's first call.
+// ---------------------------------------------------------------------------
+constexpr long long kNaMaxSites = 1LL << 18;  // 64^3, 512^2, ...
+constexpr size_t kNaAlign = 256;
+constexpr size_t na_up(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr int kNaMaxSide = 512;
+constexpr int kNaMaxBlocks = 1024;
+constexpr size_t kNaPsi = 0;
+constexpr size_t kNaF = kNaPsi + na_up(3 * kNaMaxSites * 16);
+constexpr size_t kNaOut = kNaF + na_up(kNaMaxSites * 8);
+constexpr size_t kNaK = kNaOut + na_up(3 * kNaMaxSites * 8);
+constexpr size_t kNaX = kNaK + na_up(3 * kNaMaxSide * 8);
+constexpr size_t kNaAx = kNaX + na_up(3 * kNaMaxSide * 8);
+constexpr size_t kNaParams = kNaAx + na_up(3 * kNaMaxSide * 8);
+constexpr size_t kNaActive = kNaParams + na_up(sizeof(InstParams));
+constexpr size_t kNaRed = kNaActive + na_up(sizeof(int));
+constexpr size_t kNaPart = kNaRed + na_up(RS_COUNT * 8);
+constexpr size_t kNaTw = kNaPart + na_up((size_t)RS_COUNT * kNaMaxBlocks * 8);
+constexpr size_t na_tw_bytes() {
+  size_t b = 0;
+  for (int i = 0; i < kNumFftSizes; ++i) b += na_up(2 * (size_t)kFftSizes[i] * 8);
+  return b;
+}
+constexpr size_t kNaBytes = kNaTw + na_tw_bytes();  // about 21.1 MB
+
+static __device__ __align__(256) unsigned char noalloc_scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_mu;
+static char* g_na_ptr[kNaMaxDevices] = {};  // device address of noalloc_scratch, per device
+
+// Device address of this device's array, with the twiddle tables (which never change)
+// written on the first call.  Caller holds g_na_mu.
+static int noalloc_setup(int dev, char** base) {
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    char* b = static_cast<char*>(p);
+    size_t off = kNaTw;
+    double buf[2 * kNaMaxSide];
+    for (int l = 0; l < kNumFftSizes; ++l) {
+      const int N = kFftSizes[l];
+      for (int m = 0; m < N; ++m) {
+        const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+        buf[2 * m] = (double)cosl(ang);
+        buf[2 * m + 1] = (double)sinl(ang);
+      }
+      HIPCHK(hipMemcpy(b + off, buf, 2 * (size_t)N * 8, hipMemcpyHostToDevice));
+      off += na_up(2 * (size_t)N * 8);
+    }
+    g_na_ptr[dev] = b;
+  }
+  *base = g_na_ptr[dev];
+  return BEC_OK;
+}
+
+int bec_fft_gradient_noalloc(const bec_config* grid, const double* f, double* out) {
+  if (!grid || !f || !out) return fail(BEC_EINVAL, "null argument");
+  const bec_config& c = *grid;
+  if (c.ndim < 1 || c.ndim > 3) return fail(BEC_EINVAL, "ndim must be 1, 2 or 3");
+  long long N = 1;
+  for (int d = 0; d < c.ndim; ++d) {
+    if (c.n[d] <= 0 || (c.n[d] % 2) != 0)
+      return fail(BEC_EINVAL, "n_points must be positive even integers");
+    if (!(c.L[d] > 0)) return fail(BEC_EINVAL, "box_size must be positive");
+    if (!side_supported(c.n[d]))
+      return fail(BEC_EUNSUP, "grid side without a built FFT line on this build");
+    N *= c.n[d];
+  }
+  if (N > kNaMaxSites)
+    return fail(BEC_EINVAL, "bec_fft_gradient_noalloc: grid of " + std::to_string(N) +
+                                " sites exceeds the fixed buffer of " +
+                                std::to_string(kNaMaxSites) +
+                                " sites (use bec_fft_gradient for larger grids)");
+  if (!c.x_tab || !c.k_tab) return fail(BEC_EINVAL, "missing tables");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) return fail(BEC_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  char* b = nullptr;
+  CHK(noalloc_setup(dev, &b));
+  const hipStream_t st = nullptr;
+  Geom g;
+  fill_geom(&c, 3, g);
+  const int nd = g.ndim;
+  // axis tables [3][nmax]: only the first n[d] entries of row d are read
+  int off = 0;
+  for (int d = 0; d < nd; ++d) {
+    HIPCHK(hipMemcpy(b + kNaK + (size_t)d * g.nmax * 8, c.k_tab + off, (size_t)c.n[d] * 8,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b + kNaX + (size_t)d * g.nmax * 8, c.x_tab + off, (size_t)c.n[d] * 8,
+                     hipMemcpyHostToDevice));
+    off += c.n[d];
+  }
+  InstParams ip{};
+  const int active0 = 0;
+  HIPCHK(hipMemcpy(b + kNaParams, &ip, sizeof(ip), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b + kNaActive, &active0, sizeof(int), hipMemcpyHostToDevice));
+  DevTables tb{};
+  tb.ktab = reinterpret_cast<const double*>(b + kNaK);
+  tb.xtab = reinterpret_cast<const double*>(b + kNaX);
+  tb.sktab = tb.ktab;
+  tb.axtab = reinterpret_cast<const double*>(b + kNaAx);
+  tb.Varr = nullptr;
+  tb.params = reinterpret_cast<const InstParams*>(b + kNaParams);
+  tb.active = reinterpret_cast<const int*>(b + kNaActive);
+  size_t toff = kNaTw;
+  for (int l = 0; l < kNumFftSizes; ++l) {
+    tb.tw[l] = b + toff;
+    toff += na_up(2 * (size_t)kFftSizes[l] * 8);
+  }
+  tb.red = reinterpret_cast<const double*>(b + kNaRed);
+  tb.partials = reinterpret_cast<double*>(b + kNaPart);
+  tb.max_blocks = kNaMaxBlocks;
+  tb.maxB = 1;
+  tb.work = nullptr;
+  void* psi = b + kNaPsi;
+  double* fdev = reinterpret_cast<double*>(b + kNaF);
+  double* odev = reinterpret_cast<double*>(b + kNaOut);
+  // every component of psi is written by the packing kernel (the array keeps the previous
+  // call's contents)
+  HIPCHK(hipMemcpy(fdev, f, (size_t)N * 8, hipMemcpyHostToDevice));
+  HIPCHK(launch_grad_pack(st, fdev, psi, N, nd));
+  // the schedule of bec_fft_gradient: forward transforms, i k_d / N on component d, inverse
+  XProgram p{};
+  if (nd == 1) {
+    p.stages = XS_KIN_1D;
+    p.deriv = 1;
+    HIPCHK(launch_xpass<double>(st, g, tb, p, psi, psi, nullptr, 1, nullptr));
+  } else {
+    p.stages = XS_FWD_FFT;
+    HIPCHK(launch_xpass<double>(st, g, tb, p, psi, psi, nullptr, 1, nullptr));
+    if (nd == 3) {
+      HIPCHK(launch_strided<double>(st, g, tb, 1, SM_FWD, psi, 1, 0.0, 0.0, 0, 0, nullptr));
+      HIPCHK(launch_strided<double>(st, g, tb, 2, SM_DERIV, psi, 1, 0.0, 0.0, 0, 0, nullptr));
+      HIPCHK(launch_strided<double>(st, g, tb, 1, SM_INV, psi, 1, 0.0, 0.0, 0, 0, nullptr));
+    } else {
+      HIPCHK(launch_strided<double>(st, g, tb, 1, SM_DERIV, psi, 1, 0.0, 0.0, 0, 0, nullptr));
+    }
+    XProgram q{};
+    q.stages = XS_INV_FFT;
+    HIPCHK(launch_xpass<double>(st, g, tb, q, psi, psi, nullptr, 1, nullptr));
+  }
+  HIPCHK(launch_grad_unpack(st, psi, odev, N, nd));
+  HIPCHK(hipMemcpy(out, odev, (size_t)nd * N * 8, hipMemcpyDeviceToHost));
+  return BEC_OK;
 }
 
 // Tile autotuner.  The x pass's rows per block set its LDS tile, its thread count and the

@@ -191,6 +191,15 @@ int bec_profile_work(bec_ws* ws, double* out);
  * (src/fft_utils.jl:25-44) and writes out[ndim][N]. */
 int bec_fft_derivative(const bec_config* cfg, const double* f, int dim, double* out);
 int bec_fft_gradient(const bec_config* cfg, const double* f, double* out);
+/* bec_fft_gradient without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, result and errors as
+ * bec_fft_gradient, computed by the same kernels; every device buffer it needs lives in one
+ * fixed array per GPU (about 21 MB, held until the process ends) that keeps the last call's
+ * contents.  Grids above 262144 sites (64^3) are rejected with BEC_EINVAL.  Calls on one GPU
+ * share that array: this library serialises them with a process-wide lock, and work that a
+ * caller issues on the array by other means would race with them. */
+int bec_fft_gradient_noalloc(const bec_config* cfg, const double* f, double* out);
 
 int bec_synchronize(bec_ws* ws);
 /* Kernel schedule switch (no counterpart in the reference): on (the default, or the
