@@ -16,7 +16,12 @@ REPO = os.path.dirname(ROOT)
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG_DIR, "libbecgpu.so")
 
-SOURCES = ["bec_kernels.hip", "bec_strided.hip", "bec_xsite.hip", "bec_aux.hip", "bec_capi.hip"]
+# (source, object, defines): bec_kernels.hip is compiled once for its shared code and once
+# per component count D = 2F+1, so that the parts compile in parallel
+KERNEL_DS = (1, 3, 5, 7, 9, 11, 13)
+UNITS = ([("bec_kernels.hip", f"bec_kernels_p{d}.o", [f"-DBEC_PART={d}"]) for d in (0,) + KERNEL_DS]
+         + [(src, os.path.splitext(src)[0] + ".o", [])
+            for src in ("bec_strided.hip", "bec_xsite.hip", "bec_aux.hip", "bec_capi.hip")])
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [
@@ -28,19 +33,20 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(unit) -> str:
+    src, objname, defs = unit
+    obj = os.path.join(BUILD, objname)
+    cmd = [HIPCC, *FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"hipcc failed for {src} {' '.join(defs)}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 8)) as ex:
+        objs = list(ex.map(_compile, UNITS))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -623,7 +623,7 @@ static void fill_geom(const bec_config* c, int D, Geom& g) {
   for (int d = 0; d < c->ndim; ++d) dV *= c->L[d] / c->n[d];
   g.dV = dV;
   g.nmax = std::max(g.n[0], std::max(g.n[1], g.n[2]));
-  g.spad = c->enable_ddi && c->ddi_padding ? 1 : 0;
+  g.spad = c->enable_ddi && c->ddi_padding && c->F > 0 ? 1 : 0;
   for (int d = 0; d < 3; ++d) g.sn[d] = d < c->ndim ? g.n[d] * (g.spad ? 2 : 1) : 1;
   g.smax = std::max(g.sn[0], std::max(g.sn[1], g.sn[2]));
   g.nxh = g.sn[0] / 2 + 1;
@@ -649,8 +649,12 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     if (!side_supported(c->n[d]))
       return fail(BEC_EUNSUP, "grid sides must be 2^k in [8, 512], 3*2^k in [24, 192], 5*2^k in [40, 160] or 7*2^k in [56, 224] on this build");
   }
-  if (!(c->F == 1 || c->F == 2 || c->F == 6)) return fail(BEC_EUNSUP, "F must be 1, 2 or 6");
-  if (c->F == 6 && c->n[0] > 256) return fail(BEC_EUNSUP, "F=6 supports x sides up to 256");
+  if (c->F < 0) return fail(BEC_EINVAL, "F must be non-negative");  // types.jl:43
+  if (c->F > 6) return fail(BEC_EUNSUP, "F above 6 (more than 13 components) is not built");
+  // one x-pass block holds whole x lines of every component, line_e(n_x) values per thread
+  if ((2 * c->F + 1) * (c->n[0] / line_e(c->n[0])) > 512)
+    return fail(BEC_EUNSUP, "x side too long for this many components (the x pass holds "
+                            "2F+1 whole lines in one block of at most 512 threads)");
   if (c->n_instances < 1) return fail(BEC_EINVAL, "n_instances must be >= 1");
   if (c->precision != BEC_FP64 && c->precision != BEC_FP32)
     return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
@@ -668,11 +672,13 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   ws->prec = c->precision;
   ws->itp = c->imaginary_time;
   ws->normalize_every = c->normalize_every;
-  ws->ddi = c->enable_ddi;
+  // The scalar gas (F = 0) has no spin density, so its dipolar potential and the rotation
+  // by it are exactly zero and the identity: the convolution is left out (ddi.jl:108-201)
+  ws->ddi = c->enable_ddi && c->F > 0;
   ws->tensor = c->tensor;
-  if (ws->tensor && ws->F != 6) {
+  if (ws->tensor && ws->F < 1) {
     delete ws;
-    return fail(BEC_EUNSUP, "the general-F tensor step is built for F=6");
+    return fail(BEC_EINVAL, "the tensor step needs F >= 1");
   }
   ws->csz = ws->prec == BEC_FP32 ? 8 : 16;
   Geom& g = ws->g;
@@ -1668,8 +1674,15 @@ int bec_profile_work(bec_ws* ws, double* out) {
   }
   out[0] = (double)nmv;
   out[1] = (double)nrot;
-  out[2] = ws->tensor && ws->F == 6 ? tensor_matvec_flops<6>() : 0.0;
-  out[3] = ws->F == 6 ? euler_flops<6>() : ws->F == 2 ? euler_flops<2>() : euler_flops<1>();
+  static const double kTensorFlops[7] = {0.0,
+                                         tensor_matvec_flops<1>(), tensor_matvec_flops<2>(),
+                                         tensor_matvec_flops<3>(), tensor_matvec_flops<4>(),
+                                         tensor_matvec_flops<5>(), tensor_matvec_flops<6>()};
+  static const double kEulerFlops[7] = {euler_flops<0>(), euler_flops<1>(), euler_flops<2>(),
+                                        euler_flops<3>(), euler_flops<4>(), euler_flops<5>(),
+                                        euler_flops<6>()};
+  out[2] = ws->tensor ? kTensorFlops[ws->F] : 0.0;
+  out[3] = kEulerFlops[ws->F];
   for (int k = 0; k < BEC_K_COUNT; ++k) {
     out[4 + k] = ws->prof_bytes[k];
     ws->prof_bytes[k] = 0.0;

@@ -104,6 +104,10 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
     for (int k = 0; k < TWK; ++k) tws[min(tid + k * nthr, NX - 1)] = twv[k];
   }
 
+  // Blocks of fewer than 3 NX / EX threads (the scalar gas, F = 0, on one short line: one
+  // x line per block) cover the rest of the twiddle table here; elsewhere this loop is empty.
+  for (int i = tid + TWK * nthr; i < NX; i += nthr) tws[i] = twg[i];
+
   // Op lists in LDS (A at [0, kMaxOps), B at [kMaxOps, 2 kMaxOps)), copied with
   // compile-time indices so the by-value kernel argument is never indexed at run time.
   // Slot i is copied by thread i, or by thread 0 when the block is smaller than that (1-D
@@ -381,7 +385,7 @@ energy_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ psi, const T* __
       e_ram += phx * (double)fx + phy * (double)fy + phz * (double)fz;
     }
     if (with_tensor) {
-      if constexpr (D == 13) e_ten += tensor_energy_site<T, D>(sp, P);
+      if constexpr (D >= 3) e_ten += tensor_energy_site<T, D>(sp, P);
     } else if (P.c2 != 0.0) {  // energy.jl:43-46,106-109: 1/2 c2 |A00|^2, tensor slot
       double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -448,7 +452,7 @@ __device__ __forceinline__ void site_ops(const Geom& g, const DevTables& tb, con
         nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
       }
     } else if (op.kind == OP_TENSOR) {
-      if constexpr (D == 13) nmv += op_tensor<T, D>(sp, P, dtf, itp);
+      if constexpr (D >= 3) nmv += op_tensor<T, D>(sp, P, dtf, itp);
     } else if (op.kind == OP_SCALE) {
       const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 #pragma unroll
@@ -505,6 +509,7 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
 }
 
 
+#if BEC_PART == 0  // shape-independent kernels: in the shared part only
 // ---------------------------------------------------------------------------
 // reductions and small passes
 // ---------------------------------------------------------------------------
@@ -678,37 +683,18 @@ __global__ void diff_kernel(Geom g, DevTables tb, const cplx<T>* __restrict__ a,
   }
 }
 
+#endif  // BEC_PART == 0
+
 // ---------------------------------------------------------------------------
 // launchers
+//
+// This file is compiled once per part: BEC_PART = 0 builds the shape-independent code (the
+// dispatchers, reductions, small passes); BEC_PART = D builds the x-pass, energy and
+// lane-per-site kernels of D = 2F+1 components only, so the parts compile in parallel.
 // ---------------------------------------------------------------------------
-static int xpass_tile_elems() {
-  static int v = [] {
-    const char* e = getenv("BEC_XPASS_TILE");
-    const int t = e ? atoi(e) : 0;
-    return t >= 256 ? t : 1536;
-  }();
-  return v;
-}
-
-static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int AL);
-
-int xpass_rows(int D, int NX, int nrows) {
-  // the element budget is for 8 values per thread; lines with E = 12, 20 or 28 take up to
-  // twice it so that their blocks keep about the same thread count (at most twice: an FP64
-  // spin-1 tile with zero-padded DDI lines then still fits in LDS, about 130 KB)
-  const int e = line_e(NX);
-  int target = xpass_tile_elems() * std::min(e, 16) / 8 / (D * NX);
-  const int maxthr = 512;
-  while (target > 1 && D * target * (NX / e) > maxthr) --target;
-  if (target < 1) target = 1;
-  int r = 1;
-  while (r * 2 <= target) r *= 2;
-  while (r > 1 && (nrows % r) != 0) r /= 2;
-  if (r > nrows) r = nrows;
-  // LDS bound for the largest program (FP64, zero-padded DDI lines)
-  while (r > 1 && xpass_lds(D, r, NX, 16, true, 2 * NX) > 160 * 1024) r /= 2;
-  return r;
-}
+#ifndef BEC_PART
+#error "compile with -DBEC_PART=0 (shared code) or -DBEC_PART=D (kernels of D components)"
+#endif
 
 static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int AL) {
   // padded tiles: 9 slots per 8 elements; AL: length of the DDI auxiliary lines
@@ -717,18 +703,6 @@ static size_t xpass_lds(int D, int rows, int NX, size_t csz, bool has_aux, int A
                 (has_aux ? 2 * (size_t)rows * AL / 8 * 9 : 0)) +
          sizeof(cplx<double>) * 2 * kMaxOps * D + 16 * sizeof(double) +
          sizeof(cplx<double>) * NX;  // twiddles
-}
-
-int xpass_rows_g(const Geom& g) {
-  return g.xrows > 0 ? g.xrows : xpass_rows(g.D, g.n[0], g.nrows);
-}
-
-bool xpass_rows_ok(const Geom& g, int rows) {
-  // the LDS bound is taken for FP64 with the DDI lines, so one choice fits every program
-  const int AL = g.spad ? 2 * g.n[0] : g.n[0];
-  return rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
-         g.D * rows * (g.n[0] / line_e(g.n[0])) <= 512 &&
-         xpass_lds(g.D, rows, g.n[0], sizeof(cplx<double>), true, AL) <= 160 * 1024;
 }
 
 template <typename T, int NX, int D>
@@ -748,10 +722,8 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
   return hipGetLastError();
 }
 
-int xpass_grid(const Geom& g) { return g.nrows / xpass_rows_g(g); }
-
 template <typename T, int D>
-static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
+hipError_t xpass_launch_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
                           const void* src, void* dst, void* fhat, int n_active, void* phi) {
   switch (g.n[0]) {
     case 8: return xpass_n<T, 8, D>(st, g, tb, prog, src, dst, fhat, n_active, phi);
@@ -775,15 +747,103 @@ static hipError_t xpass_d(hipStream_t st, const Geom& g, const DevTables& tb, co
   }
 }
 
+template <typename T, int D>
+hipError_t energy_launch_d(hipStream_t st, const Geom& g, const DevTables& tb, const void* psi,
+                           const void* phi, int with_tensor, dim3 grid) {
+  hipLaunchKernelGGL((energy_kernel<T, D>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
+                     (const T*)phi, with_tensor);
+  return hipGetLastError();
+}
+
+template <typename T, int D>
+hipError_t sites_launch_d(hipStream_t st, const Geom& g, const DevTables& tb, const SProgram& prog,
+                          void* psi, const void* phi, dim3 grid) {
+  hipLaunchKernelGGL((site_kernel<T, D>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
+                     (const T*)phi);
+  return hipGetLastError();
+}
+
+#define BEC_D_INST(PRE, T, D)                                                                 \
+  PRE template hipError_t xpass_launch_d<T, D>(hipStream_t, const Geom&, const DevTables&,     \
+                                               const XProgram&, const void*, void*, void*, int,\
+                                               void*);                                         \
+  PRE template hipError_t energy_launch_d<T, D>(hipStream_t, const Geom&, const DevTables&,    \
+                                                const void*, const void*, int, dim3);          \
+  PRE template hipError_t sites_launch_d<T, D>(hipStream_t, const Geom&, const DevTables&,     \
+                                               const SProgram&, void*, const void*, dim3);
+
+#if BEC_PART == 0
+// the per-D launchers live in the other parts
+#define BEC_D_EXTERN(D) BEC_D_INST(extern, float, D) BEC_D_INST(extern, double, D)
+BEC_D_EXTERN(1)
+BEC_D_EXTERN(3)
+BEC_D_EXTERN(5)
+BEC_D_EXTERN(7)
+BEC_D_EXTERN(9)
+BEC_D_EXTERN(11)
+BEC_D_EXTERN(13)
+
+static int xpass_tile_elems() {
+  static int v = [] {
+    const char* e = getenv("BEC_XPASS_TILE");
+    const int t = e ? atoi(e) : 0;
+    return t >= 256 ? t : 1536;
+  }();
+  return v;
+}
+
+int xpass_rows(int D, int NX, int nrows) {
+  // the element budget is for 8 values per thread; lines with E = 12, 20 or 28 take up to
+  // twice it so that their blocks keep about the same thread count (at most twice: an FP64
+  // spin-1 tile with zero-padded DDI lines then still fits in LDS, about 130 KB)
+  const int e = line_e(NX);
+  int target = xpass_tile_elems() * std::min(e, 16) / 8 / (D * NX);
+  const int maxthr = 512;
+  while (target > 1 && D * target * (NX / e) > maxthr) --target;
+  if (target < 1) target = 1;
+  int r = 1;
+  while (r * 2 <= target) r *= 2;
+  while (r > 1 && (nrows % r) != 0) r /= 2;
+  if (r > nrows) r = nrows;
+  // LDS bound for the largest program (FP64, zero-padded DDI lines)
+  while (r > 1 && xpass_lds(D, r, NX, 16, true, 2 * NX) > 160 * 1024) r /= 2;
+  return r;
+}
+
+
+int xpass_rows_g(const Geom& g) {
+  return g.xrows > 0 ? g.xrows : xpass_rows(g.D, g.n[0], g.nrows);
+}
+
+bool xpass_rows_ok(const Geom& g, int rows) {
+  // the LDS bound is taken for FP64 with the DDI lines, so one choice fits every program
+  const int AL = g.spad ? 2 * g.n[0] : g.n[0];
+  return rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
+         g.D * rows * (g.n[0] / line_e(g.n[0])) <= 512 &&
+         xpass_lds(g.D, rows, g.n[0], sizeof(cplx<double>), true, AL) <= 160 * 1024;
+}
+
+int xpass_grid(const Geom& g) { return g.nrows / xpass_rows_g(g); }
+
+// Component counts with built kernels: every spin F = 0 .. 6.
+#define BEC_D_SWITCH(CALL)       \
+  switch (g.D) {                 \
+    case 1: return CALL(1);      \
+    case 3: return CALL(3);      \
+    case 5: return CALL(5);      \
+    case 7: return CALL(7);      \
+    case 9: return CALL(9);      \
+    case 11: return CALL(11);    \
+    case 13: return CALL(13);    \
+    default: return hipErrorInvalidValue; \
+  }
+
 template <typename T>
 hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
                         const void* src, void* dst, void* fhat, int n_active, void* phi) {
-  switch (g.D) {
-    case 3: return xpass_d<T, 3>(st, g, tb, prog, src, dst, fhat, n_active, phi);
-    case 5: return xpass_d<T, 5>(st, g, tb, prog, src, dst, fhat, n_active, phi);
-    case 13: return xpass_d<T, 13>(st, g, tb, prog, src, dst, fhat, n_active, phi);
-    default: return hipErrorInvalidValue;
-  }
+#define BEC_CALL(D) xpass_launch_d<T, D>(st, g, tb, prog, src, dst, fhat, n_active, phi)
+  BEC_D_SWITCH(BEC_CALL)
+#undef BEC_CALL
 }
 
 template <typename T>
@@ -793,42 +853,18 @@ hipError_t launch_energy(hipStream_t st, const Geom& g, const DevTables& tb, con
   if (nb > 1024) nb = 1024;
   *nblocks_out = (int)nb;
   const dim3 grid((unsigned)nb, n_active);
-  switch (g.D) {
-    case 13:
-      hipLaunchKernelGGL((energy_kernel<T, 13>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
-                         (const T*)phi, with_tensor);
-      return hipGetLastError();
-    case 5:
-      hipLaunchKernelGGL((energy_kernel<T, 5>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
-                         (const T*)phi, with_tensor);
-      return hipGetLastError();
-    case 3:
-      hipLaunchKernelGGL((energy_kernel<T, 3>), grid, dim3(256), 0, st, g, tb, (const cplx<T>*)psi,
-                         (const T*)phi, with_tensor);
-      return hipGetLastError();
-    default: return hipErrorInvalidValue;
-  }
+#define BEC_CALL(D) energy_launch_d<T, D>(st, g, tb, psi, phi, with_tensor, grid)
+  BEC_D_SWITCH(BEC_CALL)
+#undef BEC_CALL
 }
 
 template <typename T>
 hipError_t launch_sites(hipStream_t st, const Geom& g, const DevTables& tb, const SProgram& prog,
                         void* psi, const void* phi, int n_active) {
   const dim3 grid((unsigned)((g.nsites + 255) / 256), n_active);
-  switch (g.D) {
-    case 13:
-      hipLaunchKernelGGL((site_kernel<T, 13>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
-                         (const T*)phi);
-      return hipGetLastError();
-    case 5:
-      hipLaunchKernelGGL((site_kernel<T, 5>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
-                         (const T*)phi);
-      return hipGetLastError();
-    case 3:
-      hipLaunchKernelGGL((site_kernel<T, 3>), grid, dim3(256), 0, st, g, tb, prog, (cplx<T>*)psi,
-                         (const T*)phi);
-      return hipGetLastError();
-    default: return hipErrorInvalidValue;
-  }
+#define BEC_CALL(D) sites_launch_d<T, D>(st, g, tb, prog, psi, phi, grid)
+  BEC_D_SWITCH(BEC_CALL)
+#undef BEC_CALL
 }
 
 hipError_t launch_reduce(hipStream_t st, const DevTables& tb, double* red, int n_active,
@@ -903,5 +939,10 @@ hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n)
                                                  double*);
 BEC_INST(double)
 BEC_INST(float)
+
+#else  // BEC_PART == D: the kernels of D components
+BEC_D_INST(, float, BEC_PART)
+BEC_D_INST(, double, BEC_PART)
+#endif
 
 }  // namespace bec

@@ -147,13 +147,13 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
           nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
         }
       } else if (op.kind == OP_TENSOR) {
-        if constexpr (D == 13 && sizeof(T) == 8) {
+        if constexpr (D >= 3 && sizeof(T) == 8) {
           // FP64: the Taylor sum lives in this thread's own tile slots (written back
           // below), which keeps the step within 256 VGPRs (2 waves/SIMD)
           auto at = [TS, tid](int c) __attribute__((always_inline)) { return p8(c * TS + tid); };
           LdsAcc<T, decltype(at)> acc{tile, at};
           nmv += op_tensor<T, D>(sp, P, dtf, itp, acc);
-        } else if constexpr (D == 13) {
+        } else if constexpr (D >= 3) {
           nmv += op_tensor<T, D>(sp, P, dtf, itp);
         }
       } else if (op.kind == OP_SCALE) {

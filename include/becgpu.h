@@ -53,7 +53,9 @@ typedef struct bec_config {
                           [56, 224]; other even sides (valid in the reference) return
                           BEC_EUNSUP */
   double L[3];         /* box size per axis */
-  int F;               /* spin (2F+1 components); F in {1, 2, 6} */
+  int F;               /* spin (2F+1 components), 0 <= F <= 6 (SpinSystem, types.jl:36-46);
+                          2F+1 whole x lines must fit one block: (2F+1) * n[0] / E <= 512
+                          with E = 8 (2^k sides), 12, 20 or 28 */
   int n_instances;     /* batch size B */
   int precision;       /* BEC_FP64 or BEC_FP32 */
   int imaginary_time;  /* SimParams.imaginary_time */
@@ -62,7 +64,9 @@ typedef struct bec_config {
   int ddi_secular;     /* make_ddi_params(secular=...) */
   int ddi_padding;     /* make_workspace(ddi_padding=...): 2x zero-padded DDI, sides <= 256 */
   int tensor;          /* general-F tensor step replaces spin mixing (TensorInteractionCache
-                        * active, initialization.jl:164-185); couplings via bec_set_tensor */
+                        * active, initialization.jl:164-185; any F >= 1, also a cache built
+                        * directly, _make_tensor_cache_from_channels); couplings via
+                        * bec_set_tensor */
   /* Cell-centred coordinates and fftfreq wavenumbers of each axis, concatenated
    * x then y then z (grid.jl:1-23). */
   const double* x_tab;

@@ -616,11 +616,16 @@ class OracleWS:
 def make_workspace(grid, F, psi, dt, itp, *, omega=None, V=None, p=0.0, q=0.0,
                    c0=0.0, c1=0.0, c_lhy=0.0, c_extra=(), ddi=False, cdd=0.0,
                    secular=False, padding=False, n_steps=1, normalize_every=None,
-                   save_every=None, loss=None, raman=None):
-    """initialization.jl:86-191 (the subset on the hot path)."""
+                   save_every=None, loss=None, raman=None, channels=None):
+    """initialization.jl:86-191 (the subset on the hot path).  channels: an explicit
+    tensor cache {S: g_S} (_make_tensor_cache_from_channels, tensor_interaction.jl:106-119)
+    used instead of the activation rule; c0 and c1 are then left to the tensor step."""
     if V is None:
         V = harmonic_potential(grid, omega) if omega is not None else np.zeros(grid.shape)
-    ch = tensor_channels(F, c0, c1, list(c_extra))
+    if channels is not None:
+        ch = {S: float(g) for S, g in channels.items() if abs(g) > 1e-30}
+    else:
+        ch = tensor_channels(F, c0, c1, list(c_extra))
     T = None
     c2 = float(c_extra[0]) if len(c_extra) >= 1 else 0.0   # get_cn(ip, 2), types.jl:157-162
     if ch is not None:

@@ -55,7 +55,9 @@ def _cfg(**kw):
 @pytest.mark.parametrize("kw,code", [
     (dict(n=(63, 1, 1)), L.BEC_EINVAL),
     (dict(ndim=4), L.BEC_EINVAL),
-    (dict(F=3), L.BEC_EUNSUP),
+    (dict(F=7), L.BEC_EUNSUP),                  # more than 13 components: not built
+    (dict(F=-1), L.BEC_EINVAL),                 # SpinSystem(F) needs F >= 0 (types.jl:43)
+    (dict(F=6, n=(512, 1, 1)), L.BEC_EUNSUP),   # 13 whole 512-point lines exceed one block
     (dict(n=(72, 1, 1)), L.BEC_EUNSUP),
     (dict(n=(320, 1, 1)), L.BEC_EUNSUP),
     (dict(n=(120, 1, 1)), L.BEC_EUNSUP),
