@@ -63,6 +63,8 @@ struct bec_ws {
   int* d_active = nullptr;
   unsigned long long* d_work = nullptr;  // profiling counters (tensor matvecs, spin rotations)
   double* d_sdens = nullptr;  // bec_spin_density output staging, [3][nsites] (first use)
+  double* d_kfold = nullptr;  // [B][tens::kKfoldMax] channel-folded tensor kernels (tensor ws)
+  std::vector<double> h_kfold;
   double* d_cpart = nullptr;   // constrained normalisation: per-component partials, targets,
   double* d_ctarget = nullptr; // factors (allocated on first use)
   double* d_cfac = nullptr;
@@ -151,12 +153,16 @@ static void refresh_tables(bec_ws* ws) {
   t.max_blocks = ws->max_blocks;
   t.maxB = ws->B;
   t.work = ws->prof_on ? ws->d_work : nullptr;
+  t.kfold = ws->d_kfold;
 }
 
 static int sync_params(bec_ws* ws) {
   if (!ws->params_dirty) return BEC_OK;
   HIPCHK(hipMemcpyAsync(ws->d_params, ws->hp.data(), sizeof(InstParams) * ws->B,
                         hipMemcpyHostToDevice, ws->stream));
+  if (ws->d_kfold)
+    HIPCHK(hipMemcpyAsync(ws->d_kfold, ws->h_kfold.data(), ws->h_kfold.size() * 8,
+                          hipMemcpyHostToDevice, ws->stream));
   HIPCHK(hipStreamSynchronize(ws->stream));
   ws->params_dirty = false;
   return BEC_OK;
@@ -554,6 +560,24 @@ static double tensor_matvec_flops() {
   return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
 }
 
+// The same for the channel-folded matvec of the FP64 path (tensor_matvec_folded): pair
+// products, K_M P (complex*real: 2 flop for the first entry of a row, 4 after), w updates.
+template <int F>
+static double tensor_matvec_folded_flops() {
+  double f = 0.0;
+  for (int M = -2 * F; M <= 2 * F; ++M) {
+    const int np = tens::npairs(F, M);
+    for (int i = 0; i < np; ++i) {
+      const int m1 = tens::m1lo(F, M) + i;
+      const bool off = m1 != M - m1;
+      f += off ? 6 + 8 : 6;            // P_i
+      f += 2.0 + 4.0 * (np - 1);       // C_i = sum_j K_ij P_j
+      f += off ? 16 : 8;               // w update(s) conj(psi) * C
+    }
+  }
+  return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
+}
+
 // One Euler rotation (op_euler): 4 products of the real matrix d^F(pi/2) with a complex
 // spinor (4 flop per nonzero entry) and 5 diagonal phase sweeps (12 flop per component).
 template <int F>
@@ -562,6 +586,36 @@ static double euler_flops() {
   int nz = 0;
   for (double x : d.v) nz += x != 0.0;
   return 16.0 * nz + 60.0 * (2 * F + 1);
+}
+
+// K_M[i][j] = sum_S CG(m1_i, M - m1_i | S M) g_S CG(m1_j, M - m1_j | S M) for every M, in
+// tens::kfold_off order (the kernel of tensor_matvec_folded), from the compile-time table.
+template <int F>
+static void fold_channels_f(const double* gS, double* out) {
+  constexpr tens::CGTab<F> cg{};
+  for (int M = -2 * F; M <= 2 * F; ++M) {
+    const int lo = tens::m1lo(F, M), np = tens::npairs(F, M), s0 = tens::s0(M);
+    const int ns = tens::ns(F, M);
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < np; ++j) {
+        double k = 0.0;
+        for (int q = 0; q < ns; ++q)
+          k += cg.v[tens::off(F, M, lo + i, s0) + q] * gS[s0 / 2 + q] *
+               cg.v[tens::off(F, M, lo + j, s0) + q];
+        out[tens::kfold_off(F, M) + i * np + j] = k;
+      }
+  }
+}
+static void fold_channels(int F, const double* gS, double* out) {
+  switch (F) {
+    case 1: fold_channels_f<1>(gS, out); break;
+    case 2: fold_channels_f<2>(gS, out); break;
+    case 3: fold_channels_f<3>(gS, out); break;
+    case 4: fold_channels_f<4>(gS, out); break;
+    case 5: fold_channels_f<5>(gS, out); break;
+    case 6: fold_channels_f<6>(gS, out); break;
+    default: break;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -756,6 +810,13 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (hipMemset(ws->d_axtab, 0, axb) != hipSuccess) return cleanup(BEC_EDEVICE, "memset");
   if (hipMalloc(&ws->d_params, sizeof(InstParams) * ws->B) != hipSuccess)
     return cleanup(BEC_ENOMEM, "params");
+  if (ws->tensor) {
+    ws->h_kfold.assign((size_t)ws->B * tens::kKfoldMax, 0.0);
+    if (hipMalloc(&ws->d_kfold, ws->h_kfold.size() * 8) != hipSuccess)
+      return cleanup(BEC_ENOMEM, "tensor kernels");
+    if (hipMemset(ws->d_kfold, 0, ws->h_kfold.size() * 8) != hipSuccess)
+      return cleanup(BEC_EDEVICE, "memset");
+  }
   if (hipMalloc(&ws->d_active, sizeof(int) * ws->B) != hipSuccess) return cleanup(BEC_ENOMEM, "active");
   if (hipMemcpy(ws->d_active, ws->active.data(), sizeof(int) * ws->B, hipMemcpyHostToDevice) !=
       hipSuccess)
@@ -805,7 +866,7 @@ void bec_ws_destroy(bec_ws* ws) {
   if (ws->d_params) (void)hipFree(ws->d_params);
   if (ws->d_active) (void)hipFree(ws->d_active);
   if (ws->d_work) (void)hipFree(ws->d_work);
-  for (double* q : {ws->d_cpart, ws->d_ctarget, ws->d_cfac, ws->d_sdens})
+  for (double* q : {ws->d_cpart, ws->d_ctarget, ws->d_cfac, ws->d_sdens, ws->d_kfold})
     if (q) (void)hipFree(q);
   for (int i = 0; i < kNumFftSizes; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
@@ -869,6 +930,7 @@ int bec_set_tensor(bec_ws* ws, int inst, const double* gS, int n) {
   if (!ws->tensor) return fail(BEC_EINVAL, "workspace was created without the tensor step");
   if (n != ws->F + 1) return fail(BEC_EINVAL, "expected F+1 channel couplings (S = 0, 2, ..., 2F)");
   for (int k = 0; k < 7; ++k) ws->hp[inst].gS[k] = k < n ? gS[k] : 0.0;
+  fold_channels(ws->F, gS, ws->h_kfold.data() + (size_t)inst * tens::kKfoldMax);
   ws->params_dirty = true;
   return BEC_OK;
 }
@@ -1681,7 +1743,11 @@ int bec_profile_work(bec_ws* ws, double* out) {
   static const double kEulerFlops[7] = {euler_flops<0>(), euler_flops<1>(), euler_flops<2>(),
                                         euler_flops<3>(), euler_flops<4>(), euler_flops<5>(),
                                         euler_flops<6>()};
-  out[2] = ws->tensor ? kTensorFlops[ws->F] : 0.0;
+  static const double kFoldedFlops[7] = {
+      0.0, tensor_matvec_folded_flops<1>(), tensor_matvec_folded_flops<2>(),
+      tensor_matvec_folded_flops<3>(), tensor_matvec_folded_flops<4>(),
+      tensor_matvec_folded_flops<5>(), tensor_matvec_folded_flops<6>()};
+  out[2] = !ws->tensor ? 0.0 : ws->prec == BEC_FP64 ? kFoldedFlops[ws->F] : kTensorFlops[ws->F];
   out[3] = kEulerFlops[ws->F];
   for (int k = 0; k < BEC_K_COUNT; ++k) {
     out[4 + k] = ws->prof_bytes[k];

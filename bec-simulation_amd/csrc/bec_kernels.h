@@ -66,6 +66,8 @@ struct DevTables {
   // blocks of a launch spread their atomics over many addresses; slot s holds
   // [s * kWorkStride] tensor matvecs and [s * kWorkStride + 1] spin rotations
   unsigned long long* work;
+  // [B][tens::kKfoldMax] channel-folded tensor kernels (FP64 tensor step), or null
+  const double* kfold;
   int max_blocks;
   int maxB;
 };

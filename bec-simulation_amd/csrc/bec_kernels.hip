@@ -452,7 +452,8 @@ __device__ __forceinline__ void site_ops(const Geom& g, const DevTables& tb, con
         nrot += op_euler<T, D>(sp, phx, phy, phz, dtf, itp);
       }
     } else if (op.kind == OP_TENSOR) {
-      if constexpr (D >= 3) nmv += op_tensor<T, D>(sp, P, dtf, itp);
+      if constexpr (D >= 3)
+        nmv += op_tensor<T, D>(sp, P, dtf, itp, tb.kfold + (size_t)inst * tens::kKfoldMax);
     } else if (op.kind == OP_SCALE) {
       const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 #pragma unroll

@@ -112,6 +112,16 @@ __host__ __device__ constexpr int base(int F, int M) {
   return b;
 }
 __host__ __device__ constexpr int size(int F) { return base(F, 2 * F + 1); }
+// Channel-folded kernel K_M[i][j] = sum_S CG(i | S M) g_S CG(j | S M) over the unordered
+// pairs i, j of each M, stored M by M as full npairs x npairs blocks (kfold_off(F, M) is the
+// start of block M).
+__host__ __device__ constexpr int kfold_off(int F, int M) {
+  int b = 0;
+  for (int q = -2 * F; q < M; ++q) b += npairs(F, q) * npairs(F, q);
+  return b;
+}
+__host__ __device__ constexpr int kfold_size(int F) { return kfold_off(F, 2 * F + 1); }
+constexpr int kKfoldMax = 413;  // kfold_size(6)
 __host__ __device__ constexpr int off(int F, int M, int m1, int S) {
   return base(F, M) + (m1 - m1lo(F, M)) * ns(F, M) + (S - s0(M)) / 2;
 }

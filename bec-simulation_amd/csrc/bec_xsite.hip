@@ -152,9 +152,9 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
           // below), which keeps the step within 256 VGPRs (2 waves/SIMD)
           auto at = [TS, tid](int c) __attribute__((always_inline)) { return p8(c * TS + tid); };
           LdsAcc<T, decltype(at)> acc{tile, at};
-          nmv += op_tensor<T, D>(sp, P, dtf, itp, acc);
+          nmv += op_tensor<T, D>(sp, P, dtf, itp, acc, tb.kfold + (size_t)inst * tens::kKfoldMax);
         } else if constexpr (D >= 3) {
-          nmv += op_tensor<T, D>(sp, P, dtf, itp);
+          nmv += op_tensor<T, D>(sp, P, dtf, itp, tb.kfold + (size_t)inst * tens::kKfoldMax);
         }
       } else if (op.kind == OP_SCALE) {
         const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);

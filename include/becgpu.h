@@ -179,7 +179,8 @@ int bec_profile_enable(bec_ws* ws, int on);
 int bec_profile_read(bec_ws* ws, double* ms, long long* count);
 /* Work executed by the lane-per-site pass since bec_profile_enable (or the last call):
  * out[0] tensor matvecs (Taylor terms of the tensor step), out[1] Euler spin rotations,
- * out[2] flops per tensor matvec, out[3] flops per rotation, out[4 + k] algorithmic HBM
+ * out[2] flops per tensor matvec (the channel-folded form on FP64 workspaces), out[3] flops
+ * per rotation, out[4 + k] algorithmic HBM
  * bytes issued by kernel class k (BEC_K_*; 0 for BEC_K_REDUCE and BEC_K_ENERGY), so
  * out[4 + k] / time of class k is its achieved algorithmic bandwidth;
  * out[4 + BEC_K_COUNT] FFT flops of the fused x/site launches (5 N log2 N per complex
