@@ -17,6 +17,7 @@ BEC_FP64, BEC_FP32 = 0, 1
  E_TOTAL, E_COUNT) = range(13)
 E_NAMES = ["kin", "trap", "zeeman", "c0", "c1", "ddi", "lhy", "tensor", "norm", "mz", "raman",
            "total"]
+CHUNK_E, CHUNK_DPSI, CHUNK_NORM, CHUNK_PHIMAX, CHUNK_STATUS, CHUNK_COUNT = range(6)
 K_NAMES = ["xpass", "fwd", "kin", "inv", "ke", "ddi_fft", "ddi_conv", "reduce", "sites", "energy",
            "xsite"]
 
@@ -66,6 +67,7 @@ SIGNATURES = {
     "bec_diff": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "bec_itp_guard": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_spin_density": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "bec_itp_chunk": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_profile_enable": ([C.c_void_p, C.c_int], C.c_int),
     "bec_profile_read": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "bec_profile_work": ([C.c_void_p, C.c_void_p], C.c_int),

@@ -253,6 +253,14 @@ class Workspace:
     def copy_slot(self, dst, src):
         L.check(L.lib().bec_copy_slot(self.handle, dst, src), "bec_copy_slot")
 
+    def itp_chunk(self, n_steps):
+        """bec_itp_chunk: n_steps ITP split steps, then per instance
+        [energy, dpsi vs slot 1, norm, max |Phi|, status] and slot 1 <- psi."""
+        out = np.zeros((self.B, L.CHUNK_COUNT))
+        L.check(L.lib().bec_itp_chunk(self.handle, int(n_steps), out.ctypes.data),
+                "bec_itp_chunk")
+        return out
+
     def spin_density(self, inst=0):
         """_compute_spin_density! (observables.jl:61-88): (Fx, Fy, Fz), each of the grid's
         numpy shape, in FP64."""
@@ -740,6 +748,20 @@ def _check_itp_overflow(ws: Workspace, step: int, insts=None):
                                  f"{ITP_EXPONENT_LIMIT}. Reduce dt (current={ws.params[i].dt}).")
 
 
+def _check_chunk_overflow(ws: Workspace, ch, step: int, insts):
+    """ground_state.jl:4-29 on the guard values that bec_itp_chunk returns."""
+    for i in insts:
+        if ch[i, L.CHUNK_STATUS] != 0.0:
+            raise ValueError(f"NaN detected in ITP at step {step}. Likely DDI potential overflow. "
+                             "Reduce dt.")
+        if ws.enable_ddi:
+            exponent = ch[i, L.CHUNK_PHIMAX] * ws.params[i].dt / 2
+            if exponent > ITP_EXPONENT_LIMIT:
+                raise ValueError(f"DDI potential overflow in ITP at step {step}: "
+                                 f"max|Phi|={ch[i, L.CHUNK_PHIMAX]:.3g}, exponent={exponent:.1f} > "
+                                 f"{ITP_EXPONENT_LIMIT}. Reduce dt (current={ws.params[i].dt}).")
+
+
 def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int, target_mz=None):
     """The ITP convergence loop of find_ground_state for every active instance of ws
     (ground_state.jl:132-162).  Instances retire independently when converged.  With
@@ -758,21 +780,35 @@ def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int, target_m
             k = 1
         else:
             k = min(save_every - step % save_every, n_steps - step)
-        split_step(ws, k)
-        step += k
-        if step <= 10:
-            _check_itp_overflow(ws, step, running)
-        if target_mz is not None:
-            ws.normalize_constrained(target_mz)
-        if step % save_every == 0:
-            et = ws.energy_terms()
-            dd = ws.diff(0, 1, 1)
+        check = (step + k) % save_every == 0
+        if check and target_mz is None:
+            # steps, energy, dpsi and psi_prev <- psi with one synchronisation
+            ch = ws.itp_chunk(k)
+            step += k
+            if step <= 10:
+                _check_chunk_overflow(ws, ch, step, running)
+        else:
+            split_step(ws, k)
+            step += k
+            if step <= 10:
+                _check_itp_overflow(ws, step, running)
+            if target_mz is not None:
+                ws.normalize_constrained(target_mz)
+            ch = None
+            if check:
+                et = ws.energy_terms()
+                dd = ws.diff(0, 1, 1)
+        if check:
             still = []
             for i in running:
-                E = float(et[i, L.E_TOTAL])
+                if ch is not None:
+                    E = float(ch[i, L.CHUNK_E])
+                    dpsi = float(ch[i, L.CHUNK_DPSI])
+                else:
+                    E = float(et[i, L.E_TOTAL])
+                    pmax = dd[i, 1]
+                    dpsi = float(dd[i, 0] / pmax) if pmax > 0 else 0.0
                 dE = abs(E - E_prev[i])
-                pmax = dd[i, 1]
-                dpsi = float(dd[i, 0] / pmax) if pmax > 0 else 0.0
                 out[i].update(dE=dE, dpsi=dpsi, steps=step)
                 if dE < tol and dpsi < tol:
                     out[i]["converged"] = True
@@ -783,7 +819,7 @@ def _itp_loop(ws: Workspace, n_steps: int, tol: float, save_every: int, target_m
                 running = still
                 if running:
                     ws.set_active(running)
-            if running:
+            if running and ch is None:
                 ws.copy_slot(1, 0)
         else:
             for i in running:

@@ -69,7 +69,8 @@ struct bec_ws {
   double* d_ctarget = nullptr; // factors (allocated on first use)
   double* d_cfac = nullptr;
   void* d_tw[kNumFftSizes] = {};
-  double* h_red = nullptr;  // pinned
+  double* h_red = nullptr;   // pinned
+  double* h_red2 = nullptr;  // pinned, second staging buffer of bec_itp_chunk (first use)
   int max_blocks = 0;
   int n_xblocks = 0;
   bool params_dirty = true;
@@ -871,6 +872,7 @@ void bec_ws_destroy(bec_ws* ws) {
   for (int i = 0; i < kNumFftSizes; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
   if (ws->h_red) (void)hipHostFree(ws->h_red);
+  if (ws->h_red2) (void)hipHostFree(ws->h_red2);
   for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
   drop_graphs(ws);
   if (ws->stream) (void)hipStreamDestroy(ws->stream);
@@ -1584,9 +1586,8 @@ static int energy_partials_fp64(bec_ws* ws, const DevTables& tb0, void* psi, voi
   return BEC_OK;
 }
 
-int bec_energy(bec_ws* ws, double* out) {
-  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
-  if (n_active(ws) == 0) return BEC_OK;
+// Energy partials of every active instance, reduced into d_red (no synchronisation).
+static int energy_enqueue(bec_ws* ws) {
   CHK(sync_params(ws));
   const Geom& g = ws->g;
   const int pe = prof_begin(ws, 9);
@@ -1612,10 +1613,15 @@ int bec_energy(bec_ws* ws, double* out) {
     CHK(energy_partials_fp64(ws, tb, ws->psi64, ws->scratch64, ws->fhat64, ws->phi64));
   }
   prof_end(ws, pe);
-  CHK(fetch_red(ws));
+  return BEC_OK;
+}
+
+// Energy terms (BEC_E_*) of the active instances from reduced sums red[B][RS_COUNT].
+static void energy_terms_from(const bec_ws* ws, const double* red, double* out) {
+  const Geom& g = ws->g;
   const double dV = g.dV;
   for (int inst : ws->active) {
-    const double* r = ws->h_red + (size_t)inst * RS_COUNT;
+    const double* r = red + (size_t)inst * RS_COUNT;
     const InstParams& q = ws->hp[inst];
     double* o = out + (size_t)inst * BEC_E_COUNT;
     o[BEC_E_KIN] = 0.5 * (r[RS_KIN] * dV / (double)g.nsites);
@@ -1632,6 +1638,14 @@ int bec_energy(bec_ws* ws, double* out) {
     o[BEC_E_TOTAL] = o[BEC_E_KIN] + o[BEC_E_TRAP] + o[BEC_E_ZEEMAN] + o[BEC_E_C0] + o[BEC_E_C1] +
                      o[BEC_E_DDI] + o[BEC_E_LHY] + o[BEC_E_TENSOR] + o[BEC_E_RAMAN];
   }
+}
+
+int bec_energy(bec_ws* ws, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  if (n_active(ws) == 0) return BEC_OK;
+  CHK(energy_enqueue(ws));
+  CHK(fetch_red(ws));
+  energy_terms_from(ws, ws->h_red, out);
   return BEC_OK;
 }
 
@@ -1689,6 +1703,49 @@ int bec_spin_density(bec_ws* ws, int inst, double* out) {
                               : launch_spin_density<double>(ws->stream, psi, ws->d_sdens, N, ws->D));
   HIPCHK(hipMemcpyAsync(out, ws->d_sdens, 3 * (size_t)N * 8, hipMemcpyDeviceToHost, ws->stream));
   HIPCHK(hipStreamSynchronize(ws->stream));
+  return BEC_OK;
+}
+
+int bec_itp_chunk(bec_ws* ws, int n_steps, double* out) {
+  if (!ws || !out) return fail(BEC_EINVAL, "null argument");
+  if (!ws->itp) return fail(BEC_EINVAL, "bec_itp_chunk needs an imaginary-time workspace");
+  if (n_steps < 0) return fail(BEC_EINVAL, "n_steps < 0");
+  if (n_active(ws) == 0) return BEC_OK;
+  CHK(ensure_slot(ws, 1));
+  const size_t redb = (size_t)ws->B * RS_COUNT * 8;
+  if (!ws->h_red2 && hipHostMalloc(&ws->h_red2, redb) != hipSuccess)
+    return fail(BEC_ENOMEM, "pinned reduction buffer");
+  CHK(bec_split_step(ws, n_steps));
+  // max |Phi| of the chunk's last convolution, before the energy's own convolution
+  // rewrites the per-block partials (the energy does not reduce that slot)
+  if (ws->ddi) CHK(do_reduce(ws, xpass_grid(ws->g), 1u << RS_PHIMAX, 1u << RS_PHIMAX));
+  CHK(energy_enqueue(ws));
+  HIPCHK(hipMemcpyAsync(ws->h_red, ws->d_red, redb, hipMemcpyDeviceToHost, ws->stream));
+  // max |psi - psi_prev| and max |psi| (ground_state.jl:148-149), then psi_prev <- psi
+  const int nb = 512;
+  HIPCHK(ws->prec == BEC_FP32
+             ? launch_diff<float>(ws->stream, ws->g, ws->tb, ws->slot[0], ws->slot[1],
+                                  n_active(ws), 1, nb)
+             : launch_diff<double>(ws->stream, ws->g, ws->tb, ws->slot[0], ws->slot[1],
+                                   n_active(ws), 1, nb));
+  const uint32_t m = (1u << RS_NORM) | (1u << RS_KIN);
+  CHK(do_reduce(ws, nb, m, m));
+  HIPCHK(hipMemcpyAsync(ws->h_red2, ws->d_red, redb, hipMemcpyDeviceToHost, ws->stream));
+  CHK(bec_copy_slot(ws, 1, 0));
+  HIPCHK(hipStreamSynchronize(ws->stream));
+  std::vector<double> e((size_t)ws->B * BEC_E_COUNT, 0.0);
+  energy_terms_from(ws, ws->h_red, e.data());
+  for (int inst : ws->active) {
+    const double* r2 = ws->h_red2 + (size_t)inst * RS_COUNT;
+    double* o = out + (size_t)inst * BEC_CHUNK_COUNT;
+    const double E = e[(size_t)inst * BEC_E_COUNT + BEC_E_TOTAL];
+    const double nrm = e[(size_t)inst * BEC_E_COUNT + BEC_E_NORM];
+    o[BEC_CHUNK_E] = E;
+    o[BEC_CHUNK_DPSI] = r2[RS_KIN] > 0.0 ? r2[RS_NORM] / r2[RS_KIN] : 0.0;
+    o[BEC_CHUNK_NORM] = nrm;
+    o[BEC_CHUNK_PHIMAX] = ws->ddi ? ws->h_red[(size_t)inst * RS_COUNT + RS_PHIMAX] : 0.0;
+    o[BEC_CHUNK_STATUS] = (std::isfinite(E) && std::isfinite(nrm)) ? 0.0 : 1.0;
+  }
   return BEC_OK;
 }
 

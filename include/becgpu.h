@@ -151,6 +151,16 @@ int bec_diff(bec_ws* ws, int slot_a, int slot_b, int mode, double* out);
  * out[3][N] = Fx, Fy, Fz per site in psi's spatial memory order, with
  * Fz = sum_m m |psi_m|^2 and Fx + i Fy = sum_m f+(m) conj(psi_{m+1}) psi_m. */
 int bec_spin_density(bec_ws* ws, int inst, double* out);
+/* One check interval of find_ground_state's loop (ground_state.jl:132-162) with a single
+ * host synchronisation: n_steps ITP split steps, then for every active instance
+ * out[B][BEC_CHUNK_COUNT] = {total energy, dpsi = max|psi - psi_prev| / max|psi| against
+ * slot 1, norm, max |Phi| of the last DDI convolution (0 without DDI; the overflow guard's
+ * input, ground_state.jl:4-29), status (0, or 1 when the energy or norm is not finite)},
+ * and slot 1 <- psi for the next interval.  The caller seeds slot 1 with the starting psi
+ * (bec_copy_slot(ws, 1, 0)) and keeps E_prev; inactive rows are untouched. */
+enum { BEC_CHUNK_E = 0, BEC_CHUNK_DPSI, BEC_CHUNK_NORM, BEC_CHUNK_PHIMAX, BEC_CHUNK_STATUS,
+       BEC_CHUNK_COUNT };
+int bec_itp_chunk(bec_ws* ws, int n_steps, double* out);
 /* ITP guard (ground_state.jl:4-29): out[B][2] = {sum |psi|^2 (NaN if any NaN),
  * max |Phi| of the last DDI convolution (0 without DDI)}. */
 int bec_itp_guard(bec_ws* ws, double* out);

@@ -151,3 +151,36 @@ def test_itp_guard_detects_ddi_overflow():
         else:
             assert phimax == pytest.approx(ref, rel=0.2)   # Phi of the evolved state
             A._check_itp_overflow(ws, 1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_itp_chunk_equals_separate_calls(precision):
+    """bec_itp_chunk (ground_state.jl:132-162 with one synchronisation) against the same
+    interval issued as split steps, bec_energy, bec_diff (max mode) and a slot copy, on a
+    two-instance dipolar workspace: identical energies, dpsi, norms and max |Phi|."""
+    n, L = (32, 16), (10.0, 8.0)
+    og, gg = grids(n, L)
+    ws = [G.Workspace(grid=gg, F=1, n_instances=2, precision=precision, enable_ddi=True,
+                      sim_params=G.SimParams(dt=0.004, n_steps=100, imaginary_time=True))
+          for _ in range(2)]
+    for w in ws:
+        for i, (c0, c1) in enumerate(((8.0, -0.4), (12.0, 0.6))):
+            w.set_instance(i, interactions=G.InteractionParams(c0, c1),
+                           potential=G.HarmonicTrap(1.0, 1.0), c_dd=0.5)
+            w.set_psi(G.init_psi(gg, G.SpinSystem(1), ("ferromagnetic", "uniform")[i]), i)
+        w.copy_slot(1, 0)
+    a, b = ws
+    for rep in range(3):
+        ch = a.itp_chunk(7)
+        A.split_step(b, 7)
+        guard = b.itp_guard()
+        et = b.energy_terms()
+        dd = b.diff(0, 1, 1)
+        b.copy_slot(1, 0)
+        for i in range(2):
+            assert ch[i, 0] == et[i, G.api.L.E_TOTAL]
+            assert ch[i, 1] == dd[i, 0] / dd[i, 1]
+            assert ch[i, 2] == et[i, G.api.L.E_NORM]
+            assert ch[i, 3] == guard[i, 1] and ch[i, 3] > 0
+            assert ch[i, 4] == 0.0
+        assert np.array_equal(a.get_psi(0), b.get_psi(0))

@@ -92,3 +92,66 @@ def test_realtime_arrivals_and_budget():
         q = reqs[r.rid]
         a = solve_alone(q)
         assert r.energy == a.energy and np.array_equal(r.psi, a.psi)
+
+
+def _stream_slice(precision, seed=7):
+    """A seeded slice of the serving workload's stream (poisson_mix: Poisson arrivals at
+    200/s, dimension 1/2/3 w.p. 0.5/0.35/0.15, F = 1/2/6 w.p. 0.6/0.25/0.15, sides in the
+    configuration's ranges, 500-5000 steps): the first 3-D spin-6 request and the first seven
+    1-D/2-D requests whose FP64 oracle run stays under 1.5e8 component-site-steps."""
+    from becgpu.serve import poisson_mix
+    f6, rest = None, []
+    for q in poisson_mix(300, seed=seed, precision=precision):
+        if len(q.n) == 3 and q.F == 6 and q.n[0] >= 48 and f6 is None:
+            f6 = q
+        elif len(q.n) < 3 and q.sites * q.n_steps * (2 * q.F + 1) <= 1.5e8 and len(rest) < 7:
+            rest.append(q)
+    assert f6 is not None and len(rest) == 7
+    return sorted([f6] + rest, key=lambda q: q.arrival)
+
+
+def _oracle_request(q):
+    """The oracle's result for a served request: find_ground_state (ITP) or n_steps
+    split_step! (real time) from init_psi(state) in the harmonic trap."""
+    from oracle import spinor_oracle as O
+    og = O.make_grid(q.n, q.L)
+    itp = q.kind == "itp"
+    ows = O.make_workspace(og, q.F, O.init_psi(og, q.F, q.state), q.dt, itp,
+                           omega=(q.omega,) * len(q.n), c0=q.c0, c1=q.c1, n_steps=q.n_steps)
+    if itp:
+        r = O.find_ground_state(ows, q.n_steps, q.tol)
+        return r["energy"], r["steps"], r["converged"], ows.psi
+    for _ in range(q.n_steps):
+        O.split_step(ows)
+    return O.total_energy(ows), q.n_steps, False, ows.psi
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_serving_stream_slice(precision):
+    """Eight requests of the serving configuration's own stream, arriving on the wall clock:
+    each result equals the same request served alone, bit for bit, including a 3-D spin-6
+    request; in FP64 the 1-D and 2-D requests also equal the oracle (energy 1e-12, psi 1e-10,
+    the same convergence step)."""
+    from oracle import spinor_oracle as O
+    reqs = _stream_slice(precision)
+    t0 = reqs[0].arrival
+    for q in reqs:
+        q.arrival -= t0
+    srv = Server(slots_per_pool=4, chunk_ms=5.0, keep_psi=True)
+    res, stats = srv.run(reqs, realtime=True)
+    print(stats)
+    by = {r.rid: r for r in res}
+    assert sorted(by) == sorted(q.rid for q in reqs)
+    for q in reqs:
+        m = by[q.rid]
+        assert m.error is None and math.isfinite(m.energy)
+        assert m.finished >= m.admitted >= m.arrival - 1e-9
+        a = solve_alone(q)
+        assert (m.steps, m.converged) == (a.steps, a.converged)
+        assert m.energy == a.energy and np.array_equal(m.psi, a.psi), q.rid
+        if precision == "fp64" and len(q.n) < 3:
+            e, steps, conv, psi = _oracle_request(q)
+            assert abs(m.energy - e) <= 1e-12 * abs(e), (q.rid, m.energy, e)
+            if q.kind == "itp":
+                assert (m.steps, m.converged) == (steps, conv)
+            assert O.relative_l2(m.psi, psi) <= 1e-10
