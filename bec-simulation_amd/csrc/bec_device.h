@@ -71,10 +71,16 @@ __device__ __forceinline__ void sincos_d(double a, double* s, double* c) {
     sincos_lib(a, s, c);
     return;
   }
-  const double k = rint(a * 0.6366197723675814);  // 2/pi
-  double x = fma(-k, 1.5707963267948966, a);       // pi/2 = hi + mid + lo
-  x = fma(-k, 6.123233995736766e-17, x);
-  x = fma(-k, -1.4973849048591698e-33, x);
+  // |a| <= pi/4 (every phase argument of a small step): k = 0 and x = a exactly, so the
+  // reduction and the quadrant selection are skipped with the same result
+  const bool small = fabs(a) <= 0.7853981633974483;
+  const double k = small ? 0.0 : rint(a * 0.6366197723675814);  // 2/pi
+  double x = a;
+  if (!small) {
+    x = fma(-k, 1.5707963267948966, a);  // pi/2 = hi + mid + lo
+    x = fma(-k, 6.123233995736766e-17, x);
+    x = fma(-k, -1.4973849048591698e-33, x);
+  }
   const double z = x * x;
   double ps = 2.8114572543455206e-15;  // 1/17!, alternating signs down to -1/3!
   ps = fma(ps, z, -7.647163731819816e-13);
@@ -95,6 +101,11 @@ __device__ __forceinline__ void sincos_d(double a, double* s, double* c) {
   pc = fma(pc, z, 0.041666666666666664);
   pc = fma(pc, z, -0.5);
   const double cs = fma(z, pc, 1.0);
+  if (small) {
+    *s = sn;
+    *c = cs;
+    return;
+  }
   const int q = (int)(long long)k & 3;
   double rs = (q & 1) ? cs : sn, rc = (q & 1) ? sn : cs;
   if (q == 1 || q == 2) rc = -rc;
@@ -472,8 +483,9 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
   T fmag = sqrt(fx * fx + fy * fy + fz * fz);
   if (!(fmag >= (T)1e-30)) return;
   const T is2 = (T)0.70710678118654752440;
-  const T c = fz / fmag;
-  const cplx<T> u = mk(fx / fmag * is2, fy / fmag * is2), uc = conjc(u);
+  const T inv = (T)1 / fmag;  // one division for the three components of the axis
+  const T c = fz * inv;
+  const cplx<T> u = mk(fx * inv * is2, fy * inv * is2), uc = conjc(u);
   auto nF = [&](const cplx<T>* v, cplx<T>* o) __attribute__((always_inline)) {
     o[0] = v[0] * c + uc * v[1];
     o[1] = u * v[0] + uc * v[2];
@@ -552,13 +564,15 @@ __device__ __forceinline__ int op_euler(cplx<T>* s, double phx, double phy, doub
   // sin beta = rho/|phi| >= 0, (cos alpha, sin alpha) = (phi_x, phi_y)/rho, with atan2's
   // signed-zero convention when rho = 0.  cos/sin of F*x by De Moivre powers.
   const double rho = sqrt(phx * phx + phy * phy);
-  double cb = phz / pm;
+  const double ipm = 1.0 / pm;  // one division per radius
+  double cb = phz * ipm;
   cb = cb < -1.0 ? -1.0 : (cb > 1.0 ? 1.0 : cb);
-  const double sb = rho / pm;
+  const double sb = rho * ipm;
   double cal, sal;
   if (rho > 0.0) {
-    cal = phx / rho;
-    sal = phy / rho;
+    const double irho = 1.0 / rho;
+    cal = phx * irho;
+    sal = phy * irho;
   } else {
     cal = signbit(phx) ? -1.0 : 1.0;
     sal = 0.0;
