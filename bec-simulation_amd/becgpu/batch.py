@@ -201,3 +201,64 @@ def multistart_batched(*, initial_states=("polar", "ferromagnetic", "uniform",
     return dict(converged=res[best]["converged"], energy=res[best]["energy"],
                 initial_state=jobs[best].tag[0], psi=res[best]["psi"],
                 all_results=all_results, stats=stats)
+
+
+def scan_continuation_batched(chains, *, grid, atom, initial_state="polar",
+                              energy_jump_threshold=0.1, n_steps_continuation=500,
+                              n_steps_fresh=5000, zeeman=ZeemanParams(), potential=NoPotential(),
+                              dt=0.001, tol=1e-10, precision="fp64", keep_psi=True):
+    """scan_continuation (ground_state.jl:328-378) for several independent chains at once.
+
+    chains: a list of (param_values, make_interactions).  Point i of every chain that has
+    one is solved in the same batch (solve_batch), continued from that chain's previous
+    ground state (n_steps_continuation) or, for the first point, from `initial_state`
+    (n_steps_fresh); the chains whose energy jumped by more than energy_jump_threshold
+    (relative) are solved again by multistart over the reference's four named states, all
+    such fallbacks of the point in one more batch.  Each chain's records are those of
+    scan_continuation: param, energy, converged, phase, psi, fallback."""
+    from .physics import classify_phase
+    sys_ = SpinSystem(atom.F)
+    named = ("polar", "ferromagnetic", "uniform", "antiferromagnetic")
+    psi_named = {s: init_psi(grid, sys_, s) for s in named + (initial_state,)}
+    out = [[] for _ in chains]
+    prev_psi = [None] * len(chains)
+    prev_E = [float("nan")] * len(chains)
+    n_points = max(len(vals) for vals, _ in chains)
+    for i in range(n_points):
+        live = [c for c, (vals, _) in enumerate(chains) if i < len(vals)]
+        jobs = []
+        for c in live:
+            vals, mk = chains[c]
+            fresh = prev_psi[c] is None
+            jobs.append(ITPJob(grid=grid, F=atom.F, interactions=mk(vals[i]),
+                               psi0=psi_named[initial_state] if fresh else prev_psi[c].copy(),
+                               zeeman=zeeman, potential=potential, dt=dt,
+                               n_steps=n_steps_fresh if fresh else n_steps_continuation,
+                               tol=tol, tag=c))
+        res, _ = solve_batch(jobs, precision=precision, keep_psi=True)
+        point = {c: r for c, r in zip(live, res)}
+        jumped = [c for c in live
+                  if not math.isnan(prev_E[c])
+                  and abs(point[c]["energy"] - prev_E[c]) / max(abs(prev_E[c]), 1e-30)
+                  > energy_jump_threshold]
+        if jumped:
+            fjobs = [ITPJob(grid=grid, F=atom.F, interactions=chains[c][1](chains[c][0][i]),
+                            psi0=psi_named[s], zeeman=zeeman, potential=potential, dt=dt,
+                            n_steps=n_steps_fresh, tol=tol, tag=(c, s))
+                     for c in jumped for s in named]
+            fres, _ = solve_batch(fjobs, precision=precision, keep_psi=True)
+            for c in jumped:
+                rs = [r for j, r in zip(fjobs, fres) if j.tag[0] == c]
+                best = min(range(len(rs)), key=lambda k: rs[k]["energy"])   # first minimum
+                point[c] = dict(rs[best], fallback=True)
+        for c in live:
+            r = point[c]
+            psi = r["psi"]
+            out[c].append(dict(param=chains[c][0][i], energy=r["energy"],
+                               converged=r["converged"],
+                               phase=classify_phase(psi, atom.F, grid)["phase"],
+                               psi=psi.copy() if keep_psi else None,
+                               fallback=bool(r.get("fallback", False))))
+            prev_psi[c] = psi
+            prev_E[c] = r["energy"]
+    return out

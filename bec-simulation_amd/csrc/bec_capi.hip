@@ -1635,6 +1635,17 @@ static void energy_terms_from(const bec_ws* ws, const double* red, double* out) 
     o[BEC_E_NORM] = r[RS_NORM] * dV;
     o[BEC_E_MZ] = r[RS_MZ] * dV;
     o[BEC_E_RAMAN] = q.has_raman ? r[RS_RAMAN] * dV : 0.0;
+    if (ws->prec == BEC_FP32 && ws->itp && ws->normalize_every > 0 && o[BEC_E_NORM] > 0.0) {
+      // FP32-bulk imaginary time: the FP64 renormalisation factor is applied with one
+      // rounding per element, and for a smooth state those roundings share a sign, so the
+      // stored state's norm is off by up to ~1e-7.  Its energies are those of the state
+      // the renormalisation means, psi / sqrt(norm): quadratic terms scale by 1/N,
+      // quartic ones by 1/N^2, the LHY term by N^-5/2.
+      const double in = 1.0 / o[BEC_E_NORM];
+      for (int k : {BEC_E_KIN, BEC_E_TRAP, BEC_E_ZEEMAN, BEC_E_RAMAN, BEC_E_MZ}) o[k] *= in;
+      for (int k : {BEC_E_C0, BEC_E_C1, BEC_E_DDI, BEC_E_TENSOR}) o[k] *= in * in;
+      o[BEC_E_LHY] *= in * in * std::sqrt(in);
+    }
     o[BEC_E_TOTAL] = o[BEC_E_KIN] + o[BEC_E_TRAP] + o[BEC_E_ZEEMAN] + o[BEC_E_C0] + o[BEC_E_C1] +
                      o[BEC_E_DDI] + o[BEC_E_LHY] + o[BEC_E_TENSOR] + o[BEC_E_RAMAN];
   }

@@ -139,6 +139,9 @@ int bec_normalize(bec_ws* ws);
 int bec_normalize_constrained(bec_ws* ws, const double* target_mz);
 
 /* total_energy terms (energy.jl:6-121) + norm and Mz (observables.jl:1-33),
+ * evaluated in FP64.  FP32-bulk imaginary-time workspaces (normalize_every > 0) report
+ * every term except the norm for psi / sqrt(norm), the state that their FP64
+ * renormalisation describes (its FP32 copy carries a rounding of the norm of up to 1e-7);
  * out[B][BEC_E_COUNT] for every instance (inactive rows untouched). */
 int bec_energy(bec_ws* ws, double* out);
 /* Copies between psi slots: slot 0 is psi, slots 1..3 are device-resident

@@ -66,3 +66,25 @@ def test_multistart_batched_matches_sequential():
     assert b["initial_state"] == s["initial_state"]
     assert b["energy"] == s["energy"]
     assert [r["energy"] for r in b["all_results"]] == [r["energy"] for r in s["all_results"]]
+
+
+def test_continuation_chains_batched_equal_sequential():
+    """Three continuation chains (c1 sweeps at different c0) through the batched path equal
+    scan_continuation chain by chain, bit for bit, fallbacks included."""
+    from becgpu.batch import scan_continuation_batched
+    grid = G.make_grid(G.GridConfig((16,), (10.0,)))
+    chains = [([-1.0, 0.5, 1.0], lambda v: G.InteractionParams(10.0, v)),
+              ([0.2, 0.3], lambda v: G.InteractionParams(6.0, v)),
+              ([10.0, 10.5, 20.0], lambda v: G.InteractionParams(v, -0.5))]
+    kw = dict(grid=grid, atom=G.Rb87, initial_state="polar", n_steps_continuation=300,
+              n_steps_fresh=600, dt=0.01, tol=1e-8, potential=G.HarmonicTrap(1.0))
+    got = scan_continuation_batched(chains, **kw)
+    for (vals, mk), rec in zip(chains, got):
+        ref = A.scan_continuation(param_values=vals, make_interactions=mk, **kw)
+        assert len(rec) == len(ref) == len(vals)
+        for g, r in zip(rec, ref):
+            assert (g["param"], g["converged"], g["phase"], g["fallback"]) == \
+                (r["param"], r["converged"], r["phase"], r["fallback"])
+            assert g["energy"] == r["energy"]
+            assert np.array_equal(g["psi"], r["psi"])
+    assert any(r["fallback"] for r in got[2])

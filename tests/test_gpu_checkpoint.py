@@ -51,3 +51,35 @@ def test_checkpoint_callback_cadence(tmp_path):
     run_simulation_checkpointed(ws, checkpoint_dir=str(tmp_path), checkpoint_every=25,
                                 callback=lambda w, s: count.append(s))
     assert count == [25, 50]
+
+
+def test_resumed_run_matches_oracle(tmp_path):
+    """A real-time run interrupted at step 40 and resumed from its checkpoint
+    (run_simulation_checkpointed!, simulation.jl:96-148) equals the oracle's uninterrupted
+    run_simulation! with the same save points (leapfrog, simulation.jl:49-94): psi to 1e-10
+    and the energy of the final state to 1e-12."""
+    from oracle import spinor_oracle as O
+    n, L = (32,), (10.0,)
+    og = O.make_grid(n, L)
+    grid = G.make_grid(G.GridConfig(n, L))
+    psi0 = G.init_psi(grid, G.SpinSystem(1), "uniform")
+    psi0[0] *= np.exp(0.4j * grid.x[0])                       # a spin texture that moves
+
+    def ws(n_steps):
+        sp = G.SimParams(dt=DT, n_steps=n_steps, save_every=20)
+        return A.make_workspace(grid=grid, atom=G.Rb87, interactions=G.InteractionParams(4.0, -0.3),
+                                zeeman=G.ZeemanParams(0.1, 0.05), potential=G.HarmonicTrap(1.0),
+                                sim_params=sp, psi_init=psi0)
+
+    first = ws(40)
+    run_simulation_checkpointed(first, checkpoint_dir=str(tmp_path), checkpoint_every=20)
+    second = ws(100)
+    run_simulation_checkpointed(second, checkpoint_dir=str(tmp_path), checkpoint_every=20,
+                                resume=True)
+    ows = O.make_workspace(og, 1, psi0, DT, False, omega=(1.0,), p=0.1, q=0.05, c0=4.0, c1=-0.3,
+                           n_steps=100, save_every=20)
+    O.run_simulation(ows)
+    assert int(second.step[0]) == 100
+    assert O.relative_l2(second.get_psi(0), ows.psi) <= 1e-10
+    eo = O.total_energy(ows)
+    assert abs(A.total_energy(second) - eo) <= 1e-12 * abs(eo)
