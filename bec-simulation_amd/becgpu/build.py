@@ -21,7 +21,8 @@ LIB = os.path.join(PKG_DIR, "libbecgpu.so")
 KERNEL_DS = (1, 3, 5, 7, 9, 11, 13)
 UNITS = ([("bec_kernels.hip", f"bec_kernels_p{d}.o", [f"-DBEC_PART={d}"]) for d in (0,) + KERNEL_DS]
          + [(src, os.path.splitext(src)[0] + ".o", [])
-            for src in ("bec_strided.hip", "bec_xsite.hip", "bec_aux.hip", "bec_capi.hip")])
+            for src in ("bec_strided.hip", "bec_xsite.hip", "bec_aux.hip", "bec_generic.hip",
+                        "bec_capi.hip")])
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

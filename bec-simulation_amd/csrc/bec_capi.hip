@@ -69,6 +69,8 @@ struct bec_ws {
   double* d_ctarget = nullptr; // factors (allocated on first use)
   double* d_cfac = nullptr;
   void* d_tw[kNumFftSizes] = {};
+  void* d_gtw[kGenTabs] = {};  // runtime-length twiddle tables (sides without a built FFT)
+  int gtw_n[kGenTabs] = {};
   double* h_red = nullptr;   // pinned
   double* h_red2 = nullptr;  // pinned, second staging buffer of bec_itp_chunk (first use)
   int max_blocks = 0;
@@ -149,6 +151,10 @@ static void refresh_tables(bec_ws* ws) {
   t.params = ws->d_params;
   t.active = ws->d_active;
   for (int i = 0; i < kNumFftSizes; ++i) t.tw[i] = ws->d_tw[i];
+  for (int i = 0; i < kGenTabs; ++i) {
+    t.gtw[i] = ws->d_gtw[i];
+    t.gtw_n[i] = ws->gtw_n[i];
+  }
   t.red = ws->d_red;
   t.partials = ws->d_partials;
   t.max_blocks = ws->max_blocks;
@@ -626,44 +632,6 @@ extern "C" {
 
 const char* bec_last_error(void) { return g_err.c_str(); }
 
-static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out) {
-  const int D = 2 * c->F + 1;
-  long long ns = 1;
-  for (int d = 0; d < 3; ++d) ns *= (d < c->ndim ? c->n[d] : 1);
-  const long long csz = c->precision == BEC_FP32 ? 8 : 16;
-  const int B = c->n_instances;
-  const int nx = c->n[0];
-  const int ny = c->ndim >= 2 ? c->n[1] : 1, nz = c->ndim >= 3 ? c->n[2] : 1;
-  const int nrows = ny * nz;
-  const int rows = xpass_rows(D, nx, nrows);
-  int nxb = nrows / rows;
-  // the tile autotuner may pick down to a quarter of the default rows per block
-  int maxb = std::max(std::min(nrows, 4 * nxb), 1024);
-  // strided SM_KE partial count
-  for (int axis = 1; axis < c->ndim; ++axis) {
-    const int N = axis == 1 ? ny : nz;
-    const int tbw = strided_tb(N, 1, (int)csz);
-    const int other = axis == 1 ? nz : ny;
-    maxb = std::max(maxb, ((nx + tbw - 1) / tbw) * other * D);
-  }
-  if (max_blocks_out) *max_blocks_out = maxb;
-  if (nxb_out) *nxb_out = nxb;
-  long long bytes = 2LL * B * D * ns * csz;  // psi + scratch
-  if (c->enable_ddi) {
-    const int nxh = nx / 2 + 1;
-    const int pitch = (nxh + 15) / 16 * 16;
-    const int pf = c->ddi_padding ? 2 : 1;  // padded spectra: (2nx/2+1) x 2ny x 2nz
-    const int ppitch = (pf * nx / 2 + 1 + 15) / 16 * 16;
-    bytes += 3LL * B * ppitch * (pf * ny) * (c->ndim >= 3 ? pf * nz : nz) * csz;
-    bytes += 3LL * B * ns * (csz / 2);
-    if (c->precision == BEC_FP32) bytes += 3LL * B * pitch * ny * nz * 16;
-  }
-  if (c->precision == BEC_FP32) bytes += 2LL * B * D * ns * 16;  // FP64 energy copies
-  bytes += (long long)RS_COUNT * B * maxb * 8 + (long long)B * RS_COUNT * 8;
-  bytes += (long long)B * 3 * std::max(nx, std::max(ny, nz)) * 8;
-  return bytes;
-}
-
 // Geometry of a workspace with D components on the grid c describes (validated by the caller).
 static void fill_geom(const bec_config* c, int D, Geom& g) {
   g = Geom{};
@@ -688,8 +656,67 @@ static void fill_geom(const bec_config* c, int D, Geom& g) {
   g.ddi_secular = c->ddi_secular;
 }
 
+
+// The same grid with the unpadded DDI spectra (energy evaluation).
+static Geom unpadded_geom(const Geom& g0) {
+  Geom g = g0;
+  g.spad = 0;
+  for (int d = 0; d < 3; ++d) g.sn[d] = g.n[d];
+  g.smax = g.nmax;
+  g.nxh = g.n[0] / 2 + 1;
+  g.nxh_pitch = (g.nxh + 15) / 16 * 16;
+  g.fhat_field_stride = (long long)g.nxh_pitch * g.n[1] * g.n[2];
+  g.fhat_inst_stride = 3 * g.fhat_field_stride;
+  return g;
+}
+
+static long long ws_bytes(const bec_config* c, int* max_blocks_out, int* nxb_out) {
+  const int D = 2 * c->F + 1;
+  long long ns = 1;
+  for (int d = 0; d < 3; ++d) ns *= (d < c->ndim ? c->n[d] : 1);
+  const long long csz = c->precision == BEC_FP32 ? 8 : 16;
+  const int B = c->n_instances;
+  const int nx = c->n[0];
+  const int ny = c->ndim >= 2 ? c->n[1] : 1, nz = c->ndim >= 3 ? c->n[2] : 1;
+  const int nrows = ny * nz;
+  Geom g;
+  fill_geom(c, D, g);
+  const Geom gu = unpadded_geom(g);
+  const int nxb = xpass_grid(g);
+  // the tile autotuner may pick down to a quarter of the default rows per block
+  int maxb = std::max(std::min(nrows, 4 * nxb), 1024);
+  maxb = std::max(maxb, xpass_grid(gu));  // energy evaluation on the unpadded spectra
+  // strided SM_KE partial count (the energy runs in FP64 for both precisions)
+  for (int axis = 1; axis < c->ndim; ++axis) {
+    const int N = axis == 1 ? ny : nz;
+    const int other = axis == 1 ? nz : ny;
+    for (int cs : {(int)csz, 16}) {
+      const int tbw = std::max(1, strided_tb(N, 1, cs));
+      maxb = std::max(maxb, ((nx + tbw - 1) / tbw) * other * D);
+    }
+  }
+  if (max_blocks_out) *max_blocks_out = maxb;
+  if (nxb_out) *nxb_out = nxb;
+  long long bytes = 2LL * B * D * ns * csz;  // psi + scratch
+  if (c->enable_ddi) {
+    const int nxh = nx / 2 + 1;
+    const int pitch = (nxh + 15) / 16 * 16;
+    const int pf = c->ddi_padding ? 2 : 1;  // padded spectra: (2nx/2+1) x 2ny x 2nz
+    const int ppitch = (pf * nx / 2 + 1 + 15) / 16 * 16;
+    bytes += 3LL * B * ppitch * (pf * ny) * (c->ndim >= 3 ? pf * nz : nz) * csz;
+    bytes += 3LL * B * ns * (csz / 2);
+    if (c->precision == BEC_FP32) bytes += 3LL * B * pitch * ny * nz * 16;
+  }
+  if (c->precision == BEC_FP32) bytes += 2LL * B * D * ns * 16;  // FP64 energy copies
+  bytes += (long long)RS_COUNT * B * maxb * 8 + (long long)B * RS_COUNT * 8;
+  bytes += (long long)B * 3 * std::max(nx, std::max(ny, nz)) * 8;
+  return bytes;
+}
+
 long long bec_workspace_bytes(const bec_config* cfg) {
-  if (!cfg) return -1;
+  if (!cfg || cfg->ndim < 1 || cfg->ndim > 3 || cfg->F < 0 || cfg->F > 6) return -1;
+  for (int d = 0; d < cfg->ndim; ++d)
+    if (cfg->n[d] <= 0) return -1;
   return ws_bytes(cfg, nullptr, nullptr);
 }
 
@@ -701,24 +728,27 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     if (c->n[d] <= 0 || (c->n[d] % 2) != 0)
       return fail(BEC_EINVAL, "n_points must be positive even integers");
     if (!(c->L[d] > 0)) return fail(BEC_EINVAL, "box_size must be positive");
-    if (!side_supported(c->n[d]))
-      return fail(BEC_EUNSUP, "grid sides must be 2^k in [8, 512], 3*2^k in [24, 192], 5*2^k in [40, 160] or 7*2^k in [56, 224] on this build");
   }
   if (c->F < 0) return fail(BEC_EINVAL, "F must be non-negative");  // types.jl:43
   if (c->F > 6) return fail(BEC_EUNSUP, "F above 6 (more than 13 components) is not built");
-  // one x-pass block holds whole x lines of every component, line_e(n_x) values per thread
-  if ((2 * c->F + 1) * (c->n[0] / line_e(c->n[0])) > 512)
-    return fail(BEC_EUNSUP, "x side too long for this many components (the x pass holds "
-                            "2F+1 whole lines in one block of at most 512 threads)");
   if (c->n_instances < 1) return fail(BEC_EINVAL, "n_instances must be >= 1");
   if (c->precision != BEC_FP64 && c->precision != BEC_FP32)
     return fail(BEC_EINVAL, "precision must be BEC_FP64 or BEC_FP32");
   if (c->ddi_padding && !c->enable_ddi) return fail(BEC_EINVAL, "ddi_padding needs enable_ddi");
-  if (c->ddi_padding)
-    for (int d = 0; d < c->ndim; ++d)
-      if (c->n[d] > 256 || fft_size_id(2 * c->n[d]) < 0)
-        return fail(BEC_EUNSUP, "zero-padded DDI needs built lines of twice the side (sides up to 256, not 224)");
   if (!c->x_tab || !c->k_tab) return fail(BEC_EINVAL, "missing tables");
+  {
+    // site indices are 32-bit within an instance; every line pass holds whole lines (all
+    // 2F+1 components of a row for the x pass) in one block's LDS
+    long long ns = 1;
+    for (int d = 0; d < c->ndim; ++d) ns *= c->n[d];
+    if (ns * (2 * c->F + 1) >= (1LL << 31))
+      return fail(BEC_EUNSUP, "more than 2^31 complex values per instance");
+    Geom g0;
+    fill_geom(c, 2 * c->F + 1, g0);
+    if (!generic_fits(g0, c->precision == BEC_FP32 ? 8 : 16, c->enable_ddi && c->F > 0))
+      return fail(BEC_EUNSUP, "grid side too long: one x row of all 2F+1 components (with the "
+                              "dipolar lines) or one y/z line does not fit a block's LDS");
+  }
 
   bec_ws* ws = new bec_ws();
   ws->F = c->F;
@@ -846,6 +876,30 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
     if (hipMemcpy(ws->d_tw[l], buf.data(), buf.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(BEC_EDEVICE, "twiddle copy");
   }
+  {
+    // runtime-length tables: every side and zero-padded spectral length of the grid
+    std::vector<int> lens;
+    for (int d = 0; d < c->ndim; ++d) {
+      lens.push_back(g.n[d]);
+      if (g.spad) lens.push_back(g.sn[d]);
+    }
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    for (size_t l = 0; l < lens.size() && l < (size_t)kGenTabs; ++l) {
+      const int N = lens[l];
+      std::vector<double> buf(2 * N);
+      for (int m = 0; m < N; ++m) {
+        const long double ang = -2.0L * 3.14159265358979323846264338327950288L * m / N;
+        buf[2 * m] = (double)cosl(ang);
+        buf[2 * m + 1] = (double)sinl(ang);
+      }
+      if (hipMalloc(&ws->d_gtw[l], buf.size() * 8) != hipSuccess)
+        return cleanup(BEC_ENOMEM, "twiddles");
+      if (hipMemcpy(ws->d_gtw[l], buf.data(), buf.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(BEC_EDEVICE, "twiddle copy");
+      ws->gtw_n[l] = N;
+    }
+  }
   refresh_tables(ws);
   *out = ws;
   return BEC_OK;
@@ -871,6 +925,8 @@ void bec_ws_destroy(bec_ws* ws) {
     if (q) (void)hipFree(q);
   for (int i = 0; i < kNumFftSizes; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
+  for (int i = 0; i < kGenTabs; ++i)
+    if (ws->d_gtw[i]) (void)hipFree(ws->d_gtw[i]);
   if (ws->h_red) (void)hipHostFree(ws->h_red);
   if (ws->h_red2) (void)hipHostFree(ws->h_red2);
   for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
@@ -1432,6 +1488,10 @@ static std::map<std::tuple<int, int, int, int>, int> g_tune;
 
 int bec_autotune(bec_ws* ws, int* rows_out) {
   if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (xpass_generic(ws->g)) {  // the runtime-length x pass keeps its fixed partition
+    if (rows_out) *rows_out = xpass_rows_g(ws->g);
+    return BEC_OK;
+  }
   const auto key = std::make_tuple((int)ws->prec, ws->g.n[0], ws->g.D, ws->g.nrows);
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tune.find(key);
@@ -1520,17 +1580,6 @@ static int fetch_red(bec_ws* ws) {
 // Energy partial sums of psi (FP64 arithmetic on FP64 data), reduced into d_red.
 // The DDI energy always uses the unpadded convolution (energy.jl:111-121), also in a
 // workspace whose steps use the zero-padded one: same grid, spectra of the unpadded shape.
-static Geom unpadded_geom(const Geom& g0) {
-  Geom g = g0;
-  g.spad = 0;
-  for (int d = 0; d < 3; ++d) g.sn[d] = g.n[d];
-  g.smax = g.nmax;
-  g.nxh = g.n[0] / 2 + 1;
-  g.nxh_pitch = (g.nxh + 15) / 16 * 16;
-  g.fhat_field_stride = (long long)g.nxh_pitch * g.n[1] * g.n[2];
-  g.fhat_inst_stride = 3 * g.fhat_field_stride;
-  return g;
-}
 
 static int energy_partials_fp64(bec_ws* ws, const DevTables& tb0, void* psi, void* scratch,
                                 void* fhat, void* phi) {

@@ -51,6 +51,10 @@ constexpr int fft_size_id(int n) {
 // 20 for 5 * 2^k, 28 for 7 * 2^k, else 8.
 constexpr int line_e(int n) { return n % 3 == 0 ? 12 : n % 5 == 0 ? 20 : n % 7 == 0 ? 28 : 8; }
 
+// Runtime-length twiddle tables (bec_generic.hip): one per distinct side and zero-padded
+// spectral length of a grid.
+constexpr int kGenTabs = 6;
+
 struct DevTables {
   const double* ktab;     // [3][nmax] fftfreq k values per axis
   const double* xtab;     // [3][nmax] cell-centred coordinates per axis
@@ -60,6 +64,8 @@ struct DevTables {
   const InstParams* params;  // [B]
   const int* active;      // active instance list
   const void* tw[kNumFftSizes];  // per line length: exp(-2 pi i m/N), FP64 for every T
+  const void* gtw[kGenTabs];     // runtime-length tables exp(-2 pi i m/N), m < gtw_n[i]
+  int gtw_n[kGenTabs];           // their lengths (0: unused)
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
   // profiling only (else null): kWorkSlots counter pairs, one per 64-byte line, so that the
@@ -131,6 +137,24 @@ int xpass_rows_g(const Geom& g);
 bool xpass_rows_ok(const Geom& g, int rows);
 // Blocks per instance of an x-pass launch (= number of per-block partial sums it writes).
 int xpass_grid(const Geom& g);
+
+// Line passes of runtime length (bec_generic.hip) for grid sides without a built FFT:
+// the x pass (same stages as launch_xpass) and the y/z passes (same modes as
+// launch_strided, over `other_lines` lines of the other transverse axis).  launch_xpass
+// and launch_strided hand over to them by themselves.
+bool xpass_generic(const Geom& g);     // the x pass of this geometry runs the runtime kernel
+int xgen_rows(const Geom& g);          // its rows per block
+int sgen_tb(int N, int fields, int csize);  // y/z tile columns of a runtime length (0: too long)
+// every pass of this geometry fits one block's LDS (csz: bytes per complex element)
+bool generic_fits(const Geom& g, int csz, int ddi);
+template <typename T>
+hipError_t launch_xpass_gen(hipStream_t st, const Geom& g, const DevTables& tb,
+                            const XProgram& prog, const void* src, void* dst, void* fhat,
+                            int n_active, void* phi);
+template <typename T>
+hipError_t launch_strided_gen(hipStream_t st, const Geom& g, const DevTables& tb, int axis,
+                              int mode, void* data, int n_active, double kin_frac,
+                              double dt_override, int itp, int ddi_data, int other_lines);
 
 // Spin density (Fx, Fy, Fz) of one instance's psi into out[3][nsites], FP64 (bec_aux.hip).
 template <typename T>

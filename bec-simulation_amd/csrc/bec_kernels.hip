@@ -813,13 +813,15 @@ int xpass_rows(int D, int NX, int nrows) {
 
 
 int xpass_rows_g(const Geom& g) {
-  return g.xrows > 0 ? g.xrows : xpass_rows(g.D, g.n[0], g.nrows);
+  if (g.xrows > 0) return g.xrows;
+  return xpass_generic(g) ? xgen_rows(g) : xpass_rows(g.D, g.n[0], g.nrows);
 }
 
 bool xpass_rows_ok(const Geom& g, int rows) {
   // the LDS bound is taken for FP64 with the DDI lines, so one choice fits every program
+  // (the runtime-length x pass keeps its own partition)
   const int AL = g.spad ? 2 * g.n[0] : g.n[0];
-  return rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
+  return !xpass_generic(g) && rows >= 1 && rows <= g.nrows && g.nrows % rows == 0 &&
          g.D * rows * (g.n[0] / line_e(g.n[0])) <= 512 &&
          xpass_lds(g.D, rows, g.n[0], sizeof(cplx<double>), true, AL) <= 160 * 1024;
 }
@@ -842,6 +844,7 @@ int xpass_grid(const Geom& g) { return g.nrows / xpass_rows_g(g); }
 template <typename T>
 hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
                         const void* src, void* dst, void* fhat, int n_active, void* phi) {
+  if (xpass_generic(g)) return launch_xpass_gen<T>(st, g, tb, prog, src, dst, fhat, n_active, phi);
 #define BEC_CALL(D) xpass_launch_d<T, D>(st, g, tb, prog, src, dst, fhat, n_active, phi)
   BEC_D_SWITCH(BEC_CALL)
 #undef BEC_CALL

@@ -189,7 +189,9 @@ strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, doubl
   }
 }
 
-int strided_tb(int N, int fields, int csize) { return strided_tb_c(N, fields, csize); }
+int strided_tb(int N, int fields, int csize) {
+  return fft_size_id(N) >= 0 ? strided_tb_c(N, fields, csize) : sgen_tb(N, fields, csize);
+}
 
 // Lines of the other transverse axis that a y/z pass visits.  Zero-padded DDI spectra: the
 // y passes only touch the z < n[2] planes that hold data (or are kept); the z pass all sn[1].
@@ -277,7 +279,9 @@ hipError_t launch_strided(hipStream_t st, const Geom& g, const DevTables& tb, in
     case 128: return strided_mode<T, 128>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 256: return strided_mode<T, 256>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
     case 512: return strided_mode<T, 512>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override, itp, ddi_data);
-    default: return hipErrorInvalidValue;
+    default:  // no built FFT of this length
+      return launch_strided_gen<T>(st, g, tb, axis, mode, data, n_active, kin_frac, dt_override,
+                                   itp, ddi_data, strided_other(g, axis, ddi_data));
   }
 }
 

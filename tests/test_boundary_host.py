@@ -57,13 +57,13 @@ def _cfg(**kw):
     (dict(ndim=4), L.BEC_EINVAL),
     (dict(F=7), L.BEC_EUNSUP),                  # more than 13 components: not built
     (dict(F=-1), L.BEC_EINVAL),                 # SpinSystem(F) needs F >= 0 (types.jl:43)
-    (dict(F=6, n=(512, 1, 1)), L.BEC_EUNSUP),   # 13 whole 512-point lines exceed one block
-    (dict(n=(72, 1, 1)), L.BEC_EUNSUP),
-    (dict(n=(320, 1, 1)), L.BEC_EUNSUP),
-    (dict(n=(120, 1, 1)), L.BEC_EUNSUP),
-    (dict(n=(448, 1, 1)), L.BEC_EUNSUP),
+    # any even side is built; what one block's LDS cannot hold is refused: 13 whole x rows
+    # of 2048 points, a 16384-point line, a y line of 16384 points
+    (dict(F=6, n=(2048, 1, 1)), L.BEC_EUNSUP),
+    (dict(n=(16384, 1, 1)), L.BEC_EUNSUP),
+    (dict(ndim=2, n=(8, 16384, 1)), L.BEC_EUNSUP),
+    (dict(ndim=3, n=(1024, 1024, 1024)), L.BEC_EUNSUP),  # 2^31 values per instance
     (dict(B=0), L.BEC_EINVAL),
-    (dict(n=(4, 1, 1)), L.BEC_EUNSUP),
 ])
 def test_create_rejects_invalid_configs(kw, code):
     c, buf = _cfg(**kw)
