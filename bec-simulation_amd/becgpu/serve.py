@@ -334,11 +334,10 @@ def solve_alone(r: Request, keep_psi=True):
     return res[0]
 
 
-# Built grid sides inside the stream's ranges ({64 ... 256} for 1-D and 2-D requests,
-# {48 ... 128} for 3-D): 2^k, 3 * 2^k, 5 * 2^k and 7 * 2^k (every multiple of 16 in the
-# 3-D range).
-SIDES12 = (64, 80, 96, 112, 128, 160, 192, 224, 256)
-SIDES3 = (48, 64, 80, 96, 112, 128)
+# The stream's side ranges: every even side in {64 ... 256} for 1-D and 2-D requests and in
+# {48 ... 128} for 3-D (sides without a built radix plan run the runtime-length passes).
+SIDES12 = tuple(range(64, 257, 2))
+SIDES3 = tuple(range(48, 129, 2))
 
 
 def poisson_mix(n_req, *, rate=200.0, seed=42, steps=(500, 5000), sides12=SIDES12,
