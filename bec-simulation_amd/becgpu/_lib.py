@@ -83,6 +83,10 @@ SIGNATURES = {
     "bec_fft_derivative": ([C.POINTER(BecConfig), C.c_void_p, C.c_int, C.c_void_p], C.c_int),
     "bec_fft_gradient": ([C.POINTER(BecConfig), C.c_void_p, C.c_void_p], C.c_int),
     "bec_fft_gradient_noalloc": ([C.POINTER(BecConfig), C.c_void_p, C.c_void_p], C.c_int),
+    "bec_group_create": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "bec_group_split_step": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "bec_group_grouped": ([C.c_void_p], C.c_int),
+    "bec_group_destroy": ([C.c_void_p], None),
     "bec_last_error": ([], C.c_char_p),
     "bec_workspace_bytes": ([C.POINTER(BecConfig)], C.c_longlong),
 }

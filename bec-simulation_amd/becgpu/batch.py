@@ -16,6 +16,7 @@ sequential driver.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 from dataclasses import dataclass, field
@@ -26,6 +27,40 @@ from . import _lib as L
 from .api import Workspace, _check_itp_overflow, _validate_itp_interactions, _validate_itp_zeeman
 from .physics import (Grid, HarmonicTrap, InteractionParams, NoPotential, SimParams, SpinSystem,
                       ZeemanParams, init_psi)
+
+
+class WorkspaceGroup:
+    """bec_group_*: several workspaces of different grid shapes (same precision, spin,
+    dimension and time kind) advanced together, each position of their common split-step
+    launch sequence issued as one kernel launch over all of them.  The group keeps
+    references to its members and must be closed before they are."""
+
+    def __init__(self, workspaces):
+        self.members = list(workspaces)
+        arr = (C.c_void_p * len(self.members))(*[w.handle for w in self.members])
+        h = C.c_void_p()
+        L.check(L.lib().bec_group_create(arr, len(self.members), C.byref(h)), "bec_group_create")
+        self._h = h
+
+    def split_step(self, n_steps, graph=True):
+        L.check(L.lib().bec_group_split_step(self._h, int(n_steps), int(bool(graph))),
+                "bec_group_split_step")
+
+    @property
+    def grouped(self):
+        """Whether the last split_step issued grouped launches."""
+        return bool(L.lib().bec_group_grouped(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            L.lib().bec_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -51,7 +86,7 @@ def _group_key(job: ITPJob, precision):
 class _Group:
     """The instances of one workspace and their ITP loop state."""
 
-    def __init__(self, jobs, idx, precision, graphs=False):
+    def __init__(self, jobs, idx, precision, graphs=False, autotune=True):
         j0 = jobs[0]
         self.graphs = graphs
         self.idx = idx
@@ -62,7 +97,7 @@ class _Group:
         sp = SimParams(dt=j0.dt, n_steps=j0.n_steps, imaginary_time=True, normalize_every=1,
                        save_every=self.save_every)
         self.ws = Workspace(grid=j0.grid, F=j0.F, n_instances=len(jobs), sim_params=sp,
-                            precision=precision)
+                            precision=precision, autotune=autotune)
         for i, j in enumerate(jobs):
             self.ws.set_instance(i, interactions=j.interactions, zeeman=j.zeeman,
                                  potential=j.potential, dt=j.dt)
@@ -80,12 +115,15 @@ class _Group:
     def done(self):
         return not self.running or self.step >= self.n_steps
 
-    def issue(self):
-        """Queue the steps up to the next check point (1 at a time for the first 10)."""
+    def next_k(self):
+        """Steps up to the next check point (1 at a time for the first 10)."""
         if self.step < 10:
-            k = 1
-        else:
-            k = min(self.save_every - self.step % self.save_every, self.n_steps - self.step)
+            return 1
+        return min(self.save_every - self.step % self.save_every, self.n_steps - self.step)
+
+    def issue(self):
+        """Queue the steps up to the next check point."""
+        k = self.next_k()
         if self.graphs:
             L.check(L.lib().bec_split_step_graph(self.ws.handle, int(k)), "bec_split_step_graph")
         else:
@@ -140,9 +178,18 @@ class _Group:
         return res
 
 
-def solve_batch(jobs, *, precision="fp64", keep_psi=False, graphs=False):
+def solve_batch(jobs, *, precision="fp64", keep_psi=False, graphs=False, grouped=False,
+                autotune=None):
     """Run every job to convergence or n_steps.  Returns (results in job order, stats).
-    graphs: issue each group's steps between check points as captured HIP graphs."""
+    graphs: issue each group's steps between check points as captured HIP graphs.
+    grouped: when the live groups are due at the same check point, advance all of them
+    with grouped launches (WorkspaceGroup: one launch per pass over every shape; captured
+    as one graph with `graphs`).  A grouped x-pass launch needs one block size for every
+    shape, which the default tiles give (the same number of sites per block), so grouped
+    runs keep them: autotune (per-shape x tiles, default on) is off with `grouped`.
+    Results are bitwise the same in every mode with the same tiles."""
+    if autotune is None:
+        autotune = not grouped
     for j in jobs:
         _validate_itp_zeeman(j.zeeman, j.F, j.dt)
         _validate_itp_interactions(j.interactions, j.F, j.dt, psi=j.psi0)
@@ -150,16 +197,33 @@ def solve_batch(jobs, *, precision="fp64", keep_psi=False, graphs=False):
     for k, j in enumerate(jobs):
         groups.setdefault(_group_key(j, precision), []).append(k)
     t0 = time.perf_counter()
-    gs = [_Group([jobs[k] for k in idx], idx, precision, graphs) for idx in groups.values()]
+    gs = [_Group([jobs[k] for k in idx], idx, precision, graphs, autotune)
+          for idx in groups.values()]
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     live = list(gs)
+    wgroup, wmembers = None, None
+    n_grouped = 0
     while live:
-        for g in live:          # queue work on every stream first ...
-            g.issue()
+        ks = {g.next_k() for g in live}
+        if grouped and len(live) > 1 and len(ks) == 1:
+            if wmembers != live:  # the group follows the live set
+                if wgroup is not None:
+                    wgroup.close()
+                wgroup, wmembers = WorkspaceGroup([g.ws for g in live]), list(live)
+            k = ks.pop()
+            wgroup.split_step(k, graph=graphs)
+            n_grouped += int(wgroup.grouped)
+            for g in live:
+                g.step += k
+        else:
+            for g in live:      # queue work on every stream first ...
+                g.issue()
         for g in live:          # ... then wait on each group at its check point
             g.check(t0)
         live = [g for g in live if not g.done]
+    if wgroup is not None:
+        wgroup.close()
     for g in gs:
         g.ws.synchronize()
     wall = time.perf_counter() - t0
@@ -172,7 +236,7 @@ def solve_batch(jobs, *, precision="fp64", keep_psi=False, graphs=False):
     stats = dict(wall_s=wall, setup_s=t_setup, n_groups=len(gs), solves=len(jobs),
                  solves_per_s=len(jobs) / wall,
                  group_sizes=[len(g.jobs) for g in gs],
-                 group_steps=[g.step for g in gs])
+                 group_steps=[g.step for g in gs], grouped_intervals=n_grouped)
     return results, stats
 
 

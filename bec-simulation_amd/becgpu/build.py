@@ -19,7 +19,10 @@ LIB = os.path.join(PKG_DIR, "libbecgpu.so")
 # (source, object, defines): bec_kernels.hip is compiled once for its shared code and once
 # per component count D = 2F+1, so that the parts compile in parallel
 KERNEL_DS = (1, 3, 5, 7, 9, 11, 13)
+# bec_group.hip likewise: shared part and the grouped x pass of 1, 3 and 5 components
+GROUP_DS = (1, 3, 5)
 UNITS = ([("bec_kernels.hip", f"bec_kernels_p{d}.o", [f"-DBEC_PART={d}"]) for d in (0,) + KERNEL_DS]
+         + [("bec_group.hip", f"bec_group_p{d}.o", [f"-DBEC_PART={d}"]) for d in (0,) + GROUP_DS]
          + [(src, os.path.splitext(src)[0] + ".o", [])
             for src in ("bec_strided.hip", "bec_xsite.hip", "bec_aux.hip", "bec_generic.hip",
                         "bec_capi.hip")])

@@ -34,7 +34,24 @@ static int fail(int code, const std::string& msg) {
       return fail(BEC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));    \
   } while (0)
 
+// One launch of a split-step schedule, as recorded for a grouped launch (bec_group_*).
+enum RecKind { R_X, R_STRIDED, R_REDUCE, R_SCALE, R_OTHER };
+struct Rec {
+  int kind;
+  XProgram xp;          // R_X
+  const void* src;
+  void* dst;
+  int axis, mode, itp, ddi;  // R_STRIDED
+  void* data;
+  double kin_frac, dt_ovr;
+  int nblocks;          // R_REDUCE
+  uint32_t slots, maxmask;
+};
+
 struct bec_ws {
+  // set while a grouped launch records this workspace's schedule: the launch helpers append
+  // to it instead of launching
+  std::vector<Rec>* rec = nullptr;
   Geom g{};
   int F = 0, D = 0, B = 0, prec = 0, itp = 0, normalize_every = 1, ddi = 0, tensor = 0;
   size_t csz = 16;
@@ -198,6 +215,15 @@ static double phi_bytes(const bec_ws* ws) {  // 3 real fields
 }
 
 static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
+  if (ws->rec) {
+    Rec r{};
+    r.kind = R_X;
+    r.xp = p;
+    r.src = src;
+    r.dst = dst;
+    ws->rec->push_back(r);
+    return BEC_OK;
+  }
   double b = 0.0;
   if (ws->prof_on) {
     const double pb = psi_bytes(ws);
@@ -218,6 +244,10 @@ static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
   return BEC_OK;
 }
 static int do_sites(bec_ws* ws, const SProgram& p) {
+  if (ws->rec) {
+    ws->rec->push_back(Rec{R_OTHER});
+    return BEC_OK;
+  }
   double b = 0.0;
   if (ws->prof_on) {
     b = 2.0 * psi_bytes(ws);
@@ -239,6 +269,10 @@ static int do_sites(bec_ws* ws, const SProgram& p) {
 }
 // Fused x pass + lane-per-site pass on psi in place (slot 0).
 static int do_xs(bec_ws* ws, const XProgram& p) {
+  if (ws->rec) {
+    ws->rec->push_back(Rec{R_OTHER});
+    return BEC_OK;
+  }
   double b = 0.0;
   if (ws->prof_on) {
     b = 2.0 * psi_bytes(ws);
@@ -265,6 +299,19 @@ static int do_xs(bec_ws* ws, const XProgram& p) {
 }
 static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_frac, double dt_ovr,
                       int itp, int ddi_data, int* nb = nullptr) {
+  if (ws->rec) {
+    Rec r{};
+    r.kind = R_STRIDED;
+    r.axis = axis;
+    r.mode = mode;
+    r.data = data;
+    r.kin_frac = kin_frac;
+    r.dt_ovr = dt_ovr;
+    r.itp = itp;
+    r.ddi = ddi_data;
+    ws->rec->push_back(r);
+    return BEC_OK;
+  }
   hipError_t e;
   const int cls = ddi_data ? 5 + (mode == SM_DDI ? 1 : 0)
                            : 1 + (mode == SM_KIN ? 1 : mode == SM_INV ? 2 : mode == SM_KE ? 3 : 0);
@@ -284,6 +331,10 @@ static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_fra
 }
 // 1-D DDI: the Q_ab product of the three half spectra (read and written once)
 static int do_ddi_line(bec_ws* ws) {
+  if (ws->rec) {
+    ws->rec->push_back(Rec{R_OTHER});
+    return BEC_OK;
+  }
   const int pe = prof_begin(ws, 6, ws->prof_on ? 2.0 * half_field_bytes(ws) : 0.0);
   const hipError_t e =
       ws->prec == BEC_FP32
@@ -294,6 +345,15 @@ static int do_ddi_line(bec_ws* ws) {
   return BEC_OK;
 }
 static int do_reduce(bec_ws* ws, int nblocks, uint32_t slots, uint32_t maxmask = 0) {
+  if (ws->rec) {
+    Rec r{};
+    r.kind = R_REDUCE;
+    r.nblocks = nblocks;
+    r.slots = slots;
+    r.maxmask = maxmask;
+    ws->rec->push_back(r);
+    return BEC_OK;
+  }
   const int pe = prof_begin(ws, 7);
   HIPCHK(launch_reduce(ws->stream, ws->tb, ws->d_red, n_active(ws), nblocks, slots, maxmask));
   prof_end(ws, pe);
@@ -517,6 +577,10 @@ struct Sched {
     CHK(flush());
     if (scale_pending) {
       scale_pending = false;
+      if (ws->rec) {
+        ws->rec->push_back(Rec{R_SCALE});
+        return BEC_OK;
+      }
       hipError_t e = ws->prec == BEC_FP32
                          ? launch_scale_norm<float>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws))
                          : launch_scale_norm<double>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws));
@@ -1896,5 +1960,335 @@ int bec_profile_read(bec_ws* ws, double* ms, long long* count) {
   ws->prof_cls.clear();
   return BEC_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Grouped split steps over several workspaces (batched solves of different grid shapes):
+// each position of the common launch sequence is one kernel launch for all workspaces.
+// ---------------------------------------------------------------------------
+struct bec_group {
+  std::vector<bec_ws*> ws;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> ready;
+  void* d_blob = nullptr;  // descriptors of the uncaptured path
+  size_t blob_cap = 0;
+  struct Graph {
+    int n;
+    std::vector<unsigned char> blob;
+    void* d_blob;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  int grouped = 0;  // whether the last call issued grouped launches
+};
+
+namespace {
+struct GPos {
+  int kind, mode;
+  size_t off;
+  int nd, blocks, threads;
+  size_t lds;
+  // x pass: launches of at most kXGroupMax descriptors each (kernel arguments; the copy in
+  // the blob only keys the captured graphs)
+  std::vector<std::vector<XGroupDesc>> xchunks;
+  std::vector<int> xblocks;
+};
+
+size_t blob_align(size_t b) { return (b + 255) / 256 * 256; }
+
+template <typename Desc>
+size_t blob_put(std::vector<unsigned char>& blob, const std::vector<Desc>& d) {
+  const size_t off = blob_align(blob.size());
+  blob.resize(off + sizeof(Desc) * d.size());
+  std::memcpy(blob.data() + off, d.data(), sizeof(Desc) * d.size());
+  return off;
+}
+
+// The launch plan of n_steps split steps for every workspace with active instances, or an
+// empty plan when a position has no grouped kernel (shape, pass or schedule mismatch).
+int group_plan(bec_group* grp, int n_steps, std::vector<bec_ws*>& live, std::vector<GPos>& plan,
+               std::vector<unsigned char>& blob) {
+  plan.clear();
+  blob.clear();
+  live.clear();
+  for (bec_ws* w : grp->ws)
+    if (n_active(w) > 0) live.push_back(w);
+  if (live.empty()) return BEC_OK;
+  std::vector<std::vector<Rec>> recs(live.size());
+  for (size_t i = 0; i < live.size(); ++i) {
+    bec_ws* w = live[i];
+    if (w->prof_on) return BEC_OK;
+    w->rec = &recs[i];
+    const int rc = bec_split_step(w, n_steps);
+    w->rec = nullptr;
+    CHK(rc);
+  }
+  const size_t npos = recs[0].size();
+  for (auto& r : recs)
+    if (r.size() != npos) return BEC_OK;
+  const bool f32 = live[0]->prec == BEC_FP32;
+  const size_t csz = f32 ? 8 : 16;
+  std::vector<GPos> out;
+  for (size_t p = 0; p < npos; ++p) {
+    const Rec& r0 = recs[0][p];
+    for (auto& r : recs)
+      if (r[p].kind != r0.kind || (r0.kind == R_STRIDED && (r[p].axis != r0.axis ||
+                                                             r[p].mode != r0.mode)))
+        return BEC_OK;
+    GPos pos{};
+    pos.kind = r0.kind;
+    pos.mode = r0.mode;
+    pos.nd = (int)live.size();
+    int blocks = 0;
+    if (r0.kind == R_X) {
+      std::vector<XGroupDesc> d(live.size());
+      for (size_t i = 0; i < live.size(); ++i) {
+        bec_ws* w = live[i];
+        if (!xgroup_ok(w->g)) return BEC_OK;
+        const int thr = xpass_threads_g(w->g);
+        if (i == 0) pos.threads = thr;
+        else if (thr != pos.threads) return BEC_OK;
+        XGroupDesc& x = d[i];
+        std::memset(&x, 0, sizeof(x));
+        x.g = w->g;
+        x.tb = w->tb;
+        x.prog = recs[i][p].xp;
+        x.src = recs[i][p].src;
+        x.dst = recs[i][p].dst;
+        x.fhat = w->fhat;
+        x.phi = w->phi;
+        x.rows = xpass_rows_g(w->g);
+        x.gx = xpass_grid(w->g);
+        if (i % kXGroupMax == 0) {
+          pos.xchunks.emplace_back();
+          pos.xblocks.push_back(0);
+        }
+        x.block0 = pos.xblocks.back();
+        pos.xblocks.back() += x.gx * n_active(w);
+        blocks += x.gx * n_active(w);
+        pos.xchunks.back().push_back(x);
+        pos.lds = std::max(pos.lds, xpass_lds_g(w->g, x.prog, csz));
+      }
+      pos.off = blob_put(blob, d);
+    } else if (r0.kind == R_STRIDED) {
+      std::vector<SGroupDesc> d(live.size());
+      for (size_t i = 0; i < live.size(); ++i) {
+        bec_ws* w = live[i];
+        const Rec& r = recs[i][p];
+        if (!sgroup_ok(w->g, r.axis, r.mode, r.ddi)) return BEC_OK;
+        SGroupDesc& x = d[i];
+        std::memset(&x, 0, sizeof(x));
+        x.g = w->g;
+        x.tb = w->tb;
+        x.data = r.data;
+        x.kin_frac = r.kin_frac;
+        x.dt_override = r.dt_ovr;
+        x.axis = r.axis;
+        x.itp = r.itp;
+        const int TB = 2048 / w->g.n[r.axis];
+        const int n_other = r.axis == 1 ? w->g.n[2] : w->g.n[1];
+        x.gx = ((w->g.n[0] + TB - 1) / TB) * n_other;
+        x.gy = w->g.D;
+        x.block0 = blocks;
+        blocks += x.gx * x.gy * n_active(w);
+      }
+      pos.threads = kSGroupThreads;
+      pos.off = blob_put(blob, d);
+    } else if (r0.kind == R_REDUCE) {
+      std::vector<RGroupDesc> d(live.size());
+      for (size_t i = 0; i < live.size(); ++i) {
+        bec_ws* w = live[i];
+        const Rec& r = recs[i][p];
+        RGroupDesc& x = d[i];
+        std::memset(&x, 0, sizeof(x));
+        x.tb = w->tb;
+        x.red = w->d_red;
+        x.nblocks = r.nblocks;
+        x.slot_mask = r.slots;
+        x.max_mask = r.maxmask;
+        x.gx = __builtin_popcount(r.slots);
+        x.block0 = blocks;
+        blocks += x.gx * n_active(w);
+      }
+      pos.off = blob_put(blob, d);
+    } else if (r0.kind == R_SCALE) {
+      std::vector<CGroupDesc> d(live.size());
+      for (size_t i = 0; i < live.size(); ++i) {
+        bec_ws* w = live[i];
+        CGroupDesc& x = d[i];
+        std::memset(&x, 0, sizeof(x));
+        x.g = w->g;
+        x.tb = w->tb;
+        x.psi = w->slot[0];
+        x.gx = (int)std::min<long long>(1024, (w->g.inst_stride + 255) / 256);
+        x.block0 = blocks;
+        blocks += x.gx * n_active(w);
+      }
+      pos.off = blob_put(blob, d);
+    } else {
+      return BEC_OK;
+    }
+    pos.blocks = blocks;
+    out.push_back(pos);
+  }
+  plan.swap(out);
+  return BEC_OK;
+}
+
+int group_issue(bec_group* grp, const std::vector<GPos>& plan, const unsigned char* d_blob,
+                int prec, int D) {
+  const hipStream_t st = grp->stream;
+  for (const GPos& p : plan) {
+    const void* d = d_blob + p.off;
+    hipError_t e = hipSuccess;
+    if (p.kind == R_X) {
+      for (size_t c = 0; c < p.xchunks.size() && e == hipSuccess; ++c) {
+        const auto& xc = p.xchunks[c];
+        e = prec == BEC_FP32 ? launch_xpass_group<float>(st, D, xc.data(), (int)xc.size(),
+                                                         p.xblocks[c], p.threads, p.lds)
+                             : launch_xpass_group<double>(st, D, xc.data(), (int)xc.size(),
+                                                          p.xblocks[c], p.threads, p.lds);
+      }
+    } else if (p.kind == R_STRIDED) {
+      e = prec == BEC_FP32
+              ? launch_strided_group<float>(st, p.mode, (const SGroupDesc*)d, p.nd, p.blocks)
+              : launch_strided_group<double>(st, p.mode, (const SGroupDesc*)d, p.nd, p.blocks);
+    } else if (p.kind == R_REDUCE) {
+      e = launch_reduce_group(st, (const RGroupDesc*)d, p.nd, p.blocks);
+    } else {
+      e = prec == BEC_FP32 ? launch_scale_group<float>(st, (const CGroupDesc*)d, p.nd, p.blocks)
+                           : launch_scale_group<double>(st, (const CGroupDesc*)d, p.nd, p.blocks);
+    }
+    HIPCHK(e);
+  }
+  return BEC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int bec_group_create(bec_ws* const* wss, int n, bec_group** out) {
+  if (!wss || !out || n < 1) return fail(BEC_EINVAL, "bec_group_create: null argument or n < 1");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!wss[i]) return fail(BEC_EINVAL, "bec_group_create: null workspace");
+    if (wss[i]->prec != wss[0]->prec || wss[i]->D != wss[0]->D ||
+        wss[i]->g.ndim != wss[0]->g.ndim || wss[i]->itp != wss[0]->itp)
+      return fail(BEC_EINVAL, "bec_group_create: workspaces of a group share precision, spin, "
+                              "dimension and time kind");
+    for (int j = 0; j < i; ++j)
+      if (wss[j] == wss[i]) return fail(BEC_EINVAL, "bec_group_create: workspace listed twice");
+  }
+  bec_group* g = new bec_group();
+  g->ws.assign(wss, wss + n);
+  auto cleanup = [&](const char* m) {
+    bec_group_destroy(g);
+    return fail(BEC_EDEVICE, m);
+  };
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup("hipStreamCreate failed");
+  if (hipEventCreateWithFlags(&g->done, hipEventDisableTiming) != hipSuccess)
+    return cleanup("hipEventCreate failed");
+  g->ready.assign(n, nullptr);
+  for (int i = 0; i < n; ++i)
+    if (hipEventCreateWithFlags(&g->ready[i], hipEventDisableTiming) != hipSuccess)
+      return cleanup("hipEventCreate failed");
+  *out = g;
+  return BEC_OK;
+}
+
+void bec_group_destroy(bec_group* g) {
+  if (!g) return;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (auto& e : g->graphs) {
+    (void)hipGraphExecDestroy(e.exec);
+    (void)hipFree(e.d_blob);
+  }
+  if (g->d_blob) (void)hipFree(g->d_blob);
+  for (hipEvent_t e : g->ready)
+    if (e) (void)hipEventDestroy(e);
+  if (g->done) (void)hipEventDestroy(g->done);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+int bec_group_split_step(bec_group* grp, int n_steps, int use_graph) {
+  if (!grp) return fail(BEC_EINVAL, "null group");
+  if (n_steps < 0) return fail(BEC_EINVAL, "n_steps < 0");
+  grp->grouped = 0;
+  if (n_steps == 0) return BEC_OK;
+  std::vector<bec_ws*> live;
+  std::vector<GPos> plan;
+  std::vector<unsigned char> blob;
+  CHK(group_plan(grp, n_steps, live, plan, blob));
+  if (live.empty()) return BEC_OK;
+  if (plan.empty()) {  // no grouped kernel for some position: each workspace on its own
+    for (bec_ws* w : live) CHK(use_graph ? bec_split_step_graph(w, n_steps) : bec_split_step(w, n_steps));
+    return BEC_OK;
+  }
+  // the group's stream runs after the work already queued on every member's stream, and
+  // the members' streams after the group's launches
+  for (size_t i = 0; i < grp->ws.size(); ++i) {
+    HIPCHK(hipEventRecord(grp->ready[i], grp->ws[i]->stream));
+    HIPCHK(hipStreamWaitEvent(grp->stream, grp->ready[i], 0));
+  }
+  const int prec = live[0]->prec, D = live[0]->D;
+  if (use_graph) {
+    bec_group::Graph* hit = nullptr;
+    for (auto& e : grp->graphs)
+      if (e.n == n_steps && e.blob == blob) hit = &e;
+    if (!hit) {
+      constexpr size_t kMaxGraphs = 16;
+      if (grp->graphs.size() >= kMaxGraphs) {
+        HIPCHK(hipStreamSynchronize(grp->stream));
+        (void)hipGraphExecDestroy(grp->graphs.front().exec);
+        (void)hipFree(grp->graphs.front().d_blob);
+        grp->graphs.erase(grp->graphs.begin());
+      }
+      void* d = nullptr;
+      HIPCHK(hipMalloc(&d, std::max<size_t>(blob.size(), 256)));
+      if (hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return fail(BEC_EDEVICE, "group descriptor copy failed");
+      }
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(grp->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = group_issue(grp, plan, (const unsigned char*)d, prec, D);
+      const hipError_t ec = hipStreamEndCapture(grp->stream, &graph);
+      hipGraphExec_t exec = nullptr;
+      hipError_t ei = hipErrorUnknown;
+      if (rc == BEC_OK && ec == hipSuccess) ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc != BEC_OK || ec != hipSuccess || ei != hipSuccess) {
+        (void)hipFree(d);
+        if (rc != BEC_OK) return rc;
+        HIPCHK(ec);
+        HIPCHK(ei);
+      }
+      grp->graphs.push_back({n_steps, blob, d, exec});
+      hit = &grp->graphs.back();
+    }
+    HIPCHK(hipGraphLaunch(hit->exec, grp->stream));
+  } else {
+    HIPCHK(hipStreamSynchronize(grp->stream));  // the previous call's launches read d_blob
+    if (blob.size() > grp->blob_cap) {
+      if (grp->d_blob) (void)hipFree(grp->d_blob);
+      grp->d_blob = nullptr;
+      grp->blob_cap = 0;
+      HIPCHK(hipMalloc(&grp->d_blob, blob.size()));
+      grp->blob_cap = blob.size();
+    }
+    HIPCHK(hipMemcpy(grp->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    CHK(group_issue(grp, plan, (const unsigned char*)grp->d_blob, prec, D));
+  }
+  HIPCHK(hipEventRecord(grp->done, grp->stream));
+  for (bec_ws* w : grp->ws) HIPCHK(hipStreamWaitEvent(w->stream, grp->done, 0));
+  grp->grouped = 1;
+  return BEC_OK;
+}
+
+int bec_group_grouped(const bec_group* grp) { return grp ? grp->grouped : 0; }
 
 }  // extern "C"

@@ -156,6 +156,71 @@ hipError_t launch_strided_gen(hipStream_t st, const Geom& g, const DevTables& tb
                               int mode, void* data, int n_active, double kin_frac,
                               double dt_override, int itp, int ddi_data, int other_lines);
 
+// Grouped launches (bec_group.hip): one launch runs a pass of the same schedule position
+// for several workspaces (different grid shapes), each block picking its workspace from a
+// device array of descriptors by its linear index (block0 = first block of the descriptor).
+struct XGroupDesc {
+  Geom g;
+  DevTables tb;
+  XProgram prog;
+  const void* src;
+  void* dst;
+  void* fhat;
+  void* phi;
+  int rows;    // x rows per tile
+  int gx;      // tiles per instance (blocks = gx * active instances)
+  int block0;
+  int pad_;
+};
+struct SGroupDesc {
+  Geom g;
+  DevTables tb;
+  void* data;
+  double kin_frac, dt_override;
+  int axis, itp;
+  int gx, gy;  // x chunks * lines of the other axis, component groups (blocks = gx*gy*active)
+  int block0;
+  int pad_;
+};
+struct RGroupDesc {
+  DevTables tb;
+  double* red;
+  int nblocks;
+  uint32_t slot_mask, max_mask;
+  int gx;      // slots reduced (blocks = gx * active instances)
+  int block0;
+  int pad_;
+};
+struct CGroupDesc {  // ITP renormalisation psi *= 1/sqrt(norm dV)
+  Geom g;
+  DevTables tb;
+  void* psi;
+  int gx;
+  int block0;
+};
+// Shapes the grouped kernels are built for: x sides 32, 64, 128, 256 with 1, 3 or 5
+// components (x pass); lines of 32 ... 256 points (y/z pass, modes SM_FWD, SM_INV, SM_KIN).
+bool xgroup_ok(const Geom& g);
+bool sgroup_ok(const Geom& g, int axis, int mode, int ddi_data);
+constexpr int kSGroupThreads = 256;  // y/z tiles of 2048 elements: 2048 / N columns
+// x pass: descs is a HOST array of nd <= kXGroupMax descriptors (passed as the kernel
+// argument), block0 counted from the first; the other grouped launches read theirs from
+// device memory.
+constexpr int kXGroupMax = 4;
+template <typename T>
+hipError_t launch_xpass_group(hipStream_t st, int D, const XGroupDesc* descs, int nd, int blocks,
+                              int threads, size_t lds);
+template <typename T>
+hipError_t launch_strided_group(hipStream_t st, int mode, const SGroupDesc* descs, int nd,
+                                int blocks);
+hipError_t launch_reduce_group(hipStream_t st, const RGroupDesc* descs, int nd, int blocks);
+template <typename T>
+hipError_t launch_scale_group(hipStream_t st, const CGroupDesc* descs, int nd, int blocks);
+// LDS bytes of an x-pass launch of this geometry and program (as launch_xpass sizes it)
+size_t xpass_lds_g(const Geom& g, const XProgram& prog, size_t csz);
+// threads of an x-pass block of this geometry
+int xpass_threads_g(const Geom& g);
+
 // Spin density (Fx, Fy, Fz) of one instance's psi into out[3][nsites], FP64 (bec_aux.hip).
 template <typename T>
 hipError_t launch_spin_density(hipStream_t st, const void* psi, double* out, long long nsites,

@@ -10,6 +10,15 @@
 
 namespace bec {
 
+// Threads per x-pass block are capped so that the per-site operators of large spins
+// (13-component Euler rotations and the tensor step) keep their spinors in VGPRs.
+template <int D>
+constexpr int xpass_max_threads() { return 512; }
+// Values per FFT thread along x (the line's radix plan: 8, 12, 20 or 28).  With E <= D
+// every tile has at most one site per thread.
+template <int NX, int D>
+constexpr int xpass_e() { return fft_e<NX>(); }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

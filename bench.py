@@ -333,13 +333,13 @@ def converged_error(G, n_jobs=24, n_steps=200000):
                 fp64_last_rel_dE_max=max(b["dE"] / abs(b["energy"]) for b in r64))
 
 
-def run_batch2d(G, A, n_jobs, n_plain, graphs=False):
+def run_batch2d(G, A, n_jobs, n_plain, graphs=False, grouped=False):
     """Batched FP32-bulk and FP64 solves/s of the 256-job set, against the plain per-solve
     FP64 path (find_ground_state one job at a time) on an evenly spaced subsample."""
     from becgpu.batch import solve_batch
     jobs = cfg4_jobs(G, n_jobs)
-    r32, s32 = solve_batch(jobs, precision="fp32", graphs=graphs)
-    r64, s64 = solve_batch(jobs, precision="fp64", graphs=graphs)
+    r32, s32 = solve_batch(jobs, precision="fp32", graphs=graphs, grouped=grouped)
+    r64, s64 = solve_batch(jobs, precision="fp64", graphs=graphs, grouped=grouped)
     sub = jobs[::max(1, n_jobs // n_plain)][:n_plain]
     t0 = time.perf_counter()
     for j in sub:
@@ -520,6 +520,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serve_mix: mean arrivals per s")
     ap.add_argument("--graphs", action="store_true",
                     help="batch2d_itp_256: issue each group's steps as captured HIP graphs")
+    ap.add_argument("--grouped", action="store_true",
+                    help="batch2d_itp_256: one launch per pass over all grid shapes")
     ap.add_argument("--sequential", action="store_true",
                     help="serve_mix: also time every request alone, one after another")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32"])
@@ -590,7 +592,7 @@ def main():
         print(json.dumps(line))
         return
     if args.workload == "batch2d_itp_256":
-        b = run_batch2d(G, A, args.jobs, n_plain=12, graphs=args.graphs)
+        b = run_batch2d(G, A, args.jobs, n_plain=12, graphs=args.graphs, grouped=args.grouped)
         line = {
             "metric": "batched ITP solves/s (256 x 2-D spin-1, mixed 64^2/128^2/256^2)",
             "value": b["fp32"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
@@ -600,7 +602,9 @@ def main():
             "config": {"workload": "batched find_ground_state, 2-D spin-1", "jobs": args.jobs,
                        "sides": [64, 128, 256], "box": 20.0, "c0": 10.0, "c1": "U[-1,1] seed 42",
                        "trap": "none",
-                       "dt": 0.005, "tol": 1e-8, "n_steps": 10000, "graphs": bool(args.graphs)},
+                       "dt": 0.005, "tol": 1e-8, "n_steps": 10000, "graphs": bool(args.graphs),
+                       "grouped": bool(args.grouped)},
+            "grouped_intervals": b["fp32"].get("grouped_intervals", 0),
             "fp64_batched_solves_per_s": b["fp64"]["solves_per_s"],
             "plain_fp64_solves_per_s": b["plain_fp64_solves_per_s"],
             "speedup_vs_plain_fp64": b["fp32"]["solves_per_s"] / b["plain_fp64_solves_per_s"],

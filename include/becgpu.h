@@ -48,14 +48,15 @@ typedef struct bec_ws bec_ws;
  * GridConfig types.jl:1-23; SimParams types.jl:217-237). */
 typedef struct bec_config {
   int ndim;            /* 1, 2 or 3 */
-  int n[3];            /* grid points per axis (x, y, z); unused axes 1.  Built: 2^k in
-                          [8, 512], 3*2^k in [24, 192], 5*2^k in [40, 160], 7*2^k in
-                          [56, 224]; other even sides (valid in the reference) return
-                          BEC_EUNSUP */
+  int n[3];            /* grid points per axis (x, y, z); unused axes 1.  Any positive
+                          even side (types.jl:12): 2^k in [8, 512], 3*2^k in [24, 192],
+                          5*2^k in [40, 160] and 7*2^k in [56, 224] have compiled radix
+                          plans, every other side runs line passes of run-time length.
+                          BEC_EUNSUP when one x row of all 2F+1 components (with the DDI
+                          lines) or one y/z line does not fit a block's LDS, or an
+                          instance has 2^31 complex values or more */
   double L[3];         /* box size per axis */
-  int F;               /* spin (2F+1 components), 0 <= F <= 6 (SpinSystem, types.jl:36-46);
-                          2F+1 whole x lines must fit one block: (2F+1) * n[0] / E <= 512
-                          with E = 8 (2^k sides), 12, 20 or 28 */
+  int F;               /* spin (2F+1 components), 0 <= F <= 6 (SpinSystem, types.jl:36-46) */
   int n_instances;     /* batch size B */
   int precision;       /* BEC_FP64 or BEC_FP32 */
   int imaginary_time;  /* SimParams.imaginary_time */
@@ -241,6 +242,29 @@ int bec_query(bec_ws* ws);
 const char* bec_last_error(void);
 /* Bytes of device memory held by a workspace of this configuration (admission). */
 long long bec_workspace_bytes(const bec_config* cfg);
+
+/* Grouped split steps.  No counterpart in the reference, whose batched drivers
+ * (find_ground_state_multistart, scan_continuation: ground_state.jl:239-271, 328-378) loop
+ * over solves one at a time.  A group holds workspaces of different grid shapes with the
+ * same precision, spin, dimension and time kind (n >= 1, each listed once; the group does
+ * not own them and must be destroyed before them).  bec_group_split_step advances every
+ * member's active instances by n_steps split steps (bec_split_step) with each position of
+ * their common launch sequence issued as ONE kernel launch over all members (the x pass:
+ * one launch per four members, whose descriptors travel as kernel arguments; use_graph:
+ * the whole sequence replayed as one captured HIP graph, re-captured when a member's
+ * active set, tiles or operator set change).  Results are bitwise those of bec_split_step
+ * on each member.  Members whose shapes or passes have no grouped kernel (x sides other
+ * than 32, 64, 128, 256; more than 5 components; dipolar, tensor or lane-per-site passes;
+ * unequal x-pass block sizes) are advanced one workspace at a time instead, with the same
+ * results.  Ordering: the group's launches follow the work queued on every member's
+ * stream, and later work on a member's stream follows them. */
+typedef struct bec_group bec_group;
+int bec_group_create(bec_ws* const* ws, int n, bec_group** out);
+int bec_group_split_step(bec_group* g, int n_steps, int use_graph);
+/* 1 when the last bec_group_split_step issued grouped launches, 0 when it advanced the
+ * members one at a time (or had nothing to do). */
+int bec_group_grouped(const bec_group* g);
+void bec_group_destroy(bec_group* g);
 
 #ifdef __cplusplus
 }
