@@ -429,6 +429,21 @@ struct Sched {
     reset();
     return r;
   }
+  // In real time two adjacent spin-1 spin-mixing steps are one step of the summed time:
+  // exp(-i c1 dt F.S) rotates the spinor about its own spin vector F, which the rotation
+  // leaves unchanged.  The two halves of _half_potential_step!'s spin mixing are adjacent
+  // when no DDI, nematic or Raman step sits between them, and then run as one Rodrigues
+  // rotation.  (Only these: they are adjacent within one potential step, so a run's
+  // launches do not depend on how its steps are split into calls.  In imaginary time the
+  // factor changes F and both halves run as written; so do the Euler rotations of larger
+  // spins, whose composed rounding drifts from the reference's two halves by more than the
+  // FP64 parity bar over long adaptive runs.)
+  bool merge(Op* ops, int n, int kind, double frac) const {
+    if (itp || ws->D != 3 || n == 0 || kind != OP_SPINMIX || ops[n - 1].kind != kind)
+      return false;
+    ops[n - 1].frac += frac;
+    return true;
+  }
   int add_op(int kind, double frac) {
     if (scale_pending) {
       scale_pending = false;
@@ -436,6 +451,7 @@ struct Sched {
     }
     if (fuse) {  // one op list; the x stages after it close the program
       if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
+      if (merge(p.A, p.nA, kind, frac)) return BEC_OK;
       if (p.nA >= kMaxOps) CHK(flush());
       p.A[p.nA].kind = kind;
       p.A[p.nA].frac = frac;
@@ -450,6 +466,7 @@ struct Sched {
         CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
         reset();
       }
+      if (merge(sp.ops, sp.n, kind, frac)) return BEC_OK;
       if (sp.n >= kMaxOps) CHK(flush_s());
       sp.ops[sp.n].kind = kind;
       sp.ops[sp.n].frac = frac;
@@ -457,6 +474,10 @@ struct Sched {
       return BEC_OK;
     }
     if (p.stages & (XS_FWD_FFT | XS_NORM | XS_WRITE_FHAT)) CHK(flush());
+    {
+      const bool toB = (p.stages & XS_KIN_1D) != 0;
+      if (merge(toB ? p.B : p.A, toB ? p.nB : p.nA, kind, frac)) return BEC_OK;
+    }
     if (((p.stages & XS_KIN_1D) ? p.nB : p.nA) >= kMaxOps) CHK(flush());
     const bool toB = (p.stages & XS_KIN_1D) != 0;
     Op* ops = toB ? p.B : p.A;
