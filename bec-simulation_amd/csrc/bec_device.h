@@ -57,6 +57,30 @@ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) {
   return INV ? mk(-a.y, a.x) : mk(a.y, -a.x);
 }
 
+// Element i of an array at a wave-uniform base, with the per-lane part as a 32-bit byte
+// offset (i * sizeof(T) < 4 GiB): the access takes the scalar-base addressing form, one VGPR
+// for the lane offset instead of a 64-bit address per array.
+template <typename T>
+__device__ __forceinline__ T* lane_elem(T* base, int i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + (unsigned)i * (unsigned)sizeof(T));
+}
+
+// Coefficients of sincos_d: the Taylor coefficients of sin (1/17! down to -1/3!, alternating
+// signs), of cos (-1/18! down to -1/2!), 2/pi and pi/2 split into three doubles.
+struct SinCosCoef {
+  double v[20] = {2.8114572543455206e-15, -7.647163731819816e-13, 1.6059043836821613e-10,
+                  -2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
+                  0.008333333333333333,   -0.16666666666666666,
+                  -1.5619206968586225e-16, 4.779477332387385e-14, -1.1470745597729725e-11,
+                  2.08767569878681e-09,    -2.755731922398589e-07, 2.48015873015873e-05,
+                  -0.001388888888888889,   0.041666666666666664,  -0.5,
+                  0.6366197723675814,      1.5707963267948966,    6.123233995736766e-17};
+};
+template <int I = 0>
+__constant__ SinCosCoef kSinCosConst = SinCosCoef{};
+using const_f64_ptr = const __attribute__((address_space(4))) double*;
+
 // Out-of-line library sine/cosine for the rare arguments |a| > 1e6 (and NaN), so that
 // its large-argument reduction does not set the register budget of the calling kernels.
 static __device__ __noinline__ void sincos_lib(double a, double* s, double* c) { sincos(a, s, c); }
@@ -66,40 +90,34 @@ static __device__ __noinline__ void sincos_lib(double a, double* s, double* c) {
 // and cos (to x^18) on |x| <= pi/4 in Horner form (truncation < 1e-19); the quadrant k mod 4
 // selects and signs the pair.  Measured against long-double sinl/cosl: <= 1.5 ulp for
 // |a| <= 1e6.  The phase arguments of the split-step operators are far below that bound.
+// The coefficients are read from constant memory through a pointer the compiler cannot see
+// through: as literals, every FP64 coefficient is a register pair that the compiler hoists
+// out of the operator loops of the site kernels, where they spill.
 __device__ __forceinline__ void sincos_d(double a, double* s, double* c) {
   if (!(fabs(a) <= 1.0e6)) {
     sincos_lib(a, s, c);
     return;
   }
+  const_f64_ptr cf = (const_f64_ptr)kSinCosConst<>.v;
+  asm volatile("" : "+s"(cf));
   // |a| <= pi/4 (every phase argument of a small step): k = 0 and x = a exactly, so the
   // reduction and the quadrant selection are skipped with the same result
   const bool small = fabs(a) <= 0.7853981633974483;
-  const double k = small ? 0.0 : rint(a * 0.6366197723675814);  // 2/pi
+  const double k = small ? 0.0 : rint(a * cf[17]);  // 2/pi
   double x = a;
   if (!small) {
-    x = fma(-k, 1.5707963267948966, a);  // pi/2 = hi + mid + lo
-    x = fma(-k, 6.123233995736766e-17, x);
+    x = fma(-k, cf[18], a);  // pi/2 = hi + mid + lo
+    x = fma(-k, cf[19], x);
     x = fma(-k, -1.4973849048591698e-33, x);
   }
   const double z = x * x;
-  double ps = 2.8114572543455206e-15;  // 1/17!, alternating signs down to -1/3!
-  ps = fma(ps, z, -7.647163731819816e-13);
-  ps = fma(ps, z, 1.6059043836821613e-10);
-  ps = fma(ps, z, -2.505210838544172e-08);
-  ps = fma(ps, z, 2.7557319223985893e-06);
-  ps = fma(ps, z, -0.0001984126984126984);
-  ps = fma(ps, z, 0.008333333333333333);
-  ps = fma(ps, z, -0.16666666666666666);
+  double ps = cf[0];
+#pragma unroll
+  for (int i = 1; i < 8; ++i) ps = fma(ps, z, cf[i]);
   const double sn = fma(x * z, ps, x);
-  double pc = -1.5619206968586225e-16;  // -1/18!, alternating signs down to -1/2!
-  pc = fma(pc, z, 4.779477332387385e-14);
-  pc = fma(pc, z, -1.1470745597729725e-11);
-  pc = fma(pc, z, 2.08767569878681e-09);
-  pc = fma(pc, z, -2.755731922398589e-07);
-  pc = fma(pc, z, 2.48015873015873e-05);
-  pc = fma(pc, z, -0.001388888888888889);
-  pc = fma(pc, z, 0.041666666666666664);
-  pc = fma(pc, z, -0.5);
+  double pc = cf[8];
+#pragma unroll
+  for (int i = 9; i < 17; ++i) pc = fma(pc, z, cf[i]);
   const double cs = fma(z, pc, 1.0);
   if (small) {
     *s = sn;
@@ -730,7 +748,6 @@ inline constexpr tens::CGTab<F> kCG{};
 // fetches eight values per s_load_dwordx16 instead.
 template <int F>
 __constant__ tens::CGTab<F> kCGConst = tens::CGTab<F>{};
-using const_f64_ptr = const __attribute__((address_space(4))) double*;
 
 // CG value at table index I as an operand of type T (FP32: literal; FP64: loaded through
 // cgp, a pointer to kCGConst<F> that the caller makes opaque on every matvec, so the loads
@@ -801,6 +818,9 @@ template <typename T, int D>
 __device__ __forceinline__ void tensor_matvec_folded(const cplx<T>* psi, const cplx<T>* v,
                                                      cplx<T>* w, const_f64_ptr kp) {
   constexpr int F = (D - 1) / 2;
+  // Every product is written as a chain of fused multiply-adds (a complex multiply-add is
+  // four FMAs): 2.2k VALU instructions per matvec at F = 6 instead of 2.8k with the
+  // separate multiplies and adds that the complex operators leave after contraction.
 #pragma unroll
   for (int c = 0; c < D; ++c) w[c] = mk((T)0, (T)0);
   static_for<-2 * F, 2 * F + 1>([&](auto Mc) __attribute__((always_inline)) {
@@ -812,21 +832,40 @@ __device__ __forceinline__ void tensor_matvec_folded(const cplx<T>* psi, const c
     static_for<0, NP>([&](auto ic) __attribute__((always_inline)) {
       constexpr int m1 = LO + decltype(ic)::value;
       constexpr int m2 = M - m1;
-      cplx<T> p = psi[F - m2] * v[F - m1];
-      if constexpr (m1 != m2) p = p + psi[F - m1] * v[F - m2];
-      Pp[decltype(ic)::value] = p;
+      const cplx<T> a = psi[F - m2], b = v[F - m1];
+      T pr = fma(a.x, b.x, -(a.y * b.y));
+      T pi = fma(a.x, b.y, a.y * b.x);
+      if constexpr (m1 != m2) {
+        const cplx<T> c = psi[F - m1], d = v[F - m2];
+        pr = fma(c.x, d.x, pr);
+        pr = fma(-c.y, d.y, pr);
+        pi = fma(c.x, d.y, pi);
+        pi = fma(c.y, d.x, pi);
+      }
+      Pp[decltype(ic)::value] = mk(pr, pi);
     });
     static_for<0, NP>([&](auto ic) __attribute__((always_inline)) {
       constexpr int i = decltype(ic)::value;
       constexpr int m1 = LO + i;
       constexpr int m2 = M - m1;
-      cplx<T> C = Pp[0] * (T)kp[KB + i * NP];
+      const T k0 = (T)kp[KB + i * NP];
+      T cr = Pp[0].x * k0, ci = Pp[0].y * k0;
       static_for<1, NP>([&](auto jc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
-        C = C + Pp[j] * (T)kp[KB + i * NP + j];
+        const T kj = (T)kp[KB + i * NP + j];
+        cr = fma(Pp[j].x, kj, cr);
+        ci = fma(Pp[j].y, kj, ci);
       });
-      w[F - m1] = w[F - m1] + conjc(psi[F - m2]) * C;
-      if constexpr (m1 != m2) w[F - m2] = w[F - m2] + conjc(psi[F - m1]) * C;
+      // w_{m1} += conj(psi_{m2}) C (and m1 <-> m2)
+      auto acc = [&](int c, cplx<T> a) __attribute__((always_inline)) {
+        T wr = fma(a.x, cr, w[c].x);
+        wr = fma(a.y, ci, wr);
+        T wi = fma(a.x, ci, w[c].y);
+        wi = fma(-a.y, cr, wi);
+        w[c] = mk(wr, wi);
+      };
+      acc(F - m1, psi[F - m2]);
+      if constexpr (m1 != m2) acc(F - m2, psi[F - m1]);
     });
   });
 }

@@ -58,8 +58,9 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
   {
     const cplx<double> twv = twg[min(tid, NX - 1)];
     cplx<T> ld[D];
+    const cplx<T>* pb = psi + ibase;
 #pragma unroll
-    for (int c = 0; c < D; ++c) ld[c] = psi[ibase + (size_t)c * g.nsites + tid];
+    for (int c = 0; c < D; ++c) ld[c] = *lane_elem(pb + (size_t)c * g.nsites, tid);
     cplx<T> ph[2][3];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
@@ -188,9 +189,16 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
     __syncthreads();
     tile_fft(std::false_type{});
   }
-  // without the forward FFT these are the thread's own writes; the FFT ends on a barrier
+  // without the forward FFT these are the thread's own writes; the FFT ends on a barrier.
+  // The addresses are formed again here from an opaque base and lane index (kept from the
+  // prologue, the 13 FP64 addresses would spill across the site program).
+  {
+    cplx<T>* pb = psi + ibase;
+    int lane = tid;
+    asm volatile("" : "+s"(pb), "+v"(lane));
 #pragma unroll
-  for (int c = 0; c < D; ++c) psi[ibase + (size_t)c * g.nsites + tid] = tile[p8(c * TS + tid)];
+    for (int c = 0; c < D; ++c) *lane_elem(pb + (size_t)c * g.nsites, lane) = tile[p8(c * TS + tid)];
+  }
 
   // per-block partial (every thread takes part)
   if (st & XS_NORM) {
