@@ -1809,6 +1809,12 @@ int bec_copy_slot(bec_ws* ws, int dst, int src) {
   CHK(ensure_slot(ws, dst));
   CHK(ensure_slot(ws, src));
   const size_t b = ws->csz * (size_t)ws->g.inst_stride;
+  if (dst == src || n_active(ws) == 0) return BEC_OK;
+  if (b % 16 == 0) {
+    HIPCHK(launch_copy_slot(ws->stream, ws->tb, ws->slot[src], ws->slot[dst], (long long)b,
+                            n_active(ws)));
+    return BEC_OK;
+  }
   for (int inst : ws->active)
     HIPCHK(hipMemcpyAsync((char*)ws->slot[dst] + b * inst, (char*)ws->slot[src] + b * inst, b,
                           hipMemcpyDeviceToDevice, ws->stream));

@@ -245,6 +245,10 @@ hipError_t launch_constrained_norm(hipStream_t st, const Geom& g, const DevTable
                                    int n_active, double* part, int nblk, const double* target,
                                    double* fac);
 
+// dst <- src for every active instance (bytes_per_inst a multiple of 16)
+hipError_t launch_copy_slot(hipStream_t st, const DevTables& tb, const void* src, void* dst,
+                            long long bytes_per_inst, int n_active);
+
 // partials: RS_NORM <- sum|a-b|^2 (or max|a-b| when want_max), RS_KIN <- sum|b|^2 (or max|a|)
 template <typename T>
 hipError_t launch_diff(hipStream_t st, const Geom& g, const DevTables& tb, const void* a,
