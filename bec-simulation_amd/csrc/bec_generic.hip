@@ -42,12 +42,105 @@ static bool has_gen_tw(const DevTables& tb, int N) {
   return false;
 }
 
-// One radix-R Stockham pass over nl lines (line l at src + l * P), into dst.
+// One radix-R Stockham pass over nl lines (line l at src + l * P), into dst: output
+//   out[k] = sum_t in[j + t N/R] * w^(t * (k mod Ns R) * N/(Ns R)),   w = exp(-+2 pi i / N)
+// with j = (k div Ns R) Ns + k mod Ns (Ns = product of the radices before).
+//
+// Radices 2, 3, 4, 5 and 7 run as butterflies: one thread takes the R outputs
+// k = jh Ns R + jl + q Ns (q < R) that share the inputs in[j + t N/R], j = jh Ns + jl; it
+// multiplies input t by w^(t jl N/(Ns R)) (an exponent below N) and takes the R-point DFT of
+// the products in registers (constants cos/sin(2 pi m / R)).  Larger prime radices keep one
+// R-term sum per output.  Sums in FP64 for both storage precisions.
+template <int R>
+struct SmallDft;  // e^(-2 pi i m / R), m < R
+template <> struct SmallDft<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, -0.8660254037844386, 0.8660254037844386};
+};
+template <> struct SmallDft<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494745, -0.8090169943749475,
+                                  -0.8090169943749475, 0.30901699437494745};
+  static constexpr double s[5] = {0.0, -0.9510565162951535, -0.5877852522924731,
+                                  0.5877852522924731, 0.9510565162951535};
+};
+template <> struct SmallDft<7> {
+  static constexpr double c[7] = {1.0, 0.6234898018587336, -0.22252093395631434,
+                                  -0.9009688679024191, -0.9009688679024191,
+                                  -0.22252093395631434, 0.6234898018587336};
+  static constexpr double s[7] = {0.0, -0.7818314824680298, -0.9749279121818236,
+                                  -0.43388373911755823, 0.43388373911755823,
+                                  0.9749279121818236, 0.7818314824680298};
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void gen_butterflies(const cplx<T>* __restrict__ src,
+                                                cplx<T>* __restrict__ dst, int P, int nl, int N,
+                                                int Ns, const cplx<double>* __restrict__ tw,
+                                                bool inv, int tid, int nthr) {
+  const int NB = N / R, step = N / (Ns * R);
+  for (int b = tid; b < nl * NB; b += nthr) {
+    const int l = b / NB, j = b - l * NB;
+    const int jh = j / Ns, jl = j - jh * Ns;
+    const cplx<T>* in = src + (size_t)l * P + j;
+    double vx[R], vy[R];
+    vx[0] = (double)in[0].x;
+    vy[0] = (double)in[0].y;
+    const int e1 = jl * step;
+#pragma unroll
+    for (int t = 1; t < R; ++t) {
+      const cplx<double> w = tw[t * e1];
+      const double wy = inv ? -w.y : w.y;
+      const cplx<T> v = in[t * NB];
+      vx[t] = (double)v.x * w.x - (double)v.y * wy;
+      vy[t] = (double)v.x * wy + (double)v.y * w.x;
+    }
+    cplx<T>* out = dst + (size_t)l * P + jh * Ns * R + jl;
+    if constexpr (R == 2) {
+      out[0] = mk((T)(vx[0] + vx[1]), (T)(vy[0] + vy[1]));
+      out[Ns] = mk((T)(vx[0] - vx[1]), (T)(vy[0] - vy[1]));
+    } else if constexpr (R == 4) {
+      // X_q = (a0 + a2) + (-+i)^q (a1 + a3) ... with -i for the forward transform
+      const double s0x = vx[0] + vx[2], s0y = vy[0] + vy[2];
+      const double d0x = vx[0] - vx[2], d0y = vy[0] - vy[2];
+      const double s1x = vx[1] + vx[3], s1y = vy[1] + vy[3];
+      const double d1x = vx[1] - vx[3], d1y = vy[1] - vy[3];
+      // (-i) d1 forward, (+i) d1 inverse
+      const double rx = inv ? -d1y : d1y, ry = inv ? d1x : -d1x;
+      out[0] = mk((T)(s0x + s1x), (T)(s0y + s1y));
+      out[Ns] = mk((T)(d0x + rx), (T)(d0y + ry));
+      out[2 * Ns] = mk((T)(s0x - s1x), (T)(s0y - s1y));
+      out[3 * Ns] = mk((T)(d0x - rx), (T)(d0y - ry));
+    } else {
+      using W = SmallDft<R>;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        double ax = vx[0], ay = vy[0];
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+          const int m = (t * q) % R;
+          const double c = W::c[m], sn = inv ? -W::s[m] : W::s[m];
+          ax += vx[t] * c - vy[t] * sn;
+          ay += vx[t] * sn + vy[t] * c;
+        }
+        out[q * Ns] = mk((T)ax, (T)ay);
+      }
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void gen_stage(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst,
                                           int P, int nl, int N, int R, int Ns,
                                           const cplx<double>* __restrict__ tw, bool inv, int tid,
                                           int nthr) {
+  switch (R) {
+    case 2: gen_butterflies<T, 2>(src, dst, P, nl, N, Ns, tw, inv, tid, nthr); return;
+    case 3: gen_butterflies<T, 3>(src, dst, P, nl, N, Ns, tw, inv, tid, nthr); return;
+    case 4: gen_butterflies<T, 4>(src, dst, P, nl, N, Ns, tw, inv, tid, nthr); return;
+    case 5: gen_butterflies<T, 5>(src, dst, P, nl, N, Ns, tw, inv, tid, nthr); return;
+    case 7: gen_butterflies<T, 7>(src, dst, P, nl, N, Ns, tw, inv, tid, nthr); return;
+    default: break;
+  }
   const int blk = Ns * R, step = N / blk, stride = N / R;
   for (int o = tid; o < nl * N; o += nthr) {
     const int l = o / N, k = o - l * N;
@@ -74,7 +167,7 @@ __device__ __forceinline__ void gen_stage(const cplx<T>* __restrict__ src, cplx<
 // place, through a scratch region of `cap` lines of the same pitch (groups of cap lines).
 // Every thread of the block calls it; it ends with the block synchronised.
 template <typename T>
-__device__ void gen_fft(cplx<T>* buf, cplx<T>* scr, int nl, int N, int P, int cap,
+__device__ __forceinline__ void gen_fft(cplx<T>* buf, cplx<T>* scr, int nl, int N, int P, int cap,
                         const cplx<double>* tw, bool inv, int tid, int nthr) {
   for (int l0 = 0; l0 < nl; l0 += cap) {
     const int ng = min(cap, nl - l0);

@@ -573,7 +573,7 @@ def main():
             "config": {"workload": "Poisson request stream (becgpu.serve.poisson_mix, seed 42)",
                        "requests": args.requests, "rate_per_s": args.rate,
                        "dims": "1/2/3 w.p. 0.5/0.35/0.15", "F": "1/2/6 w.p. 0.6/0.25/0.15",
-                       "sides": "1-D/2-D {64,80,96,112,128,160,192,224,256}, 3-D {48,64,80,96,112,128}", "c0": "U[5,20]",
+                       "sides": "every even side: 1-D/2-D 64..256, 3-D 48..128", "c0": "U[5,20]",
                        "c1": "U[-1,1]", "n_steps": "U[500,5000]", "itp_tol": 1e-8,
                        "slots_per_pool": 8, "hbm_budget_GiB": 200},
             "latency_p50_s": b["p50_s"], "latency_p99_s": b["p99_s"],
