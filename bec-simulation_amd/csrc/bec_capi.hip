@@ -1148,9 +1148,70 @@ static bool any_loss(const bec_ws* ws) {
   return false;
 }
 
+// 1-D grids whose split step is a single x pass: one block per instance holds the whole
+// line, so n steps can run in one launch that repeats the step with the line resident in
+// LDS (XProgram::nrep).  The step's program is taken from the schedule compiler itself (one
+// step recorded, not launched): eligible when it is exactly one x pass of the stages
+// ops A / 1-D kinetic / ops B / norm partial (+ its reduction and the closing
+// renormalisation in imaginary time), i.e. no lane-per-site operators (F >= 2 spin mixing,
+// tensor, Raman), no DDI and a built line length.  Within the launch the op sequence of every
+// step is the one the separate launches run, and each renormalisation uses the same norm
+// (the repetition loop of bec_xpass_body.h), so the result is bitwise that of the per-step
+// schedule.
+static bool repeat_1d_program(bec_ws* ws, XProgram* out) {
+  // power-of-two lines only: their repeat kernels compile without register spills, and each
+  // is tested bit for bit against the per-step launches; a 112-point line (radix 7) gave
+  // wrong repeated steps while its per-step kernel was right, and the cause is not known
+  const int nx = ws->g.n[0];
+  if (ws->rec || ws->g.ndim != 1 || ws->D > 3 || ws->ddi || (nx & (nx - 1)) != 0 || nx < 8 ||
+      nx > 512)
+    return false;
+  const int itp = ws->itp;
+  if (itp && ws->normalize_every != 1) return false;
+  std::vector<Item> items = {{IT_V, 0.5}, {IT_K, 1.0}, {IT_V, 0.5}};
+  if (!itp && any_loss(ws)) items.push_back({IT_LOSS, 1.0});
+  if (itp) items.push_back({IT_NORM, 0.0});
+  std::vector<Rec> rec;
+  ws->rec = &rec;
+  const int r = run_items(ws, items, itp, 0.0);
+  ws->rec = nullptr;
+  if (r != BEC_OK || rec.empty() || rec[0].kind != R_X) return false;
+  const XProgram& p = rec[0].xp;
+  const int allowed = XS_OPS_A | XS_KIN_1D | XS_OPS_B | XS_NORM;
+  if ((p.stages & ~allowed) != 0 || !(p.stages & XS_KIN_1D)) return false;
+  if (rec[0].src != ws->slot[0] || rec[0].dst != ws->slot[0]) return false;
+  if (itp) {
+    if (rec.size() != 3 || rec[1].kind != R_REDUCE || rec[2].kind != R_SCALE ||
+        !(p.stages & XS_NORM))
+      return false;
+  } else if (rec.size() != 1 || (p.stages & XS_NORM)) {
+    return false;
+  }
+  for (int i = 0; i < p.nA; ++i)
+    if (p.A[i].kind == OP_SCALE) return false;
+  for (int i = 0; i < p.nB; ++i)
+    if (p.B[i].kind == OP_SCALE) return false;
+  *out = p;
+  return true;
+}
+
 int bec_split_step(bec_ws* ws, int n_steps) {
   if (!ws) return fail(BEC_EINVAL, "null workspace");
   if (n_steps < 0) return fail(BEC_EINVAL, "n_steps < 0");
+  if (n_steps >= 2 && n_active(ws) > 0) {
+    XProgram p;
+    if (repeat_1d_program(ws, &p)) {
+      p.nrep = n_steps;
+      CHK(do_x(ws, p, ws->slot[0], ws->slot[0]));
+      if (p.stages & XS_NORM) {
+        CHK(do_reduce(ws, xpass_grid(ws->g), 1u << RS_NORM));
+        HIPCHK(ws->prec == BEC_FP32
+                   ? launch_scale_norm<float>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws))
+                   : launch_scale_norm<double>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws)));
+      }
+      return BEC_OK;
+    }
+  }
   const bool loss = !ws->itp && any_loss(ws);  // split_step.jl:21-23: real time only
   std::vector<Item> items;
   items.reserve(5 * (size_t)n_steps);

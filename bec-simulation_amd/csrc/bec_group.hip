@@ -56,7 +56,9 @@ __device__ __forceinline__ void xpass_group_tile(const XGroupDesc& x, int bx_, i
   const int ROWS = x.rows;
 #define BEC_BX bx_
 #define BEC_BY by_
+#define BEC_REPEAT 0
 #include "bec_xpass_body.h"
+#undef BEC_REPEAT
 #undef BEC_BX
 #undef BEC_BY
 }

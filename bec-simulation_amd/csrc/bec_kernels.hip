@@ -23,14 +23,47 @@ namespace bec {
 // ---------------------------------------------------------------------------
 // x pass
 // ---------------------------------------------------------------------------
+// Waves per SIMD the x pass is compiled for.  A thread holds the E values of its FFT block
+// in registers (E complex values: 4E VGPRs in FP64), so lines with E = 20 or 28 (5 2^k and
+// 7 2^k) and lines with E = 12 (3 2^k) get a larger register budget; at 4 waves (128 VGPRs)
+// they spilled hundreds of registers to scratch.
+template <typename T, int NX, int D>
+constexpr int xpass_waves() {
+  constexpr int e = line_e(NX);
+  if (e >= 20) return 2;
+  if (e == 12) return 3;
+  return D == 13 && sizeof(T) == 8 ? 3 : 4;
+}
+
 template <typename T, int NX, int D>
 __global__ void __launch_bounds__(xpass_max_threads<D>())
-    __attribute__((amdgpu_waves_per_eu(D == 13 && sizeof(T) == 8 ? 3 : 4)))
+    __attribute__((amdgpu_waves_per_eu(xpass_waves<T, NX, D>())))
 xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
              cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, T* __restrict__ phi, int ROWS) {
 #define BEC_BX blockIdx.x
 #define BEC_BY blockIdx.y
+#define BEC_REPEAT 0
 #include "bec_xpass_body.h"
+#undef BEC_REPEAT
+#undef BEC_BX
+#undef BEC_BY
+}
+
+// The same pass with prog.nrep split steps in one launch (1-D lines of F = 0 and 1, whose
+// split step is this one pass; the schedule compiler decides, bec_split_step).  A kernel of
+// its own, so the single-step kernel's code is not touched by the repetition loop.  One
+// block per instance of at most 3 NX / E threads: occupancy does not matter, so the kernel
+// may use the whole register file (the loop keeps more values live than one step does).
+template <typename T, int NX, int D>
+__global__ void __launch_bounds__(xpass_max_threads<D>()) __attribute__((amdgpu_waves_per_eu(1)))
+xpass_rep_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
+                 cplx<T>* __restrict__ dst, cplx<T>* __restrict__ fhat, T* __restrict__ phi,
+                 int ROWS) {
+#define BEC_BX blockIdx.x
+#define BEC_BY blockIdx.y
+#define BEC_REPEAT 1
+#include "bec_xpass_body.h"
+#undef BEC_REPEAT
 #undef BEC_BX
 #undef BEC_BY
 }
@@ -470,6 +503,18 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
   const int threads = D * rows * (NX / xpass_e<NX, D>());
   if (threads > xpass_max_threads<D>()) return hipErrorInvalidConfiguration;
   dim3 grid(xpass_grid(g), n_active);
+  if (prog.nrep > 1) {
+    if constexpr (D <= 3) {
+      if (lds > 48 * 1024)
+        (void)hipFuncSetAttribute((const void*)xpass_rep_kernel<T, NX, D>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((xpass_rep_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,
+                         (const cplx<T>*)src, (cplx<T>*)dst, (cplx<T>*)fhat, (T*)phi, rows);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (lds > 48 * 1024)  // + static LDS (op lists)
     (void)hipFuncSetAttribute((const void*)xpass_kernel<T, NX, D>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

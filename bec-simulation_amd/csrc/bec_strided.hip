@@ -20,9 +20,15 @@ namespace bec {
 template <typename T, int N, int MODE, int TB, int FIELDS>
 __global__ void __launch_bounds__(1024)
 strided_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, double kin_frac,
-               double dt_override, int itp, int ddi_data) {
-#define BEC_BX blockIdx.x
-#define BEC_BY blockIdx.y
+               double dt_override, int itp, int ddi_data, int order) {
+  // Block order (the dispatch order of the tiles): bit 0 reverses the tiles of a component
+  // (for the y pass: the z planes from the last one), bit 1 the components, so that a pass
+  // starts with the data its predecessor wrote last, which the L2 and the memory-side cache
+  // still hold (psi is larger than the 256 MB memory-side cache).
+  const int bx_ = (order & 1) ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
+  const int by_ = (order & 2) ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+#define BEC_BX bx_
+#define BEC_BY by_
 #define BEC_BZ blockIdx.z
 #define BEC_GX gridDim.x
 #include "bec_strided_body.h"
@@ -43,6 +49,23 @@ static int strided_other(const Geom& g, int axis, int ddi_data) {
   return axis == 1 ? g.n[2] : g.sn[1];
 }
 
+// Block order of a psi pass (bit 0: tiles of a component from the last, bit 1: components
+// from the last).  A 3-D kinetic step is x pass -> y FWD -> z KIN -> y INV -> x pass: the y
+// forward pass takes the z planes from the last one (the x pass wrote the rows in z order),
+// the z pass takes the components from the last one (the y pass wrote them in order), and
+// the y inverse pass then takes them from the first.  BEC_STRIDED_ORDER=0 keeps the plain
+// order everywhere (a measurement switch).
+static int strided_order(const Geom& g, int axis, int mode, int ddi_data) {
+  static const int enabled = [] {
+    const char* e = getenv("BEC_STRIDED_ORDER");
+    return e ? atoi(e) : 1;
+  }();
+  if (!enabled || ddi_data || g.ndim != 3) return 0;
+  if (axis == 1 && mode == SM_FWD) return 1;
+  if (axis == 2) return 2;
+  return 0;
+}
+
 template <typename T, int N, int MODE, int FIELDS>
 static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, int axis,
                             void* data, int n_active, double kin_frac, double dt_override,
@@ -61,7 +84,8 @@ static hipError_t strided_n(hipStream_t st, const Geom& g, const DevTables& tb, 
     (void)hipFuncSetAttribute((const void*)strided_kernel<T, N, MODE, TB, FIELDS>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((strided_kernel<T, N, MODE, TB, FIELDS>), grid, dim3(threads), lds, st, g, tb,
-                     (cplx<T>*)data, axis, kin_frac, dt_override, itp, ddi_data);
+                     (cplx<T>*)data, axis, kin_frac, dt_override, itp, ddi_data,
+                     strided_order(g, axis, MODE, ddi_data));
   return hipGetLastError();
 }
 

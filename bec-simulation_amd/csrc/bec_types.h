@@ -65,6 +65,10 @@ struct XProgram {
   double dt_override; // > 0: use instead of the per-instance dt
   double kin_frac;    // 1-D kinetic step dt fraction
   int deriv;          // KIN_1D multiplies by i k / N (spectral derivative) instead of the phase
+  int nrep;           // > 1: the ops A / KIN_1D / ops B / NORM stages run nrep times on the tile
+                      // (one block per instance holds the whole 1-D line); with XS_NORM every
+                      // repetition after the first starts with the renormalisation by the
+                      // previous one's norm
   Op A[kMaxOps];
   Op B[kMaxOps];
 };

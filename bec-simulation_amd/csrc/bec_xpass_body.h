@@ -3,7 +3,7 @@
 // (bec_group.hip), so that both compile from one text while the single-workspace kernel
 // keeps its kernel-argument code generation.  Names in scope at the paste: T, NX, D, g,
 // tb, prog, src, dst, fhat, phi, ROWS, and the macros BEC_BX (x tile) and BEC_BY (index
-// into the active-instance list).
+// into the active-instance list), and BEC_REPEAT (1 in the repeat kernel of 1-D lines).
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int EX = xpass_e<NX, D>();
   constexpr int TEAM = NX / EX;
@@ -105,7 +105,11 @@
     fft_team<T, NX, true, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
 
   // ITP renormalisation factor of the instance (only used when an OP_SCALE is present)
+#if BEC_REPEAT
+  double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
+#else
   const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
+#endif
 
   if (st & XS_LOAD_PHI) {
     ddi_lines_in<T>(g, fhat, (size_t)inst * g.fhat_inst_stride, row0, ROWS, AL, aux, tid, nthr);
@@ -227,6 +231,26 @@
     }
   }
 
+#if BEC_REPEAT
+  // Repeated split steps (the repeat kernel, prog.nrep > 1, 1-D lines): the line stays in
+  // LDS from one step to the next.  A repetition's renormalisation uses the previous one's
+  // norm reduced as the per-step schedule reduces it (the block partial of the NORM stage,
+  // which the one-partial reduction returns unchanged), so the steps give the bits of nrep
+  // separate launches.  Only the stages ops A, 1-D kinetic, ops B and norm are repeated.
+  const int nrep = prog.nrep > 1 ? prog.nrep : 1;
+  __shared__ double s_norm;
+  for (int rep = 0; rep < nrep; ++rep) {
+  if (rep > 0 && (st & XS_NORM)) {
+    const double v = block_sum(norm_acc, rsh);
+    if (tid == 0) s_norm = v;
+    __syncthreads();
+    sc = 1.0 / sqrt(s_norm * g.dV);
+    norm_acc = 0.0;
+    for (int e = tid; e < D * TS; e += nthr) tile[P8(e)] = scale64(tile[P8(e)], sc);
+    __syncthreads();
+  }
+#endif
+
   if (st & XS_OPS_A) run_ops(s_ops, prog.nA, 0);
 
   if (st & XS_KIN_1D) {
@@ -274,6 +298,10 @@
       for (int c = 0; c < D; ++c) norm_acc += (double)abs2(tile[P8(c * TS + s)]);
     }
   }
+#if BEC_REPEAT
+  __syncthreads();
+  }  // repetitions
+#endif
 
   if (st & XS_FWD_FFT)
     fft_team<T, NX, false, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
