@@ -50,20 +50,21 @@ static int strided_other(const Geom& g, int axis, int ddi_data) {
 }
 
 // Block order of a psi pass (bit 0: tiles of a component from the last, bit 1: components
-// from the last).  A 3-D kinetic step is x pass -> y FWD -> z KIN -> y INV -> x pass: the y
-// forward pass takes the z planes from the last one (the x pass wrote the rows in z order),
-// the z pass takes the components from the last one (the y pass wrote them in order), and
-// the y inverse pass then takes them from the first.  BEC_STRIDED_ORDER=0 keeps the plain
-// order everywhere (a measurement switch).
+// from the last).  A 3-D kinetic step is x pass -> y FWD -> z KIN -> y INV -> x pass, and
+// psi (436 MB at 128^3 spin-6 FP64) is larger than the memory-side cache: the y passes
+// visit the components in order, so the z pass takes them from the last one (the data the
+// y pass wrote last), and the y inverse pass, in order again, starts with what the z pass
+// wrote last.  Measured at 128^3 spin-6 FP64: z pass 186 -> 164 us, y inverse 171 -> 135 us;
+// reversing the z planes of the y forward pass as well was 5 % slower.
+// BEC_STRIDED_ORDER=0 keeps the plain order everywhere (a measurement switch).
 static int strided_order(const Geom& g, int axis, int mode, int ddi_data) {
   static const int enabled = [] {
     const char* e = getenv("BEC_STRIDED_ORDER");
     return e ? atoi(e) : 1;
   }();
+  (void)mode;
   if (!enabled || ddi_data || g.ndim != 3) return 0;
-  if (axis == 1 && mode == SM_FWD) return 1;
-  if (axis == 2) return 2;
-  return 0;
+  return axis == 2 ? 2 : 0;
 }
 
 template <typename T, int N, int MODE, int FIELDS>

@@ -312,23 +312,29 @@ def cfg4_jobs(G, n_jobs=256, seed=42, n_steps=10000, sizes=(64, 128, 256)):
     return jobs
 
 
-def converged_error(G, n_jobs=24, n_steps=200000):
+def converged_error(G, n_jobs=24, n_steps=200000, refine_steps=2000):
     """FP32-bulk against FP64 on relaxed ground states: the first n_jobs solves of the same
     seeded set on its 64^2 grid, run for n_steps in both precisions through the batched path
     with the convergence test switched off (tol = 0), so that both reach their fixed points
     instead of stopping at different checks (tol = 1e-8 on |dE| is a relative 1e-6 on these
     energies).  The error is taken over the solves whose FP64 energy changed by less than
-    1e-10 (relative) over the last check interval."""
-    from becgpu.batch import solve_batch
+    1e-10 (relative) over the last check interval, for the FP32-bulk solves and for the same
+    solves finished by refine_steps FP64 steps (batch.solve_batch_refined)."""
+    from becgpu.batch import solve_batch, solve_batch_refined
     jobs = cfg4_jobs(G, n_jobs, n_steps=n_steps, sizes=(64,))
     for j in jobs:
         j.tol = 0.0
     r32, _ = solve_batch(jobs, precision="fp32")
+    r32r, _ = solve_batch_refined(jobs, refine_steps=refine_steps)
     r64, _ = solve_batch(jobs, precision="fp64")
-    relaxed = [(a, b) for a, b in zip(r32, r64) if b["dE"] < 1e-10 * abs(b["energy"])]
-    errs = [abs(a["energy"] - b["energy"]) / abs(b["energy"]) for a, b in relaxed]
+    relaxed = [k for k, b in enumerate(r64) if b["dE"] < 1e-10 * abs(b["energy"])]
+    rel = lambda a, b: abs(a["energy"] - b["energy"]) / abs(b["energy"])
+    errs = [rel(r32[k], r64[k]) for k in relaxed]
+    errs_r = [rel(r32r[k], r64[k]) for k in relaxed]
     return dict(max_rel_energy_err=max(errs) if errs else None,
                 median_rel_energy_err=float(np.median(errs)) if errs else None,
+                max_rel_energy_err_refined=max(errs_r) if errs_r else None,
+                refine_steps_fp64=refine_steps,
                 n_relaxed=len(relaxed), n_jobs=n_jobs, grid=[64, 64], n_steps=n_steps,
                 fp64_last_rel_dE_max=max(b["dE"] / abs(b["energy"]) for b in r64))
 
@@ -338,7 +344,9 @@ def run_batch2d(G, A, n_jobs, n_plain, graphs=False, grouped=False):
     FP64 path (find_ground_state one job at a time) on an evenly spaced subsample."""
     from becgpu.batch import solve_batch
     jobs = cfg4_jobs(G, n_jobs)
+    from becgpu.batch import solve_batch_refined
     r32, s32 = solve_batch(jobs, precision="fp32", graphs=graphs, grouped=grouped)
+    _, s32r = solve_batch_refined(jobs, refine_steps=2000, graphs=graphs, grouped=grouped)
     r64, s64 = solve_batch(jobs, precision="fp64", graphs=graphs, grouped=grouped)
     sub = jobs[::max(1, n_jobs // n_plain)][:n_plain]
     t0 = time.perf_counter()
@@ -353,7 +361,8 @@ def run_batch2d(G, A, n_jobs, n_plain, graphs=False, grouped=False):
     err = max(conv) if conv else None   # null in the JSON line when no pair converged
     lat = np.array([r["time_s"] for r in r32])
     conv_err = converged_error(G)
-    return dict(fp32=s32, fp64=s64, plain_fp64_solves_per_s=plain, plain_sample=len(sub),
+    return dict(fp32=s32, fp32_refined=s32r, fp64=s64, plain_fp64_solves_per_s=plain,
+                plain_sample=len(sub),
                 converged_error=conv_err,
                 max_rel_energy_err_fp32=err, max_rel_energy_err_all=max(rel),
                 n_both_converged=len(conv), converged_fp32=sum(r["converged"] for r in r32),
@@ -606,6 +615,11 @@ def main():
                        "grouped": bool(args.grouped)},
             "grouped_intervals": b["fp32"].get("grouped_intervals", 0),
             "fp64_batched_solves_per_s": b["fp64"]["solves_per_s"],
+            "fp32_refined_solves_per_s": b["fp32_refined"]["solves_per_s"],
+            "fp32_refined_note": "FP32-bulk solves finished by 2000 FP64 steps "
+                                 "(batch.solve_batch_refined); error in max_rel_energy_err_note",
+            "max_rel_energy_err_fp32_refined_vs_fp64":
+                b["converged_error"]["max_rel_energy_err_refined"],
             "plain_fp64_solves_per_s": b["plain_fp64_solves_per_s"],
             "speedup_vs_plain_fp64": b["fp32"]["solves_per_s"] / b["plain_fp64_solves_per_s"],
             "max_rel_energy_err_fp32_vs_fp64": b["converged_error"]["max_rel_energy_err"],

@@ -9,7 +9,7 @@ import pytest
 from oracle import spinor_oracle as O
 import becgpu as G
 from becgpu import api as A
-from becgpu.batch import ITPJob, multistart_batched, solve_batch
+from becgpu.batch import ITPJob, multistart_batched, solve_batch, solve_batch_refined
 from helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -88,3 +88,31 @@ def test_continuation_chains_batched_equal_sequential():
             assert g["energy"] == r["energy"]
             assert np.array_equal(g["psi"], r["psi"])
     assert any(r["fallback"] for r in got[2])
+
+
+def test_refined_fp32_batch_reaches_fp64_fixed_point():
+    """batch.solve_batch_refined: an untrapped 2-D box (uniform ground state) relaxed in
+    FP32-bulk keeps its slowest mode frozen (the per-step decay rounds away); the FP64 steps
+    that finish the solve must bring the energy to the FP64 fixed point's within 1e-7
+    (relative), and each refined result must equal an FP64 batch started from the FP32
+    state."""
+    grid = G.make_grid(G.GridConfig((32, 32), (20.0, 20.0)))
+    jobs = []
+    for k, kind in enumerate(("polar", "ferromagnetic")):
+        psi0 = G.init_psi(grid, G.SpinSystem(1), kind, seed=7 + k)
+        jobs.append(ITPJob(grid=grid, F=1, interactions=G.InteractionParams(10.0, -0.5),
+                           psi0=psi0, dt=0.005, n_steps=40000, tol=0.0))
+    r32, _ = solve_batch(jobs, precision="fp32", keep_psi=True)
+    rr, stats = solve_batch_refined(jobs, refine_steps=3000, keep_psi=True)
+    r64, _ = solve_batch(jobs, precision="fp64")
+    for a, b, c in zip(r32, rr, r64):
+        print("fp32", abs(a["energy"] - c["energy"]) / abs(c["energy"]),
+              "refined", abs(b["energy"] - c["energy"]) / abs(c["energy"]))
+        assert abs(b["energy"] - c["energy"]) <= 1e-7 * abs(c["energy"])
+        assert b["steps_fp32"] == a["steps"] and b["energy_fp32"] == a["energy"]
+    again = [ITPJob(grid=j.grid, F=1, interactions=j.interactions, psi0=a["psi"], dt=j.dt,
+                    n_steps=3000, tol=0.0) for j, a in zip(jobs, r32)]
+    r2, _ = solve_batch(again, precision="fp64", keep_psi=True)
+    for b, c in zip(rr, r2):
+        assert b["energy"] == c["energy"]
+        assert np.array_equal(b["psi"], c["psi"])
