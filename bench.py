@@ -90,8 +90,9 @@ def _kernel_class(name):
 
 def pmc_summary(fetch_csv, write_csv, copy_bytes, workload, out_path, precision):
     """HBM bytes per launch by kernel class from two rocprofv3 --pmc runs (FETCH_SIZE, then
-    WRITE_SIZE; KiB per dispatch), each scaled so that the largest __amd_rocclr_copyBuffer
-    dispatch, a copy of `copy_bytes`, reads and writes exactly that many bytes."""
+    WRITE_SIZE; KiB per dispatch), each scaled so that the largest psi copy of the run (the
+    slot copy kernel or the runtime's copyBuffer), a copy of `copy_bytes`, reads and writes
+    exactly that many bytes."""
     import csv
     import collections
 
@@ -104,8 +105,9 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes, workload, out_path, precision)
         return d
 
     fe, wr = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
-    cf = max(max(v) for k, v in fe.items() if "copyBuffer" in k)
-    cw = max(max(v) for k, v in wr.items() if "copyBuffer" in k)
+    is_copy = lambda k: "copyBuffer" in k or "copy_slot_kernel" in k
+    cf = max(max(v) for k, v in fe.items() if is_copy(k))
+    cw = max(max(v) for k, v in wr.items() if is_copy(k))
     sf, sw = copy_bytes / 1024 / cf, copy_bytes / 1024 / cw
     cls = collections.defaultdict(lambda: [0, 0.0, 0.0])
     for k, v in fe.items():
