@@ -64,6 +64,7 @@ SIGNATURES = {
     "bec_normalize": ([C.c_void_p], C.c_int),
     "bec_energy": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_copy_slot": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "bec_swap_slots": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "bec_diff": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p], C.c_int),
     "bec_itp_guard": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_spin_density": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),

@@ -253,6 +253,10 @@ class Workspace:
     def copy_slot(self, dst, src):
         L.check(L.lib().bec_copy_slot(self.handle, dst, src), "bec_copy_slot")
 
+    def swap_slots(self, a, b):
+        """Exchange two slots for every instance (bec_swap_slots: no data moves)."""
+        L.check(L.lib().bec_swap_slots(self.handle, a, b), "bec_swap_slots")
+
     def itp_chunk(self, n_steps):
         """bec_itp_chunk: n_steps ITP split steps, then per instance
         [energy, dpsi vs slot 1, norm, max |Phi|, status] and slot 1 <- psi."""
@@ -583,7 +587,10 @@ def run_simulation_yoshida(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end:
         clamped = dt_step < dt * 0.99
         ws.copy_slot(2, 0)                 # psi_old
         strang_core(ws, dt_step)
-        ws.copy_slot(3, 0)                 # psi_strang
+        if len(ws.active) == ws.B:         # psi_strang: slot 0's result moves to slot 3
+            ws.swap_slots(3, 0)
+        else:                              # (inactive instances keep their slots in place)
+            ws.copy_slot(3, 0)
         ws.copy_slot(0, 2)
         yoshida_core(ws, dt_step)
         d = ws.diff(0, 3, 0)[0]

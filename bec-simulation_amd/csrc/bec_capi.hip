@@ -1882,6 +1882,16 @@ int bec_copy_slot(bec_ws* ws, int dst, int src) {
   return BEC_OK;
 }
 
+int bec_swap_slots(bec_ws* ws, int a, int b) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  CHK(ensure_slot(ws, a));
+  CHK(ensure_slot(ws, b));
+  if (a == b) return BEC_OK;
+  std::swap(ws->slot[a], ws->slot[b]);
+  drop_graphs(ws);  // captured launches hold the slot addresses
+  return BEC_OK;
+}
+
 int bec_diff(bec_ws* ws, int a, int b, int mode, double* out) {
   if (!ws || !out) return fail(BEC_EINVAL, "null argument");
   CHK(ensure_slot(ws, a));

@@ -750,23 +750,26 @@ hipError_t launch_widen(hipStream_t st, const void* src, void* dst, long long n)
   return hipGetLastError();
 }
 
-// Slot copy of the active instances (bec_copy_slot): 16-byte vectors, four loads per thread
-// in flight before their stores (the runtime's copy kernel runs a 436 MB psi copy at about
-// half of HBM bandwidth).  Grid: (blocks, active instances); n16 vectors per instance.
+// Slot copy of the active instances (bec_copy_slot): 16-byte vectors, eight loads per thread
+// in flight before their stores, non-temporal (the copy is read and written once; keeping it
+// out of the caches leaves them to the passes around it).  Grid: (blocks, active
+// instances); n16 vectors per instance.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256)
-copy_slot_kernel(DevTables tb, const uint4* __restrict__ src, uint4* __restrict__ dst,
+copy_slot_kernel(DevTables tb, const u32x4* __restrict__ src, u32x4* __restrict__ dst,
                  long long n16) {
+  constexpr int U = 8;
   const int inst = tb.active[blockIdx.y];
-  const uint4* s = src + (size_t)inst * n16;
-  uint4* d = dst + (size_t)inst * n16;
+  const u32x4* s = src + (size_t)inst * n16;
+  u32x4* d = dst + (size_t)inst * n16;
   const long long stride = (long long)gridDim.x * blockDim.x;
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    const uint4 a = s[i], b = s[i + stride], c = s[i + 2 * stride], e = s[i + 3 * stride];
-    d[i] = a;
-    d[i + stride] = b;
-    d[i + 2 * stride] = c;
-    d[i + 3 * stride] = e;
+  for (; i + (U - 1) * stride < n16; i += U * stride) {
+    u32x4 v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = __builtin_nontemporal_load(&s[i + k * stride]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) __builtin_nontemporal_store(v[k], &d[i + k * stride]);
   }
   for (; i < n16; i += stride) d[i] = s[i];
 }
@@ -775,11 +778,11 @@ hipError_t launch_copy_slot(hipStream_t st, const DevTables& tb, const void* src
                             long long bytes_per_inst, int n_active) {
   if (bytes_per_inst % 16 != 0) return hipErrorInvalidValue;
   const long long n16 = bytes_per_inst / 16;
-  long long nb = (n16 + 1023) / 1024;
-  if (nb > 4096) nb = 4096;
+  long long nb = (n16 + 2047) / 2048;
+  if (nb > 2048) nb = 2048;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(copy_slot_kernel, dim3((unsigned)nb, n_active), dim3(256), 0, st, tb,
-                     (const uint4*)src, (uint4*)dst, n16);
+                     (const u32x4*)src, (u32x4*)dst, n16);
   return hipGetLastError();
 }
 

@@ -148,6 +148,11 @@ int bec_energy(bec_ws* ws, double* out);
 /* Copies between psi slots: slot 0 is psi, slots 1..3 are device-resident
  * copies (psi_prev / psi_old / psi_strang of the drivers). */
 int bec_copy_slot(bec_ws* ws, int dst_slot, int src_slot);
+/* Exchanges the contents of two slots for EVERY instance of the workspace (a pointer swap:
+ * no data moves), where the drivers would copy psi out of a slot they overwrite next
+ * (the psi_strang copy of run_simulation_yoshida!, yoshida.jl:36-44).  Callers with inactive
+ * instances whose slots must stay in place use bec_copy_slot instead. */
+int bec_swap_slots(bec_ws* ws, int slot_a, int slot_b);
 /* mode 0: out[B][2] = {sum |a-b|^2, sum |b|^2}  (_wavefunction_l2_change, adaptive.jl:42-50)
  * mode 1: out[B][2] = {max |a-b|, max |a|}      (dpsi test, ground_state.jl:148-149) */
 int bec_diff(bec_ws* ws, int slot_a, int slot_b, int mode, double* out);
