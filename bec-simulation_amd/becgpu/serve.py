@@ -43,8 +43,7 @@ from .physics import (GridConfig, HarmonicTrap, InteractionParams, SimParams, Sp
 class Request:
     rid: int
     kind: str                 # "itp" or "rt"
-    n: tuple                  # grid points per axis (2^k 8..512, 3*2^k 24..192, 5*2^k 40..160,
-                              # 7*2^k 56..224)
+    n: tuple                  # grid points per axis (any even side)
     L: tuple                  # box length per axis
     F: int
     c0: float
@@ -123,7 +122,8 @@ class Server:
     powers of two so that each pool replays a few cached graphs."""
 
     def __init__(self, *, hbm_budget_bytes=200 << 30, slots_per_pool=8, chunk_ms=20.0,
-                 max_chunk=512, keep_psi=False, graphs=False):
+                 max_chunk=512, keep_psi=False, graphs=False, max_in_flight=8):
+        self.max_in_flight = int(max_in_flight)
         self.budget = int(hbm_budget_bytes)
         self.slots_per_pool = int(slots_per_pool)
         self.chunk_ms = float(chunk_ms)
@@ -201,6 +201,15 @@ class Server:
         p.ws.set_active(p.active)
 
     # -- stepping ------------------------------------------------------------------
+    @staticmethod
+    def _remaining_ms(p: _Pool):
+        """Work left in the pool for the scheduling order: its longest slot's remaining steps
+        times a static per-step cost (a launch overhead plus a term in the sites and
+        components; the measured pace is not used, because a pool sharing the device with
+        large ones measures their time as its own)."""
+        steps = max(st["req"].n_steps - st["step"] for st in p.slot.values())
+        return steps * (0.02 + 2e-6 * int(np.prod(p.grid.shape)) * (2 * p.F + 1))
+
     def _issue(self, p: _Pool):
         """Queue one chunk on the pool's stream: up to the nearest event of its slots, and
         about chunk_ms of the pool's measured pace (a few steps while it is unmeasured)."""
@@ -306,10 +315,19 @@ class Server:
                     self._admit(p, r, time.perf_counter() - t0, results)
                     progressed = True
             queue = waiting
-            for p in list(self.pools.values()):
-                if p.slot and not p.in_flight:
-                    self._issue(p)
-                    progressed = True
+            # At most max_in_flight pools have a chunk on the device (the process has a few
+            # hardware queues; more streams only interleave their chunks), chosen shortest
+            # remaining work first, so that small requests are not queued behind the chunks
+            # of large ones.  Results do not depend on this order (chunking does not change
+            # an instance's arithmetic).
+            busy = sum(1 for p in self.pools.values() if p.in_flight)
+            ready = [p for p in self.pools.values() if p.slot and not p.in_flight]
+            for p in sorted(ready, key=self._remaining_ms):
+                if busy >= self.max_in_flight:
+                    break
+                self._issue(p)
+                busy += 1
+                progressed = True
             if not any(p.in_flight for p in self.pools.values()):
                 if queue:
                     raise RuntimeError("requests wait for memory that no pool will release")

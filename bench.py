@@ -586,7 +586,8 @@ def main():
                        "dims": "1/2/3 w.p. 0.5/0.35/0.15", "F": "1/2/6 w.p. 0.6/0.25/0.15",
                        "sides": "every even side: 1-D/2-D 64..256, 3-D 48..128", "c0": "U[5,20]",
                        "c1": "U[-1,1]", "n_steps": "U[500,5000]", "itp_tol": 1e-8,
-                       "slots_per_pool": 8, "hbm_budget_GiB": 200},
+                       "slots_per_pool": 8, "hbm_budget_GiB": 200,
+                       "scheduling": "8 pools in flight, shortest remaining work first"},
             "latency_p50_s": b["p50_s"], "latency_p99_s": b["p99_s"],
             "admission_wait_p50_s": b["wait_p50_s"], "admission_wait_p99_s": b["wait_p99_s"],
             "latency_by_dim": b["by_dim"], "arrivals_span_s": b["arrivals_span_s"],
