@@ -77,6 +77,7 @@ SIGNATURES = {
     "bec_loss_step": ([C.c_void_p, C.c_double], C.c_int),
     "bec_normalize_constrained": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_synchronize": ([C.c_void_p], C.c_int),
+    "bec_line_plan": ([C.c_int], C.c_int),
     "bec_set_xsite": ([C.c_void_p, C.c_int], C.c_int),
     "bec_query": ([C.c_void_p], C.c_int),
     "bec_autotune": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),

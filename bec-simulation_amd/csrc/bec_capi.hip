@@ -88,6 +88,8 @@ struct bec_ws {
   void* d_tw[kNumFftSizes] = {};
   void* d_gtw[kGenTabs] = {};  // runtime-length twiddle tables (sides without a built FFT)
   int gtw_n[kGenTabs] = {};
+  void* d_gbl[kGenTabs] = {};  // Bluestein tables of the same lengths (or null)
+  int gbl_m[kGenTabs] = {};
   double* h_red = nullptr;   // pinned
   double* h_red2 = nullptr;  // pinned, second staging buffer of bec_itp_chunk (first use)
   int max_blocks = 0;
@@ -171,6 +173,8 @@ static void refresh_tables(bec_ws* ws) {
   for (int i = 0; i < kGenTabs; ++i) {
     t.gtw[i] = ws->d_gtw[i];
     t.gtw_n[i] = ws->gtw_n[i];
+    t.gbl[i] = ws->d_gbl[i];
+    t.gbl_m[i] = ws->gbl_m[i];
   }
   t.red = ws->d_red;
   t.partials = ws->d_partials;
@@ -805,6 +809,69 @@ long long bec_workspace_bytes(const bec_config* cfg) {
   return ws_bytes(cfg, nullptr, nullptr);
 }
 
+// Bluestein tables of a runtime line length N with M-point convolutions (layout as the
+// device reads them: chirp, chirp transform / M, M-point twiddles; interleaved re, im):
+// c[n] = exp(i pi n^2 / N) with n^2 reduced mod 2N (exact phase), the wrapped chirp
+// (c[m] at m and M - m, zero between) transformed by a radix-2 FFT in long double.
+static std::vector<double> bluestein_tables(int N, int M) {
+  const long double pi = 3.14159265358979323846264338327950288L;
+  std::vector<long double> cr(N), ci(N);
+  for (int n = 0; n < N; ++n) {
+    const long long q = ((long long)n * n) % (2LL * N);
+    cr[n] = cosl(pi * q / N);
+    ci[n] = sinl(pi * q / N);
+  }
+  std::vector<long double> wr(M, 0.0L), wi(M, 0.0L);
+  for (int n = 0; n < N; ++n) {
+    wr[n] = cr[n];
+    wi[n] = ci[n];
+    if (n > 0) {
+      wr[M - n] = cr[n];
+      wi[M - n] = ci[n];
+    }
+  }
+  // in-place iterative radix-2 FFT (forward, exp(-2 pi i jk / M))
+  for (int i = 1, j = 0; i < M; ++i) {
+    int bit = M >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(wr[i], wr[j]);
+      std::swap(wi[i], wi[j]);
+    }
+  }
+  for (int len = 2; len <= M; len <<= 1) {
+    for (int i = 0; i < M; i += len) {
+      for (int k = 0; k < len / 2; ++k) {
+        const long double ang = -2.0L * pi * k / len;
+        const long double tr = cosl(ang), ti = sinl(ang);
+        const long double xr = wr[i + k + len / 2] * tr - wi[i + k + len / 2] * ti;
+        const long double xi = wr[i + k + len / 2] * ti + wi[i + k + len / 2] * tr;
+        wr[i + k + len / 2] = wr[i + k] - xr;
+        wi[i + k + len / 2] = wi[i + k] - xi;
+        wr[i + k] += xr;
+        wi[i + k] += xi;
+      }
+    }
+  }
+  std::vector<double> out(2 * ((size_t)N + 2 * (size_t)M));
+  size_t o = 0;
+  for (int n = 0; n < N; ++n) {
+    out[o++] = (double)cr[n];
+    out[o++] = (double)ci[n];
+  }
+  for (int k = 0; k < M; ++k) {
+    out[o++] = (double)(wr[k] / M);
+    out[o++] = (double)(wi[k] / M);
+  }
+  for (int m = 0; m < M; ++m) {
+    const long double ang = -2.0L * pi * m / M;
+    out[o++] = (double)cosl(ang);
+    out[o++] = (double)sinl(ang);
+  }
+  return out;
+}
+
 int bec_ws_create(const bec_config* c, bec_ws** out) {
   if (!c || !out) return fail(BEC_EINVAL, "null argument");
   *out = nullptr;
@@ -983,6 +1050,15 @@ int bec_ws_create(const bec_config* c, bec_ws** out) {
       if (hipMemcpy(ws->d_gtw[l], buf.data(), buf.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(BEC_EDEVICE, "twiddle copy");
       ws->gtw_n[l] = N;
+      const int M = gen_blue_m(N);
+      if (M > 0) {
+        const std::vector<double> bt = bluestein_tables(N, M);
+        if (hipMalloc(&ws->d_gbl[l], bt.size() * 8) != hipSuccess)
+          return cleanup(BEC_ENOMEM, "Bluestein tables");
+        if (hipMemcpy(ws->d_gbl[l], bt.data(), bt.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+          return cleanup(BEC_EDEVICE, "Bluestein table copy");
+        ws->gbl_m[l] = M;
+      }
     }
   }
   refresh_tables(ws);
@@ -1010,8 +1086,10 @@ void bec_ws_destroy(bec_ws* ws) {
     if (q) (void)hipFree(q);
   for (int i = 0; i < kNumFftSizes; ++i)
     if (ws->d_tw[i]) (void)hipFree(ws->d_tw[i]);
-  for (int i = 0; i < kGenTabs; ++i)
+  for (int i = 0; i < kGenTabs; ++i) {
     if (ws->d_gtw[i]) (void)hipFree(ws->d_gtw[i]);
+    if (ws->d_gbl[i]) (void)hipFree(ws->d_gbl[i]);
+  }
   if (ws->h_red) (void)hipHostFree(ws->h_red);
   if (ws->h_red2) (void)hipHostFree(ws->h_red2);
   for (hipEvent_t e : ws->ev_pool) (void)hipEventDestroy(e);
@@ -1706,6 +1784,12 @@ int bec_autotune(bec_ws* ws, int* rows_out) {
   ws->g.xrows = best;
   if (rows_out) *rows_out = xpass_rows_g(ws->g);
   return BEC_OK;
+}
+
+int bec_line_plan(int n) {
+  if (n < 2) return fail(BEC_EINVAL, "line length below 2");
+  if (fft_size_id(n) >= 0) return 0;
+  return gen_blue_m(n) > 0 ? 2 : 1;
 }
 
 int bec_query(bec_ws* ws) {

@@ -42,6 +42,27 @@ static bool has_gen_tw(const DevTables& tb, int N) {
   return false;
 }
 
+// Bluestein tables of a line length (M = 0: the length has none and runs mixed-radix passes).
+struct GenBlue {
+  int M;
+  const cplx<double>* c;   // chirp exp(i pi n^2 / N), n < N
+  const cplx<double>* ch;  // M-point transform of the wrapped chirp, divided by M
+  const cplx<double>* tm;  // exp(-2 pi i m / M), m < M
+};
+__device__ __forceinline__ GenBlue gen_blue(const DevTables& tb, int N) {
+  GenBlue b{0, nullptr, nullptr, nullptr};
+#pragma unroll
+  for (int i = 0; i < kGenTabs; ++i) {
+    if (tb.gtw_n[i] == N && tb.gbl_m[i] > 0) {
+      b.M = tb.gbl_m[i];
+      b.c = reinterpret_cast<const cplx<double>*>(tb.gbl[i]);
+      b.ch = b.c + N;
+      b.tm = b.ch + b.M;
+    }
+  }
+  return b;
+}
+
 // One radix-R Stockham pass over nl lines (line l at src + l * P), into dst: output
 //   out[k] = sum_t in[j + t N/R] * w^(t * (k mod Ns R) * N/(Ns R)),   w = exp(-+2 pi i / N)
 // with j = (k div Ns R) Ns + k mod Ns (Ns = product of the radices before).
@@ -163,12 +184,85 @@ __device__ __forceinline__ void gen_stage(const cplx<T>* __restrict__ src, cplx<
   }
 }
 
+// Bluestein's algorithm for lengths with a large prime factor, whose radix-p pass would cost
+// N p complex multiply-adds per line: with nk = (n^2 + k^2 - (k - n)^2) / 2 and
+// c[n] = exp(i pi n^2 / N), the forward DFT is X[k] = conj(c[k]) sum_n x[n] conj(c[n]) c[k - n],
+// a cyclic convolution with the chirp once both are zero-padded to M >= 2N - 1 points
+// (M a power of two): transform, multiply by the chirp's transform, transform back.  The
+// inverse DFT is the same with c and conj(c) exchanged (the wrapped chirp is symmetric, so
+// its transform is conjugated).  Lines in groups that fit the scratch region (two M-point
+// work lines each); every product in FP64.
+template <typename T>
+__device__ __forceinline__ void gen_fft_blue(cplx<T>* buf, cplx<T>* scr, int nl, int N, int P,
+                                             int cap_elems, const GenBlue& bl, bool inv, int tid,
+                                             int nthr) {
+  // work lines in FP64 for both storage precisions (the M-point values grow to sqrt(M)
+  // times the input, which FP32 storage between passes would round at that scale), from
+  // the first 16-byte boundary of the scratch region
+  const int M = bl.M, msh = __ffs(M) - 1;
+  cplx<double>* wk = reinterpret_cast<cplx<double>*>(
+      (reinterpret_cast<uintptr_t>(scr) + 15) & ~static_cast<uintptr_t>(15));
+  const long long bytes = (long long)cap_elems * (long long)sizeof(cplx<T>) - 16;
+  const int gcap = max(1, (int)(bytes / (2LL * M * (long long)sizeof(cplx<double>))));
+  auto cmul = [](cplx<double> x, cplx<double> w) __attribute__((always_inline)) {
+    return mk(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+  };
+  for (int l0 = 0; l0 < nl; l0 += gcap) {
+    const int ng = min(gcap, nl - l0);
+    cplx<double>* a = wk;
+    cplx<double>* b = wk + (size_t)ng * M;
+    for (int e = tid; e < ng * M; e += nthr) {
+      const int l = e >> msh, n = e & (M - 1);
+      cplx<double> v = mk(0.0, 0.0);
+      if (n < N) {
+        const cplx<double> c = bl.c[n];
+        const cplx<T> x = buf[(size_t)(l0 + l) * P + n];
+        v = cmul(mk((double)x.x, (double)x.y), mk(c.x, inv ? c.y : -c.y));
+      }
+      a[e] = v;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      int rem = M, Ns = 1;
+      while (rem > 1) {
+        const int R = (rem & 3) == 0 ? 4 : 2;
+        gen_stage<double>(a, b, M, ng, M, R, Ns, bl.tm, pass == 1, tid, nthr);
+        __syncthreads();
+        cplx<double>* t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+        rem /= R;
+      }
+      if (pass == 0) {
+        for (int e = tid; e < ng * M; e += nthr) {
+          const cplx<double> h = bl.ch[e & (M - 1)];
+          a[e] = cmul(a[e], mk(h.x, inv ? -h.y : h.y));
+        }
+        __syncthreads();
+      }
+    }
+    for (int e = tid; e < ng * N; e += nthr) {
+      const int l = e / N, k = e - l * N;
+      const cplx<double> c = bl.c[k];
+      const cplx<double> y = cmul(a[((size_t)l << msh) + k], mk(c.x, inv ? c.y : -c.y));
+      buf[(size_t)(l0 + l) * P + k] = mk((T)y.x, (T)y.y);
+    }
+    __syncthreads();
+  }
+}
+
 // Unnormalised FFT (forward: exp(-2 pi i jk/N)) of nl lines of length N at buf + l * P, in
-// place, through a scratch region of `cap` lines of the same pitch (groups of cap lines).
+// place, through a scratch region of `cap` lines of the same pitch (groups of cap lines);
+// lengths with Bluestein tables (bl.M > 0) go through gen_fft_blue.
 // Every thread of the block calls it; it ends with the block synchronised.
 template <typename T>
 __device__ __forceinline__ void gen_fft(cplx<T>* buf, cplx<T>* scr, int nl, int N, int P, int cap,
-                        const cplx<double>* tw, bool inv, int tid, int nthr) {
+                        const cplx<double>* tw, bool inv, int tid, int nthr, const GenBlue& bl) {
+  if (bl.M > 0) {
+    gen_fft_blue<T>(buf, scr, nl, N, P, cap * P, bl, inv, tid, nthr);
+    return;
+  }
   for (int l0 = 0; l0 < nl; l0 += cap) {
     const int ng = min(cap, nl - l0);
     cplx<T>* a = buf + (size_t)l0 * P;
@@ -229,6 +323,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
   cplx<T>* aux = tile + (size_t)NL * LP;
   cplx<T>* scr = aux + (has_aux ? (size_t)2 * ROWS * ALP : 0);
   const int capx = max(1, cap_elems / LP), capa = max(1, cap_elems / ALP);
+  const GenBlue blx = gen_blue(tb, NX), bla = gen_blue(tb, AL);
 
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
@@ -269,7 +364,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
   __syncthreads();
   double norm_acc = 0.0, ke_acc = 0.0, phimax = 0.0;
 
-  if (st & XS_INV_FFT) gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, true, tid, nthr);
+  if (st & XS_INV_FFT) gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, true, tid, nthr, blx);
 
   const double sc = 1.0 / sqrt(tb.red[(size_t)inst * RS_COUNT + RS_NORM] * g.dV);
 
@@ -291,7 +386,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
       aux[(size_t)(2 * r + 1) * ALP + x] = yz;
     }
     __syncthreads();
-    gen_fft<T>(aux, scr, 2 * ROWS, AL, ALP, capa, twa, true, tid, nthr);
+    gen_fft<T>(aux, scr, 2 * ROWS, AL, ALP, capa, twa, true, tid, nthr, bla);
     if (!g.spad) {  // max |Phi| (ground_state.jl:4-29), unpadded convolutions only
       for (int s = tid; s < TS; s += nthr) {
         const int r = s / NX, x = s - r * NX;
@@ -349,7 +444,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
   if (st & XS_OPS_A) run_ops(s_ops, prog.nA, 0);
 
   if (st & XS_KIN_1D) {
-    gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, false, tid, nthr);
+    gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, false, tid, nthr, blx);
     const double kdt = prog.kin_frac * dt;
     for (int e = tid; e < NL * NX; e += nthr) {
       const int l = e / NX, x = e - l * NX;
@@ -366,7 +461,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
       v = mul64(v, ph);
     }
     __syncthreads();
-    gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, true, tid, nthr);
+    gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, true, tid, nthr, blx);
   }
 
   if (st & XS_OPS_B) run_ops(s_ops + kMaxOps, prog.nB, kMaxOps);
@@ -390,7 +485,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
       }
     }
     __syncthreads();
-    gen_fft<T>(aux, scr, 2 * ROWS, AL, ALP, capa, twa, false, tid, nthr);
+    gen_fft<T>(aux, scr, 2 * ROWS, AL, ALP, capa, twa, false, tid, nthr, bla);
     // R2C output: line 2r holds F_x + i F_y (split two-for-one), line 2r+1 holds F_z
     const size_t fb = (size_t)inst * g.fhat_inst_stride;
     const int NH = AL / 2 + 1;
@@ -414,7 +509,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
     }
   }
 
-  if (st & XS_FWD_FFT) gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, false, tid, nthr);
+  if (st & XS_FWD_FFT) gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, false, tid, nthr, blx);
 
   if (st & XS_KE_1D) {
     for (int e = tid; e < NL * NX; e += nthr) {
@@ -465,6 +560,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(rsh + 16);
   cplx<T>* scr = tile + (size_t)FIELDS * TB * LP;
   auto at = [&](int f, int xi, int e) -> cplx<T>& { return tile[(size_t)(f * TB + xi) * LP + e]; };
+  const GenBlue bl = gen_blue(tb, N);
 
   const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
   const int valid = ddi_data ? g.nxh : g.n[0];
@@ -520,7 +616,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
   }
   __syncthreads();
 
-  if (MODE != SM_INV) gen_fft<T>(tile, scr, FIELDS * TB, N, LP, cap, tw, false, tid, nthr);
+  if (MODE != SM_INV) gen_fft<T>(tile, scr, FIELDS * TB, N, LP, cap, tw, false, tid, nthr, bl);
 
   if (MODE == SM_KIN) {
     for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
@@ -586,7 +682,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
   }
 
   if (MODE == SM_INV || MODE == SM_KIN || MODE == SM_DDI || MODE == SM_DERIV)
-    gen_fft<T>(tile, scr, FIELDS * TB, N, LP, cap, tw, true, tid, nthr);
+    gen_fft<T>(tile, scr, FIELDS * TB, N, LP, cap, tw, true, tid, nthr, bl);
 
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
     const int xi2 = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
@@ -601,17 +697,59 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
 static constexpr size_t kGenLdsSoft = 64 * 1024;   // shapes are shrunk towards this
 static constexpr size_t kGenLdsHard = 160 * 1024;  // and must fit this
 
+// Bluestein or not: the mixed-radix passes cost about N (p - 1) complex multiply-adds per
+// line for each odd prime factor p > 7 (one p-term sum per output), Bluestein about
+// M (1.5 log2 M + 3) (two M-point radix-4 transforms and three pointwise products) in that
+// unit, but its FP64 work lines take LDS from the line tiles and its passes synchronise the
+// block more often: measured on MI355X, it was slower than the mixed-radix passes at a model
+// ratio of 1.4 to 1.9 (3-D sides 106, 118, 122) and 1.5 times faster at 3.8 (side 254), so it
+// is taken from a ratio of 2.5 (sides 214, 218, 226, 254, and 25 sides from 314 to 508).
+// Lengths whose M would exceed 4096 keep the mixed-radix passes.
+int gen_blue_m(int N) {
+  static const bool off = [] {
+    const char* e = getenv("BEC_BLUESTEIN");
+    return e && atoi(e) == 0;
+  }();
+  if (N < 2 || off) return 0;
+  long long direct = 0;
+  int rem = N;
+  while (rem % 2 == 0) rem /= 2;
+  for (int p = 3; p * p <= rem || rem > 1; p += 2) {
+    if (p * p > rem) p = rem;
+    while (rem % p == 0) {
+      if (p > 7) direct += (long long)N * (p - 1);
+      rem /= p;
+    }
+  }
+  int M = 1, lg = 0;
+  while (M < 2 * N - 1) {
+    M *= 2;
+    ++lg;
+  }
+  if (M > 4096) return 0;
+  const long long blue = (long long)M * (3 * lg + 6) / 2;
+  return 2 * direct > 5 * blue ? M : 0;
+}
+// scratch elements (of csz bytes) Bluestein needs for one line of length N (0: none): two
+// FP64 M-point work lines after a 16-byte alignment, plus a line pitch for the rounding of
+// the scratch region into whole lines
+static long long blue_need(int N, int pitch, size_t csz) {
+  const int M = gen_blue_m(N);
+  return M ? (32LL * M + 16 + (long long)csz - 1) / (long long)csz + pitch : 0;
+}
+
 static size_t xgen_lds(int D, int NX, int AL, int rows, size_t csz, bool has_aux, long long cap) {
   return sizeof(cplx<double>) * ((size_t)NX + (AL != NX ? AL : 0) + 2 * kMaxOps * D) +
          16 * sizeof(double) +
          csz * ((size_t)D * rows * (NX + 1) + (has_aux ? (size_t)2 * rows * (AL + 1) : 0) +
                 (size_t)cap);
 }
-// scratch elements for a full ping-pong of the largest FFT of a launch
-static long long xgen_full_cap(int D, int NX, int AL, int rows, bool has_aux) {
+// scratch elements for a full ping-pong of the largest FFT of a launch (at least one
+// Bluestein line group of each length that has one)
+static long long xgen_full_cap(int D, int NX, int AL, int rows, bool has_aux, size_t csz) {
   const long long t = (long long)D * rows * (NX + 1);
   const long long a = has_aux ? 2LL * rows * (AL + 1) : 0;
-  return std::max(t, a);
+  return std::max({t, a, blue_need(NX, NX + 1, csz), has_aux ? blue_need(AL, AL + 1, csz) : 0LL});
 }
 
 // x sides with a built line FFT for the x pass (zero-padded DDI also needs lines of twice
@@ -635,7 +773,7 @@ int xgen_rows(const Geom& g) {
   int r = 1;
   while (r * 2 <= target) r *= 2;
   while (r > 1 && (g.nrows % r) != 0) r /= 2;
-  while (r > 1 && xgen_lds(g.D, NX, AL, r, 16, true, xgen_full_cap(g.D, NX, AL, r, true)) >
+  while (r > 1 && xgen_lds(g.D, NX, AL, r, 16, true, xgen_full_cap(g.D, NX, AL, r, true, 16)) >
                       kGenLdsSoft)
     r /= 2;
   return r;
@@ -644,8 +782,10 @@ int xgen_rows(const Geom& g) {
 // scratch elements of an x-pass launch (0: the tile does not fit in LDS)
 static long long xgen_cap(const Geom& g, int rows, size_t csz, bool has_aux) {
   const int NX = g.n[0], AL = g.sn[0];
-  long long cap = xgen_full_cap(g.D, NX, AL, rows, has_aux);
-  const long long minc = has_aux ? std::max(NX, AL) + 1 : NX + 1;
+  long long cap = xgen_full_cap(g.D, NX, AL, rows, has_aux, csz);
+  const long long minc =
+      std::max({(long long)(has_aux ? std::max(NX, AL) + 1 : NX + 1), blue_need(NX, NX + 1, csz),
+                has_aux ? blue_need(AL, AL + 1, csz) : 0LL});
   while (cap > minc && xgen_lds(g.D, NX, AL, rows, csz, has_aux, cap) > kGenLdsHard)
     cap = std::max(minc, cap / 2);
   return xgen_lds(g.D, NX, AL, rows, csz, has_aux, cap) <= kGenLdsHard ? cap : 0;
@@ -660,12 +800,16 @@ static size_t sgen_lds(int N, int tb, int fields, size_t csz, int cap) {
          csz * (size_t)(fields * tb + cap) * (N + 1);
 }
 static SGenShape sgen_shape(int N, int fields, int csz) {
+  // scratch lines (pitch N + 1) that hold one Bluestein line group
+  const long long bn = blue_need(N, N + 1, (size_t)csz);
+  const int bl = (int)((bn + N) / (N + 1));
   for (int tb = 16; tb >= 1; tb /= 2) {
-    const size_t l = sgen_lds(N, tb, fields, csz, fields * tb);
-    if (l <= kGenLdsSoft) return {tb, fields * tb, l};
+    const int cap = std::max(fields * tb, bl);
+    const size_t l = sgen_lds(N, tb, fields, csz, cap);
+    if (l <= kGenLdsSoft) return {tb, cap, l};
   }
-  int cap = fields;
-  while (cap > 1 && sgen_lds(N, 1, fields, csz, cap) > kGenLdsHard) --cap;
+  int cap = std::max(fields, bl);
+  while (cap > std::max(1, bl) && sgen_lds(N, 1, fields, csz, cap) > kGenLdsHard) --cap;
   const size_t l = sgen_lds(N, 1, fields, csz, cap);
   if (l > kGenLdsHard) return {0, 0, 0};
   return {1, cap, l};

@@ -66,6 +66,11 @@ struct DevTables {
   const void* tw[kNumFftSizes];  // per line length: exp(-2 pi i m/N), FP64 for every T
   const void* gtw[kGenTabs];     // runtime-length tables exp(-2 pi i m/N), m < gtw_n[i]
   int gtw_n[kGenTabs];           // their lengths (0: unused)
+  // Bluestein tables of the same lengths (gbl_m[i] = 0: none): chirp c[n] = exp(i pi n^2/N)
+  // (n < N), the transform of the wrapped chirp over M points divided by M, and
+  // exp(-2 pi i m/M) (m < M), M = gbl_m[i] a power of two >= 2N - 1
+  const void* gbl[kGenTabs];
+  int gbl_m[kGenTabs];
   const double* red;      // [B][RS_COUNT] reduced scalars (norm for OP_SCALE)
   double* partials;       // [RS_COUNT][B][max_blocks]
   // profiling only (else null): kWorkSlots counter pairs, one per 64-byte line, so that the
@@ -145,6 +150,8 @@ int xpass_grid(const Geom& g);
 bool xpass_generic(const Geom& g);     // the x pass of this geometry runs the runtime kernel
 int xgen_rows(const Geom& g);          // its rows per block
 int sgen_tb(int N, int fields, int csize);  // y/z tile columns of a runtime length (0: too long)
+// Bluestein length M of a runtime line length N (0: N keeps its mixed-radix Stockham passes)
+int gen_blue_m(int N);
 // every pass of this geometry fits one block's LDS (csz: bytes per complex element)
 bool generic_fits(const Geom& g, int csz, int ddi);
 template <typename T>

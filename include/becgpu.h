@@ -244,6 +244,11 @@ int bec_autotune(bec_ws* ws, int* rows_out);
  * No counterpart in the reference (its calls are synchronous); a request loop polls it
  * to serve each workspace as soon as its own work ends. */
 int bec_query(bec_ws* ws);
+/* Transform a line of n points runs through (no counterpart in the reference, whose FFTW
+ * plans any length, src/grid.jl:37-53): 0 a built radix plan (2^k 8..512, 3 2^k 24..192,
+ * 5 2^k 40..160, 7 2^k 56..224), 1 runtime mixed-radix passes, 2 Bluestein's algorithm
+ * (lengths with a large prime factor, from about 214 points), BEC_EINVAL for n < 2. */
+int bec_line_plan(int n);
 const char* bec_last_error(void);
 /* Bytes of device memory held by a workspace of this configuration (admission). */
 long long bec_workspace_bytes(const bec_config* cfg);

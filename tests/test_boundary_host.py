@@ -33,6 +33,19 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(L.SIGNATURES), set(syms) ^ set(L.SIGNATURES)
 
 
+def test_line_plan_choice():
+    """bec_line_plan (host only): built radix plans, runtime mixed-radix passes, and
+    Bluestein's algorithm from a cost ratio of 2.5 (sides 214, 218, 226, 254, ...)."""
+    lib = L.lib()
+    for n in (8, 24, 40, 56, 128, 224, 512):
+        assert lib.bec_line_plan(n) == 0, n
+    for n in (6, 10, 22, 62, 72, 100, 118, 122, 188, 202, 1024):
+        assert lib.bec_line_plan(n) == 1, n
+    for n in (214, 218, 226, 254, 314, 452, 502, 508):
+        assert lib.bec_line_plan(n) == 2, n
+    assert lib.bec_line_plan(1) == L.BEC_EINVAL
+
+
 def _cfg(**kw):
     c = L.BecConfig()
     c.ndim = kw.get("ndim", 1)

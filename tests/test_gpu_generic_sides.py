@@ -7,7 +7,9 @@ have factors 2, 3, 5, 7 and larger odd primes (11, 13, 101), alone and beside bu
 on the other axes; zero-padded DDI lines of twice such sides, and of 224 and 512 points,
 whose doubles have no built plan; 13 components on 512-point x rows (more than one block
 of built-plan threads holds); FP32 bulk; the spectral gradient (every mode, Nyquist
-included) to 1e-12.  Bars: per-component relative L2 1e-10 and total energy 1e-12 in FP64.
+included) to 1e-12.  Sides with a large prime factor (214 = 2 107, 218, 226, 254, 502 and
+the padded 452) run Bluestein's algorithm instead of a radix-p pass, on x lines, y/z lines
+and DDI spectra lines.  Bars: per-component relative L2 1e-10 and total energy 1e-12 in FP64.
 """
 import math
 
@@ -177,3 +179,77 @@ def test_ground_state_cfg_side_100():
     assert r["converged"] == o["converged"] and r["steps"] == o["steps"]
     assert abs(r["energy"] - o["energy"]) <= 1e-12 * abs(o["energy"])
     assert rel_l2(r["workspace"].get_psi(), ows.psi) <= 1e-10
+
+
+
+def _bluestein(*sides):
+    """The sides run Bluestein's algorithm (bec_line_plan 2)."""
+    from becgpu import _lib as L
+    for n in sides:
+        assert L.lib().bec_line_plan(n) == 2, n
+
+
+@pytest.mark.parametrize("n", [214, 226, 254, 502])
+def test_bluestein_1d_lines(n):
+    _bluestein(n)
+    L = (16.0,)
+    psi = _psi((n,), L, 1, n)
+    for itp in (False, True):
+        ows, gws = pair((n,), L, 1, psi, 0.004, itp, omega=(1.0,), p=0.1, q=0.2, c0=8.0,
+                        c1=-0.4)
+        _check(ows, gws, 20)
+
+
+@pytest.mark.parametrize("n,L,F,itp", [((254, 30), (20.0, 8.0), 1, True),
+                                       ((30, 226), (8.0, 20.0), 2, False),
+                                       ((218, 12), (20.0, 6.0), 1, False)])
+def test_bluestein_2d(n, L, F, itp):
+    _bluestein(max(n))
+    psi = _psi(n, L, F, 21)
+    ows, gws = pair(n, L, F, psi, 0.004, itp, omega=(1.0, 1.3), p=0.1, q=0.2, c0=8.0, c1=-0.4)
+    _check(ows, gws, 15)
+
+
+@pytest.mark.parametrize("n,L,F,itp", [((10, 8, 254), (6.0, 5.0, 20.0), 1, True),
+                                       ((12, 226, 6), (7.0, 20.0, 5.0), 2, False)])
+def test_bluestein_3d(n, L, F, itp):
+    _bluestein(max(n))
+    psi = _psi(n, L, F, 23)
+    ows, gws = pair(n, L, F, psi, 0.002, itp, omega=(1.0, 1.1, 1.2), q=0.1, c0=8.0, c1=-0.4)
+    _check(ows, gws, 10)
+
+
+@pytest.mark.parametrize("padding", [False, True])
+@pytest.mark.parametrize("n,L", [((226,), (20.0,)), ((8, 226), (6.0, 20.0))])
+def test_bluestein_ddi(n, L, padding):
+    """Spectra of 226-point lines; zero-padded lines of 452 points (Bluestein) on the x axis
+    (the x pass's auxiliary lines) and on the y axis (the DDI spectra pass)."""
+    _bluestein(226, 452)
+    psi = _psi(n, L, 1, 25)
+    ows, gws = pair(n, L, 1, psi, 0.002, False, omega=(1.0,) * len(n), c0=10.0, c1=-0.5,
+                    ddi=True, cdd=2.0, padding=padding)
+    _check(ows, gws, 8)
+
+
+def test_bluestein_fp32():
+    """FP32 bulk on 254 x 30 with DDI: error stated against the FP64 oracle (the Bluestein
+    work lines are FP64 in both precisions)."""
+    n, L = (254, 30), (20.0, 8.0)
+    psi = _psi(n, L, 1, 27)
+    ows, gws = pair(n, L, 1, psi, 0.002, False, omega=(1.0, 1.0), q=0.1, c0=10.0, c1=0.5,
+                    ddi=True, cdd=1.0, precision="fp32")
+    _check(ows, gws, 10, psi_tol=1e-5, e_tol=1e-6)
+
+
+@pytest.mark.parametrize("n,L", [((254,), (20.0,)), ((30, 226), (8.0, 20.0)),
+                                 ((6, 8, 218), (5.0, 6.0, 20.0))])
+def test_bluestein_gradient(n, L):
+    _bluestein(max(n))
+    og, gg = grids(n, L)
+    f = np.random.default_rng(9).normal(size=og.shape)
+    ref = O.fft_gradient(f, og)
+    got = G.fft_gradient(f, gg)
+    for d in range(og.ndim):
+        err = np.max(np.abs(got[d] - ref[d])) / np.max(np.abs(ref[d]))
+        print(n, d, err)
+        assert err < 1e-12
