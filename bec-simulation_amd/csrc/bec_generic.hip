@@ -26,6 +26,22 @@ namespace bec {
 
 constexpr int kGenThreads = 256;
 
+// Division by a block-uniform runtime divisor d >= 1 for 0 <= x < 2^22: a float reciprocal
+// gives the quotient to within one, and one remainder test fixes it (exact; about a
+// quarter of the instructions of a general 32-bit division).
+struct FastDiv {
+  int d;
+  float r;
+  __device__ __forceinline__ explicit FastDiv(int dv) : d(dv), r(1.0f / (float)dv) {}
+  __device__ __forceinline__ int div(int x) const {
+    int q = (int)((float)x * r);
+    const int rem = x - q * d;
+    q += rem >= d ? 1 : 0;
+    q -= rem < 0 ? 1 : 0;
+    return q;
+  }
+};
+
 // Twiddle table exp(-2 pi i m / N), m < N, of a runtime line length (the workspace uploads
 // one for every side and zero-padded spectral length of its grid).
 // (The launchers check on the host that the table exists.)
@@ -99,9 +115,10 @@ __device__ __forceinline__ void gen_butterflies(const cplx<T>* __restrict__ src,
                                                 int Ns, const cplx<double>* __restrict__ tw,
                                                 bool inv, int tid, int nthr) {
   const int NB = N / R, step = N / (Ns * R);
+  const FastDiv fnb(NB), fns(Ns);
   for (int b = tid; b < nl * NB; b += nthr) {
-    const int l = b / NB, j = b - l * NB;
-    const int jh = j / Ns, jl = j - jh * Ns;
+    const int l = fnb.div(b), j = b - l * NB;
+    const int jh = fns.div(j), jl = j - jh * Ns;
     const cplx<T>* in = src + (size_t)l * P + j;
     double vx[R], vy[R];
     vx[0] = (double)in[0].x;
@@ -163,10 +180,11 @@ __device__ __forceinline__ void gen_stage(const cplx<T>* __restrict__ src, cplx<
     default: break;
   }
   const int blk = Ns * R, step = N / blk, stride = N / R;
+  const FastDiv fn(N), fblk(blk), fns(Ns);
   for (int o = tid; o < nl * N; o += nthr) {
-    const int l = o / N, k = o - l * N;
-    const int jh = k / blk, rem = k - jh * blk;
-    const int jl = rem - (rem / Ns) * Ns;
+    const int l = fn.div(o), k = o - l * N;
+    const int jh = fblk.div(k), rem = k - jh * blk;
+    const int jl = rem - fns.div(rem) * Ns;
     const cplx<T>* in = src + (size_t)l * P + jh * Ns + jl;
     const int s = rem * step;
     double ax = (double)in[0].x, ay = (double)in[0].y;
@@ -242,8 +260,9 @@ __device__ __forceinline__ void gen_fft_blue(cplx<T>* buf, cplx<T>* scr, int nl,
         __syncthreads();
       }
     }
+    const FastDiv fn(N);
     for (int e = tid; e < ng * N; e += nthr) {
-      const int l = e / N, k = e - l * N;
+      const int l = fn.div(e), k = e - l * N;
       const cplx<double> c = bl.c[k];
       const cplx<double> y = cmul(a[((size_t)l << msh) + k], mk(c.x, inv ? c.y : -c.y));
       buf[(size_t)(l0 + l) * P + k] = mk((T)y.x, (T)y.y);
@@ -283,8 +302,9 @@ __device__ __forceinline__ void gen_fft(cplx<T>* buf, cplx<T>* scr, int nl, int 
       rem /= R;
     }
     if (in_scr) {
+      const FastDiv fn(N);
       for (int o = tid; o < ng * N; o += nthr) {
-        const int l = o / N, k = o - l * N;
+        const int l = fn.div(o), k = o - l * N;
         a[(size_t)l * P + k] = scr[(size_t)l * P + k];
       }
       __syncthreads();
@@ -327,6 +347,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
 
   const int row0 = blockIdx.x * ROWS;
   const size_t ibase = (size_t)inst * g.inst_stride + (size_t)row0 * NX;
+  const FastDiv fNX(NX), fRows(ROWS);
   {
     const cplx<double>* t = gen_tw(tb, NX);
     for (int i = tid; i < NX; i += nthr) twx[i] = t[i];
@@ -337,8 +358,8 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
   }
   if (!(st & XS_NO_LOAD)) {
     for (int e = tid; e < NL * NX; e += nthr) {
-      const int l = e / NX, x = e - l * NX;
-      const int c = l / ROWS, r = l - c * ROWS;
+      const int l = fNX.div(e), x = e - l * NX;
+      const int c = fRows.div(l), r = l - c * ROWS;
       tile[(size_t)l * LP + x] = src[ibase + (size_t)c * g.nsites + (size_t)r * NX + x];
     }
   }
@@ -389,7 +410,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
     gen_fft<T>(aux, scr, 2 * ROWS, AL, ALP, capa, twa, true, tid, nthr, bla);
     if (!g.spad) {  // max |Phi| (ground_state.jl:4-29), unpadded convolutions only
       for (int s = tid; s < TS; s += nthr) {
-        const int r = s / NX, x = s - r * NX;
+        const int r = fNX.div(s), x = s - r * NX;
         const cplx<T> a = aux[(size_t)(2 * r) * ALP + x], b = aux[(size_t)(2 * r + 1) * ALP + x];
         phimax = fmax(phimax, fmax(fabs((double)a.x), fmax(fabs((double)a.y), fabs((double)b.x))));
       }
@@ -397,7 +418,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
     if (st & XS_STORE_PHI) {
       T* ph = phi + (size_t)inst * 3 * g.nsites + (size_t)row0 * NX;
       for (int s = tid; s < TS; s += nthr) {
-        const int r = s / NX, x = s - r * NX;
+        const int r = fNX.div(s), x = s - r * NX;
         const cplx<T> a = aux[(size_t)(2 * r) * ALP + x], b = aux[(size_t)(2 * r + 1) * ALP + x];
         ph[s] = a.x;
         ph[g.nsites + s] = a.y;
@@ -413,13 +434,13 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
       const double dtf = op.frac * dt;
       if (op.kind == OP_SCALE) {
         for (int e = tid; e < NL * NX; e += nthr) {
-          const int l = e / NX, x = e - l * NX;
+          const int l = fNX.div(e), x = e - l * NX;
           cplx<T>& v = tile[(size_t)l * LP + x];
           v = scale64(v, sc);
         }
       } else {
         for (int s = tid; s < TS; s += nthr) {
-          const int r = s / NX, x = s - r * NX;
+          const int r = fNX.div(s), x = s - r * NX;
           cplx<T> sp[D];
 #pragma unroll
           for (int c = 0; c < D; ++c) sp[c] = tile[(size_t)(c * ROWS + r) * LP + x];
@@ -447,7 +468,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
     gen_fft<T>(tile, scr, NL, NX, LP, capx, twx, false, tid, nthr, blx);
     const double kdt = prog.kin_frac * dt;
     for (int e = tid; e < NL * NX; e += nthr) {
-      const int l = e / NX, x = e - l * NX;
+      const int l = fNX.div(e), x = e - l * NX;
       const double k = tb.ktab[x];
       cplx<double> ph;
       if (prog.deriv) {  // fft_utils.jl:4-19 along x: i k_x / N on component 0
@@ -468,7 +489,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
 
   if (st & XS_WRITE_FHAT) {
     for (int s = tid; s < TS; s += nthr) {
-      const int r = s / NX, x = s - r * NX;
+      const int r = fNX.div(s), x = s - r * NX;
       cplx<T> sp[D];
 #pragma unroll
       for (int c = 0; c < D; ++c) sp[c] = tile[(size_t)(c * ROWS + r) * LP + x];
@@ -503,7 +524,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
 
   if (st & XS_NORM) {
     for (int s = tid; s < TS; s += nthr) {
-      const int r = s / NX, x = s - r * NX;
+      const int r = fNX.div(s), x = s - r * NX;
 #pragma unroll
       for (int c = 0; c < D; ++c) norm_acc += (double)abs2(tile[(size_t)(c * ROWS + r) * LP + x]);
     }
@@ -513,7 +534,7 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
 
   if (st & XS_KE_1D) {
     for (int e = tid; e < NL * NX; e += nthr) {
-      const int l = e / NX, x = e - l * NX;
+      const int l = fNX.div(e), x = e - l * NX;
       const double k = tb.ktab[x];
       ke_acc += (k * k) * (double)abs2(tile[(size_t)l * LP + x]);
     }
@@ -521,8 +542,8 @@ xgen_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src
 
   if (!(st & XS_NO_STORE)) {
     for (int e = tid; e < NL * NX; e += nthr) {
-      const int l = e / NX, x = e - l * NX;
-      const int c = l / ROWS, r = l - c * ROWS;
+      const int l = fNX.div(e), x = e - l * NX;
+      const int c = fRows.div(l), r = l - c * ROWS;
       dst[ibase + (size_t)c * g.nsites + (size_t)r * NX + x] = tile[(size_t)l * LP + x];
     }
   }
@@ -561,6 +582,8 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
   cplx<T>* scr = tile + (size_t)FIELDS * TB * LP;
   auto at = [&](int f, int xi, int e) -> cplx<T>& { return tile[(size_t)(f * TB + xi) * LP + e]; };
   const GenBlue bl = gen_blue(tb, N);
+  const int tbs = __ffs(TB) - 1;  // TB is a power of two
+  const FastDiv fN(N);
 
   const int pitch = ddi_data ? g.nxh_pitch : g.n[0];
   const int valid = ddi_data ? g.nxh : g.n[0];
@@ -580,7 +603,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
                       (size_t)other * ostride + x0;
 
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
-    const int xi = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
+    const int xi = idx & (TB - 1), rest = idx >> tbs, f = fN.div(rest), e = rest - f * N;
     const bool in = x0 + xi < valid && e < n_in;
     at(f, xi, e) = in ? data[base + (size_t)f * fstride + (size_t)e * estride + xi]
                       : mk((T)0, (T)0);
@@ -620,7 +643,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
 
   if (MODE == SM_KIN) {
     for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
-      const int xi2 = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
+      const int xi2 = idx & (TB - 1), rest = idx >> tbs, f = fN.div(rest), e = rest - f * N;
       const cplx<double> pa = mk(tabs[2 * e], tabs[2 * e + 1]);
       const cplx<double> px = mk(tabs[2 * (N + xi2)], tabs[2 * (N + xi2) + 1]);
       cplx<T>& v = at(f, xi2, e);
@@ -631,7 +654,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
     const double C = tb.params[inst].cdd;
     const double s = C / ((double)g.sn[0] * (double)g.sn[1] * (double)g.sn[2]);
     for (int idx = tid; idx < N * TB; idx += nthr) {
-      const int xi2 = idx % TB, e = idx / TB;
+      const int xi2 = idx & (TB - 1), e = idx >> tbs;
       const int x = x0 + xi2;
       double kx = 0.0;  // rfftfreq along x (Nyquist +), fftfreq along y and z (ddi.jl:66-68)
       if (x < valid) kx = (x == g.sn[0] / 2) ? -tb.sktab[g.sn[0] / 2] : tb.sktab[x];
@@ -648,7 +671,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
   } else if (MODE == SM_DERIV) {
     const int c = blockIdx.y;  // fft_utils.jl:4-19,25-44: component c takes i k_c / N
     for (int idx = tid; idx < N * TB; idx += nthr) {
-      const int xi2 = idx % TB, e = idx / TB;
+      const int xi2 = idx & (TB - 1), e = idx >> tbs;
       const int x = x0 + xi2;
       double k = 0.0;
       if (c == 0) k = x < valid ? tb.ktab[x] : 0.0;
@@ -661,7 +684,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
   } else if (MODE == SM_KE) {
     double acc = 0.0;
     for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
-      const int xi2 = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
+      const int xi2 = idx & (TB - 1), rest = idx >> tbs, f = fN.div(rest), e = rest - f * N;
       const int x = x0 + xi2;
       if (x >= valid) continue;
       const double kx = tb.ktab[x];
@@ -685,7 +708,7 @@ sgen_kernel(Geom g, DevTables tb, cplx<T>* __restrict__ data, int axis, int TB, 
     gen_fft<T>(tile, scr, FIELDS * TB, N, LP, cap, tw, true, tid, nthr, bl);
 
   for (int idx = tid; idx < FIELDS * N * TB; idx += nthr) {
-    const int xi2 = idx % TB, rest = idx / TB, e = rest % N, f = rest / N;
+    const int xi2 = idx & (TB - 1), rest = idx >> tbs, f = fN.div(rest), e = rest - f * N;
     if (x0 + xi2 < valid && e < n_out)
       data[base + (size_t)f * fstride + (size_t)e * estride + xi2] = at(f, xi2, e);
   }
