@@ -52,8 +52,9 @@ xpass_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ sr
 // The same pass with prog.nrep split steps in one launch (1-D lines of F = 0 and 1, whose
 // split step is this one pass; the schedule compiler decides, bec_split_step).  A kernel of
 // its own, so the single-step kernel's code is not touched by the repetition loop.  One
-// block per instance of at most 3 NX / E threads: occupancy does not matter, so the kernel
-// may use the whole register file (the loop keeps more values live than one step does).
+// block per instance of max(D NX / E, NX) threads: one thread per site for the per-site
+// operators, whose dependent chains (exp, sinh, cosh, sqrt) set the step's latency on a
+// single block.  Occupancy does not matter, so the kernel may use the whole register file.
 template <typename T, int NX, int D>
 __global__ void __launch_bounds__(xpass_max_threads<D>()) __attribute__((amdgpu_waves_per_eu(1)))
 xpass_rep_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
@@ -508,7 +509,11 @@ static hipError_t xpass_n(hipStream_t st, const Geom& g, const DevTables& tb, co
       if (lds > 48 * 1024)
         (void)hipFuncSetAttribute((const void*)xpass_rep_kernel<T, NX, D>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((xpass_rep_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,
+      // one thread per site for the per-site operators (the tile's FFT threads are the first
+      // D rows NX / E of them)
+      const int rthreads = std::max(threads, rows * NX);
+      if (rthreads > xpass_max_threads<D>()) return hipErrorInvalidConfiguration;
+      hipLaunchKernelGGL((xpass_rep_kernel<T, NX, D>), grid, dim3(rthreads), lds, st, g, tb, prog,
                          (const cplx<T>*)src, (cplx<T>*)dst, (cplx<T>*)fhat, (T*)phi, rows);
       return hipGetLastError();
     } else {

@@ -8,7 +8,18 @@
   constexpr int EX = xpass_e<NX, D>();
   constexpr int TEAM = NX / EX;
   const int tid = threadIdx.x;
+#if BEC_REPEAT
+  // The repeat kernel runs one thread per site (at least): the tile's FFT threads
+  // (nthr = D ROWS NX / EX, the same assignment of loads, stores and norm partials as the
+  // single-step kernel) plus threads that only take part in the per-site operators and
+  // elementwise loops (nsite-strided) and in the barriers.
+  const int nthr = D * ROWS * NX / EX;
+  const bool own = threadIdx.x < nthr;
+#else
   const int nthr = blockDim.x;
+  constexpr bool own = true;
+#endif
+  const int nsite = blockDim.x;
   const int inst = tb.active[BEC_BY];
   const InstParams& P = tb.params[inst];
   const double dt = prog.dt_override > 0.0 ? prog.dt_override : P.dt;
@@ -54,7 +65,7 @@
   // the load into the guard and wait for it there): the launch has exactly D*TS/EX threads,
   // so every tile index is in range, and a twiddle index past NX - 1 rewrites entry NX - 1
   // with the value it already holds.
-  if (!(st & XS_NO_LOAD)) {
+  if (!(st & XS_NO_LOAD) && own) {
     cplx<T> ld[EX];
 #pragma unroll
     for (int k = 0; k < EX; ++k) {
@@ -66,7 +77,7 @@
     for (int k = 0; k < TWK; ++k) tws[min(tid + k * nthr, NX - 1)] = twv[k];
 #pragma unroll
     for (int k = 0; k < EX; ++k) tile[P8(tid + k * nthr)] = ld[k];
-  } else {
+  } else if (own) {
 #pragma unroll
     for (int k = 0; k < TWK; ++k) tws[min(tid + k * nthr, NX - 1)] = twv[k];
   }
@@ -80,6 +91,12 @@
   // Slot i is copied by thread i, or by thread 0 when the block is smaller than that (1-D
   // lines of 16 or 32 points run blocks of 6 or 12 threads).
   __shared__ Op s_ops[2 * kMaxOps];
+#if BEC_REPEAT
+  // per-launch constants of the repeated step: the trap potential of every site and the
+  // 1-D kinetic factor of every x (lines of at most 512 points)
+  __shared__ double s_vtr[512];
+  __shared__ cplx<double> s_kph[512];
+#endif
 #pragma unroll
   for (int i = 0; i < kMaxOps; ++i) {
     if (tid == (i < nthr ? i : 0)) s_ops[i] = prog.A[i];
@@ -87,7 +104,7 @@
   }
   __syncthreads();
   // Zeeman factors of every diagonal op (uniform over the block).
-  for (int idx = tid; idx < 2 * kMaxOps * D; idx += nthr) {
+  for (int idx = tid; idx < 2 * kMaxOps * D; idx += nsite) {
     const int i = idx / D, c = idx % D;
     const bool used = i < kMaxOps ? i < prog.nA : (i - kMaxOps) < prog.nB;
     const Op op = s_ops[i];
@@ -144,7 +161,7 @@
     if constexpr (EX <= D) {
       if (tid < TS) body(tid);
     } else {
-      for (int s = tid; s < TS; s += nthr) body(s);
+      for (int s = tid; s < TS; s += nsite) body(s);
     }
   };
   auto run_ops = [&](const Op* ops, int nops, int zbase) {
@@ -160,7 +177,11 @@
           T n = 0;
 #pragma unroll
           for (int c = 0; c < D; ++c) n = density_acc(n, tile[P8(c * TS + s)]);
+#if BEC_REPEAT
+          const cplx<double> base = diag_base((double)n, s_vtr[s], P, dtf, itp);
+#else
           const cplx<double> base = diag_base((double)n, trap_at(s), P, dtf, itp);
+#endif
           __asm__ volatile("" ::: "memory");
 #pragma unroll
           for (int c = 0; c < D; ++c) {
@@ -203,7 +224,7 @@
           for (int c = 0; c < D; ++c) tile[P8(c * TS + s)] = sp[c];
         });
       } else if (op.kind == OP_SCALE) {
-        for (int e = tid; e < D * TS; e += nthr) tile[P8(e)] = scale64(tile[P8(e)], sc);
+        for (int e = tid; e < D * TS; e += nsite) tile[P8(e)] = scale64(tile[P8(e)], sc);
         __syncthreads();  // the scale loop's element ownership differs from the site loops
       }
     }
@@ -238,6 +259,18 @@
   // which the one-partial reduction returns unchanged), so the steps give the bits of nrep
   // separate launches.  Only the stages ops A, 1-D kinetic, ops B and norm are repeated.
   const int nrep = prog.nrep > 1 ? prog.nrep : 1;
+  // the same values as the single-step kernel computes per use
+  for (int s = tid; s < TS; s += nsite) s_vtr[s] = trap_at(s);
+  if (st & XS_KIN_1D) {
+    const double kdt = prog.kin_frac * dt;
+    for (int x = tid; x < NX; x += nsite) {
+      const double k = tb.ktab[x];
+      const double a = 0.5 * (k * k) * kdt;
+      cplx<double> ph = itp ? mk(exp(-a), 0.0) : cis_neg<double>(a);
+      s_kph[x] = ph * g.inv_ntot;
+    }
+  }
+  __syncthreads();
   __shared__ double s_norm;
   for (int rep = 0; rep < nrep; ++rep) {
   if (rep > 0 && (st & XS_NORM)) {
@@ -246,7 +279,7 @@
     __syncthreads();
     sc = 1.0 / sqrt(s_norm * g.dV);
     norm_acc = 0.0;
-    for (int e = tid; e < D * TS; e += nthr) tile[P8(e)] = scale64(tile[P8(e)], sc);
+    for (int e = tid; e < D * TS; e += nsite) tile[P8(e)] = scale64(tile[P8(e)], sc);
     __syncthreads();
   }
 #endif
@@ -256,16 +289,20 @@
   if (st & XS_KIN_1D) {
     fft_team<T, NX, false, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
     const double kdt = prog.kin_frac * dt;
-    for (int e = tid; e < D * TS; e += nthr) {
+    for (int e = tid; e < D * TS; e += nsite) {
       const int x = e % NX;
       const double k = tb.ktab[x];
       cplx<double> ph;
       if (prog.deriv) {  // fft_utils.jl:4-19 along x: i k_x / N on component 0
         ph = mk(0.0, e < TS ? k * g.inv_ntot : 0.0);
       } else {
+#if BEC_REPEAT
+        ph = s_kph[x];
+#else
         const double a = 0.5 * (k * k) * kdt;
         ph = itp ? mk(exp(-a), 0.0) : cis_neg<double>(a);
         ph = ph * g.inv_ntot;
+#endif
       }
       tile[P8(e)] = mul64(tile[P8(e)], ph);
     }
@@ -292,7 +329,7 @@
     ddi_lines_out<T>(g, fhat, (size_t)inst * g.fhat_inst_stride, row0, ROWS, AL, aux, tid, nthr);
   }
 
-  if (st & XS_NORM) {
+  if ((st & XS_NORM) && own) {
     for (int s = tid; s < TS; s += nthr) {
 #pragma unroll
       for (int c = 0; c < D; ++c) norm_acc += (double)abs2(tile[P8(c * TS + s)]);
@@ -306,14 +343,14 @@
   if (st & XS_FWD_FFT)
     fft_team<T, NX, false, EX, true>(tile + P8(line * NX), 1, tt, line < D * ROWS, tw);
 
-  if (st & XS_KE_1D) {
+  if ((st & XS_KE_1D) && own) {
     for (int e = tid; e < D * TS; e += nthr) {
       const double k = tb.ktab[e % NX];
       ke_acc += (k * k) * (double)abs2(tile[P8(e)]);
     }
   }
 
-  if (!(st & XS_NO_STORE)) {  // EX elements per thread (exact launch shape)
+  if (!(st & XS_NO_STORE) && own) {  // EX elements per thread (exact launch shape)
 #pragma unroll
     for (int k = 0; k < EX; ++k) {
       const int e = tid + k * nthr;
