@@ -61,6 +61,8 @@ SIGNATURES = {
     "bec_kinetic": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
     "bec_strang_core": ([C.c_void_p, C.c_double], C.c_int),
     "bec_yoshida_core": ([C.c_void_p, C.c_double], C.c_int),
+    "bec_strang_core_from": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
+    "bec_yoshida_core_from": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
     "bec_normalize": ([C.c_void_p], C.c_int),
     "bec_energy": ([C.c_void_p, C.c_void_p], C.c_int),
     "bec_copy_slot": ([C.c_void_p, C.c_int, C.c_int], C.c_int),

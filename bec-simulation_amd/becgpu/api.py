@@ -554,14 +554,25 @@ def _run_leapfrog_time_dependent(ws, res, callback, keep_snapshots):
     return res
 
 
-def strang_core(ws: Workspace, dt: float):
+def strang_core(ws: Workspace, dt: float, src_slot: int = 0):
+    """_strang_core! (split_step.jl:146-153); src_slot > 0 starts from that slot's psi
+    (kept) and leaves the result in slot 0."""
     ws.sync_zeeman()
-    L.check(L.lib().bec_strang_core(ws.handle, float(dt)), "bec_strang_core")
+    if src_slot:
+        L.check(L.lib().bec_strang_core_from(ws.handle, float(dt), int(src_slot)),
+                "bec_strang_core_from")
+    else:
+        L.check(L.lib().bec_strang_core(ws.handle, float(dt)), "bec_strang_core")
 
 
-def yoshida_core(ws: Workspace, dt: float):
+def yoshida_core(ws: Workspace, dt: float, src_slot: int = 0):
+    """_yoshida_core! (split_step.jl:155-183); src_slot as strang_core."""
     ws.sync_zeeman()
-    L.check(L.lib().bec_yoshida_core(ws.handle, float(dt)), "bec_yoshida_core")
+    if src_slot:
+        L.check(L.lib().bec_yoshida_core_from(ws.handle, float(dt), int(src_slot)),
+                "bec_yoshida_core_from")
+    else:
+        L.check(L.lib().bec_yoshida_core(ws.handle, float(dt)), "bec_yoshida_core")
 
 
 def run_simulation_yoshida(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end: float,
@@ -585,14 +596,19 @@ def run_simulation_yoshida(ws: Workspace, *, adaptive=AdaptiveDtParams(), t_end:
             dt_step = rem
         dt_step = max(dt_step, adaptive.dt_min)
         clamped = dt_step < dt * 0.99
-        ws.copy_slot(2, 0)                 # psi_old
-        strang_core(ws, dt_step)
-        if len(ws.active) == ws.B:         # psi_strang: slot 0's result moves to slot 3
+        if len(ws.active) == ws.B:
+            # no psi copies: psi_old moves to slot 2, both cores start from it and leave
+            # their result in slot 0; psi_strang moves to slot 3
+            ws.swap_slots(2, 0)
+            strang_core(ws, dt_step, src_slot=2)
             ws.swap_slots(3, 0)
+            yoshida_core(ws, dt_step, src_slot=2)
         else:                              # (inactive instances keep their slots in place)
-            ws.copy_slot(3, 0)
-        ws.copy_slot(0, 2)
-        yoshida_core(ws, dt_step)
+            ws.copy_slot(2, 0)             # psi_old
+            strang_core(ws, dt_step)
+            ws.copy_slot(3, 0)             # psi_strang
+            ws.copy_slot(0, 2)
+            yoshida_core(ws, dt_step)
         d = ws.diff(0, 3, 0)[0]
         err = float(d[0] / max(d[1], 1e-300))
         if (not clamped) and err > adaptive.tol and dt_step > adaptive.dt_min * 1.01:

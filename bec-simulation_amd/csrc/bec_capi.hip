@@ -61,6 +61,9 @@ struct bec_ws {
   std::vector<double> xtab, ktab;  // host copies, concatenated per axis
   std::vector<int> has_varr;
   void* slot[4] = {nullptr, nullptr, nullptr, nullptr};
+  // slot the next launch of a core reads psi from (bec_strang_core_from and
+  // bec_yoshida_core_from; -1: slot 0 in place)
+  int src_once = -1;
   void* scratch = nullptr;
   void* fhat = nullptr;
   void* phi = nullptr;  // Phi(x) fields for the lane-per-site DDI rotation (F=6)
@@ -218,7 +221,40 @@ static double phi_bytes(const bec_ws* ws) {  // 3 real fields
   return 3.0 * (double)ws->g.nsites * (ws->csz / 2) * n_active(ws);
 }
 
+#define CHK(expr)                 \
+  do {                            \
+    int _r = (expr);              \
+    if (_r != BEC_OK) return _r;  \
+  } while (0)
+
+// A core that starts from another slot (src_once): x passes and fused passes that load
+// and store psi read that slot directly and write slot 0; before any other first launch
+// the slot is copied into slot 0.
+static int flush_src(bec_ws* ws) {
+  if (ws->src_once < 0) return BEC_OK;
+  const int s = ws->src_once;
+  ws->src_once = -1;
+  const size_t b = ws->csz * (size_t)ws->g.inst_stride;
+  if (b % 16 == 0) {
+    HIPCHK(launch_copy_slot(ws->stream, ws->tb, ws->slot[s], ws->slot[0], (long long)b,
+                            n_active(ws)));
+    return BEC_OK;
+  }
+  for (int inst : ws->active)
+    HIPCHK(hipMemcpyAsync((char*)ws->slot[0] + b * inst, (char*)ws->slot[s] + b * inst, b,
+                          hipMemcpyDeviceToDevice, ws->stream));
+  return BEC_OK;
+}
+
 static int do_x(bec_ws* ws, const XProgram& p, const void* src, void* dst) {
+  if (ws->src_once >= 0 && !ws->rec) {
+    if (src == ws->slot[0] && dst == ws->slot[0] && !(p.stages & (XS_NO_LOAD | XS_NO_STORE))) {
+      src = ws->slot[ws->src_once];
+      ws->src_once = -1;
+    } else {
+      CHK(flush_src(ws));
+    }
+  }
   if (ws->rec) {
     Rec r{};
     r.kind = R_X;
@@ -252,6 +288,7 @@ static int do_sites(bec_ws* ws, const SProgram& p) {
     ws->rec->push_back(Rec{R_OTHER});
     return BEC_OK;
   }
+  CHK(flush_src(ws));
   double b = 0.0;
   if (ws->prof_on) {
     b = 2.0 * psi_bytes(ws);
@@ -291,12 +328,14 @@ static int do_xs(bec_ws* ws, const XProgram& p) {
     const int naux = ((p.stages & XS_LOAD_PHI) ? 1 : 0) + ((p.stages & XS_WRITE_FHAT) ? 1 : 0);
     ws->prof_fft_flops += rows * (ntile * g.D * fl(g.n[0]) + naux * 2.0 * fl(g.sn[0]));
   }
+  const void* src = ws->src_once >= 0 ? ws->slot[ws->src_once] : ws->slot[0];
+  ws->src_once = -1;
   const int pe = prof_begin(ws, BEC_K_XSITE, b);
   hipError_t e = ws->prec == BEC_FP32
-                     ? launch_xsite<float>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->fhat,
-                                           n_active(ws))
-                     : launch_xsite<double>(ws->stream, ws->g, ws->tb, p, ws->slot[0], ws->fhat,
-                                            n_active(ws));
+                     ? launch_xsite<float>(ws->stream, ws->g, ws->tb, p, src, ws->slot[0],
+                                           ws->fhat, n_active(ws))
+                     : launch_xsite<double>(ws->stream, ws->g, ws->tb, p, src, ws->slot[0],
+                                            ws->fhat, n_active(ws));
   prof_end(ws, pe);
   HIPCHK(e);
   return BEC_OK;
@@ -316,6 +355,7 @@ static int do_strided(bec_ws* ws, int axis, int mode, void* data, double kin_fra
     ws->rec->push_back(r);
     return BEC_OK;
   }
+  CHK(flush_src(ws));
   hipError_t e;
   const int cls = ddi_data ? 5 + (mode == SM_DDI ? 1 : 0)
                            : 1 + (mode == SM_KIN ? 1 : mode == SM_INV ? 2 : mode == SM_KE ? 3 : 0);
@@ -339,6 +379,7 @@ static int do_ddi_line(bec_ws* ws) {
     ws->rec->push_back(Rec{R_OTHER});
     return BEC_OK;
   }
+  CHK(flush_src(ws));
   const int pe = prof_begin(ws, 6, ws->prof_on ? 2.0 * half_field_bytes(ws) : 0.0);
   const hipError_t e =
       ws->prec == BEC_FP32
@@ -358,17 +399,13 @@ static int do_reduce(bec_ws* ws, int nblocks, uint32_t slots, uint32_t maxmask =
     ws->rec->push_back(r);
     return BEC_OK;
   }
+  CHK(flush_src(ws));
   const int pe = prof_begin(ws, 7);
   HIPCHK(launch_reduce(ws->stream, ws->tb, ws->d_red, n_active(ws), nblocks, slots, maxmask));
   prof_end(ws, pe);
   return BEC_OK;
 }
 
-#define CHK(expr)                 \
-  do {                            \
-    int _r = (expr);              \
-    if (_r != BEC_OK) return _r;  \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // Schedule compiler
@@ -606,6 +643,7 @@ struct Sched {
         ws->rec->push_back(Rec{R_SCALE});
         return BEC_OK;
       }
+      CHK(flush_src(ws));
       hipError_t e = ws->prec == BEC_FP32
                          ? launch_scale_norm<float>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws))
                          : launch_scale_norm<double>(ws->stream, ws->g, ws->tb, ws->slot[0], n_active(ws));
@@ -1460,6 +1498,33 @@ int bec_yoshida_core(bec_ws* ws, double dt) {
                    {{IT_V, w1 / 2}, {IT_K, w1}, {IT_V, wm}, {IT_K, w0}, {IT_V, wm}, {IT_K, w1},
                     {IT_V, w1 / 2}},
                    0, dt);
+}
+
+// The cores from another slot: slot src holds the state, slot 0 receives the result, src
+// is left unchanged.  Only with every instance active (slot 0's inactive instances would
+// keep stale data) and outside recording; otherwise slot src is copied first.
+static int ensure_slot(bec_ws* ws, int s);
+static int core_from(bec_ws* ws, int src, int (*core)(bec_ws*, double), double dt) {
+  if (!ws) return fail(BEC_EINVAL, "null workspace");
+  if (src < 0 || src > 3) return fail(BEC_EINVAL, "slot out of range");
+  if (src == 0) return core(ws, dt);
+  CHK(ensure_slot(ws, src));
+  ws->src_once = src;
+  if (ws->rec || n_active(ws) != ws->B) CHK(flush_src(ws));
+  const int r = core(ws, dt);
+  if (r != BEC_OK) {
+    ws->src_once = -1;
+    return r;
+  }
+  return flush_src(ws);  // a core that launched nothing (no active instance)
+}
+
+int bec_strang_core_from(bec_ws* ws, double dt, int src_slot) {
+  return core_from(ws, src_slot, bec_strang_core, dt);
+}
+
+int bec_yoshida_core_from(bec_ws* ws, double dt, int src_slot) {
+  return core_from(ws, src_slot, bec_yoshida_core, dt);
 }
 
 int bec_normalize(bec_ws* ws) {

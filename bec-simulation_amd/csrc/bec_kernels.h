@@ -98,11 +98,12 @@ hipError_t launch_xpass(hipStream_t st, const Geom& g, const DevTables& tb, cons
                         const void* src, void* dst, void* fhat, int n_active, void* phi = nullptr);
 
 // Fused x pass + lane-per-site pass (bec_xsite.hip): stages XS_INV_FFT, XS_LOAD_PHI,
-// XS_OPS_A (any operator), XS_WRITE_FHAT, XS_NORM, XS_FWD_FFT on psi in place, with the
-// x-pass tiling (xpass_grid blocks, so its partials line up with the x pass's).
+// XS_OPS_A (any operator), XS_WRITE_FHAT, XS_NORM, XS_FWD_FFT from src into psi (the same
+// buffer, or another slot of the same layout), with the x-pass tiling (xpass_grid blocks,
+// so its partials line up with the x pass's).
 template <typename T>
 hipError_t launch_xsite(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                        void* psi, void* fhat, int n_active);
+                        const void* src, void* psi, void* fhat, int n_active);
 // Whether the fused kernel covers this geometry (F = 2 or 6, 2-D/3-D, one site per thread).
 bool xsite_ok(const Geom& g);
 // Lane-per-site pass: one thread per grid site applies `prog` to its spinor in VGPRs

@@ -24,8 +24,12 @@ constexpr int kXsiteMaxThreads = 256;
 
 template <typename T, int NX, int D>
 __global__ void __launch_bounds__(kXsiteMaxThreads) __attribute__((amdgpu_waves_per_eu(2)))
-xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
-             cplx<T>* __restrict__ fhat, int ROWS) {
+xsite_kernel(Geom g, DevTables tb, XProgram prog, const cplx<T>* __restrict__ src,
+             cplx<T>* __restrict__ psi, cplx<T>* __restrict__ fhat, int ROWS) {
+  // src is psi itself except for the first launch of a core started from another slot.
+  // In place, each thread reads its own site's D values in the prologue, and the block
+  // writes them only after its last barrier, from LDS: no load of src can move past a
+  // store to psi.  (Without __restrict__ on src the FP64 spin-6 launch ran 6 % slower.)
   extern __shared__ __align__(16) unsigned char smem[];
   // values per thread in the tile FFTs: 16 for 13 components, so one round of TS/TEAM =
   // 16*ROWS lines covers all 13*ROWS (a butterfly's arithmetic does not depend on E)
@@ -58,7 +62,7 @@ xsite_kernel(Geom g, DevTables tb, XProgram prog, cplx<T>* __restrict__ psi,
   {
     const cplx<double> twv = twg[min(tid, NX - 1)];
     cplx<T> ld[D];
-    const cplx<T>* pb = psi + ibase;
+    const cplx<T>* pb = src + ibase;  // psi itself, or the slot a core starts from
 #pragma unroll
     for (int c = 0; c < D; ++c) ld[c] = *lane_elem(pb + (size_t)c * g.nsites, tid);
     cplx<T> ph[2][3];
@@ -242,7 +246,7 @@ bool xsite_ok(const Geom& g) {
 
 template <typename T, int NX, int D>
 static hipError_t xsite_n(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                          void* psi, void* fhat, int n_active) {
+                          const void* src, void* psi, void* fhat, int n_active) {
   const int rows = xpass_rows_g(g);
   const int threads = rows * NX;
   if (threads < 16 || threads > kXsiteMaxThreads) return hipErrorInvalidConfiguration;
@@ -253,39 +257,39 @@ static hipError_t xsite_n(hipStream_t st, const Geom& g, const DevTables& tb, co
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const dim3 grid(g.nrows / rows, n_active);
   hipLaunchKernelGGL((xsite_kernel<T, NX, D>), grid, dim3(threads), lds, st, g, tb, prog,
-                     (cplx<T>*)psi, (cplx<T>*)fhat, rows);
+                     (const cplx<T>*)src, (cplx<T>*)psi, (cplx<T>*)fhat, rows);
   return hipGetLastError();
 }
 
 template <typename T, int D>
 static hipError_t xsite_d(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                          void* psi, void* fhat, int n_active) {
+                          const void* src, void* psi, void* fhat, int n_active) {
   switch (g.n[0]) {
-    case 16: return xsite_n<T, 16, D>(st, g, tb, prog, psi, fhat, n_active);
-    case 32: return xsite_n<T, 32, D>(st, g, tb, prog, psi, fhat, n_active);
-    case 64: return xsite_n<T, 64, D>(st, g, tb, prog, psi, fhat, n_active);
-    case 128: return xsite_n<T, 128, D>(st, g, tb, prog, psi, fhat, n_active);
-    case 256: return xsite_n<T, 256, D>(st, g, tb, prog, psi, fhat, n_active);
+    case 16: return xsite_n<T, 16, D>(st, g, tb, prog, src, psi, fhat, n_active);
+    case 32: return xsite_n<T, 32, D>(st, g, tb, prog, src, psi, fhat, n_active);
+    case 64: return xsite_n<T, 64, D>(st, g, tb, prog, src, psi, fhat, n_active);
+    case 128: return xsite_n<T, 128, D>(st, g, tb, prog, src, psi, fhat, n_active);
+    case 256: return xsite_n<T, 256, D>(st, g, tb, prog, src, psi, fhat, n_active);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename T>
 hipError_t launch_xsite(hipStream_t st, const Geom& g, const DevTables& tb, const XProgram& prog,
-                        void* psi, void* fhat, int n_active) {
+                        const void* src, void* psi, void* fhat, int n_active) {
   // the stages this kernel implements; the schedule never asks it for others
   constexpr int kStages = XS_INV_FFT | XS_LOAD_PHI | XS_OPS_A | XS_WRITE_FHAT | XS_NORM | XS_FWD_FFT;
   if ((prog.stages & ~kStages) != 0 || prog.nB != 0 || prog.nA > kMaxOps) return hipErrorInvalidValue;
   switch (g.D) {
-    case 5: return xsite_d<T, 5>(st, g, tb, prog, psi, fhat, n_active);
-    case 13: return xsite_d<T, 13>(st, g, tb, prog, psi, fhat, n_active);
+    case 5: return xsite_d<T, 5>(st, g, tb, prog, src, psi, fhat, n_active);
+    case 13: return xsite_d<T, 13>(st, g, tb, prog, src, psi, fhat, n_active);
     default: return hipErrorInvalidValue;
   }
 }
 
 template hipError_t launch_xsite<float>(hipStream_t, const Geom&, const DevTables&, const XProgram&,
-                                        void*, void*, int);
+                                        const void*, void*, void*, int);
 template hipError_t launch_xsite<double>(hipStream_t, const Geom&, const DevTables&,
-                                         const XProgram&, void*, void*, int);
+                                         const XProgram&, const void*, void*, void*, int);
 
 }  // namespace bec

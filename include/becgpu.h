@@ -133,6 +133,12 @@ int bec_kinetic(bec_ws* ws, double dt, int imaginary_time);
 /* _strang_core! / _yoshida_core! (split_step.jl:146-183), real time. */
 int bec_strang_core(bec_ws* ws, double dt);
 int bec_yoshida_core(bec_ws* ws, double dt);
+/* The same cores started from psi in slot src_slot (1..3; 0: the cores above): slot 0
+ * receives the result and src_slot keeps the starting state, as run_simulation_yoshida!
+ * needs for psi_old (src/yoshida.jl:36-44) without a copy of psi: the first launch of the
+ * core reads src_slot (with every instance active; otherwise the slot is copied first). */
+int bec_strang_core_from(bec_ws* ws, double dt, int src_slot);
+int bec_yoshida_core_from(bec_ws* ws, double dt, int src_slot);
 /* _normalize_psi! (split_step.jl:185-195). */
 int bec_normalize(bec_ws* ws);
 /* _normalize_psi_constrained! (ground_state.jl:279-320) of every active instance, with
