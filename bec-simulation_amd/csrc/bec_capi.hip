@@ -1793,8 +1793,14 @@ int bec_autotune(bec_ws* ws, int* rows_out) {
   Geom g = ws->g;
   g.xrows = 0;
   const int r0 = xpass_rows_g(g);
+  // Shapes that run the fused x/site kernel by default keep its default tile: the timing
+  // below is of the x pass, whose best tile is not the fused kernel's (128^3 spin 6 FP64,
+  // one box: 624-627 us per fused launch at one row per block, 662-673 us at the two rows
+  // that the x-pass timing picked on some runs), and a tile above kXsiteMaxThreads sites
+  // would switch the shape to the separate passes for the rest of the process.
+  const bool fusable = xsite_ok(g);
   std::vector<int> cand;
-  for (int r = 1; r <= g.nrows; r *= 2) {
+  for (int r = 1; r <= g.nrows && !fusable; r *= 2) {
     g.xrows = r;
     if (xpass_rows_ok(g, r) && g.nrows / r <= ws->max_blocks && r * 4 >= r0) cand.push_back(r);
   }
