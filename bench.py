@@ -262,6 +262,10 @@ def run_yoshida3d(G, A, steps, warmup, precision, padding=False, profiled=True):
     prof = ws.profile_read()
     work = ws.profile_work()
     ws.profile(False)
+    # one whole-psi copy after the measurements: the adaptive driver copies no psi, and the
+    # --pmc traffic summary calibrates its counters on a copy of known size
+    ws.copy_slot(1, 0)
+    ws.synchronize()
     return ws, grid, (t1 - t0), prof, psi, r2, work
 
 
