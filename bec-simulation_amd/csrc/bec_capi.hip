@@ -712,14 +712,26 @@ static double tensor_matvec_folded_flops() {
   return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
 }
 
-// One Euler rotation (op_euler): 4 products of the real matrix d^F(pi/2) with a complex
-// spinor (4 flop per nonzero entry) and 5 diagonal phase sweeps (12 flop per component).
+// One Euler rotation (op_euler): 4 products of the real matrix d^F(pi/2) (or its
+// transpose) with a complex spinor, as dhalf_apply forms them (the 2F sums and differences
+// v_c +- v_{D-1-c}, 2 flop each, then 4 flop per nonzero entry of each row's half: c <= F
+// for an even row, c < F for an odd one, all D columns for a row of neither parity), and
+// 5 diagonal phase sweeps (12 flop per component).
+template <int F, bool TRANS>
+static double dhalf_flops() {
+  constexpr int D = 2 * F + 1;
+  constexpr tens::DHalf<F> d{};
+  constexpr tens::DHalfParity<F, TRANS> par{};
+  double f = 4.0 * F;
+  for (int i = 0; i < D; ++i) {
+    const int ncol = par.s[i] == 1 ? F + 1 : par.s[i] == -1 ? F : D;
+    for (int j = 0; j < ncol; ++j) f += (TRANS ? d.v[j * D + i] : d.v[i * D + j]) != 0.0 ? 4.0 : 0.0;
+  }
+  return f;
+}
 template <int F>
 static double euler_flops() {
-  constexpr tens::DHalf<F> d{};
-  int nz = 0;
-  for (double x : d.v) nz += x != 0.0;
-  return 16.0 * nz + 60.0 * (2 * F + 1);
+  return 2.0 * (dhalf_flops<F, false>() + dhalf_flops<F, true>()) + 60.0 * (2 * F + 1);
 }
 
 // K_M[i][j] = sum_S CG(m1_i, M - m1_i | S M) g_S CG(m1_j, M - m1_j | S M) for every M, in

@@ -537,19 +537,45 @@ __device__ __forceinline__ void op_rodrigues(cplx<T>* s, double c1, double dtf, 
 template <int F>
 inline constexpr tens::DHalf<F> kDHalf{};
 
-// w = d v (TRANS false) or w = d^T v with d = d^F(pi/2): a real matrix with
-// compile-time entries (zeros skipped), so each entry costs two FMAs on literals.
+template <int F, bool TRANS>
+inline constexpr tens::DHalfParity<F, TRANS> kDHalfPar{};
+
+// w = d v (TRANS false) or w = d^T v with d = d^F(pi/2): a real matrix with compile-time
+// entries.  An even row takes the F + 1 sums v_c + v_{D-1-c} (c <= F), an odd row the F
+// differences v_c - v_{D-1-c} (c < F): about half the products of the plain row sums, each
+// an FMA pair on literals (zeros skipped).
 template <typename T, int D, bool TRANS>
 __device__ __forceinline__ void dhalf_apply(const cplx<T>* v, cplx<T>* w) {
   constexpr int F = (D - 1) / 2;
+  constexpr int NP = F > 0 ? F : 1;
+  cplx<T> ps[NP], ms[NP];
+  static_for<0, F>([&](auto jc) __attribute__((always_inline)) {
+    constexpr int j = decltype(jc)::value;
+    ps[j] = v[j] + v[D - 1 - j];
+    ms[j] = v[j] - v[D - 1 - j];
+  });
   static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
     constexpr int i = decltype(ic)::value;
+    constexpr int par = kDHalfPar<F, TRANS>.s[i];
     cplx<T> acc = mk((T)0, (T)0);
-    static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
-      constexpr int j = decltype(jc)::value;
-      constexpr T c = (T)kDHalf<F>.v[TRANS ? j * D + i : i * D + j];
-      if constexpr (c != (T)0) acc = mk(acc.x + c * v[j].x, acc.y + c * v[j].y);
-    });
+    if constexpr (par == 0) {
+      static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        constexpr T c = (T)kDHalf<F>.v[TRANS ? j * D + i : i * D + j];
+        if constexpr (c != (T)0) acc = mk(acc.x + c * v[j].x, acc.y + c * v[j].y);
+      });
+    } else {
+      if constexpr (par == 1) {
+        constexpr T c = (T)kDHalf<F>.v[TRANS ? F * D + i : i * D + F];
+        if constexpr (c != (T)0) acc = mk(acc.x + c * v[F].x, acc.y + c * v[F].y);
+      }
+      static_for<0, F>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        constexpr T c = (T)kDHalf<F>.v[TRANS ? j * D + i : i * D + j];
+        const cplx<T> u = par == 1 ? ps[j] : ms[j];
+        if constexpr (c != (T)0) acc = mk(acc.x + c * u.x, acc.y + c * u.y);
+      });
+    }
     w[i] = acc;
   });
 }

@@ -194,6 +194,29 @@ struct DHalf {
   }
 };
 
+// Rows of d = d^F(pi/2) and of d^T are even or odd under m -> -m (column c -> D-1-c):
+// d_{m',-m}(pi/2) = (-1)^(F+m') d_{m',m}(pi/2), and d^T = P d P with P = diag((-1)^m).  The
+// parity of each row is read off the table (+1 even, -1 odd, 0 neither, which keeps the
+// row's full sum; equal within 1e-14, the closed form's rounding).
+template <int F, bool TRANS>
+struct DHalfParity {
+  int s[2 * F + 1];
+  __host__ __device__ constexpr DHalfParity() : s() {
+    constexpr int D = 2 * F + 1;
+    const DHalf<F> d{};
+    for (int i = 0; i < D; ++i) {
+      bool ev = true, od = true;
+      for (int j = 0; j < D; ++j) {
+        const double a = TRANS ? d.v[j * D + i] : d.v[i * D + j];
+        const double b = TRANS ? d.v[(D - 1 - j) * D + i] : d.v[i * D + D - 1 - j];
+        if (a - b > 1e-14 || b - a > 1e-14) ev = false;
+        if (a + b > 1e-14 || -(a + b) > 1e-14) od = false;
+      }
+      s[i] = ev ? 1 : (od ? -1 : 0);
+    }
+  }
+};
+
 // The table in off() order, with |c| < 1e-15 set to 0 (clebsch_gordan.jl:174-175).
 template <int F>
 struct CGTab {
