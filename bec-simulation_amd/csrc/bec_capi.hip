@@ -716,7 +716,7 @@ static double tensor_matvec_folded_flops() {
 // transpose) with a complex spinor, as dhalf_apply forms them (the 2F sums and differences
 // v_c +- v_{D-1-c}, 2 flop each, then 4 flop per nonzero entry of each row's half: c <= F
 // for an even row, c < F for an odd one, all D columns for a row of neither parity), and
-// 5 diagonal phase sweeps (12 flop per component).
+// 5 diagonal phase sweeps (F - 1 powers and 2F component products, 6 flop each).
 template <int F, bool TRANS>
 static double dhalf_flops() {
   constexpr int D = 2 * F + 1;
@@ -731,7 +731,7 @@ static double dhalf_flops() {
 }
 template <int F>
 static double euler_flops() {
-  return 2.0 * (dhalf_flops<F, false>() + dhalf_flops<F, true>()) + 60.0 * (2 * F + 1);
+  return 2.0 * (dhalf_flops<F, false>() + dhalf_flops<F, true>()) + 30.0 * (F > 0 ? 3 * F - 1 : 0);
 }
 
 // K_M[i][j] = sum_S CG(m1_i, M - m1_i | S M) g_S CG(m1_j, M - m1_j | S M) for every M, in

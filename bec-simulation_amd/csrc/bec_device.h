@@ -580,13 +580,26 @@ __device__ __forceinline__ void dhalf_apply(const cplx<T>* v, cplx<T>* w) {
   });
 }
 
-// v_c *= ph * z^c for c = 0..D-1 (a diagonal phase sweep by recurrence).
+// v_c *= e^{i m x} for m = F - c (Rz(-x) of the reference's convention): the factors of m and
+// -m are conjugates, so the F powers e^{i m x}, m = 1..F, come from one recurrence on e.
+// The products are explicit FMAs, so that every kernel that inlines the rotation (the fused
+// x/site pass and the lane-per-site pass) rounds them alike.
 template <typename T, int D>
-__device__ __forceinline__ void phase_sweep(cplx<T>* v, cplx<T> ph, cplx<T> z) {
+__device__ __forceinline__ void phase_sweep(cplx<T>* v, cplx<T> e) {
+  constexpr int F = (D - 1) / 2;
+  auto fm = [](T x, T y, T z) __attribute__((always_inline)) {
+    if constexpr (sizeof(T) == 4) return fmaf(x, y, z);
+    else return fma(x, y, z);
+  };
+  auto cm = [&](cplx<T> a, cplx<T> b) __attribute__((always_inline)) {
+    return mk<T>(fm(a.x, b.x, -(a.y * b.y)), fm(a.x, b.y, a.y * b.x));
+  };
+  cplx<T> p = e;
 #pragma unroll
-  for (int c = 0; c < D; ++c) {
-    v[c] = ph * v[c];
-    ph = ph * z;
+  for (int m = 1; m <= F; ++m) {
+    v[F - m] = cm(p, v[F - m]);
+    v[F + m] = cm(conjc(p), v[F + m]);
+    if (m < F) p = cm(p, e);
   }
 }
 
@@ -595,7 +608,8 @@ __device__ __forceinline__ void phase_sweep(cplx<T>* v, cplx<T> ph, cplx<T> z) {
 // The reference forms Ry(+-beta) from a numerical eigenbasis V_Fy of F_y; here the eigenbasis
 // is the closed form U = Rz(pi/2) d^F(pi/2), so that, with a = alpha + pi/2,
 //   R = Rz(a) d Rz(beta) d^T Dz(theta) d Rz(-beta) d^T Rz(-a),   Rz(x) = diag(e^{-i x m}),
-// four real D x D matvecs and five phase sweeps (cis recurrences as in the reference).
+// four real D x D matvecs and five phase sweeps (cis recurrences as in the reference, one
+// power per |m|).
 // Returns 1 if the rotation was applied (0: |phi| |dt| < 1e-14, identity).
 template <typename T, int D>
 __device__ __forceinline__ int op_euler(cplx<T>* s, double phx, double phy, double phz, double dt,
@@ -606,7 +620,7 @@ __device__ __forceinline__ int op_euler(cplx<T>* s, double phx, double phy, doub
   const double n0 = (sizeof(T) == 4 && !itp) ? site_norm2<T, D>(s) : 0.0;
   // cos/sin of beta and alpha from phi itself (no acos/atan2): cos beta = phi_z/|phi| (clamped),
   // sin beta = rho/|phi| >= 0, (cos alpha, sin alpha) = (phi_x, phi_y)/rho, with atan2's
-  // signed-zero convention when rho = 0.  cos/sin of F*x by De Moivre powers.
+  // signed-zero convention when rho = 0.
   const double rho = sqrt(phx * phx + phy * phy);
   const double ipm = 1.0 / pm;  // one division per radius
   double cb = phz * ipm;
@@ -622,32 +636,13 @@ __device__ __forceinline__ int op_euler(cplx<T>* s, double phx, double phy, doub
     sal = 0.0;
   }
   const double ca = -sal, sa = cal;  // a = alpha + pi/2
-  auto powF = [](double c, double sn, double& cF, double& sF) __attribute__((always_inline)) {
-    double rc = 1.0, rs = 0.0, bc = c, bs = sn;
-#pragma unroll
-    for (int e = F; e > 0; e >>= 1) {
-      if (e & 1) {
-        const double t = rc * bc - rs * bs;
-        rs = rc * bs + rs * bc;
-        rc = t;
-      }
-      const double t = bc * bc - bs * bs;
-      bs = 2.0 * bc * bs;
-      bc = t;
-    }
-    cF = rc;
-    sF = rs;
-  };
   const double theta = pm * dt;
-  double sFa, cFa, sFb, cFb;
-  powF(ca, sa, cFa, sFa);
-  powF(cb, sb, cFb, sFb);
   cplx<T> v[D], w[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) v[c] = s[c];
-  phase_sweep<T, D>(v, mk((T)cFa, (T)sFa), mk((T)ca, (T)(-sa)));  // Rz(-a): e^{+i a m}
+  phase_sweep<T, D>(v, mk((T)ca, (T)sa));  // Rz(-a): e^{+i a m}
   dhalf_apply<T, D, true>(v, w);
-  phase_sweep<T, D>(w, mk((T)cFb, (T)sFb), mk((T)cb, (T)(-sb)));  // Rz(-beta)
+  phase_sweep<T, D>(w, mk((T)cb, (T)sb));  // Rz(-beta)
   dhalf_apply<T, D, false>(w, v);
   if (itp) {
     const double et = exp(theta);
@@ -662,15 +657,14 @@ __device__ __forceinline__ int op_euler(cplx<T>* s, double phx, double phy, doub
       dz *= st;
     }
   } else {
-    double sF, cF, st, ct;
+    double st, ct;
     sincos_d(theta, &st, &ct);
-    powF(ct, st, cF, sF);
-    phase_sweep<T, D>(v, mk((T)cF, (T)(-sF)), mk((T)ct, (T)st));  // e^{-i theta m}
+    phase_sweep<T, D>(v, mk((T)ct, (T)(-st)));  // e^{-i theta m}
   }
   dhalf_apply<T, D, true>(v, w);
-  phase_sweep<T, D>(w, mk((T)cFb, (T)(-sFb)), mk((T)cb, (T)sb));  // Rz(beta)
+  phase_sweep<T, D>(w, mk((T)cb, (T)(-sb)));  // Rz(beta)
   dhalf_apply<T, D, false>(w, v);
-  phase_sweep<T, D>(v, mk((T)cFa, (T)(-sFa)), mk((T)ca, (T)sa));  // Rz(a)
+  phase_sweep<T, D>(v, mk((T)ca, (T)(-sa)));  // Rz(a)
 #pragma unroll
   for (int c = 0; c < D; ++c) s[c] = v[c];
   if (!itp) restore_norm<T, D>(s, n0);
