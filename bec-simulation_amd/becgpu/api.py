@@ -294,10 +294,13 @@ class Workspace:
         {"tensor_matvecs", "rotations", "flops_per_matvec", "flops_per_rotation", "flops"},
         "fft_flops" of the fused x/site launches, and "bytes": algorithmic HBM bytes issued
         per kernel class."""
-        out = np.zeros(5 + len(L.K_NAMES))
+        K = len(L.K_NAMES)
+        out = np.zeros(7 + K)
         L.check(L.lib().bec_profile_work(self.handle, out.ctypes.data), "bec_profile_work")
+        tensor = out[0] * out[2] - out[5 + K] * (out[2] - out[6 + K])
         return {"tensor_matvecs": out[0], "rotations": out[1], "flops_per_matvec": out[2],
-                "flops_per_rotation": out[3], "flops": out[0] * out[2] + out[1] * out[3],
+                "first_term_matvecs": out[5 + K], "flops_per_first_term": out[6 + K],
+                "flops_per_rotation": out[3], "flops": tensor + out[1] * out[3],
                 "fft_flops": float(out[4 + len(L.K_NAMES)]),
                 "bytes": {n: float(out[4 + i]) for i, n in enumerate(L.K_NAMES)}}
 

@@ -711,6 +711,16 @@ static double tensor_matvec_folded_flops() {
   }
   return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
 }
+// The first matvec of a site's series (v = psi): each pair sum of m1 != m2 is one complex
+// product doubled (6 + 2 flop instead of 6 + 8)
+template <int F>
+static double tensor_matvec_folded_sym_flops() {
+  double f = tensor_matvec_folded_flops<F>();
+  for (int M = -2 * F; M <= 2 * F; ++M)
+    for (int i = 0; i < tens::npairs(F, M); ++i)
+      if (tens::m1lo(F, M) + i != M - (tens::m1lo(F, M) + i)) f -= 6.0;
+  return f;
+}
 
 // One Euler rotation (op_euler): 4 products of the real matrix d^F(pi/2) (or its
 // transpose) with a complex spinor, as dhalf_apply forms them (the 2F sums and differences
@@ -2175,10 +2185,11 @@ int bec_profile_work(bec_ws* ws, double* out) {
   HIPCHK(hipStreamSynchronize(ws->stream));
   std::vector<unsigned long long> h((size_t)kWorkSlots * kWorkStride);
   HIPCHK(hipMemcpy(h.data(), ws->d_work, kWorkBytes, hipMemcpyDeviceToHost));
-  unsigned long long nmv = 0, nrot = 0;
+  unsigned long long nmv = 0, nrot = 0, nsym = 0;
   for (int sl = 0; sl < kWorkSlots; ++sl) {
     nmv += h[(size_t)sl * kWorkStride];
     nrot += h[(size_t)sl * kWorkStride + 1];
+    nsym += h[(size_t)sl * kWorkStride + 2];
   }
   out[0] = (double)nmv;
   out[1] = (double)nrot;
@@ -2201,6 +2212,12 @@ int bec_profile_work(bec_ws* ws, double* out) {
   }
   out[4 + BEC_K_COUNT] = ws->prof_fft_flops;
   ws->prof_fft_flops = 0.0;
+  static const double kFoldedSymFlops[7] = {
+      0.0, tensor_matvec_folded_sym_flops<1>(), tensor_matvec_folded_sym_flops<2>(),
+      tensor_matvec_folded_sym_flops<3>(), tensor_matvec_folded_sym_flops<4>(),
+      tensor_matvec_folded_sym_flops<5>(), tensor_matvec_folded_sym_flops<6>()};
+  out[5 + BEC_K_COUNT] = (double)nsym;
+  out[6 + BEC_K_COUNT] = !ws->tensor ? 0.0 : ws->prec == BEC_FP64 ? kFoldedSymFlops[ws->F] : out[2];
   HIPCHK(hipMemset(ws->d_work, 0, kWorkBytes));
   return BEC_OK;
 }

@@ -834,7 +834,7 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
 // per matvec at F = 6).  Its operands come from memory in both precisions' cost model for
 // FP64 (no 64-bit literals), so fewer of them pay off; kp points at this instance's
 // kernel table in constant memory, made opaque by the caller per matvec.
-template <typename T, int D>
+template <typename T, int D, bool SYM = false>
 __device__ __forceinline__ void tensor_matvec_folded(const cplx<T>* psi, const cplx<T>* v,
                                                      cplx<T>* w, const_f64_ptr kp) {
   constexpr int F = (D - 1) / 2;
@@ -856,11 +856,16 @@ __device__ __forceinline__ void tensor_matvec_folded(const cplx<T>* psi, const c
       T pr = fma(a.x, b.x, -(a.y * b.y));
       T pi = fma(a.x, b.y, a.y * b.x);
       if constexpr (m1 != m2) {
-        const cplx<T> c = psi[F - m1], d = v[F - m2];
-        pr = fma(c.x, d.x, pr);
-        pr = fma(-c.y, d.y, pr);
-        pi = fma(c.x, d.y, pi);
-        pi = fma(c.y, d.x, pi);
+        if constexpr (SYM) {  // v = psi: psi_m1 v_m2 = psi_m2 v_m1
+          pr = pr + pr;
+          pi = pi + pi;
+        } else {
+          const cplx<T> c = psi[F - m1], d = v[F - m2];
+          pr = fma(c.x, d.x, pr);
+          pr = fma(-c.y, d.y, pr);
+          pi = fma(c.x, d.y, pi);
+          pi = fma(c.y, d.x, pi);
+        }
       }
       Pp[decltype(ic)::value] = mk(pr, pi);
     });
@@ -916,6 +921,10 @@ struct LdsAcc {
 // exponential is summed as a Taylor series to working precision, with the step split into
 // nsub pieces when the norm bound theta = 2 |dt| n sum_S |g_S| exceeds 1/2.  Sites with
 // n < 1e-20 are left unchanged (tensor_interaction.jl:212).
+// op_tensor returns the matvecs it ran, plus 1 << kSymShift when the first of them took the
+// symmetric pair sums (FP64, every site that runs the series); the profiling counters of the
+// calling kernels split the two.
+constexpr int kSymShift = 24;
 template <typename T, int D, typename Acc>
 __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double dtf, bool itp,
                                          Acc& acc, const double* kfold) {
@@ -949,10 +958,41 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
     psi0[c] = s[c];
     acc.set(c, s[c]);
   }
-  for (int sub = 0; sub < nsub; ++sub) {
+  // one Taylor term: v <- (-i or -1) hstep/k w, acc += v; true once the rest is negligible
+  auto term = [&](int k) __attribute__((always_inline)) {
+    const T f = (T)(hstep / k);
+    T nv = 0, na = 0;
 #pragma unroll
-    for (int c = 0; c < D; ++c) v[c] = acc.get(c);
-    for (int k = 1; k <= 60; ++k) {
+    for (int c = 0; c < D; ++c) {
+#pragma clang fp contract(off)
+      v[c] = itp ? mk(-w[c].x * f, -w[c].y * f) : mk(w[c].y * f, -w[c].x * f);
+      const cplx<T> a = acc.get(c) + v[c];
+      acc.set(c, a);
+      nv += cabs1(v[c]);
+      na += cabs1(a);
+    }
+    return (T)(2.0 * th_sub / (k + 1)) * nv <= eps * na;
+  };
+  // FP64: the first term of the first sub-step has v = psi, whose pair sums are twice one
+  // product (peeled before the loop, so the loop keeps one matvec body)
+  bool first_done = false;
+  if constexpr (sizeof(T) == 8) {
+    const_f64_ptr kp = (const_f64_ptr)kfold;
+    asm volatile("" : "+s"(kp));
+    tensor_matvec_folded<T, D, true>(psi0, psi0, w, kp);
+    ++nmv;
+    first_done = term(1);
+  }
+  for (int sub = 0; sub < nsub; ++sub) {
+    int k = 1;
+    if (sizeof(T) == 8 && sub == 0) {
+      if (first_done) continue;
+      k = 2;  // v holds the first term
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) v[c] = acc.get(c);
+    }
+    for (; k <= 60; ++k) {
       if constexpr (sizeof(T) == 8) {
         const_f64_ptr kp = (const_f64_ptr)kfold;
         asm volatile("" : "+s"(kp));
@@ -961,24 +1001,13 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
         tensor_matvec<T, D>(psi0, v, w, g, (const_f64_ptr)kCGConst<F>.v);
       }
       ++nmv;
-      const T f = (T)(hstep / k);
-      T nv = 0, na = 0;
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-#pragma clang fp contract(off)
-        v[c] = itp ? mk(-w[c].x * f, -w[c].y * f) : mk(w[c].y * f, -w[c].x * f);
-        const cplx<T> a = acc.get(c) + v[c];
-        acc.set(c, a);
-        nv += cabs1(v[c]);
-        na += cabs1(a);
-      }
-      if ((T)(2.0 * th_sub / (k + 1)) * nv <= eps * na) break;
+      if (term(k)) break;
     }
   }
 #pragma unroll
   for (int c = 0; c < D; ++c) s[c] = acc.get(c);
   if (!itp) restore_norm<T, D>(s, n_in);
-  return nmv;
+  return nmv + (sizeof(T) == 8 ? (1 << kSymShift) : 0);
 }
 
 template <typename T, int D>

@@ -221,17 +221,20 @@ site_kernel(Geom g, DevTables tb, SProgram prog, cplx<T>* __restrict__ psi,
   if (site < g.nsites) site_ops<T, D>(g, tb, prog, psi, phi, s_ops, zf, inst, site, dt, itp, nmv, nrot);
   if (tb.work) {  // profiling: executed tensor matvecs and spin rotations
     // wave sums (no LDS, no barrier), then one atomic per wave into a spread slot
-    int a = nmv, b = nrot;
+    // matvecs, rotations, first matvecs in the symmetric form (op_tensor's tag bits)
+    int a = nmv & ((1 << kSymShift) - 1), b = nrot, c = nmv >> kSymShift;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       a += __shfl_xor(a, o);
       b += __shfl_xor(b, o);
+      c += __shfl_xor(c, o);
     }
     if ((tid & 63) == 0) {
       const unsigned slot = (blockIdx.x * (blockDim.x >> 6) + (tid >> 6)) % kWorkSlots;
       unsigned long long* w = tb.work + (size_t)slot * kWorkStride;
       if (a) atomicAdd(&w[0], (unsigned long long)a);
       if (b) atomicAdd(&w[1], (unsigned long long)b);
+      if (c) atomicAdd(&w[2], (unsigned long long)c);
     }
   }
 }

@@ -209,8 +209,11 @@ int bec_profile_read(bec_ws* ws, double* ms, long long* count);
  * bytes issued by kernel class k (BEC_K_*; 0 for BEC_K_REDUCE and BEC_K_ENERGY), so
  * out[4 + k] / time of class k is its achieved algorithmic bandwidth;
  * out[4 + BEC_K_COUNT] FFT flops of the fused x/site launches (5 N log2 N per complex
- * line of N points).  out holds 5 + BEC_K_COUNT doubles.  Resets the counters.  No
- * counterpart in the reference. */
+ * line of N points); out[5 + BEC_K_COUNT] the matvecs among out[0] that were a site's first
+ * Taylor term in the symmetric form (v = psi: one product per pair; FP64 workspaces), and
+ * out[6 + BEC_K_COUNT] the flops of such a matvec, so the tensor flops are
+ * out[0] out[2] - out[5 + K] (out[2] - out[6 + K]).  out holds 7 + BEC_K_COUNT doubles.
+ * Resets the counters.  No counterpart in the reference. */
 int bec_profile_work(bec_ws* ws, double* out);
 
 /* Spectral derivatives of a real field f[N] (memory order as psi without the component
