@@ -693,6 +693,16 @@ static double tensor_matvec_flops() {
   }
   return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
 }
+// Flops a site's first matvec (v = psi) saves in either form: per pair with m1 != m2 the
+// second complex product and the sum become one doubling (6 + 8 -> 6 + 2).
+template <int F>
+static double first_term_saving() {
+  double f = 0.0;
+  for (int M = -2 * F; M <= 2 * F; ++M)
+    for (int i = 0; i < tens::npairs(F, M); ++i)
+      if (tens::m1lo(F, M) + i != M - (tens::m1lo(F, M) + i)) f += 6.0;
+  return f;
+}
 
 // The same for the channel-folded matvec of the FP64 path (tensor_matvec_folded): pair
 // products, K_M P (complex*real: 2 flop for the first entry of a row, 4 after), w updates.
@@ -710,16 +720,6 @@ static double tensor_matvec_folded_flops() {
     }
   }
   return f + 10.0 * (2 * F + 1);  // v = -i f w, acc += v, |v|_1 and |acc|_1
-}
-// The first matvec of a site's series (v = psi): each pair sum of m1 != m2 is one complex
-// product doubled (6 + 2 flop instead of 6 + 8)
-template <int F>
-static double tensor_matvec_folded_sym_flops() {
-  double f = tensor_matvec_folded_flops<F>();
-  for (int M = -2 * F; M <= 2 * F; ++M)
-    for (int i = 0; i < tens::npairs(F, M); ++i)
-      if (tens::m1lo(F, M) + i != M - (tens::m1lo(F, M) + i)) f -= 6.0;
-  return f;
 }
 
 // One Euler rotation (op_euler): 4 products of the real matrix d^F(pi/2) (or its
@@ -2212,12 +2212,11 @@ int bec_profile_work(bec_ws* ws, double* out) {
   }
   out[4 + BEC_K_COUNT] = ws->prof_fft_flops;
   ws->prof_fft_flops = 0.0;
-  static const double kFoldedSymFlops[7] = {
-      0.0, tensor_matvec_folded_sym_flops<1>(), tensor_matvec_folded_sym_flops<2>(),
-      tensor_matvec_folded_sym_flops<3>(), tensor_matvec_folded_sym_flops<4>(),
-      tensor_matvec_folded_sym_flops<5>(), tensor_matvec_folded_sym_flops<6>()};
+  static const double kFirstSaving[7] = {0.0, first_term_saving<1>(), first_term_saving<2>(),
+                                         first_term_saving<3>(), first_term_saving<4>(),
+                                         first_term_saving<5>(), first_term_saving<6>()};
   out[5 + BEC_K_COUNT] = (double)nsym;
-  out[6 + BEC_K_COUNT] = !ws->tensor ? 0.0 : ws->prec == BEC_FP64 ? kFoldedSymFlops[ws->F] : out[2];
+  out[6 + BEC_K_COUNT] = !ws->tensor ? 0.0 : out[2] - kFirstSaving[ws->F];
   HIPCHK(hipMemset(ws->d_work, 0, kWorkBytes));
   return BEC_OK;
 }

@@ -784,7 +784,7 @@ __device__ __forceinline__ T cg_operand(const_f64_ptr cgp) {
 //   B_SM = sum_{m'} CG(m', M-m') psi_{M-m'} v_{m'},  (h v)_m = sum_{S,M} g_S CG(m, M-m) conj(psi_{M-m}) B_SM,
 // using CG(m1, m2) = CG(m2, m1) for even S to visit each unordered pair once.
 // cgt: table in tens:: order; every index below is a compile-time constant.
-template <typename T, int D>
+template <typename T, int D, bool SYM = false>
 __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>* v, cplx<T>* w,
                                               const T* g, const_f64_ptr cgp) {
   constexpr int F = (D - 1) / 2;
@@ -803,7 +803,10 @@ __device__ __forceinline__ void tensor_matvec(const cplx<T>* psi, const cplx<T>*
       constexpr int m1 = decltype(m1c)::value;
       constexpr int m2 = M - m1;
       cplx<T> p = psi[F - m2] * v[F - m1];
-      if constexpr (m1 != m2) p = p + psi[F - m1] * v[F - m2];
+      if constexpr (m1 != m2) {
+        if constexpr (SYM) p = p + p;  // v = psi
+        else p = p + psi[F - m1] * v[F - m2];
+      }
       static_for<0, NS>([&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
         constexpr int I = tens::off(F, M, m1, S0) + k;
@@ -922,7 +925,7 @@ struct LdsAcc {
 // nsub pieces when the norm bound theta = 2 |dt| n sum_S |g_S| exceeds 1/2.  Sites with
 // n < 1e-20 are left unchanged (tensor_interaction.jl:212).
 // op_tensor returns the matvecs it ran, plus 1 << kSymShift when the first of them took the
-// symmetric pair sums (FP64, every site that runs the series); the profiling counters of the
+// symmetric pair sums (every site that runs the series); the profiling counters of the
 // calling kernels split the two.
 constexpr int kSymShift = 24;
 template <typename T, int D, typename Acc>
@@ -973,19 +976,21 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
     }
     return (T)(2.0 * th_sub / (k + 1)) * nv <= eps * na;
   };
-  // FP64: the first term of the first sub-step has v = psi, whose pair sums are twice one
-  // product (peeled before the loop, so the loop keeps one matvec body)
-  bool first_done = false;
+  // The first term of the first sub-step has v = psi, whose pair sums are twice one product
+  // (peeled before the loop, so the loop keeps one matvec body)
+  bool first_done;
   if constexpr (sizeof(T) == 8) {
     const_f64_ptr kp = (const_f64_ptr)kfold;
     asm volatile("" : "+s"(kp));
     tensor_matvec_folded<T, D, true>(psi0, psi0, w, kp);
-    ++nmv;
-    first_done = term(1);
+  } else {
+    tensor_matvec<T, D, true>(psi0, psi0, w, g, (const_f64_ptr)kCGConst<F>.v);
   }
+  ++nmv;
+  first_done = term(1);
   for (int sub = 0; sub < nsub; ++sub) {
     int k = 1;
-    if (sizeof(T) == 8 && sub == 0) {
+    if (sub == 0) {
       if (first_done) continue;
       k = 2;  // v holds the first term
     } else {
@@ -1007,7 +1012,7 @@ __device__ __forceinline__ int op_tensor(cplx<T>* s, const InstParams& P, double
 #pragma unroll
   for (int c = 0; c < D; ++c) s[c] = acc.get(c);
   if (!itp) restore_norm<T, D>(s, n_in);
-  return nmv + (sizeof(T) == 8 ? (1 << kSymShift) : 0);
+  return nmv + (1 << kSymShift);
 }
 
 template <typename T, int D>

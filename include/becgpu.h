@@ -210,7 +210,7 @@ int bec_profile_read(bec_ws* ws, double* ms, long long* count);
  * out[4 + k] / time of class k is its achieved algorithmic bandwidth;
  * out[4 + BEC_K_COUNT] FFT flops of the fused x/site launches (5 N log2 N per complex
  * line of N points); out[5 + BEC_K_COUNT] the matvecs among out[0] that were a site's first
- * Taylor term in the symmetric form (v = psi: one product per pair; FP64 workspaces), and
+ * Taylor term in the symmetric form (v = psi: one product per pair), and
  * out[6 + BEC_K_COUNT] the flops of such a matvec, so the tensor flops are
  * out[0] out[2] - out[5 + K] (out[2] - out[6 + K]).  out holds 7 + BEC_K_COUNT doubles.
  * Resets the counters.  No counterpart in the reference. */
