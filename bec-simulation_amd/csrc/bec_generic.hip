@@ -12,7 +12,9 @@
 // region and written to another, with the powers of w taken from an N-entry FP64 table
 // (exact integer exponents mod N).  Sums are formed in FP64 for both storage precisions,
 // so a pass costs N * R complex multiply-adds per line: these kernels trade the built
-// lengths' register-resident radix-8 butterflies for generality.
+// lengths' register-resident radix-8 butterflies for generality.  Lengths whose large prime
+// factor would make that sum dominate run Bluestein's algorithm instead (gen_fft_blue,
+// chosen per length by gen_blue_m).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
