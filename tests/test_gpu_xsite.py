@@ -135,3 +135,26 @@ def test_fused_graph_replays_match_eager():
         A.split_step(ge, 2)
         A.split_step(gg, 2, graph=True)
         assert np.array_equal(ge.get_psi(), gg.get_psi()), rep
+
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_autotuner_keeps_fused_tile(precision):
+    """The tile autotuner times the x pass; shapes that run the fused x/site kernel keep
+    the default tile instead (the x pass's timing once picked 16 rows of 32 points for the
+    spin-2 shape below, a tile the fused kernel cannot take, so the shape lost the fused
+    kernel for the rest of the process)."""
+    import becgpu as G
+    ce = (0.3, 0.0, -0.2, 0.0, 0.1, 0.0, 0.05, 0.0, -0.1, 0.0, 0.2)
+    for n, F, rows in (((32, 16), 2, 8), ((64, 32, 16), 6, 1), ((128, 8, 8), 6, 1)):
+        grid = G.make_grid(G.GridConfig(n, tuple(10.0 for _ in n)))
+        psi = random_psi(grid.shape, 2 * F + 1, seed=3, scale=0.2)
+        ws = G.make_workspace(grid=grid, atom=G.AtomSpecies("t", 1.0, F),
+                              interactions=G.InteractionParams(
+                                  5.0, -0.3 if F == 2 else 0.0, 0.0, ce if F == 6 else ()),
+                              potential=G.HarmonicTrap(*([1.0] * len(n))),
+                              sim_params=G.SimParams(dt=1e-3, n_steps=10), psi_init=psi,
+                              precision=precision)          # autotunes (the default)
+        assert ws.set_xsite(True), n
+        assert ws.autotune() == rows, n
+        ws.close()
